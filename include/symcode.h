@@ -1,0 +1,147 @@
+/*
+ * symcode.h -- C ABI of libsymcode_hip.so, an MI355X (gfx950) implementation
+ * of the pseudo-spectral incompressible Navier-Stokes hot path of
+ * agdestein/SymmetryCode.jl.
+ *
+ * Every entry point returns an int status (0 = ok, 1 = invalid argument,
+ * 2 = unsupported size, 3 = HIP error, 9 = internal); sc_last_error() gives
+ * the message.  No exception crosses the ABI.
+ *
+ * Fields are device pointers in the device layout of the context:
+ *   spectral scalar : complex T [nz][n][nhp]  (x fastest; nhp >= n/2+1 pads
+ *                     each x-row to a 128-byte multiple, pad entries unused)
+ *   physical scalar : T [nz][n][n]            (identical to the reference)
+ * with nz = n in 3D and 1 in 2D and T = double (precision 64) or float (32).
+ * A vector / tensor field is ncomp scalars back to back (component stride =
+ * spec_elems or phys_elems from sc_info), in the reference's NamedTuple
+ * order: vectors x,y(,z); symmetric tensors xx,yy,zz,xy,yz,zx (2D xx,yy,xy);
+ * velocity gradients xx,yx,zx,xy,yy,zy,xz,yz,zz (G.ij = d u_i / d x_j).
+ * sc_upload_spectral / sc_download_spectral convert from / to the reference's
+ * host layout ((n/2+1, n, n) column-major complex).
+ *
+ * Calls are synchronous on return by default (the reference's apply!
+ * synchronises after every kernel); sc_set_sync(ctx, 0) makes them
+ * stream-ordered.  One context per host thread; not reentrant.
+ *
+ * Reference interfaces are cited as path:line in agdestein/SymmetryCode.jl.
+ */
+#ifndef SYMCODE_H
+#define SYMCODE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sc_ctx sc_ctx;
+
+/* Grid{D}(; l, n) (src/solver.jl:131-151) + getcache/getplan
+ * (src/solver.jl:956-965, 432).  precision: 64 or 32. */
+int sc_create(int D, int n, double l, int precision, int device, sc_ctx** out);
+int sc_destroy(sc_ctx* ctx);
+/* message of the last failed call on ctx (or of sc_create when ctx is NULL) */
+const char* sc_last_error(const sc_ctx* ctx);
+/* info[0..7] = D, n, nh, nhp, nz, spec_elems, phys_elems, precision */
+int sc_info(const sc_ctx* ctx, int64_t* info);
+/* launch on an external hipStream_t from now on (NULL = the null stream);
+ * the stream created by sc_create is released */
+int sc_set_stream(sc_ctx* ctx, void* hip_stream);
+int sc_set_sync(sc_ctx* ctx, int sync);
+int sc_sync(sc_ctx* ctx);
+
+/* per-launch timing with HIP events on the context's stream (no reference
+ * counterpart; replaces the reference's wall clocks around its loops).
+ * sc_profile(ctx, 1) clears and starts recording; sc_profile_report writes
+ * one line "kernel count total_ms" per kernel into buf. */
+int sc_profile(sc_ctx* ctx, int enable);
+int sc_profile_report(sc_ctx* ctx, char* buf, int buflen);
+
+/* layout conversion, host reference layout <-> device layout */
+int sc_upload_spectral(sc_ctx* ctx, void* dev, const void* host, int ncomp);
+int sc_download_spectral(sc_ctx* ctx, void* host, const void* dev, int ncomp);
+int sc_upload_physical(sc_ctx* ctx, void* dev, const void* host, int ncomp);
+int sc_download_physical(sc_ctx* ctx, void* host, const void* dev, int ncomp);
+
+/* to_phys! / to_spec! (src/solver.jl:210-215): unnormalised C2R, and R2C
+ * divided by n^D.  sc_irfft / sc_rfft are the plan's ldiv! (normalised C2R)
+ * and mul! (unnormalised R2C) used by dealias_phys! (src/solver.jl:218-223).
+ * The spectral input is left intact (the reference's C2R destroys it). */
+int sc_to_phys(sc_ctx* ctx, void* phys, const void* spec, int ncomp);
+int sc_to_spec(sc_ctx* ctx, void* spec, const void* phys, int ncomp);
+int sc_irfft(sc_ctx* ctx, void* phys, const void* spec, int ncomp);
+int sc_rfft(sc_ctx* ctx, void* spec, const void* phys, int ncomp);
+
+/* spectral kernels of src/solver.jl and src/filtering.jl */
+int sc_twothirds(sc_ctx* ctx, void* u, int ncomp);                        /* :371-389 */
+int sc_project(sc_ctx* ctx, void* u);                                     /* :350-368 */
+int sc_symmetrize(sc_ctx* ctx, void* u, int ncomp);                       /* :398-430 */
+int sc_viscosity(sc_ctx* ctx, void* sigma, const void* u, double visc);   /* :458-476 */
+int sc_tensordivergence(sc_ctx* ctx, void* du, const void* sigma);        /* :509-521 */
+int sc_vectordivergence(sc_ctx* ctx, void* div, const void* u);           /* :484-493 */
+int sc_vectorgradient(sc_ctx* ctx, void* G, const void* u);               /* :711-733 */
+int sc_strainrate(sc_ctx* ctx, void* S, const void* u);                   /* :776-794 */
+int sc_gaussianfilter(sc_ctx* ctx, void* u, int ncomp, double delta);     /* filtering.jl:44-67 */
+/* cutoff! (src/filtering.jl:28-34): ubar on the coarse context les,
+ * u on the fine context dns (same D and precision) */
+int sc_cutoff(sc_ctx* les, sc_ctx* dns, void* ubar, const void* u, int ncomp);
+int sc_inverse_cutoff(sc_ctx* les, sc_ctx* dns, void* u, const void* ubar, int ncomp); /* :36-42 */
+/* y .+= a .* x over ncomp spectral components (the axpy! of wray3!, :1002) */
+int sc_axpy(sc_ctx* ctx, void* y, const void* x, double a, int ncomp);
+/* make_tracefree! (src/solver.jl:305-317) on a spectral / physical tensor */
+int sc_make_tracefree_spectral(sc_ctx* ctx, void* tau);
+int sc_make_tracefree_physical(sc_ctx* ctx, void* tau);
+
+/* the RHS: nonlinearity! (:434-456), stress! (:478-482) and the DNS RHS
+ * convectiondiffusion!(du, u, grid, cache; visc) (:949-953) */
+int sc_nonlinearity(sc_ctx* ctx, void* sigma, const void* u);
+int sc_stress(sc_ctx* ctx, void* sigma, const void* u, double visc);
+int sc_convectiondiffusion(sc_ctx* ctx, void* du, const void* u, double visc);
+/* wray3!(convectiondiffusion!, u, dt, grid, cache; visc) (:968-1012) */
+int sc_wray3(sc_ctx* ctx, void* u, double dt, double visc);
+/* classic RK4 with projected stages (no reference counterpart; configs use it) */
+int sc_rk4(sc_ctx* ctx, void* u, double dt, double visc);
+/* propose_timestep(u, grid, visc, cache) (:931-947) */
+int sc_propose_timestep(sc_ctx* ctx, const void* u, double visc, double* dt);
+/* linear band forcing f = A u on 0 < |k| <= kband with A = injection /
+ * sum_band |u|^2 (nearest reference: forced_rhs!, src/data.jl:12-23); added
+ * inside sc_convectiondiffusion / sc_wray3 / sc_rk4.  injection <= 0 disables. */
+int sc_set_band_forcing(sc_ctx* ctx, double injection, int kband);
+
+/* FP64 reductions */
+int sc_energy(sc_ctx* ctx, const void* u, int ncomp, double* e);          /* energy, :643-646 */
+int sc_spectraldot(sc_ctx* ctx, const void* u, const void* v, int ncomp,
+                   double* re, double* im);                                /* :708-709 */
+int sc_dissipation(sc_ctx* ctx, const void* u, double visc, double* eps);  /* get_dissipation!, :796-811 */
+int sc_palinstrophy(sc_ctx* ctx, const void* u, double visc, double* p);   /* get_palinstrophy!, :813-827 */
+/* spectrum (:664-699): s[i-1] = shell i energy, i = 1..nshell */
+int sc_spectrum(sc_ctx* ctx, const void* u, int ncomp, int nshell, double* s);
+/* energy of integer shells 0..nshell-1 (getshells, :1080-1131) and their rescaling */
+int sc_shell_energies(sc_ctx* ctx, const void* u, int ncomp, int nshell, double* out);
+int sc_shell_scale(sc_ctx* ctx, void* u, int ncomp, int nshell, const double* factor);
+/* maintain_shell_energy! (:1146-1152) for shells 1..nshell with reference
+ * energies eref[0..nshell-1] from energy_shells (:1135-1141) */
+int sc_maintain_shell_energy(sc_ctx* ctx, void* u, int nshell, const double* eref);
+
+/* initial conditions */
+int sc_randn(sc_ctx* ctx, void* u, int ncomp, uint64_t seed);
+/* randomfield (:576-641) with target shell energies E0[k], k = 0..nshell-1
+ * (E0[k] = totalenergy * profile(k) / sum profile) */
+int sc_randomfield(sc_ctx* ctx, void* u, uint64_t seed, int nshell, const double* E0);
+int sc_taylorgreen(sc_ctx* ctx, void* u, double V0, int doproject);       /* :537-570 */
+
+/* closures (src/closures.jl) on physical fields; tau packed, GG nonsym */
+int sc_gradient_phys(sc_ctx* ctx, void* GG, const void* u);               /* les.jl:11-16 */
+int sc_closure_clark(sc_ctx* ctx, void* tau, const void* GG, double delta);            /* :37-42,64 */
+int sc_closure_smagorinsky(sc_ctx* ctx, void* tau, const void* GG, double cs, double delta); /* :44-51 */
+/* create_dynamic_smagorinsky model (:188-271); coeff receives c */
+int sc_closure_dynsmag(sc_ctx* ctx, void* tau, const void* u, double delta, double* coeff);
+/* les!(du, u, grid, cache; model, visc) (src/les.jl:4-33) with a built-in
+ * closure: 0 none, 1 Clark(p1 = Delta), 2 Smagorinsky(p1 = C_S, p2 = Delta),
+ * 3 dynamic Smagorinsky(p1 = Delta) */
+int sc_les_rhs(sc_ctx* ctx, void* du, const void* u, double visc, int closure, double p1, double p2);
+int sc_les_wray3(sc_ctx* ctx, void* u, double dt, double visc, int closure, double p1, double p2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

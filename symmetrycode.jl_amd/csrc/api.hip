@@ -1,0 +1,382 @@
+// C ABI of libsymcode_hip.so (declared in include/symcode.h).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include "../../include/symcode.h"
+#include "ops.hpp"
+
+struct sc_ctx : sc::Ctx {};
+
+namespace {
+thread_local std::string g_create_err;
+
+template <class F>
+int guard(sc_ctx* c, F&& f) {
+  if (!c) return 1;
+  try {
+    SC_HIP(hipSetDevice(c->device));
+    f();
+    if (c->sync_calls) SC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  } catch (sc::Error& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (std::exception& e) {
+    c->err = e.what();
+    return 9;
+  }
+}
+
+// run f with a typed zero of the context precision
+template <class F>
+void by_prec(sc::Ctx& c, F&& f) {
+  if (c.prec == 64) f(double(0));
+  else f(float(0));
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw sc::Error(1, msg);
+}
+}  // namespace
+
+#define TYPED(c, ...)                                    \
+  by_prec(*(c), [&](auto z_) {                           \
+    using T = decltype(z_);                              \
+    using CT = sc::C<T>;                                 \
+    (void)sizeof(CT);                                    \
+    __VA_ARGS__;                                         \
+  })
+
+extern "C" {
+
+int sc_create(int D, int n, double l, int precision, int device, sc_ctx** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (D != 2 && D != 3) { g_create_err = "D must be 2 or 3"; return 1; }
+  if (n <= 0 || n % 2 != 0) { g_create_err = "Only even number of grid points supported."; return 1; }
+  if (precision != 32 && precision != 64) { g_create_err = "precision must be 32 or 64"; return 1; }
+  if (!(l > 0)) { g_create_err = "l must be positive"; return 1; }
+  sc_ctx* c = new sc_ctx();
+  c->D = D; c->n = n; c->l = l; c->prec = precision; c->device = device;
+  c->nh = n / 2 + 1;
+  const int align = 128 / (precision == 64 ? 16 : 8);
+  c->nhp = (c->nh + align - 1) / align * align;
+  c->nz = D == 3 ? n : 1;
+  c->spec_elems = long(c->nhp) * n * c->nz;
+  c->phys_elems = long(n) * n * c->nz;
+  try {
+    SC_HIP(hipSetDevice(device));
+    SC_HIP(hipStreamCreate(&c->stream));
+    c->own_stream = true;
+    bool ok = precision == 64 ? sc::Ops<double>::supported(n) : sc::Ops<float>::supported(n);
+    if (ok) {
+      if (precision == 64) sc::Ops<double>::init_tables(*c);
+      else sc::Ops<float>::init_tables(*c);
+    }
+  } catch (std::exception& e) {
+    g_create_err = e.what();
+    delete c;
+    return 3;
+  }
+  *out = c;
+  return 0;
+}
+
+int sc_destroy(sc_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* bufs[] = {c->tw_n, c->tw_m, c->bufA, c->bufB, c->scr, c->ustart, c->du, c->acc, c->ust,
+                  c->dscr, c->vmax, c->forceA};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (void* b : c->ws)
+    if (b) (void)hipFree(b);
+  if (c->hbuf) (void)hipHostFree(c->hbuf);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char* sc_last_error(const sc_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int sc_info(const sc_ctx* c, int64_t* info) {
+  if (!c || !info) return 1;
+  info[0] = c->D; info[1] = c->n; info[2] = c->nh; info[3] = c->nhp; info[4] = c->nz;
+  info[5] = c->spec_elems; info[6] = c->phys_elems; info[7] = c->prec;
+  return 0;
+}
+
+int sc_set_stream(sc_ctx* c, void* s) {
+  return guard(c, [&] {
+    SC_HIP(hipStreamSynchronize(c->stream));
+    if (c->own_stream) SC_HIP(hipStreamDestroy(c->stream));
+    c->own_stream = false;
+    c->stream = (hipStream_t)s;
+  });
+}
+int sc_set_sync(sc_ctx* c, int s) { if (!c) return 1; c->sync_calls = s != 0; return 0; }
+int sc_sync(sc_ctx* c) { return guard(c, [&] { SC_HIP(hipStreamSynchronize(c->stream)); }); }
+
+int sc_profile(sc_ctx* c, int enable) {
+  return guard(c, [&] {
+    SC_HIP(hipStreamSynchronize(c->stream));
+    c->prof.clear();
+    c->ev_used = 0;
+    c->prof_on = enable != 0;
+  });
+}
+int sc_profile_report(sc_ctx* c, char* buf, int buflen) {
+  return guard(c, [&] {
+    need(buf && buflen > 0, "buffer");
+    SC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<std::string> names;
+    std::vector<double> tot;
+    std::vector<long> cnt;
+    for (auto& r : c->prof) {
+      float ms = 0;
+      SC_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      size_t i = 0;
+      while (i < names.size() && names[i] != r.name) ++i;
+      if (i == names.size()) { names.push_back(r.name); tot.push_back(0); cnt.push_back(0); }
+      tot[i] += ms;
+      cnt[i] += 1;
+    }
+    std::string out;
+    for (size_t i = 0; i < names.size(); ++i)
+      out += names[i] + " " + std::to_string(cnt[i]) + " " + std::to_string(tot[i]) + "\n";
+    std::strncpy(buf, out.c_str(), size_t(buflen) - 1);
+    buf[buflen - 1] = 0;
+  });
+}
+
+int sc_upload_spectral(sc_ctx* c, void* dev, const void* host, int ncomp) {
+  return guard(c, [&] {
+    need(dev && host && ncomp > 0, "null pointer or ncomp");
+    const size_t e = c->celem();
+    const size_t rows = size_t(c->n) * c->nz;
+    for (int k = 0; k < ncomp; ++k) {
+      char* d = (char*)dev + size_t(k) * c->spec_elems * e;
+      const char* h = (const char*)host + size_t(k) * c->nh * rows * e;
+      SC_HIP(hipMemcpy2DAsync(d, c->nhp * e, h, c->nh * e, c->nh * e, rows, hipMemcpyHostToDevice, c->stream));
+    }
+  });
+}
+int sc_download_spectral(sc_ctx* c, void* host, const void* dev, int ncomp) {
+  return guard(c, [&] {
+    need(dev && host && ncomp > 0, "null pointer or ncomp");
+    const size_t e = c->celem();
+    const size_t rows = size_t(c->n) * c->nz;
+    for (int k = 0; k < ncomp; ++k) {
+      const char* d = (const char*)dev + size_t(k) * c->spec_elems * e;
+      char* h = (char*)host + size_t(k) * c->nh * rows * e;
+      SC_HIP(hipMemcpy2DAsync(h, c->nh * e, d, c->nhp * e, c->nh * e, rows, hipMemcpyDeviceToHost, c->stream));
+    }
+    SC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+int sc_upload_physical(sc_ctx* c, void* dev, const void* host, int ncomp) {
+  return guard(c, [&] {
+    need(dev && host && ncomp > 0, "null pointer or ncomp");
+    SC_HIP(hipMemcpyAsync(dev, host, size_t(ncomp) * c->phys_elems * c->relem(), hipMemcpyHostToDevice, c->stream));
+  });
+}
+int sc_download_physical(sc_ctx* c, void* host, const void* dev, int ncomp) {
+  return guard(c, [&] {
+    need(dev && host && ncomp > 0, "null pointer or ncomp");
+    SC_HIP(hipMemcpyAsync(host, dev, size_t(ncomp) * c->phys_elems * c->relem(), hipMemcpyDeviceToHost, c->stream));
+    SC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+static void need_fft(sc_ctx* c) {
+  if (!c->tw_n) throw sc::Error(2, "no FFT plan for n = " + std::to_string(c->n));
+}
+static double nD(const sc_ctx* c) { return std::pow(double(c->n), c->D); }
+
+int sc_to_phys(sc_ctx* c, void* phys, const void* spec, int ncomp) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::c2r(*c, (T*)phys, (const CT*)spec, ncomp, 1.0, false)); });
+}
+int sc_to_spec(sc_ctx* c, void* spec, const void* phys, int ncomp) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::r2c(*c, (CT*)spec, (const T*)phys, ncomp, 1.0 / nD(c))); });
+}
+int sc_irfft(sc_ctx* c, void* phys, const void* spec, int ncomp) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::c2r(*c, (T*)phys, (const CT*)spec, ncomp, 1.0 / nD(c), false)); });
+}
+int sc_rfft(sc_ctx* c, void* spec, const void* phys, int ncomp) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::r2c(*c, (CT*)spec, (const T*)phys, ncomp, 1.0)); });
+}
+
+int sc_twothirds(sc_ctx* c, void* u, int ncomp) { return guard(c, [&] { TYPED(c, sc::Ops<T>::twothirds(*c, (CT*)u, ncomp)); }); }
+int sc_project(sc_ctx* c, void* u) { return guard(c, [&] { TYPED(c, sc::Ops<T>::project(*c, (CT*)u)); }); }
+int sc_symmetrize(sc_ctx* c, void* u, int ncomp) { return guard(c, [&] { TYPED(c, sc::Ops<T>::symmetrize(*c, (CT*)u, ncomp)); }); }
+int sc_viscosity(sc_ctx* c, void* s, const void* u, double nu) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::viscosity(*c, (CT*)s, (const CT*)u, nu)); });
+}
+int sc_tensordivergence(sc_ctx* c, void* du, const void* s) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::tensordivergence(*c, (CT*)du, (const CT*)s)); });
+}
+int sc_vectordivergence(sc_ctx* c, void* d, const void* u) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::vectordivergence(*c, (CT*)d, (const CT*)u)); });
+}
+int sc_vectorgradient(sc_ctx* c, void* G, const void* u) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::vectorgradient(*c, (CT*)G, (const CT*)u)); });
+}
+int sc_strainrate(sc_ctx* c, void* S, const void* u) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::strainrate(*c, (CT*)S, (const CT*)u)); });
+}
+int sc_gaussianfilter(sc_ctx* c, void* u, int ncomp, double delta) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::gaussian(*c, (CT*)u, ncomp, delta)); });
+}
+int sc_cutoff(sc_ctx* les, sc_ctx* dns, void* ub, const void* u, int ncomp) {
+  return guard(les, [&] {
+    need(dns && dns->D == les->D && dns->prec == les->prec && dns->n >= les->n, "incompatible contexts");
+    TYPED(les, sc::Ops<T>::cutoff(*les, *dns, (CT*)ub, (const CT*)u, ncomp));
+  });
+}
+int sc_inverse_cutoff(sc_ctx* les, sc_ctx* dns, void* u, const void* ub, int ncomp) {
+  return guard(les, [&] {
+    need(dns && dns->D == les->D && dns->prec == les->prec && dns->n >= les->n, "incompatible contexts");
+    TYPED(les, sc::Ops<T>::inverse_cutoff(*les, *dns, (CT*)u, (const CT*)ub, ncomp));
+  });
+}
+int sc_axpy(sc_ctx* c, void* y, const void* x, double a, int ncomp) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::axpy(*c, (CT*)y, (const CT*)x, a, ncomp)); });
+}
+int sc_make_tracefree_spectral(sc_ctx* c, void* tau) { return guard(c, [&] { TYPED(c, sc::Ops<T>::spec_tracefree(*c, (CT*)tau)); }); }
+int sc_make_tracefree_physical(sc_ctx* c, void* tau) { return guard(c, [&] { TYPED(c, sc::Ops<T>::phys_tracefree(*c, (T*)tau)); }); }
+
+int sc_nonlinearity(sc_ctx* c, void* s, const void* u) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::nonlinearity(*c, (CT*)s, (const CT*)u, nullptr)); });
+}
+int sc_stress(sc_ctx* c, void* s, const void* u, double nu) {
+  return guard(c, [&] {
+    need_fft(c);
+    TYPED(c, sc::Ops<T>::nonlinearity(*c, (CT*)s, (const CT*)u, nullptr); sc::Ops<T>::viscosity(*c, (CT*)s, (const CT*)u, nu));
+  });
+}
+int sc_convectiondiffusion(sc_ctx* c, void* du, const void* u, double nu) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::rhs(*c, (CT*)du, (const CT*)u, nu)); });
+}
+int sc_wray3(sc_ctx* c, void* u, double dt, double nu) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::wray3(*c, (CT*)u, dt, nu)); });
+}
+int sc_rk4(sc_ctx* c, void* u, double dt, double nu) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::rk4(*c, (CT*)u, dt, nu)); });
+}
+int sc_propose_timestep(sc_ctx* c, const void* u, double nu, double* dt) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(dt != nullptr, "null dt");
+    double vmax = 0;
+    TYPED(c, vmax = sc::Ops<T>::max_velocity(*c, (const CT*)u));
+    const double h = c->l / c->n;
+    const double dconv = h / vmax;
+    const double ddiff = h * h / c->D / 2 / nu;
+    *dt = std::fmin(dconv, ddiff);
+  });
+}
+int sc_set_band_forcing(sc_ctx* c, double injection, int kband) {
+  if (!c) return 1;
+  if (injection <= 0) { c->force_kind = 0; return 0; }
+  if (kband < 1 || kband >= c->n / 2) { c->err = "kband out of range"; return 1; }
+  c->force_kind = 1;
+  c->injection = injection;
+  c->kband = kband;
+  return 0;
+}
+
+int sc_energy(sc_ctx* c, const void* u, int ncomp, double* e) {
+  return guard(c, [&] { TYPED(c, *e = sc::Ops<T>::sum_modes(*c, (const CT*)u, ncomp, 0, 0.0) / 2); });
+}
+int sc_spectraldot(sc_ctx* c, const void* u, const void* v, int ncomp, double* re, double* im) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::dot_modes(*c, (const CT*)u, (const CT*)v, ncomp, re, im)); });
+}
+int sc_dissipation(sc_ctx* c, const void* u, double nu, double* eps) {
+  return guard(c, [&] { TYPED(c, *eps = sc::Ops<T>::sum_modes(*c, (const CT*)u, c->D, 1, nu)); });
+}
+int sc_palinstrophy(sc_ctx* c, const void* u, double nu, double* p) {
+  return guard(c, [&] { TYPED(c, *p = sc::Ops<T>::sum_modes(*c, (const CT*)u, c->D, 2, nu)); });
+}
+int sc_spectrum(sc_ctx* c, const void* u, int ncomp, int nshell, double* s) {
+  return guard(c, [&] {
+    need(nshell > 0 && s, "nshell / output");
+    std::vector<double> h(nshell + 1);
+    TYPED(c, sc::Ops<T>::shell_energy(*c, (const CT*)u, ncomp, nshell + 1, h.data()));
+    for (int i = 0; i < nshell; ++i) s[i] = h[i + 1];
+  });
+}
+int sc_shell_energies(sc_ctx* c, const void* u, int ncomp, int nshell, double* out) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::shell_energy(*c, (const CT*)u, ncomp, nshell, out)); });
+}
+int sc_shell_scale(sc_ctx* c, void* u, int ncomp, int nshell, const double* f) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::shell_scale(*c, (CT*)u, ncomp, nshell, f)); });
+}
+int sc_maintain_shell_energy(sc_ctx* c, void* u, int nshell, const double* eref) {
+  return guard(c, [&] {
+    need(nshell > 0 && eref, "nshell / eref");
+    std::vector<double> e(nshell + 1), f(nshell + 1, 1.0);
+    TYPED(c, sc::Ops<T>::shell_energy(*c, (const CT*)u, c->D, nshell + 1, e.data()));
+    for (int i = 1; i <= nshell; ++i) f[i] = std::sqrt(eref[i - 1] / e[i]);
+    TYPED(c, sc::Ops<T>::shell_scale(*c, (CT*)u, c->D, nshell + 1, f.data()));
+  });
+}
+
+int sc_randn(sc_ctx* c, void* u, int ncomp, uint64_t seed) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::randn(*c, (CT*)u, ncomp, seed)); });
+}
+int sc_randomfield(sc_ctx* c, void* u, uint64_t seed, int nshell, const double* E0) {
+  return guard(c, [&] {
+    need(nshell > 0 && E0, "nshell / E0");
+    TYPED(c, {
+      CT* uu = (CT*)u;
+      sc::Ops<T>::randn(*c, uu, c->D, seed);
+      sc::Ops<T>::symmetrize(*c, uu, c->D);
+      sc::Ops<T>::project(*c, uu);
+      sc::Ops<T>::twothirds(*c, uu, c->D);
+      std::vector<double> e(nshell), f(nshell);
+      sc::Ops<T>::shell_energy(*c, uu, c->D, nshell, e.data());
+      for (int k = 0; k < nshell; ++k) f[k] = e[k] > 0 ? std::sqrt(E0[k] / e[k]) : 0.0;
+      sc::Ops<T>::shell_scale(*c, uu, c->D, nshell, f.data());
+      sc::Ops<T>::twothirds(*c, uu, c->D);
+    });
+  });
+}
+int sc_taylorgreen(sc_ctx* c, void* u, double V0, int doproject) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::taylorgreen(*c, (CT*)u, V0, doproject)); });
+}
+
+int sc_gradient_phys(sc_ctx* c, void* GG, const void* u) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::gradient_phys(*c, (T*)GG, (const CT*)u)); });
+}
+int sc_closure_clark(sc_ctx* c, void* tau, const void* GG, double delta) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::clark(*c, (T*)tau, (const T*)GG, delta)); });
+}
+int sc_closure_smagorinsky(sc_ctx* c, void* tau, const void* GG, double cs, double delta) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::smagorinsky(*c, (T*)tau, (const T*)GG, cs, delta)); });
+}
+int sc_closure_dynsmag(sc_ctx* c, void* tau, const void* u, double delta, double* coeff) {
+  return guard(c, [&] {
+    need_fft(c);
+    double cc = 0;
+    TYPED(c, cc = sc::Ops<T>::dynsmag(*c, (T*)tau, (const CT*)u, delta));
+    if (coeff) *coeff = cc;
+  });
+}
+int sc_les_rhs(sc_ctx* c, void* du, const void* u, double nu, int closure, double p1, double p2) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(closure >= 0 && closure <= 3, "closure kind");
+    TYPED(c, sc::Ops<T>::les_rhs(*c, (CT*)du, (const CT*)u, nu, closure, p1, p2));
+  });
+}
+int sc_les_wray3(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(closure >= 0 && closure <= 3, "closure kind");
+    TYPED(c, sc::Ops<T>::les_wray3(*c, (CT*)u, dt, nu, closure, p1, p2));
+  });
+}
+
+}  // extern "C"

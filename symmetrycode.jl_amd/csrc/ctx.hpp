@@ -1,0 +1,133 @@
+// Library context: one grid (D, n, l), one precision, one device, one stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "solver_kernels.hpp"
+
+namespace sc {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SC_HIP(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::sc::Error(3, std::string(#call) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+struct Ctx {
+  int D = 3, n = 0, nh = 0, nhp = 0, nz = 0, prec = 64, device = 0;
+  double l = 0;
+  long spec_elems = 0, phys_elems = 0;  // per component
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool sync_calls = true;
+  void* tw_n = nullptr;  // forward twiddles exp(-2 pi i k / n), k < n
+  void* tw_m = nullptr;  // forward twiddles exp(-2 pi i k / (n/2)), k < n/2
+  // lazily allocated scratch (device layout)
+  void* bufA = nullptr;   // D spectral comps: inverse pipeline
+  void* bufB = nullptr;   // 6 spectral comps: forward pipeline / stress
+  void* scr = nullptr;    // general spectral scratch
+  int scr_comp = 0;
+  void* ustart = nullptr; // RK storage (D comps each)
+  void* du = nullptr;
+  void* acc = nullptr;
+  void* ust = nullptr;
+  double* dscr = nullptr; long dscr_n = 0;
+  unsigned long long* vmax = nullptr;
+  double* forceA = nullptr;
+  void* ws[16] = {};       // named workspaces (closures, LES)
+  size_t ws_bytes[16] = {};
+  double* hbuf = nullptr;  // pinned host scratch
+  long hbuf_n = 0;
+  // forcing applied inside the RHS: 0 none, 1 linear band
+  int force_kind = 0;
+  double injection = 0.1;
+  int kband = 2;
+  std::string err;
+  // per-launch HIP-event timing (sc_profile)
+  struct ProfRec { const char* name; hipEvent_t a, b; };
+  bool prof_on = false;
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      SC_HIP(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  void prof_begin(const char* name) {
+    if (!prof_on) return;
+    ProfRec r{name, next_event(), next_event()};
+    SC_HIP(hipEventRecord(r.a, stream));
+    prof.push_back(r);
+  }
+  void prof_end() {
+    if (!prof_on) return;
+    SC_HIP(hipEventRecord(prof.back().b, stream));
+  }
+
+  int tdim() const { return D == 3 ? 6 : 3; }
+  size_t celem() const { return prec == 64 ? 16 : 8; }
+  size_t relem() const { return prec == 64 ? 8 : 4; }
+  SpecGeom geom() const {
+    SpecGeom g;
+    g.D = D; g.n = n; g.nh = nh; g.nhp = nhp; g.nz = nz;
+    g.s_comp = spec_elems; g.total = long(nhp) * n * nz;
+    g.kf = 3.14159265358979323846 / l * 2;
+    return g;
+  }
+  void* ensure_spec(void*& p, int ncomp) {
+    if (!p) {
+      SC_HIP(hipMalloc(&p, size_t(ncomp) * spec_elems * celem()));
+      SC_HIP(hipMemsetAsync(p, 0, size_t(ncomp) * spec_elems * celem(), stream));
+    }
+    return p;
+  }
+  void* scratch(int ncomp) {
+    if (scr_comp < ncomp) {
+      if (scr) SC_HIP(hipFree(scr));
+      scr = nullptr;
+      SC_HIP(hipMalloc(&scr, size_t(ncomp) * spec_elems * celem()));
+      SC_HIP(hipMemsetAsync(scr, 0, size_t(ncomp) * spec_elems * celem(), stream));
+      scr_comp = ncomp;
+    }
+    return scr;
+  }
+  void* work(int slot, size_t bytes) {
+    if (ws_bytes[slot] < bytes) {
+      if (ws[slot]) SC_HIP(hipFree(ws[slot]));
+      ws[slot] = nullptr;
+      SC_HIP(hipMalloc(&ws[slot], bytes));
+      SC_HIP(hipMemsetAsync(ws[slot], 0, bytes, stream));
+      ws_bytes[slot] = bytes;
+    }
+    return ws[slot];
+  }
+  double* dscratch(long n_) {
+    if (dscr_n < n_) {
+      if (dscr) SC_HIP(hipFree(dscr));
+      SC_HIP(hipMalloc(&dscr, n_ * sizeof(double)));
+      dscr_n = n_;
+    }
+    return dscr;
+  }
+  double* hostbuf(long n_) {
+    if (hbuf_n < n_) {
+      if (hbuf) SC_HIP(hipHostFree(hbuf));
+      SC_HIP(hipHostMalloc(&hbuf, n_ * sizeof(double)));
+      hbuf_n = n_;
+    }
+    return hbuf;
+  }
+};
+
+}  // namespace sc

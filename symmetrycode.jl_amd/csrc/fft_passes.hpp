@@ -1,0 +1,262 @@
+// FFT pass kernels over the device spectral layout.
+//
+// Device layout of one spectral scalar: C<T>[nz][n][nhp] (x fastest), with
+// nhp = nh = n/2+1 rounded up to a 128-byte multiple so every x-row starts on
+// a cache line.  nz = n in 3D and 1 in 2D.  Physical scalars are T[nz][n][n]
+// (the reference layout).  Multi-component fields store components back to
+// back at a fixed component stride.
+#pragma once
+#include "fft_engine.hpp"
+
+namespace sc {
+
+// Blocks are dispatched round-robin over the 8 XCDs.  Remap so that
+// consecutive logical tiles (which share 128-byte lines and L2 working sets)
+// land on the same XCD at about the same time.
+DEV int xcd_remap(int b, int nb) {
+  int q = nb >> 3;
+  if (b >= (q << 3)) return b;
+  return (b & 7) * q + (b >> 3);
+}
+
+// pencils per block along x for the strided passes: a 128-byte row segment,
+// halved until the LDS tile fits in 64 KiB
+template <typename T, int N> constexpr int tile_x() {
+  int tx = 128 / int(sizeof(C<T>));
+  while (tx > 1 && tx * N * int(sizeof(C<T>)) > 65536) tx /= 2;
+  return tx;
+}
+
+// ------------------------------------------------------------------
+// Strided complex pass along y or z.  Each block transforms TX consecutive-x
+// pencils at one "other" index (z for a y-pass, y for a z-pass).
+struct StridedGeom {
+  int xs;             // process x in [0, xs)
+  int ntile;          // ceil(xs / TX)
+  int nother;         // number of other-axis indices processed
+  int other_lo;       // other index o maps to o if o < other_lo, else o + other_gap
+  int other_gap;
+  int n_other;        // length of the other axis (for its frequency)
+  long s_axis;        // element stride along the transform axis
+  long s_other;       // element stride along the other axis
+  long s_comp;        // component stride
+  int kin;            // read only axis positions with |freq| <= kin
+  int mask23;         // apply the 2/3 box mask (and zero mode) on load
+  int kcut;           // n / 3, for mask23
+  double scale;       // output scale
+};
+
+template <typename T, int N, int SGN>
+__global__ void __launch_bounds__((tile_x<T, N>() * Plan<N>::tpp))
+k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw, StridedGeom g) {
+  constexpr int TX = tile_x<T, N>();
+  using PE = Pencil<T, N, SGN>;
+  __shared__ C<T> lds[TX * N];
+  const int nb = g.ntile * g.nother;
+  const int b = xcd_remap(blockIdx.x, nb);
+  const int tile = b % g.ntile;
+  const int o = b / g.ntile;
+  const int oi = o < g.other_lo ? o : o + g.other_gap;
+  const int t = threadIdx.x;
+  const int xl = t % TX, p = t / TX;
+  const int x = tile * TX + xl;
+  const bool xok = x < g.xs;
+  const long base = long(blockIdx.y) * g.s_comp + long(oi) * g.s_other + x;
+  const C<T>* src = in + base;
+  C<T>* dst = out + base;
+  bool xmask = true;
+  bool zero_o = false;
+  if (g.mask23) {
+    int ko = freq(oi, g.n_other);
+    xmask = (x <= g.kcut) && (ko <= g.kcut && ko >= -g.kcut);
+    zero_o = (x == 0) && (ko == 0);
+  }
+  C<T> v[PE::MAXE];
+  PE::load(v, p, [&](int j) {
+    int kj = freq(j, N);
+    bool ok = xok && xmask && kj <= g.kin && kj >= -g.kin;
+    if (zero_o && j == 0) ok = false;
+    return ok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0));
+  });
+  PE::run(v, p, lds + xl, TX, tw);
+  if (xok) {
+    const T sc_ = T(g.scale);
+    PE::store(v, p, [&](int k, C<T> val) { dst[long(k) * g.s_axis] = scl<T>(val, sc_); });
+  }
+}
+
+// ------------------------------------------------------------------
+// x axis: real rows of length n = 2M <-> Hermitian half rows of nh = M+1,
+// through one complex FFT of length M on z[m] = x[2m] + i x[2m+1].
+struct RowGeom {
+  long nrows;      // rows processed (row r: spectral offset r*nhp, physical r*n)
+  int nhp;         // spectral row pitch
+  int n;           // physical row length / pitch
+  int kin;         // spectral inputs read only for kx <= kin
+  long s_comp_spec, s_comp_phys;
+  double scale;
+};
+
+template <int M> constexpr int rows_per_block() {
+  int tpp = Plan<M>::tpp;
+  int r = 256 / tpp;
+  return r < 1 ? 1 : r;
+}
+
+// Build the length-M FFT input of a C2R row from the half spectrum X[0..M]:
+// Zin[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/n} (X[k] - conj X[M-k]).
+// The imaginary parts of X[0] and X[M] are discarded (the convention of every
+// multi-dimensional C2R for its halved axis).
+template <typename T, int M, class Get>
+DEV C<T> c2r_pre(int k, const C<T>* __restrict__ twn, Get&& get) {
+  C<T> a = get(k);
+  C<T> b = get(M - k);
+  if (k == 0) { a.y = T(0); b.y = T(0); }
+  b = conjc<T>(b);
+  C<T> w = conjc<T>(twn[k]);  // exp(+2 pi i k / n)
+  C<T> s = add<T>(a, b), d = sub<T>(a, b);
+  return add<T>(s, muli<1, T>(cmul<T>(w, d)));
+}
+
+// R2C post-processing: X[k] = 1/2 (Z[k] + conj Z[M-k]) - i/2 W^k (Z[k] - conj Z[M-k]),
+// W = exp(-2 pi i / n), Z[M] = Z[0].
+template <typename T, int M>
+DEV C<T> r2c_post(int k, const C<T>* zl, const C<T>* __restrict__ twn) {
+  C<T> a = zl[k == M ? 0 : k];
+  C<T> b = conjc<T>(zl[k == 0 ? 0 : M - k]);
+  C<T> s = add<T>(a, b), d = sub<T>(a, b);
+  C<T> wd = cmul<T>(twn[k], d);
+  C<T> r = sub<T>(s, muli<1, T>(wd));
+  return scl<T>(r, T(0.5));
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__((rows_per_block<M>() * Plan<M>::tpp))
+k_x_c2r(const C<T>* __restrict__ in, T* out, const C<T>* __restrict__ twm,
+        const C<T>* __restrict__ twn, RowGeom g) {
+  constexpr int TY = rows_per_block<M>();
+  using PE = Pencil<T, M, +1>;
+  __shared__ C<T> lds[TY * M];
+  const int t = threadIdx.x;
+  const int ry = t / PE::TPP, p = t % PE::TPP;
+  const long row = long(blockIdx.x) * TY + ry;
+  const bool rok = (ry < TY) && row < g.nrows;
+  const C<T>* src = in + long(blockIdx.y) * g.s_comp_spec + row * g.nhp;
+  T* dst = out + long(blockIdx.y) * g.s_comp_phys + row * g.n;
+  C<T> v[PE::MAXE];
+  PE::load(v, p, [&](int k) {
+    if (!rok) return mk<T>(T(0), T(0));
+    return c2r_pre<T, M>(k, twn, [&](int kk) { return kk <= g.kin ? src[kk] : mk<T>(T(0), T(0)); });
+  });
+  PE::run(v, p, lds + ry * M, 1, twm);
+  if (rok) {
+    const T s = T(g.scale);
+    typedef C<T> C2;
+    PE::store(v, p, [&](int m, C2 z) { reinterpret_cast<C2*>(dst)[m] = scl<T>(z, s); });
+  }
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__((rows_per_block<M>() * Plan<M>::tpp))
+k_x_r2c(const T* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm,
+        const C<T>* __restrict__ twn, RowGeom g) {
+  constexpr int TY = rows_per_block<M>();
+  using PE = Pencil<T, M, -1>;
+  __shared__ C<T> lds[TY * M];
+  const int t = threadIdx.x;
+  const int ry = t / PE::TPP, p = t % PE::TPP;
+  const long row = long(blockIdx.x) * TY + ry;
+  const bool rok = (ry < TY) && row < g.nrows;
+  const T* src = in + long(blockIdx.y) * g.s_comp_phys + row * g.n;
+  C<T>* dst = out + long(blockIdx.y) * g.s_comp_spec + row * g.nhp;
+  C<T> v[PE::MAXE];
+  PE::load(v, p, [&](int m) {
+    return rok ? reinterpret_cast<const C<T>*>(src)[m] : mk<T>(T(0), T(0));
+  });
+  PE::run(v, p, lds + ry * M, 1, twm);
+  C<T>* zl = lds + ry * M;
+  PE::out_to_lds(v, p, zl, 1);
+  __syncthreads();
+  if (rok) {
+    const T s = T(g.scale);
+    for (int k = p; k <= M; k += PE::TPP) dst[k] = scl<T>(r2c_post<T, M>(k, zl, twn), s);
+  }
+}
+
+// ------------------------------------------------------------------
+// Fused x-pass of the nonlinear term: per (y,z) row, C2R of the D velocity
+// components (inputs band-limited to kx <= kin), the D(D+1)/2 pointwise
+// products v_i v_j, and R2C of each product (times scale).  Also the
+// row-wise max of |v| for the CFL time step.
+template <typename T, int M, int D>
+__global__ void __launch_bounds__((rows_per_block<M>() * Plan<M>::tpp))
+k_x_products(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm,
+             const C<T>* __restrict__ twn, RowGeom g, unsigned long long* vmax_bits) {
+  constexpr int TY = rows_per_block<M>();
+  constexpr int NP = D == 3 ? 6 : 3;
+  using PI = Pencil<T, M, +1>;
+  using PF = Pencil<T, M, -1>;
+  constexpr int E = PlanInfo<M>::E_LAST;
+  __shared__ C<T> lds[TY * M];
+  __shared__ double red[TY * Plan<M>::tpp];
+  const int t = threadIdx.x;
+  const int ry = t / PI::TPP, p = t % PI::TPP;
+  const long row = long(blockIdx.x) * TY + ry;
+  const bool rok = (ry < TY) && row < g.nrows;
+  C<T>* ll = lds + ry * M;
+  C<T> u[D][E];
+  C<T> v[PlanInfo<M>::MAXE];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const C<T>* src = in + long(c) * g.s_comp_spec + row * g.nhp;
+    PI::load(v, p, [&](int k) {
+      if (!rok) return mk<T>(T(0), T(0));
+      return c2r_pre<T, M>(k, twn, [&](int kk) { return kk <= g.kin ? src[kk] : mk<T>(T(0), T(0)); });
+    });
+    PI::run(v, p, ll, 1, twm);
+#pragma unroll
+    for (int e = 0; e < E; ++e) u[c][e] = v[e];
+  }
+  // CFL: max |v| over the row's points
+  double vm = 0.0;
+  PI::for_out(p, [&](int e, int) {
+    double a = 0, b = 0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      a += double(u[c][e].x) * double(u[c][e].x);
+      b += double(u[c][e].y) * double(u[c][e].y);
+    }
+    vm = fmax(vm, fmax(a, b));
+  });
+  if (vmax_bits != nullptr) {
+    red[t] = rok ? vm : 0.0;
+    __syncthreads();
+    for (int s = (TY * PI::TPP) / 2; s > 0; s >>= 1) {
+      if (t < s && t + s < TY * PI::TPP) red[t] = fmax(red[t], red[t + s]);
+      __syncthreads();
+    }
+    if (t == 0) atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(sqrt(red[0])));
+  }
+  constexpr int PA[6] = {0, 1, 2, 0, 1, 2};
+  constexpr int PB[6] = {0, 1, 2, 1, 2, 0};
+  constexpr int QA[3] = {0, 1, 0};
+  constexpr int QB[3] = {0, 1, 1};
+  const T s = T(g.scale);
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int i = D == 3 ? PA[q] : QA[q];
+    const int j = D == 3 ? PB[q] : QB[q];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
+    PI::out_to_in(v, p, ll, 1);
+    PF::run(v, p, ll, 1, twm);
+    PF::out_to_lds(v, p, ll, 1);
+    __syncthreads();
+    if (rok) {
+      C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
+      for (int k = p; k <= M; k += PI::TPP) dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn), s);
+    }
+  }
+}
+
+}  // namespace sc

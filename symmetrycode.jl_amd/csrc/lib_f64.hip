@@ -1,0 +1,3 @@
+// Double-precision instantiation of the typed operations.
+#include "ops_impl.hpp"
+template struct sc::Ops<double>;

@@ -1,0 +1,59 @@
+// Typed operations of the library (instantiated for float and double in
+// lib_f32.hip / lib_f64.hip).  All launch on ctx.stream; pointers are device
+// pointers in the device layout.
+#pragma once
+#include "ctx.hpp"
+
+namespace sc {
+
+template <typename T>
+struct Ops {
+  using CT = C<T>;
+  static bool supported(int n);
+  static void init_tables(Ctx& c);
+  // 1D passes and 3D transforms
+  static void c2r(Ctx& c, T* phys, const CT* spec, int ncomp, double scale, bool mask23);
+  static void r2c(Ctx& c, CT* spec, const T* phys, int ncomp, double scale);
+  // spectral kernels
+  static void twothirds(Ctx& c, CT* u, int ncomp);
+  static void project(Ctx& c, CT* u);
+  static void viscosity(Ctx& c, CT* sig, const CT* u, double nu);
+  static void tensordivergence(Ctx& c, CT* du, const CT* sig);
+  static void vectorgradient(Ctx& c, CT* G, const CT* u);
+  static void strainrate(Ctx& c, CT* S, const CT* u);
+  static void vectordivergence(Ctx& c, CT* d, const CT* u);
+  static void gaussian(Ctx& c, CT* u, int ncomp, double delta);
+  static void axpy(Ctx& c, CT* y, const CT* x, double a, int ncomp);
+  static void spec_tracefree(Ctx& c, CT* tau);
+  // the RHS
+  static void nonlinearity(Ctx& c, CT* sig, const CT* u, unsigned long long* vmax);
+  static void rhs(Ctx& c, CT* du, const CT* u, double nu);
+  static void wray3(Ctx& c, CT* u, double dt, double nu);
+  static void rk4(Ctx& c, CT* u, double dt, double nu);
+  static double max_velocity(Ctx& c, const CT* u);
+  // reductions (return to host)
+  static double sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu);
+  static void dot_modes(Ctx& c, const CT* u, const CT* v, int ncomp, double* re, double* im);
+  static void shell_energy(Ctx& c, const CT* u, int ncomp, int nshell, double* host_out);
+  static void shell_scale(Ctx& c, CT* u, int ncomp, int nshell, const double* host_factor);
+  // initial conditions
+  static void randn(Ctx& c, CT* u, int ncomp, unsigned long long seed);
+  static void symmetrize(Ctx& c, CT* u, int ncomp);
+  static void taylorgreen(Ctx& c, CT* u, double V0, int doproject);
+  // filtering between two contexts (coarse <- fine)
+  static void cutoff(Ctx& cb, Ctx& cf, CT* ub, const CT* u, int ncomp);
+  static void inverse_cutoff(Ctx& cb, Ctx& cf, CT* u, const CT* ub, int ncomp);
+  // closures (physical packed tensors)
+  static void clark(Ctx& c, T* tau, const T* GG, double delta);
+  static void smagorinsky(Ctx& c, T* tau, const T* GG, double cs, double delta);
+  static void gradient_phys(Ctx& c, T* GG, const CT* u);
+  static double dynsmag(Ctx& c, T* tau, const CT* u, double delta);
+  static void les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2);
+  static void phys_tracefree(Ctx& c, T* tau);
+  static void les_wray3(Ctx& c, CT* u, double dt, double nu, int closure, double p1, double p2);
+  static void launch_phys(Ctx& c, T* y, const T* x, T a, T b, long n);
+  static void smag_tensor(Ctx& c, T* m, const T* S, double delta);
+  static void ml_mm(Ctx& c, const T* M, const T* L, double* ml, double* mm);
+};
+
+}  // namespace sc

@@ -1,0 +1,609 @@
+// Host-side launchers of the typed operations (included once per precision).
+#pragma once
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "ops.hpp"
+#include "fft_passes.hpp"
+
+namespace sc {
+
+#define SC_FOR_N(X) X(4) X(6) X(8) X(12) X(16) X(24) X(30) X(32) X(40) X(48) X(64) X(96) X(128) X(256) X(512) X(810)
+#define SC_FOR_M(X) X(2) X(3) X(4) X(6) X(8) X(12) X(15) X(16) X(20) X(24) X(32) X(48) X(64) X(128) X(256) X(405)
+
+static inline unsigned nblk(long total, int bs = 256) { return unsigned((total + bs - 1) / bs); }
+
+template <typename T> static int tile_x_rt(int N) {
+  switch (N) {
+#define X(NN) case NN: return tile_x<T, NN>();
+    SC_FOR_N(X)
+#undef X
+  }
+  throw Error(2, "unsupported transform length " + std::to_string(N));
+}
+static int tpp_rt(int N) {
+  switch (N) {
+#define X(NN) case NN: return Plan<NN>::tpp;
+    SC_FOR_N(X)
+    X(2) X(3) X(15) X(20) X(405)
+#undef X
+  }
+  throw Error(2, "unsupported transform length " + std::to_string(N));
+}
+static int rows_rt(int M) {
+  switch (M) {
+#define X(MM) case MM: return rows_per_block<MM>();
+    SC_FOR_M(X)
+#undef X
+  }
+  throw Error(2, "unsupported half length " + std::to_string(M));
+}
+
+template <typename T>
+bool Ops<T>::supported(int n) {
+  switch (n) {
+#define X(NN) case NN: return true;
+    SC_FOR_N(X)
+#undef X
+  }
+  return false;
+}
+
+// strided complex pass along axis (1 = y, 2 = z) of length n
+template <typename T>
+static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int ncomp, int xs, bool prune_other,
+                     int kin, bool mask23, double scale) {
+  const int n = c.n;
+  const int TX = tile_x_rt<T>(n);
+  StridedGeom g;
+  g.xs = xs;
+  g.ntile = (xs + TX - 1) / TX;
+  const int kc = n / 3;
+  const int nother_full = axis == 1 ? c.nz : n;
+  if (prune_other && axis == 2) {
+    g.nother = 2 * kc + 1;
+    g.other_lo = kc + 1;
+    g.other_gap = n - 2 * kc - 1;
+  } else {
+    g.nother = nother_full;
+    g.other_lo = nother_full;
+    g.other_gap = 0;
+  }
+  g.n_other = axis == 1 ? c.nz : n;
+  g.s_axis = axis == 1 ? long(c.nhp) : long(c.nhp) * n;
+  g.s_other = axis == 1 ? long(c.nhp) * n : long(c.nhp);
+  g.s_comp = c.spec_elems;
+  g.kin = kin;
+  g.mask23 = mask23 ? 1 : 0;
+  g.kcut = kc;
+  g.scale = scale;
+  dim3 grid(unsigned(g.ntile) * unsigned(g.nother), unsigned(ncomp));
+  const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
+  static const char* names[2][2] = {{"c2c_y_fwd", "c2c_y_inv"}, {"c2c_z_fwd", "c2c_z_inv"}};
+  c.prof_begin(names[axis - 1][sign > 0]);
+  switch (n) {
+#define X(NN)                                                                                        \
+  case NN:                                                                                           \
+    if (sign < 0)                                                                                    \
+      hipLaunchKernelGGL((k_c2c_strided<T, NN, -1>), grid, dim3(tile_x<T, NN>() * Plan<NN>::tpp), 0, \
+                         c.stream, in, out, tw, g);                                                  \
+    else                                                                                             \
+      hipLaunchKernelGGL((k_c2c_strided<T, NN, 1>), grid, dim3(tile_x<T, NN>() * Plan<NN>::tpp), 0,  \
+                         c.stream, in, out, tw, g);                                                  \
+    break;
+    SC_FOR_N(X)
+#undef X
+    default: throw Error(2, "unsupported grid size " + std::to_string(n));
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static RowGeom row_geom(const Ctx& c, int kin, double scale) {
+  RowGeom g;
+  g.nrows = long(c.n) * c.nz;
+  g.nhp = c.nhp;
+  g.n = c.n;
+  g.kin = kin;
+  g.s_comp_spec = c.spec_elems;
+  g.s_comp_phys = c.phys_elems;
+  g.scale = scale;
+  return g;
+}
+
+template <typename T>
+static void x_c2r(Ctx& c, T* out, const C<T>* in, int ncomp, int kin, double scale) {
+  const int M = c.n / 2;
+  RowGeom g = row_geom<T>(c, kin, scale);
+  const int TY = rows_rt(M);
+  dim3 grid(unsigned((g.nrows + TY - 1) / TY), unsigned(ncomp));
+  auto twm = reinterpret_cast<const C<T>*>(c.tw_m);
+  auto twn = reinterpret_cast<const C<T>*>(c.tw_n);
+  c.prof_begin("x_c2r");
+  switch (M) {
+#define X(MM)                                                                                           \
+  case MM:                                                                                              \
+    hipLaunchKernelGGL((k_x_c2r<T, MM>), grid, dim3(rows_per_block<MM>() * Plan<MM>::tpp), 0, c.stream, \
+                       in, out, twm, twn, g);                                                           \
+    break;
+    SC_FOR_M(X)
+#undef X
+    default: throw Error(2, "unsupported grid size");
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void x_r2c(Ctx& c, C<T>* out, const T* in, int ncomp, double scale) {
+  const int M = c.n / 2;
+  RowGeom g = row_geom<T>(c, M, scale);
+  const int TY = rows_rt(M);
+  dim3 grid(unsigned((g.nrows + TY - 1) / TY), unsigned(ncomp));
+  auto twm = reinterpret_cast<const C<T>*>(c.tw_m);
+  auto twn = reinterpret_cast<const C<T>*>(c.tw_n);
+  c.prof_begin("x_r2c");
+  switch (M) {
+#define X(MM)                                                                                           \
+  case MM:                                                                                              \
+    hipLaunchKernelGGL((k_x_r2c<T, MM>), grid, dim3(rows_per_block<MM>() * Plan<MM>::tpp), 0, c.stream, \
+                       in, out, twm, twn, g);                                                           \
+    break;
+    SC_FOR_M(X)
+#undef X
+    default: throw Error(2, "unsupported grid size");
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale, unsigned long long* vmax) {
+  const int M = c.n / 2;
+  RowGeom g = row_geom<T>(c, kin, scale);
+  const int TY = rows_rt(M);
+  dim3 grid(unsigned((g.nrows + TY - 1) / TY), 1);
+  auto twm = reinterpret_cast<const C<T>*>(c.tw_m);
+  auto twn = reinterpret_cast<const C<T>*>(c.tw_n);
+  c.prof_begin("x_products");
+  switch (M) {
+#define X(MM)                                                                                                 \
+  case MM:                                                                                                    \
+    if (c.D == 3)                                                                                             \
+      hipLaunchKernelGGL((k_x_products<T, MM, 3>), grid, dim3(rows_per_block<MM>() * Plan<MM>::tpp), 0,       \
+                         c.stream, in, out, twm, twn, g, vmax);                                               \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_x_products<T, MM, 2>), grid, dim3(rows_per_block<MM>() * Plan<MM>::tpp), 0,       \
+                         c.stream, in, out, twm, twn, g, vmax);                                               \
+    break;
+    SC_FOR_M(X)
+#undef X
+    default: throw Error(2, "unsupported grid size");
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------- tables
+template <typename T>
+void Ops<T>::init_tables(Ctx& c) {
+  const int n = c.n, M = n / 2;
+  std::vector<C<T>> hn(n), hm(M);
+  for (int k = 0; k < n; ++k) {
+    long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+    hn[k] = mk<T>(T(std::cos(a)), T(std::sin(a)));
+  }
+  for (int k = 0; k < M; ++k) {
+    long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)M;
+    hm[k] = mk<T>(T(std::cos(a)), T(std::sin(a)));
+  }
+  SC_HIP(hipMalloc(&c.tw_n, n * sizeof(C<T>)));
+  SC_HIP(hipMalloc(&c.tw_m, (M > 0 ? M : 1) * sizeof(C<T>)));
+  SC_HIP(hipMemcpy(c.tw_n, hn.data(), n * sizeof(C<T>), hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(c.tw_m, hm.data(), M * sizeof(C<T>), hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------- transforms
+// spectral -> physical: z (3D) and y complex inverse passes into scratch,
+// then the x C2R.  mask23 applies twothirds! on the fly (inputs outside the
+// 2/3 box are never read).
+template <typename T>
+void Ops<T>::c2r(Ctx& c, T* phys, const CT* spec, int ncomp, double scale, bool mask23) {
+  CT* s = reinterpret_cast<CT*>(c.scratch(ncomp));
+  const int kc = c.n / 3;
+  const int kin = mask23 ? kc : c.n;
+  const int xs = mask23 ? kc + 1 : c.nh;
+  if (c.D == 3) {
+    c2c_axis<T>(c, 2, +1, spec, s, ncomp, xs, mask23, kin, mask23, 1.0);
+    c2c_axis<T>(c, 1, +1, s, s, ncomp, xs, false, kin, false, 1.0);
+  } else {
+    c2c_axis<T>(c, 1, +1, spec, s, ncomp, xs, false, kin, mask23, 1.0);
+  }
+  x_c2r<T>(c, phys, s, ncomp, mask23 ? kc : c.nh, scale);
+}
+
+template <typename T>
+void Ops<T>::r2c(Ctx& c, CT* spec, const T* phys, int ncomp, double scale) {
+  x_r2c<T>(c, spec, phys, ncomp, scale);
+  c2c_axis<T>(c, 1, -1, spec, spec, ncomp, c.nh, false, c.n, false, 1.0);
+  if (c.D == 3) c2c_axis<T>(c, 2, -1, spec, spec, ncomp, c.nh, false, c.n, false, 1.0);
+}
+
+// ----------------------------------------------------------- spectral ops
+#define SPEC_LAUNCH(kern, ...)                                                             \
+  do {                                                                                     \
+    SpecGeom g_ = c.geom();                                                                \
+    c.prof_begin(#kern);                                                                   \
+    hipLaunchKernelGGL(kern, dim3(nblk(g_.total)), dim3(256), 0, c.stream, __VA_ARGS__);  \
+    c.prof_end();                                                                          \
+    SC_HIP(hipGetLastError());                                                             \
+  } while (0)
+
+template <typename T> void Ops<T>::twothirds(Ctx& c, CT* u, int ncomp) { SPEC_LAUNCH(k_twothirds<T>, u, g_, ncomp); }
+template <typename T> void Ops<T>::project(Ctx& c, CT* u) {
+  if (c.D == 3) SPEC_LAUNCH((k_project<T, 3>), u, g_);
+  else SPEC_LAUNCH((k_project<T, 2>), u, g_);
+}
+template <typename T> void Ops<T>::viscosity(Ctx& c, CT* sig, const CT* u, double nu) {
+  if (c.D == 3) SPEC_LAUNCH((k_viscosity<T, 3>), sig, u, nu, g_);
+  else SPEC_LAUNCH((k_viscosity<T, 2>), sig, u, nu, g_);
+}
+template <typename T> void Ops<T>::tensordivergence(Ctx& c, CT* du, const CT* sig) {
+  if (c.D == 3) SPEC_LAUNCH((k_tensordivergence<T, 3>), du, sig, g_);
+  else SPEC_LAUNCH((k_tensordivergence<T, 2>), du, sig, g_);
+}
+template <typename T> void Ops<T>::vectorgradient(Ctx& c, CT* G, const CT* u) {
+  if (c.D == 3) SPEC_LAUNCH((k_vectorgradient<T, 3>), G, u, g_);
+  else SPEC_LAUNCH((k_vectorgradient<T, 2>), G, u, g_);
+}
+template <typename T> void Ops<T>::strainrate(Ctx& c, CT* S, const CT* u) {
+  if (c.D == 3) SPEC_LAUNCH((k_strainrate<T, 3>), S, u, g_);
+  else SPEC_LAUNCH((k_strainrate<T, 2>), S, u, g_);
+}
+template <typename T> void Ops<T>::vectordivergence(Ctx& c, CT* d, const CT* u) {
+  if (c.D == 3) SPEC_LAUNCH((k_vectordivergence<T, 3>), d, u, g_);
+  else SPEC_LAUNCH((k_vectordivergence<T, 2>), d, u, g_);
+}
+template <typename T> void Ops<T>::gaussian(Ctx& c, CT* u, int ncomp, double delta) {
+  SPEC_LAUNCH(k_gaussian<T>, u, delta, g_, ncomp);
+}
+template <typename T> void Ops<T>::axpy(Ctx& c, CT* y, const CT* x, double a, int ncomp) {
+  SPEC_LAUNCH(k_axpy<T>, y, x, T(a), g_, ncomp);
+}
+template <typename T> void Ops<T>::spec_tracefree(Ctx& c, CT* tau) { SPEC_LAUNCH(k_spec_tracefree<T>, tau, g_); }
+
+// ---------------------------------------------------------------- the RHS
+// nonlinearity! (src/solver.jl:434-456) as five passes: inverse z (pruned to
+// the 2/3 box), inverse y, fused x C2R/products/R2C, forward y, forward z.
+template <typename T>
+void Ops<T>::nonlinearity(Ctx& c, CT* sig, const CT* u, unsigned long long* vmax) {
+  CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, c.D));
+  const int kc = c.n / 3;
+  const int nt = c.tdim();
+  if (c.D == 3) {
+    c2c_axis<T>(c, 2, +1, u, A, 3, kc + 1, true, kc, true, 1.0);
+    c2c_axis<T>(c, 1, +1, A, A, 3, kc + 1, false, kc, false, 1.0);
+  } else {
+    c2c_axis<T>(c, 1, +1, u, A, 2, kc + 1, false, kc, true, 1.0);
+  }
+  const double fac = 1.0 / std::pow(double(c.n), c.D);
+  x_products<T>(c, sig, A, kc, fac, vmax);
+  c2c_axis<T>(c, 1, -1, sig, sig, nt, c.nh, false, c.n, false, 1.0);
+  if (c.D == 3) c2c_axis<T>(c, 2, -1, sig, sig, nt, c.nh, false, c.n, false, 1.0);
+}
+
+// convectiondiffusion! (src/solver.jl:949-953) plus the optional band forcing
+template <typename T>
+void Ops<T>::rhs(Ctx& c, CT* du, const CT* u, double nu) {
+  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
+  nonlinearity(c, B, u, nullptr);
+  const double* fA = nullptr;
+  if (c.force_kind == 1) {
+    if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
+    hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, u, c.D, c.geom(), c.kband, c.injection,
+                       c.forceA);
+    SC_HIP(hipGetLastError());
+    fA = c.forceA;
+  }
+  if (c.D == 3) SPEC_LAUNCH((k_div_visc<T, 3>), du, B, u, nu, g_, fA, c.kband * c.kband);
+  else SPEC_LAUNCH((k_div_visc<T, 2>), du, B, u, nu, g_, fA, c.kband * c.kband);
+}
+
+// wray3! (src/solver.jl:968-1012)
+template <typename T>
+void Ops<T>::wray3(Ctx& c, CT* u, double dt, double nu) {
+  CT* us = reinterpret_cast<CT*>(c.ensure_spec(c.ustart, c.D));
+  CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
+  const T a[3] = {T(8.0 / 15.0), T(5.0 / 12.0), T(3.0 / 4.0)};
+  const T b[2] = {T(1.0 / 4.0), T(0.0)};
+  for (int i = 0; i < 3; ++i) {
+    rhs(c, du, u, nu);
+    const CT* src = i == 0 ? u : us;
+    T adt = T(a[i] * dt);
+    T bdt = i < 2 ? T(b[i] * dt) : T(0);
+    int wr = (i < 2 && b[i] != T(0)) ? 1 : 0;
+    if (c.D == 3) SPEC_LAUNCH((k_wray3_update<T, 3>), u, us, src, du, adt, bdt, wr, g_);
+    else SPEC_LAUNCH((k_wray3_update<T, 2>), u, us, src, du, adt, bdt, wr, g_);
+  }
+}
+
+// classic RK4 with a projection of every stage value
+template <typename T>
+void Ops<T>::rk4(Ctx& c, CT* u, double dt, double nu) {
+  CT* ust = reinterpret_cast<CT*>(c.ensure_spec(c.ust, c.D));
+  CT* acc = reinterpret_cast<CT*>(c.ensure_spec(c.acc, c.D));
+  CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
+  const double cdt[4] = {dt / 2, dt / 2, dt, dt / 6};
+  for (int s = 0; s < 4; ++s) {
+    rhs(c, du, s == 0 ? u : ust, nu);
+    if (c.D == 3) SPEC_LAUNCH((k_rk4_update<T, 3>), u, ust, acc, du, T(cdt[s]), s, g_);
+    else SPEC_LAUNCH((k_rk4_update<T, 2>), u, ust, acc, du, T(cdt[s]), s, g_);
+  }
+}
+
+// max |C2R(2/3 u)| (propose_timestep, src/solver.jl:931-947)
+template <typename T>
+double Ops<T>::max_velocity(Ctx& c, const CT* u) {
+  if (!c.vmax) SC_HIP(hipMalloc(&c.vmax, sizeof(unsigned long long)));
+  SC_HIP(hipMemsetAsync(c.vmax, 0, sizeof(unsigned long long), c.stream));
+  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
+  nonlinearity(c, B, u, c.vmax);
+  unsigned long long bits = 0;
+  SC_HIP(hipMemcpyAsync(&bits, c.vmax, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof(v));
+  return v;
+}
+
+// ------------------------------------------------------------- reductions
+template <typename T>
+double Ops<T>::sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu) {
+  SpecGeom g = c.geom();
+  unsigned nb = nblk(g.total);
+  double* part = c.dscratch(long(nb) + 1);
+  hipLaunchKernelGGL(k_sum_modes<T>, dim3(nb), dim3(RB), 0, c.stream, u, ncomp, g, kind, nu, part);
+  SC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part, long(nb), part + nb);
+  double* h = c.hostbuf(1);
+  SC_HIP(hipMemcpyAsync(h, part + nb, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  return h[0];
+}
+
+template <typename T>
+void Ops<T>::dot_modes(Ctx& c, const CT* u, const CT* v, int ncomp, double* re, double* im) {
+  SpecGeom g = c.geom();
+  unsigned nb = nblk(g.total);
+  double* part = c.dscratch(2 * long(nb) + 2);
+  hipLaunchKernelGGL(k_dot_modes<T>, dim3(nb), dim3(RB), 0, c.stream, u, v, ncomp, g, part);
+  SC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part, long(nb), part + 2 * nb);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part + nb, long(nb), part + 2 * nb + 1);
+  double* h = c.hostbuf(2);
+  SC_HIP(hipMemcpyAsync(h, part + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  *re = h[0];
+  *im = h[1];
+}
+
+template <typename T>
+void Ops<T>::shell_energy(Ctx& c, const CT* u, int ncomp, int nshell, double* host_out) {
+  SpecGeom g = c.geom();
+  double* d = c.dscratch(nshell);
+  SC_HIP(hipMemsetAsync(d, 0, nshell * sizeof(double), c.stream));
+  hipLaunchKernelGGL(k_shell_energy<T>, dim3(nblk(g.total)), dim3(256), nshell * sizeof(double), c.stream, u, ncomp,
+                     g, nshell, d);
+  SC_HIP(hipGetLastError());
+  double* h = c.hostbuf(nshell);
+  SC_HIP(hipMemcpyAsync(h, d, nshell * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(host_out, h, nshell * sizeof(double));
+}
+
+template <typename T>
+void Ops<T>::shell_scale(Ctx& c, CT* u, int ncomp, int nshell, const double* host_factor) {
+  double* d = c.dscratch(nshell);
+  double* h = c.hostbuf(nshell);
+  SC_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(h, host_factor, nshell * sizeof(double));
+  SC_HIP(hipMemcpyAsync(d, h, nshell * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  SPEC_LAUNCH(k_shell_scale<T>, u, ncomp, g_, nshell, (const double*)d);
+  SC_HIP(hipStreamSynchronize(c.stream));
+}
+
+// --------------------------------------------------------- initial fields
+template <typename T> void Ops<T>::randn(Ctx& c, CT* u, int ncomp, unsigned long long seed) {
+  SPEC_LAUNCH(k_randn<T>, u, ncomp, g_, seed);
+}
+template <typename T> void Ops<T>::symmetrize(Ctx& c, CT* u, int ncomp) {
+  long tot = 2L * c.n * c.nz;
+  hipLaunchKernelGGL(k_symmetrize<T>, dim3(nblk(tot)), dim3(256), 0, c.stream, u, ncomp, c.geom());
+  SC_HIP(hipGetLastError());
+}
+template <typename T> void Ops<T>::taylorgreen(Ctx& c, CT* u, double V0, int doproject) {
+  T* v = reinterpret_cast<T*>(c.scratch(1));  // one spectral scalar holds one physical scalar
+  for (int comp = 0; comp < c.D; ++comp) {
+    hipLaunchKernelGGL(k_taylorgreen_phys<T>, dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, v, comp, c.D, c.n,
+                       c.l, V0);
+    SC_HIP(hipGetLastError());
+    r2c(c, u + comp * c.spec_elems, v, 1, 1.0 / std::pow(double(c.n), c.D));
+  }
+  if (doproject) project(c, u);
+}
+
+// ------------------------------------------------------------- filtering
+template <typename T> void Ops<T>::cutoff(Ctx& cb, Ctx& cf, CT* ub, const CT* u, int ncomp) {
+  SpecGeom gb = cb.geom(), gf = cf.geom();
+  hipLaunchKernelGGL(k_cutoff<T>, dim3(nblk(gb.total)), dim3(256), 0, cb.stream, ub, u, gb, gf, ncomp);
+  SC_HIP(hipGetLastError());
+}
+template <typename T> void Ops<T>::inverse_cutoff(Ctx& cb, Ctx& cf, CT* u, const CT* ub, int ncomp) {
+  SpecGeom gb = cb.geom(), gf = cf.geom();
+  hipLaunchKernelGGL(k_inverse_cutoff<T>, dim3(nblk(gb.total)), dim3(256), 0, cb.stream, u, ub, gb, gf, ncomp);
+  SC_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------------------- closures
+template <typename T> void Ops<T>::clark(Ctx& c, T* tau, const T* GG, double delta) {
+  if (c.D == 3) hipLaunchKernelGGL((k_clark<T, 3>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, GG, delta, c.phys_elems);
+  else hipLaunchKernelGGL((k_clark<T, 2>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, GG, delta, c.phys_elems);
+  SC_HIP(hipGetLastError());
+}
+template <typename T> void Ops<T>::smagorinsky(Ctx& c, T* tau, const T* GG, double cs, double delta) {
+  if (c.D == 3) hipLaunchKernelGGL((k_smagorinsky<T, 3>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, GG, cs, delta, c.phys_elems);
+  else hipLaunchKernelGGL((k_smagorinsky<T, 2>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, GG, cs, delta, c.phys_elems);
+  SC_HIP(hipGetLastError());
+}
+template <typename T> void Ops<T>::phys_tracefree(Ctx& c, T* tau) {
+  hipLaunchKernelGGL(k_phys_tracefree<T>, dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, c.D, c.phys_elems);
+  SC_HIP(hipGetLastError());
+}
+
+// physical velocity gradient GG (D*D comps): vectorgradient! + 2/3 + C2R
+// (src/les.jl:11-16)
+template <typename T> void Ops<T>::gradient_phys(Ctx& c, T* GG, const CT* u) {
+  const int ng = c.D * c.D;
+  CT* G = reinterpret_cast<CT*>(c.scratch(2 * ng)) + long(ng) * c.spec_elems;
+  vectorgradient(c, G, u);
+  // the twothirds mask is applied while reading the spectral gradient
+  c2r(c, GG, G, ng, 1.0, true);
+}
+
+// les! (src/les.jl:4-33) for the built-in closures:
+// closure 1 = Clark(Delta=p1), 2 = Smagorinsky(C_S=p1, Delta=p2),
+// 3 = dynamic Smagorinsky(Delta=p1), 0 = none.
+template <typename T>
+void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2) {
+  const int nt = c.tdim();
+  const int ng = c.D * c.D;
+  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, nt));
+  nonlinearity(c, B, u, nullptr);
+  viscosity(c, B, u, nu);
+  if (closure != 0) {
+    T* tau = reinterpret_cast<T*>(c.work(0, size_t(nt) * c.phys_elems * sizeof(T)));
+    if (closure == 3) {
+      dynsmag(c, tau, u, p1);
+    } else {
+      T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));
+      gradient_phys(c, GG, u);
+      if (closure == 1) clark(c, tau, GG, p1);
+      else smagorinsky(c, tau, GG, p1, p2);
+    }
+    // sigma += to_spec!(tau) per component (the closure stress is not truncated)
+    CT* th = reinterpret_cast<CT*>(c.work(2, size_t(nt) * c.spec_elems * sizeof(CT)));
+    r2c(c, th, tau, nt, 1.0 / std::pow(double(c.n), c.D));
+    axpy(c, B, th, 1.0, nt);
+  }
+  tensordivergence(c, du, B);
+}
+
+// create_dynamic_smagorinsky's model (src/closures.jl:188-271), with the
+// test-filtered strain computed from the truncated strain (the reference
+// reads a buffer its own C2R destroyed; see the oracle).
+template <typename T>
+double Ops<T>::dynsmag(Ctx& c, T* tau, const CT* u, double delta) {
+  const int D = c.D, nt = c.tdim();
+  const long sp = c.spec_elems, np = c.phys_elems;
+  const double dt = 2 * delta;
+  const double dd = std::sqrt(dt * dt + delta * delta);
+  const double inv = 1.0 / std::pow(double(c.n), D);
+  CT* UT = reinterpret_cast<CT*>(c.work(3, size_t(D) * sp * sizeof(CT)));
+  CT* SIG = reinterpret_cast<CT*>(c.work(4, size_t(nt) * sp * sizeof(CT)));
+  CT* SIGT = reinterpret_cast<CT*>(c.work(5, size_t(nt) * sp * sizeof(CT)));
+  CT* SH = reinterpret_cast<CT*>(c.work(6, size_t(nt) * sp * sizeof(CT)));
+  T* L = reinterpret_cast<T*>(c.work(7, size_t(nt) * np * sizeof(T)));
+  T* P = reinterpret_cast<T*>(c.work(8, size_t(nt) * np * sizeof(T)));
+  T* S = reinterpret_cast<T*>(c.work(9, size_t(nt) * np * sizeof(T)));
+  T* St = reinterpret_cast<T*>(c.work(10, size_t(nt) * np * sizeof(T)));
+  T* m2 = reinterpret_cast<T*>(c.work(11, size_t(nt) * np * sizeof(T)));
+  T* M = reinterpret_cast<T*>(c.work(12, size_t(nt) * np * sizeof(T)));
+  // test-filtered velocity
+  SC_HIP(hipMemcpyAsync(UT, u, size_t(D) * sp * sizeof(CT), hipMemcpyDeviceToDevice, c.stream));
+  gaussian(c, UT, D, dt);
+  twothirds(c, UT, D);
+  // Germano L = tilde(uu) - tilde(u) tilde(u)
+  nonlinearity(c, SIG, u, nullptr);
+  nonlinearity(c, SIGT, UT, nullptr);
+  gaussian(c, SIG, nt, dt);
+  twothirds(c, SIG, nt);
+  c2r(c, L, SIG, nt, 1.0, false);
+  c2r(c, P, SIGT, nt, 1.0, false);
+  launch_phys(c, L, P, T(-1), T(1), long(nt) * np);
+  // strain at grid (S) and combined (St) filter levels
+  strainrate(c, SH, u);
+  twothirds(c, SH, nt);
+  c2r(c, S, SH, nt, 1.0, false);
+  gaussian(c, SH, nt, dt);
+  twothirds(c, SH, nt);
+  c2r(c, St, SH, nt, 1.0, false);
+  // Smagorinsky tensors; m1 goes straight into tau
+  smag_tensor(c, tau, S, delta);
+  smag_tensor(c, m2, St, dd);
+  // m1 = dealias_phys!(m1); M = test_filter_phys!(copy(m1)) - m2
+  r2c(c, SH, tau, nt, 1.0);
+  twothirds(c, SH, nt);
+  c2r(c, tau, SH, nt, inv, false);
+  r2c(c, SH, tau, nt, 1.0);
+  gaussian(c, SH, nt, dt);
+  twothirds(c, SH, nt);
+  c2r(c, M, SH, nt, inv, false);
+  launch_phys(c, M, m2, T(-1), T(1), long(nt) * np);
+  // Lilly average c = max(-<ML>/<MM>, 0)
+  double ml = 0, mm = 0;
+  ml_mm(c, M, L, &ml, &mm);
+  const double coef = mm == 0.0 ? 0.0 : std::fmax(-ml / mm, 0.0);
+  hipLaunchKernelGGL(k_phys_scale<T>, dim3(nblk(long(nt) * np)), dim3(256), 0, c.stream, tau, coef, long(nt) * np);
+  SC_HIP(hipGetLastError());
+  return coef;
+}
+
+template <typename T>
+void Ops<T>::launch_phys(Ctx& c, T* y, const T* x, T a, T b, long n) {
+  hipLaunchKernelGGL(k_phys_axpby<T>, dim3(nblk(n)), dim3(256), 0, c.stream, y, x, a, b, n);
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Ops<T>::smag_tensor(Ctx& c, T* m, const T* S, double delta) {
+  if (c.D == 3) hipLaunchKernelGGL((k_smag_tensor<T, 3>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, m, S, delta, c.phys_elems);
+  else hipLaunchKernelGGL((k_smag_tensor<T, 2>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, m, S, delta, c.phys_elems);
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Ops<T>::ml_mm(Ctx& c, const T* M, const T* L, double* ml, double* mm) {
+  unsigned nb = nblk(c.phys_elems);
+  double* part = c.dscratch(2 * long(nb) + 2);
+  if (c.D == 3) hipLaunchKernelGGL((k_ml_mm<T, 3>), dim3(nb), dim3(RB), 0, c.stream, M, L, c.phys_elems, part);
+  else hipLaunchKernelGGL((k_ml_mm<T, 2>), dim3(nb), dim3(RB), 0, c.stream, M, L, c.phys_elems, part);
+  SC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part, long(nb), part + 2 * nb);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part + nb, long(nb), part + 2 * nb + 1);
+  double* h = c.hostbuf(2);
+  SC_HIP(hipMemcpyAsync(h, part + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  *ml = h[0];
+  *mm = h[1];
+}
+
+// wray3!(les!, u, dt, grid, cache; model, visc)
+template <typename T>
+void Ops<T>::les_wray3(Ctx& c, CT* u, double dt, double nu, int closure, double p1, double p2) {
+  CT* us = reinterpret_cast<CT*>(c.ensure_spec(c.ustart, c.D));
+  CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
+  const T a[3] = {T(8.0 / 15.0), T(5.0 / 12.0), T(3.0 / 4.0)};
+  const T b[2] = {T(1.0 / 4.0), T(0.0)};
+  for (int i = 0; i < 3; ++i) {
+    les_rhs(c, du, u, nu, closure, p1, p2);
+    const CT* src = i == 0 ? u : us;
+    T adt = T(a[i] * dt);
+    T bdt = i < 2 ? T(b[i] * dt) : T(0);
+    int wr = (i < 2 && b[i] != T(0)) ? 1 : 0;
+    if (c.D == 3) SPEC_LAUNCH((k_wray3_update<T, 3>), u, us, src, du, adt, bdt, wr, g_);
+    else SPEC_LAUNCH((k_wray3_update<T, 2>), u, us, src, du, adt, bdt, wr, g_);
+  }
+}
+
+}  // namespace sc

@@ -1,0 +1,15 @@
+"""symcode: MI355X-native pseudo-spectral Navier-Stokes hot path with the
+call signatures of agdestein/SymmetryCode.jl (see include/symcode.h).
+
+Every operation runs in libsymcode_hip.so (hand-written HIP for gfx950);
+importing raises if the library has not been built.
+"""
+from ._lib import LIBPATH, SymcodeError, lib  # noqa: F401
+from .grid import Grid  # noqa: F401
+from . import solver, filtering, closures  # noqa: F401
+from .solver import *  # noqa: F401,F403
+from .filtering import cutoff, gaussianfilter, sfs, inverse_cutoff  # noqa: F401
+from .closures import (create_clark, create_smagorinsky, create_dynamic_smagorinsky,  # noqa: F401
+                       les_rhs, les_wray3, solve_les, gradient_phys)
+
+lib()  # fail loudly at import if the HIP library is missing

@@ -1,0 +1,100 @@
+"""ctypes binding of libsymcode_hip.so (the C ABI in include/symcode.h)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libsymcode_hip.so")
+
+_lib = None
+
+c_int, c_double, c_void_p, c_int64, c_uint64 = (
+    ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64)
+P = c_void_p
+DP = ctypes.POINTER(c_double)
+
+# name -> argtypes (all return int status except sc_last_error)
+SIGNATURES = {
+    "sc_create": [c_int, c_int, c_double, c_int, c_int, ctypes.POINTER(P)],
+    "sc_destroy": [P],
+    "sc_info": [P, ctypes.POINTER(c_int64)],
+    "sc_set_stream": [P, P],
+    "sc_set_sync": [P, c_int],
+    "sc_sync": [P],
+    "sc_profile": [P, c_int],
+    "sc_profile_report": [P, ctypes.c_char_p, c_int],
+    "sc_upload_spectral": [P, P, P, c_int],
+    "sc_download_spectral": [P, P, P, c_int],
+    "sc_upload_physical": [P, P, P, c_int],
+    "sc_download_physical": [P, P, P, c_int],
+    "sc_to_phys": [P, P, P, c_int],
+    "sc_to_spec": [P, P, P, c_int],
+    "sc_irfft": [P, P, P, c_int],
+    "sc_rfft": [P, P, P, c_int],
+    "sc_twothirds": [P, P, c_int],
+    "sc_project": [P, P],
+    "sc_symmetrize": [P, P, c_int],
+    "sc_viscosity": [P, P, P, c_double],
+    "sc_tensordivergence": [P, P, P],
+    "sc_vectordivergence": [P, P, P],
+    "sc_vectorgradient": [P, P, P],
+    "sc_strainrate": [P, P, P],
+    "sc_gaussianfilter": [P, P, c_int, c_double],
+    "sc_cutoff": [P, P, P, P, c_int],
+    "sc_inverse_cutoff": [P, P, P, P, c_int],
+    "sc_axpy": [P, P, P, c_double, c_int],
+    "sc_make_tracefree_spectral": [P, P],
+    "sc_make_tracefree_physical": [P, P],
+    "sc_nonlinearity": [P, P, P],
+    "sc_stress": [P, P, P, c_double],
+    "sc_convectiondiffusion": [P, P, P, c_double],
+    "sc_wray3": [P, P, c_double, c_double],
+    "sc_rk4": [P, P, c_double, c_double],
+    "sc_propose_timestep": [P, P, c_double, DP],
+    "sc_set_band_forcing": [P, c_double, c_int],
+    "sc_energy": [P, P, c_int, DP],
+    "sc_spectraldot": [P, P, P, c_int, DP, DP],
+    "sc_dissipation": [P, P, c_double, DP],
+    "sc_palinstrophy": [P, P, c_double, DP],
+    "sc_spectrum": [P, P, c_int, c_int, DP],
+    "sc_shell_energies": [P, P, c_int, c_int, DP],
+    "sc_shell_scale": [P, P, c_int, c_int, DP],
+    "sc_maintain_shell_energy": [P, P, c_int, DP],
+    "sc_randn": [P, P, c_int, c_uint64],
+    "sc_randomfield": [P, P, c_uint64, c_int, DP],
+    "sc_taylorgreen": [P, P, c_double, c_int],
+    "sc_gradient_phys": [P, P, P],
+    "sc_closure_clark": [P, P, P, c_double],
+    "sc_closure_smagorinsky": [P, P, P, c_double, c_double],
+    "sc_closure_dynsmag": [P, P, P, c_double, DP],
+    "sc_les_rhs": [P, P, P, c_double, c_int, c_double, c_double],
+    "sc_les_wray3": [P, P, c_double, c_double, c_int, c_double, c_double],
+}
+
+
+class SymcodeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libsymcode_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(
+                f"{LIBPATH} is missing: build it with `python symmetrycode.jl_amd/build.py` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIBPATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = c_int
+        L.sc_last_error.argtypes = [P]
+        L.sc_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc, ctx=None):
+    if rc != 0:
+        msg = lib().sc_last_error(ctx)
+        raise SymcodeError(f"symcode error {rc}: {msg.decode() if msg else ''}")

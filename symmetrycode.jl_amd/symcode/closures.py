@@ -1,0 +1,134 @@
+"""Host-side mirror of the classical closures (src/closures.jl) and the LES
+right-hand side / rollout (src/les.jl).
+
+A closure is a callable ``model(u, GG) -> tau`` like the reference's (packed
+physical tensor, components xx, yy, zz, xy, yz, zx).  The built-in closures
+also carry an id so that ``les_rhs`` can run the whole les! on the GPU in one
+library call.
+"""
+import ctypes
+import math
+
+from ._lib import lib
+from .grid import Grid, _p
+from .solver import (_c, energy, energy_shells, maintain_shell_energy, propose_timestep, twothirds,
+                     wray3)
+
+
+class Closure:
+    kind = 0
+    p1 = 0.0
+    p2 = 0.0
+
+    def __init__(self, g: Grid):
+        self.g = g
+
+
+class Clark(Closure):
+    """create_clark (src/closures.jl:64): τ = Δ²/12 · G Gᵀ."""
+    kind = 1
+
+    def __init__(self, delta, g):
+        super().__init__(g)
+        self.p1 = float(delta)
+
+    def __call__(self, u, GG):
+        tau = self.g.spacetensorfield()
+        _c(lib().sc_closure_clark(self.g.ctx, _p(tau), _p(GG), self.p1), self.g)
+        return tau
+
+
+class Smagorinsky(Closure):
+    """create_smagorinsky (src/closures.jl:65-66)."""
+    kind = 2
+
+    def __init__(self, cs, delta, g):
+        super().__init__(g)
+        self.p1, self.p2 = float(cs), float(delta)
+
+    def __call__(self, u, GG):
+        tau = self.g.spacetensorfield()
+        _c(lib().sc_closure_smagorinsky(self.g.ctx, _p(tau), _p(GG), self.p1, self.p2), self.g)
+        return tau
+
+
+class DynamicSmagorinsky(Closure):
+    """create_dynamic_smagorinsky (src/closures.jl:188-271); ``G`` is
+    accepted for the shared interface but the model works from ``u``."""
+    kind = 3
+
+    def __init__(self, delta, g):
+        super().__init__(g)
+        self.p1 = float(delta)
+        self.last_coefficient = None
+
+    def __call__(self, u, GG=None):
+        tau = self.g.spacetensorfield()
+        c = ctypes.c_double()
+        _c(lib().sc_closure_dynsmag(self.g.ctx, _p(tau), _p(u), self.p1, ctypes.byref(c)), self.g)
+        self.last_coefficient = c.value
+        return tau
+
+
+def create_clark(delta, g):
+    return Clark(delta, g)
+
+
+def create_smagorinsky(cs, delta, g):
+    return Smagorinsky(cs, delta, g)
+
+
+def create_dynamic_smagorinsky(delta, g):
+    return DynamicSmagorinsky(delta, g)
+
+
+def gradient_phys(u, g: Grid):
+    """∇ū in physical space as les! forms it (src/les.jl:11-16)."""
+    GG = g.spacetensorfield_nonsym()
+    _c(lib().sc_gradient_phys(g.ctx, _p(GG), _p(u)), g)
+    return GG
+
+
+def les_rhs(du, u, g: Grid, model, visc):
+    """les!(du, u, grid, cache; model, visc) (src/les.jl:4-33)."""
+    kind = 0 if model is None else model.kind
+    p1 = 0.0 if model is None else model.p1
+    p2 = 0.0 if model is None else model.p2
+    _c(lib().sc_les_rhs(g.ctx, _p(du), _p(u), visc, kind, p1, p2), g)
+    return du
+
+
+def les_wray3(u, dt, g: Grid, model, visc):
+    """wray3!(les!, u, Δt, grid, cache; model, visc)."""
+    kind = 0 if model is None else model.kind
+    p1 = 0.0 if model is None else model.p1
+    p2 = 0.0 if model is None else model.p2
+    _c(lib().sc_les_wray3(g.ctx, _p(u), dt, visc, kind, p1, p2), g)
+    return u
+
+
+def solve_les(u, times, g: Grid, visc, model, cfl, forced=True, onsnapshot=None):
+    """solve_les! (src/les.jl:188-247): CFL-limited Wray3 rollout with the
+    shell-energy clamp and the blow-up guard.  Returns the number of save
+    times reached."""
+    shells = energy_shells(g, [1, 2], u) if forced else None
+    t = times[0]
+    for i, tstop in enumerate(times):
+        if i > 0:
+            while t < tstop:
+                dt = cfl * propose_timestep(u, g, visc)
+                dt = min(dt, tstop - t)
+                t += dt
+                if model is None:
+                    wray3(u, dt, g, visc)
+                else:
+                    les_wray3(u, dt, g, model, visc)
+                if forced:
+                    maintain_shell_energy(u, shells, g)
+                e = energy(u, g)
+                if dt < 1.0e-8 or e > 1.0e5 or not math.isfinite(e):
+                    return i
+        twothirds(u, g)
+        if onsnapshot is not None:
+            onsnapshot(i, u)
+    return len(times)

@@ -1,0 +1,269 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bar: FP64 within 1e-12 relative (L2 over the field) per call / per step;
+FP32 within the stated single-precision tolerances."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle_solenoidal, print_case, rand_phys, rel
+from oracle import closures as OC
+from oracle import filtering as OF
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+SIZES3 = [8, 12, 16, 30, 32, 48, 64, 96]
+SIZES2 = [16, 64, 256, 810]
+
+
+def grids(D, n, prec=64):
+    import symcode as sc
+
+    dt = torch.float64 if prec == 64 else torch.float32
+    return S.Grid(D, n, 2 * np.pi, np.float64), sc.Grid(D, n, 2 * np.pi, dt)
+
+
+def cases():
+    return [(3, n) for n in SIZES3] + [(2, n) for n in SIZES2]
+
+
+# ---------------------------------------------------------------- transforms
+@pytest.mark.parametrize("D,n", cases())
+def test_fft_parity(D, n):
+    import symcode as sc
+
+    print_case("fft", D, n)
+    go, gg = grids(D, n)
+    x = rand_phys((2,) + go.phys_shape, 1)
+    ref = np.stack([S.to_spec(x[i], go) for i in range(2)])
+    spec = sc.to_spec(gg.spec(2), gg.upload_physical(x), gg)
+    assert rel(gg.download_spectral(spec), ref) < TOL64
+    back = sc.to_phys(gg.phys(2), spec, gg)
+    assert rel(gg.download_physical(back), x) < TOL64
+    # C2R of a non-Hermitian spectrum follows the multi-dimensional C2R
+    # convention (imaginary parts of kx = 0 and kx = n/2 discarded)
+    r = np.random.default_rng(2)
+    z = r.standard_normal(go.spec_shape) + 1j * r.standard_normal(go.spec_shape)
+    p = sc.to_phys(gg.phys(1), gg.upload_spectral(z), gg)
+    assert rel(gg.download_physical(p)[0], S.to_phys(z, go)) < TOL64
+    # plan-level scalings: ldiv! normalised, mul! unnormalised
+    q = sc.irfft(gg.phys(1), gg.upload_spectral(ref[0]), gg)
+    assert rel(gg.download_physical(q)[0], S.irfft_normalised(ref[0], go)) < TOL64
+    s2 = sc.rfft(gg.spec(1), gg.upload_physical(x[0]), gg)
+    assert rel(gg.download_spectral(s2)[0], S.rfft_unnormalised(x[0], go)) < TOL64
+
+
+# ----------------------------------------------------------- spectral kernels
+@pytest.mark.parametrize("D,n", [(3, 12), (3, 30), (2, 64)])
+def test_pointwise_kernels(D, n):
+    import symcode as sc
+
+    print_case("kernels", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 3) * 1.3 + 0.1 * np.stack([S.to_spec(rand_phys(go.phys_shape, 9 + i), go) for i in range(D)])
+    ud = gg.upload_spectral(u)
+    t = gg.upload_spectral(u)
+    sc.twothirds(t, gg)
+    assert np.array_equal(gg.download_spectral(t), np.stack([S.twothirds(c, go) for c in u]))
+    t = gg.upload_spectral(u)
+    sc.project(t, gg)
+    assert rel(gg.download_spectral(t), S.project(u, go)) < TOL64
+    sig = np.stack([S.to_spec(rand_phys(go.phys_shape, 20 + i), go) for i in range(go.tensordim)])
+    t = gg.upload_spectral(sig)
+    sc.viscosity(t, ud, 0.03, gg)
+    assert rel(gg.download_spectral(t), S.viscosity(sig, u, 0.03, go)) < TOL64
+    du = sc.tensordivergence(gg.vectorfield(), gg.upload_spectral(sig), gg)
+    assert rel(gg.download_spectral(du), S.tensordivergence(sig, go)) < TOL64
+    G = sc.vectorgradient(gg.tensorfield_nonsym(), ud, gg)
+    assert rel(gg.download_spectral(G), S.vectorgradient(u, go)[0]) < TOL64
+    St = sc.strainrate(gg.tensorfield(), ud, gg)
+    assert rel(gg.download_spectral(St), S.strainrate(u, go)) < TOL64
+    dv = sc.vectordivergence(gg.scalarfield(), ud, gg)
+    assert rel(gg.download_spectral(dv)[0], S.vectordivergence(u, go)) < TOL64
+    t = gg.upload_spectral(u)
+    sc.gaussianfilter(t, 0.37, gg)
+    assert rel(gg.download_spectral(t), np.stack([OF.gaussianfilter(c, 0.37, go) for c in u])) < TOL64
+    # symmetrize on raw normals
+    z = S.random_spectral(go, 4, 1)[0]
+    t = gg.upload_spectral(z)
+    sc.symmetrize(t, gg)
+    assert np.array_equal(gg.download_spectral(t)[0], S.symmetrize(z, go))
+    # axpy and trace removal
+    t = gg.upload_spectral(sig)
+    sc.filtering.axpy(t, gg.upload_spectral(sig[::-1].copy()), -0.5, gg)
+    assert rel(gg.download_spectral(t), sig - 0.5 * sig[::-1]) < TOL64
+    t = gg.upload_spectral(sig)
+    sc.make_tracefree(t, gg)
+    assert rel(gg.download_spectral(t), OF.make_tracefree(sig, D)) < TOL64
+
+
+# --------------------------------------------------------------------- RHS
+RHS_CASES = [(3, 12), (3, 16), (3, 30), (3, 32), (3, 48), (2, 32), (2, 256), (2, 810)]
+
+
+@pytest.mark.parametrize("D,n", RHS_CASES)
+def test_rhs_parity(D, n):
+    import symcode as sc
+
+    print_case("rhs", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 5)  # untruncated: exercises the full viscous path
+    nu = 0.02
+    ud = gg.upload_spectral(u)
+    sig = sc.nonlinearity(gg.tensorfield(), ud, gg)
+    assert rel(gg.download_spectral(sig), S.nonlinearity(u, go)) < TOL64
+    st = sc.stress(gg.tensorfield(), ud, nu, gg)
+    assert rel(gg.download_spectral(st), S.stress(u, nu, go)) < TOL64
+    du = sc.convectiondiffusion(gg.vectorfield(), ud, gg, nu)
+    assert rel(gg.download_spectral(du), S.convectiondiffusion(u, go, nu)) < TOL64
+
+
+@pytest.mark.parametrize("D,n", [(3, 16), (3, 30), (2, 64)])
+def test_steppers_parity(D, n):
+    import symcode as sc
+
+    print_case("steppers", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 6, trunc=True)
+    nu, dt = 0.01, 0.02
+    f = lambda v: S.convectiondiffusion(v, go, nu)
+    for name, ostep, gstep in (("wray3", S.wray3, sc.wray3), ("rk4", S.rk4, sc.rk4)):
+        ref = u
+        ud = gg.upload_spectral(u)
+        for k in range(3):
+            ref = ostep(f, ref, dt, go)
+            gstep(ud, dt, gg, nu)
+            assert rel(gg.download_spectral(ud), ref) < TOL64 * (k + 1), (name, k)
+
+
+@pytest.mark.parametrize("D,n", [(3, 16), (2, 64)])
+def test_band_forcing_parity(D, n):
+    import symcode as sc
+
+    print_case("band", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 8, trunc=True)
+    sc.set_band_forcing(gg, 0.1, 2)
+    du = sc.convectiondiffusion(gg.vectorfield(), gg.upload_spectral(u), gg, 0.01)
+    ref = S.convectiondiffusion(u, go, 0.01, forcing=(0.1, 2))
+    assert rel(gg.download_spectral(du), ref) < TOL64
+    # injection by the forcing alone is exactly the requested rate
+    f = gg.download_spectral(du) - S.convectiondiffusion(u, go, 0.01)
+    inj = sum(S.spectraldot(u[i], f[i], go) for i in range(D)).real
+    assert inj == pytest.approx(0.1, rel=1e-11)
+    sc.set_band_forcing(gg, 0.0)
+
+
+@pytest.mark.parametrize("D,n", [(3, 16), (3, 30), (2, 64)])
+def test_reductions_and_cfl(D, n):
+    import symcode as sc
+
+    print_case("reductions", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 9)
+    ud = gg.upload_spectral(u)
+    assert sc.energy(ud, gg) == pytest.approx(S.energy(u, go), rel=1e-13)
+    assert sc.get_dissipation(ud, 0.01, gg) == pytest.approx(S.get_dissipation(u, 0.01, go), rel=1e-13)
+    v = gg.upload_spectral(u[::-1].copy())
+    dot = sc.spectraldot(ud, v, gg)
+    ref = sum(S.spectraldot(u[i], u[::-1][i], go) for i in range(D))
+    assert abs(dot - ref) < 1e-13 * abs(ref)
+    k, s = sc.spectrum(ud, gg)
+    ko, so = S.spectrum(u, go)
+    assert np.allclose(k, ko) and rel(s, so) < 1e-13
+    assert sc.propose_timestep(ud, gg, 0.01) == pytest.approx(S.propose_timestep(u, go, 0.01), rel=1e-12)
+    shells = sc.energy_shells(gg, [1, 2], ud)
+    oshells = S.energy_shells(u, go)
+    for (i, e), (j, eo) in zip(shells, oshells):
+        assert i == j and e == pytest.approx(eo, rel=1e-13)
+    w = gg.upload_spectral(1.5 * u)
+    sc.maintain_shell_energy(w, shells, gg)
+    assert rel(gg.download_spectral(w), S.maintain_shell_energy(1.5 * u, go, oshells)) < TOL64
+    stats = sc.turbulence_statistics(gg.upload_spectral(u), 0.01, gg)
+    _, ostats = S.turbulence_statistics(u, 0.01, go)
+    for key in ostats:
+        assert stats[key] == pytest.approx(ostats[key], rel=1e-12), key
+
+
+@pytest.mark.parametrize("D,n", [(3, 16), (3, 32), (2, 64)])
+def test_initial_conditions(D, n):
+    import symcode as sc
+
+    print_case("ic", D, n)
+    go, gg = grids(D, n)
+    prof = lambda k: S.peak_profile(k, 4)
+    u = sc.randomfield(lambda k: prof(k), gg, seed=3, totalenergy=1.0)
+    ref = S.randomfield(prof, go, seed=3, totalenergy=1.0)
+    assert rel(gg.download_spectral(u), ref) < TOL64
+    tg = sc.taylorgreen(gg, V0=1.3)
+    assert rel(gg.download_spectral(tg), S.taylorgreen(go, V0=1.3)) < TOL64
+
+
+# ---------------------------------------------------------------- filtering
+@pytest.mark.parametrize("nd,nl,D", [(48, 16, 3), (32, 16, 3), (256, 64, 2)])
+def test_cutoff_and_sfs(nd, nl, D):
+    import symcode as sc
+
+    print_case("sfs", nd, nl, D)
+    god, ggd = grids(D, nd)
+    gol, ggl = grids(D, nl)
+    u = oracle_solenoidal(god, 10, trunc=True)
+    ud = ggd.upload_spectral(u)
+    ub = sc.cutoff(ggl.vectorfield(), ud, ggl, ggd)
+    assert np.array_equal(ggl.download_spectral(ub), np.stack([OF.cutoff(c, gol, god) for c in u]))
+    delta = 2 * (2 * np.pi / nl)
+    ubar, tau = sc.sfs(ud, ggd, ggl, delta)
+    oubar, otau = OF.sfs(u, god, gol, delta)
+    assert rel(ggl.download_spectral(ubar), oubar) < TOL64
+    assert rel(ggl.download_spectral(tau), otau) < TOL64
+
+
+# ---------------------------------------------------------------- closures
+@pytest.mark.parametrize("D,n", [(3, 16), (2, 32)])
+def test_closures_parity(D, n):
+    import symcode as sc
+
+    print_case("closures", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 12, trunc=True)
+    ud = gg.upload_spectral(u)
+    delta = 2 * np.pi / n * 2
+    GG = sc.gradient_phys(ud, gg)
+    oGG = OC.gradient_phys(u, go)
+    assert rel(gg.download_physical(GG), oGG) < TOL64
+    tau = sc.create_clark(delta, gg)(ud, GG)
+    assert rel(gg.download_physical(tau), OC.clark(oGG, delta, D)) < TOL64
+    tau = sc.create_smagorinsky(0.1, delta, gg)(ud, GG)
+    assert rel(gg.download_physical(tau), OC.smagorinsky(oGG, 0.1, delta, D)) < TOL64
+    m = sc.create_dynamic_smagorinsky(delta, gg)
+    tau = m(ud, GG)
+    otau, oc = OC.dynamic_smagorinsky(u, delta, go, return_coeff=True)
+    assert m.last_coefficient == pytest.approx(oc, rel=1e-11)
+    assert rel(gg.download_physical(tau), otau) < 1e-11
+    nu = 0.01
+    for model, omodel in (
+        (sc.create_clark(delta, gg), lambda u_, G_: OC.clark(G_, delta, D)),
+        (sc.create_smagorinsky(0.1, delta, gg), lambda u_, G_: OC.smagorinsky(G_, 0.1, delta, D)),
+        (sc.create_dynamic_smagorinsky(delta, gg), lambda u_, G_: OC.dynamic_smagorinsky(u_, delta, go)),
+    ):
+        du = sc.les_rhs(gg.vectorfield(), ud, gg, model, nu)
+        assert rel(gg.download_spectral(du), OC.les_rhs(u, go, nu, omodel)) < 1e-11, type(model).__name__
+
+
+# ----------------------------------------------------------------- FP32 path
+@pytest.mark.parametrize("D,n", [(3, 16), (3, 30), (2, 810)])
+def test_fp32_rhs(D, n):
+    import symcode as sc
+
+    print_case("fp32", D, n)
+    go, gg = grids(D, n, prec=32)
+    u = oracle_solenoidal(go, 13)
+    du = sc.convectiondiffusion(gg.vectorfield(), gg.upload_spectral(u), gg, 0.02)
+    assert rel(gg.download_spectral(du), S.convectiondiffusion(u, go, 0.02)) < TOL32
+    x = rand_phys(go.phys_shape, 2)
+    s = sc.to_spec(gg.spec(1), gg.upload_physical(x), gg)
+    assert rel(gg.download_spectral(s)[0], S.to_spec(x, go)) < 1e-6
