@@ -97,6 +97,12 @@ struct RowGeom {
   double scale;
 };
 
+constexpr int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+
 template <int M> constexpr int rows_per_block() {
   int tpp = Plan<M>::tpp;
   int r = 256 / tpp;
@@ -231,8 +237,10 @@ k_x_products(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw
   if (vmax_bits != nullptr) {
     red[t] = rok ? vm : 0.0;
     __syncthreads();
-    for (int s = (TY * PI::TPP) / 2; s > 0; s >>= 1) {
-      if (t < s && t + s < TY * PI::TPP) red[t] = fmax(red[t], red[t + s]);
+    constexpr int NTH = TY * Plan<M>::tpp;
+    constexpr int P2 = pow2_ceil(NTH);
+    for (int s = P2 / 2; s > 0; s >>= 1) {
+      if (t < s && t + s < NTH) red[t] = fmax(red[t], red[t + s]);
       __syncthreads();
     }
     if (t == 0) atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(sqrt(red[0])));
