@@ -28,8 +28,8 @@ METRIC = "810³ DNS RK4 steps/s + %HBM roofline; 128³ a-posteriori LES grad ste
 # Algorithmic HBM bytes per launch, in units of C (one spectral scalar field,
 # 16·(n/2+1)·n² B in FP64), for the unpruned pass model of one RHS.
 ALG_C = {
-    "c2c_z_inv": 6, "c2c_y_inv": 6, "x_products": 9, "c2c_y_fwd": 12, "c2c_z_fwd": 12,
-    "(k_div_visc<T, 3>)": 12,
+    "c2c_z_inv": 6, "c2c_y_inv": 6, "x_products": 9, "y_fwd_div": 12, "z_fwd_rhs": 12,
+    "c2c_y_fwd": 12, "c2c_z_fwd": 12, "(k_div_visc<T, 3>)": 12,
 }
 RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates)
 
@@ -89,9 +89,7 @@ def main():
     L = sc.lib()
 
     def step():
-        dt = cfl * sc.propose_timestep(u, g, nu)
-        sc.rk4(u, dt, g, nu)
-        return dt
+        return sc.rk4_cfl(u, g, nu, cfl)
 
     for _ in range(args.warmup):
         step()

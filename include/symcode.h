@@ -100,6 +100,14 @@ int sc_convectiondiffusion(sc_ctx* ctx, void* du, const void* u, double visc);
 int sc_wray3(sc_ctx* ctx, void* u, double dt, double visc);
 /* classic RK4 with projected stages (no reference counterpart; configs use it) */
 int sc_rk4(sc_ctx* ctx, void* u, double dt, double visc);
+/* one step with dt = min(cfl * propose_timestep(u), dtmax), the velocity
+ * maximum taken from the first stage's own C2R (the caller's
+ * `Δt = min(cfl * propose_timestep(...), tstop - t)`, src/data.jl:61-62,
+ * src/les.jl:209-210); *dt receives the step taken */
+int sc_rk4_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, double* dt);
+int sc_wray3_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, double* dt);
+int sc_les_wray3_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, int closure, double p1,
+                     double p2, double* dt);
 /* propose_timestep(u, grid, visc, cache) (:931-947) */
 int sc_propose_timestep(sc_ctx* ctx, const void* u, double visc, double* dt);
 /* linear band forcing f = A u on 0 < |k| <= kband with A = injection /

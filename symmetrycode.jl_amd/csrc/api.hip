@@ -87,7 +87,7 @@ int sc_destroy(sc_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->tw_n, c->tw_m, c->bufA, c->bufB, c->scr, c->ustart, c->du, c->acc, c->ust,
-                  c->dscr, c->vmax, c->forceA};
+                  c->dscr, c->vmax, c->forceA, c->dtdev};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (void* b : c->ws)
@@ -264,6 +264,35 @@ int sc_wray3(sc_ctx* c, void* u, double dt, double nu) {
 }
 int sc_rk4(sc_ctx* c, void* u, double dt, double nu) {
   return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::rk4(*c, (CT*)u, dt, nu)); });
+}
+int sc_rk4_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, double* dt) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
+    double d = 0;
+    TYPED(c, d = sc::Ops<T>::rk4_dev(*c, (CT*)u, dtmax, nu, cfl));
+    if (dt) *dt = d;
+  });
+}
+int sc_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, double* dt) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
+    double d = 0;
+    TYPED(c, d = sc::Ops<T>::wray3_dev(*c, (CT*)u, dtmax, nu, cfl, -1, 0, 0));
+    if (dt) *dt = d;
+  });
+}
+int sc_les_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, int closure, double p1, double p2,
+                     double* dt) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
+    need(closure >= 0 && closure <= 3, "closure kind");
+    double d = 0;
+    TYPED(c, d = sc::Ops<T>::wray3_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
+    if (dt) *dt = d;
+  });
 }
 int sc_propose_timestep(sc_ctx* c, const void* u, double nu, double* dt) {
   return guard(c, [&] {

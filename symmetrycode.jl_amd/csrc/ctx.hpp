@@ -41,6 +41,7 @@ struct Ctx {
   double* dscr = nullptr; long dscr_n = 0;
   unsigned long long* vmax = nullptr;
   double* forceA = nullptr;
+  double* dtdev = nullptr;  // time step of the current step (device)
   void* ws[16] = {};       // named workspaces (closures, LES)
   size_t ws_bytes[16] = {};
   double* hbuf = nullptr;  // pinned host scratch

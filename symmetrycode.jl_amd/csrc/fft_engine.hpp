@@ -160,9 +160,9 @@ SC_PLAN(512, 64, 8, 8, 8)
 SC_PLAN(810, 45, 10, 9, 9)
 #undef SC_PLAN
 
-template <int N> struct PlanInfo {
+template <int N, int TPP_ = Plan<N>::tpp> struct PlanInfo {
   using P = Plan<N>;
-  static constexpr int TPP = P::tpp;
+  static constexpr int TPP = TPP_;
   static constexpr int NS = P::ns;
   static constexpr int R(int s) { return P::r[s]; }
   static constexpr int Ns(int s) { int p = 1; for (int i = 0; i < s; ++i) p *= P::r[i]; return p; }
@@ -181,8 +181,8 @@ template <int N> struct PlanInfo {
 // Pencil engine: TPP threads (index p) cooperatively transform one pencil of
 // length N.  LDS pencil storage is element i at lds[i * LS] (LS = pencil
 // interleave stride).
-template <typename T, int N, int SGN> struct Pencil {
-  using PI = PlanInfo<N>;
+template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp> struct Pencil {
+  using PI = PlanInfo<N, TPP_>;
   static constexpr int TPP = PI::TPP;
   static constexpr int NS = PI::NS;
   static constexpr int MAXE = PI::MAXE;

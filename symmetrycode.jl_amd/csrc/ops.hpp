@@ -31,6 +31,11 @@ struct Ops {
   static void wray3(Ctx& c, CT* u, double dt, double nu);
   static void rk4(Ctx& c, CT* u, double dt, double nu);
   static double max_velocity(Ctx& c, const CT* u);
+  static void max_velocity_into(Ctx& c, const CT* u);
+  static void rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax);
+  static double wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure, double p1, double p2);
+  static double rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl);
+  static double read_dt(Ctx& c);
   // reductions (return to host)
   static double sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu);
   static void dot_modes(Ctx& c, const CT* u, const CT* v, int ncomp, double* re, double* im);

@@ -5,6 +5,7 @@
 #include <vector>
 #include "ops.hpp"
 #include "fft_passes.hpp"
+#include "rhs_passes.hpp"
 
 namespace sc {
 
@@ -185,6 +186,71 @@ static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale,
   SC_HIP(hipGetLastError());
 }
 
+// geometry of a full (unpruned) strided pass along axis (1 = y, 2 = z)
+template <typename T>
+static StridedGeom full_geom(const Ctx& c, int axis) {
+  const int n = c.n;
+  const int TX = tile_x_rt<T>(n);
+  StridedGeom g;
+  g.xs = c.nh;
+  g.ntile = (c.nh + TX - 1) / TX;
+  g.nother = axis == 1 ? c.nz : n;
+  g.other_lo = g.nother;
+  g.other_gap = 0;
+  g.n_other = axis == 1 ? c.nz : n;
+  g.s_axis = axis == 1 ? long(c.nhp) : long(c.nhp) * n;
+  g.s_other = axis == 1 ? long(c.nhp) * n : long(c.nhp);
+  g.s_comp = c.spec_elems;
+  g.kin = n;
+  g.mask23 = 0;
+  g.kcut = n / 3;
+  g.scale = 1.0;
+  return g;
+}
+
+template <typename T>
+static void y_fwd_div(Ctx& c, C<T>* buf) {
+  StridedGeom g = full_geom<T>(c, 1);
+  dim3 grid(unsigned(g.ntile) * unsigned(g.nother));
+  const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
+  const double kf = c.geom().kf;
+  c.prof_begin("y_fwd_div");
+  switch (c.n) {
+#define X(NN)                                                                                          \
+  case NN:                                                                                             \
+    hipLaunchKernelGGL((k_y_fwd_div<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, tw, g, kf); \
+    break;
+    SC_FOR_N(X)
+#undef X
+    default: throw Error(2, "unsupported grid size");
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void fwd_rhs_last(Ctx& c, const C<T>* buf, const C<T>* u, C<T>* du, RhsParams rp) {
+  const int axis = c.D == 3 ? 2 : 1;
+  StridedGeom g = full_geom<T>(c, axis);
+  dim3 grid(unsigned(g.ntile) * unsigned(g.nother));
+  const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
+  c.prof_begin(c.D == 3 ? "z_fwd_rhs" : "y_fwd_rhs2d");
+  switch (c.n) {
+#define X(NN)                                                                                                  \
+  case NN:                                                                                                     \
+    if (c.D == 3)                                                                                              \
+      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, rp); \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_y_fwd_rhs2d<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, rp); \
+    break;
+    SC_FOR_N(X)
+#undef X
+    default: throw Error(2, "unsupported grid size");
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
 // ----------------------------------------------------------------- tables
 template <typename T>
 void Ops<T>::init_tables(Ctx& c) {
@@ -310,45 +376,140 @@ void Ops<T>::rhs(Ctx& c, CT* du, const CT* u, double nu) {
   else SPEC_LAUNCH((k_div_visc<T, 2>), du, B, u, nu, g_, fA, c.kband * c.kband);
 }
 
-// wray3! (src/solver.jl:968-1012)
+// Right-hand side of the time steppers: the five-pass pipeline with the
+// divergence, viscous term and forcing fused into the forward passes.  The
+// gradient part -nu k (k.u) of the viscous term is omitted (removed by the
+// projection that follows every stage).  With vmax non-null the stage-input
+// velocity maximum is accumulated for the CFL step.
 template <typename T>
-void Ops<T>::wray3(Ctx& c, CT* u, double dt, double nu) {
+void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax) {
+  CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, c.D));
+  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
+  const int kc = c.n / 3;
+  if (c.D == 3) {
+    c2c_axis<T>(c, 2, +1, u, A, 3, kc + 1, true, kc, true, 1.0);
+    c2c_axis<T>(c, 1, +1, A, A, 3, kc + 1, false, kc, false, 1.0);
+  } else {
+    c2c_axis<T>(c, 1, +1, u, A, 2, kc + 1, false, kc, true, 1.0);
+  }
+  x_products<T>(c, B, A, kc, 1.0 / std::pow(double(c.n), c.D), vmax);
+  RhsParams rp;
+  rp.kf = c.geom().kf;
+  rp.nu = nu;
+  rp.forceA = nullptr;
+  rp.kband2 = c.kband * c.kband;
+  if (c.force_kind == 1) {
+    if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
+    c.prof_begin("band_coeff");
+    hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, u, c.D, c.geom(), c.kband, c.injection,
+                       c.forceA);
+    c.prof_end();
+    SC_HIP(hipGetLastError());
+    rp.forceA = c.forceA;
+  }
+  if (c.D == 3) y_fwd_div<T>(c, B);
+  fwd_rhs_last<T>(c, B, u, du, rp);
+}
+
+template <typename T>
+static double* dt_slot(Ctx& c) {
+  if (!c.dtdev) SC_HIP(hipMalloc(&c.dtdev, 2 * sizeof(double)));
+  return c.dtdev;
+}
+
+// CFL time step computed on the device from the maximum collected in vmax
+template <typename T>
+static void cfl_dt(Ctx& c, double cfl, double nu, double dtmax) {
+  hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, c.stream, c.vmax, cfl, c.l / c.n, c.D, nu, dtmax, dt_slot<T>(c));
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void zero_vmax(Ctx& c) {
+  if (!c.vmax) SC_HIP(hipMalloc(&c.vmax, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, c.stream, c.vmax, 0ull);
+  SC_HIP(hipGetLastError());
+}
+
+// wray3! (src/solver.jl:968-1012).  cfl > 0: the step is
+// min(cfl * propose_timestep(u), dtmax) with the velocity maximum taken from
+// the first stage's x-pass (the same C2R of the truncated u).
+template <typename T>
+double Ops<T>::wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure, double p1, double p2) {
   CT* us = reinterpret_cast<CT*>(c.ensure_spec(c.ustart, c.D));
   CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
+  double* dtp = dt_slot<T>(c);
   const T a[3] = {T(8.0 / 15.0), T(5.0 / 12.0), T(3.0 / 4.0)};
   const T b[2] = {T(1.0 / 4.0), T(0.0)};
+  if (cfl > 0) zero_vmax<T>(c);
+  else hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c.stream, dtp, dt);
   for (int i = 0; i < 3; ++i) {
-    rhs(c, du, u, nu);
+    if (les_closure >= 0) {
+      if (cfl > 0 && i == 0) max_velocity_into(c, u);
+      les_rhs(c, du, u, nu, les_closure, p1, p2);
+    } else {
+      rhs_fused(c, du, u, nu, (cfl > 0 && i == 0) ? c.vmax : nullptr);
+    }
+    if (cfl > 0 && i == 0) cfl_dt<T>(c, cfl, nu, dt);
     const CT* src = i == 0 ? u : us;
-    T adt = T(a[i] * dt);
-    T bdt = i < 2 ? T(b[i] * dt) : T(0);
+    const T bi = i < 2 ? b[i] : T(0);
     int wr = (i < 2 && b[i] != T(0)) ? 1 : 0;
-    if (c.D == 3) SPEC_LAUNCH((k_wray3_update<T, 3>), u, us, src, du, adt, bdt, wr, g_);
-    else SPEC_LAUNCH((k_wray3_update<T, 2>), u, us, src, du, adt, bdt, wr, g_);
+    if (c.D == 3) SPEC_LAUNCH((k_wray3_update<T, 3>), u, us, src, du, (const double*)dtp, a[i], bi, wr, g_);
+    else SPEC_LAUNCH((k_wray3_update<T, 2>), u, us, src, du, (const double*)dtp, a[i], bi, wr, g_);
   }
+  return read_dt(c);
 }
 
 // classic RK4 with a projection of every stage value
 template <typename T>
-void Ops<T>::rk4(Ctx& c, CT* u, double dt, double nu) {
+double Ops<T>::rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl) {
   CT* ust = reinterpret_cast<CT*>(c.ensure_spec(c.ust, c.D));
   CT* acc = reinterpret_cast<CT*>(c.ensure_spec(c.acc, c.D));
   CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
-  const double cdt[4] = {dt / 2, dt / 2, dt, dt / 6};
+  double* dtp = dt_slot<T>(c);
+  const double frac[4] = {0.5, 0.5, 1.0, 1.0 / 6.0};
+  if (cfl > 0) zero_vmax<T>(c);
+  else hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c.stream, dtp, dt);
   for (int s = 0; s < 4; ++s) {
-    rhs(c, du, s == 0 ? u : ust, nu);
-    if (c.D == 3) SPEC_LAUNCH((k_rk4_update<T, 3>), u, ust, acc, du, T(cdt[s]), s, g_);
-    else SPEC_LAUNCH((k_rk4_update<T, 2>), u, ust, acc, du, T(cdt[s]), s, g_);
+    rhs_fused(c, du, s == 0 ? u : ust, nu, (cfl > 0 && s == 0) ? c.vmax : nullptr);
+    if (cfl > 0 && s == 0) cfl_dt<T>(c, cfl, nu, dt);
+    if (c.D == 3) SPEC_LAUNCH((k_rk4_update<T, 3>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
+    else SPEC_LAUNCH((k_rk4_update<T, 2>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
   }
+  return read_dt(c);
+}
+
+template <typename T>
+double Ops<T>::read_dt(Ctx& c) {
+  double* h = c.hostbuf(1);
+  SC_HIP(hipMemcpyAsync(h, c.dtdev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  return h[0];
+}
+
+template <typename T> void Ops<T>::wray3(Ctx& c, CT* u, double dt, double nu) { wray3_dev(c, u, dt, nu, 0.0, -1, 0, 0); }
+template <typename T> void Ops<T>::rk4(Ctx& c, CT* u, double dt, double nu) { rk4_dev(c, u, dt, nu, 0.0); }
+
+// velocity maximum of C2R(2/3 u) accumulated into c.vmax (no reset)
+template <typename T>
+void Ops<T>::max_velocity_into(Ctx& c, const CT* u) {
+  CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, c.D));
+  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
+  const int kc = c.n / 3;
+  if (c.D == 3) {
+    c2c_axis<T>(c, 2, +1, u, A, 3, kc + 1, true, kc, true, 1.0);
+    c2c_axis<T>(c, 1, +1, A, A, 3, kc + 1, false, kc, false, 1.0);
+  } else {
+    c2c_axis<T>(c, 1, +1, u, A, 2, kc + 1, false, kc, true, 1.0);
+  }
+  x_products<T>(c, B, A, kc, 1.0, c.vmax);
 }
 
 // max |C2R(2/3 u)| (propose_timestep, src/solver.jl:931-947)
 template <typename T>
 double Ops<T>::max_velocity(Ctx& c, const CT* u) {
-  if (!c.vmax) SC_HIP(hipMalloc(&c.vmax, sizeof(unsigned long long)));
-  SC_HIP(hipMemsetAsync(c.vmax, 0, sizeof(unsigned long long), c.stream));
-  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
-  nonlinearity(c, B, u, c.vmax);
+  zero_vmax<T>(c);
+  max_velocity_into(c, u);
   unsigned long long bits = 0;
   SC_HIP(hipMemcpyAsync(&bits, c.vmax, sizeof(bits), hipMemcpyDeviceToHost, c.stream));
   SC_HIP(hipStreamSynchronize(c.stream));
@@ -591,19 +752,7 @@ void Ops<T>::ml_mm(Ctx& c, const T* M, const T* L, double* ml, double* mm) {
 // wray3!(les!, u, dt, grid, cache; model, visc)
 template <typename T>
 void Ops<T>::les_wray3(Ctx& c, CT* u, double dt, double nu, int closure, double p1, double p2) {
-  CT* us = reinterpret_cast<CT*>(c.ensure_spec(c.ustart, c.D));
-  CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
-  const T a[3] = {T(8.0 / 15.0), T(5.0 / 12.0), T(3.0 / 4.0)};
-  const T b[2] = {T(1.0 / 4.0), T(0.0)};
-  for (int i = 0; i < 3; ++i) {
-    les_rhs(c, du, u, nu, closure, p1, p2);
-    const CT* src = i == 0 ? u : us;
-    T adt = T(a[i] * dt);
-    T bdt = i < 2 ? T(b[i] * dt) : T(0);
-    int wr = (i < 2 && b[i] != T(0)) ? 1 : 0;
-    if (c.D == 3) SPEC_LAUNCH((k_wray3_update<T, 3>), u, us, src, du, adt, bdt, wr, g_);
-    else SPEC_LAUNCH((k_wray3_update<T, 2>), u, us, src, du, adt, bdt, wr, g_);
-  }
+  wray3_dev(c, u, dt, nu, 0.0, closure, p1, p2);
 }
 
 }  // namespace sc

@@ -1,0 +1,219 @@
+// Fused forward passes of the DNS right-hand side (time-stepper path).
+//
+// After the fused x-pass the six stress components sigma_ij(kx, y, z) are
+// transformed along y and z.  The divergence du_i = -i k_j sigma_ij is split
+// over the two passes: the y-pass forms A_i = kx sigma_ix + ky sigma_iy as
+// soon as ky is known, the z-pass adds kz sigma_iz, the viscous term
+// -nu |k|^2 u_i and the band forcing, and writes du directly.  The viscous
+// term of viscosity! also carries -nu k_i (k.u), a gradient that the
+// projection of every Runge-Kutta stage removes exactly; the stepper path
+// therefore omits it (the stand-alone convectiondiffusion keeps it).
+#pragma once
+#include "fft_passes.hpp"
+
+namespace sc {
+
+// threads per pencil giving one butterfly per thread in the largest stage
+template <int N> constexpr int tpp_wide() {
+  int m = 1;
+  for (int s = 0; s < Plan<N>::ns; ++s) {
+    int b = N / Plan<N>::r[s];
+    m = b > m ? b : m;
+  }
+  return m;
+}
+template <typename T, int N> constexpr int wide_threads() { return tile_x<T, N>() * tpp_wide<N>(); }
+
+struct RhsParams {
+  double kf;             // (pi / l) * 2
+  double nu;
+  const double* forceA;  // band forcing coefficient (device) or null
+  int kband2;
+};
+
+// Output slots of the y-pass: 0 A_x, 1 A_y, 2 s_zz, 3 A_z, 4 s_yz, 5 s_zx.
+// Input slots (packed stress): 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  The
+// processing order below only ever writes slots whose input this block has
+// already consumed, so the pass runs in place.
+template <typename T, int N>
+__global__ void __launch_bounds__((wide_threads<T, N>()))
+k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
+  constexpr int TX = tile_x<T, N>();
+  constexpr int TPP = tpp_wide<N>();
+  using PE = Pencil<T, N, -1, TPP>;
+  constexpr int E = PE::PI::E_LAST;
+  __shared__ C<T> lds[TX * N];
+  const int nb = g.ntile * g.nother;
+  const int b = xcd_remap(blockIdx.x, nb);
+  const int tile = b % g.ntile;
+  const int oi = b / g.ntile;  // z index
+  const int t = threadIdx.x;
+  const int xl = t % TX, p = t / TX;
+  const int x = tile * TX + xl;
+  const bool xok = x < g.xs;
+  C<T>* base = buf + long(oi) * g.s_other + x;
+  const T kx = T(kf) * T(x);
+  C<T> v[PE::MAXE];
+  C<T> acc[E];
+  auto fft = [&](int comp) {
+    const C<T>* src = base + long(comp) * g.s_comp;
+    PE::load(v, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
+    PE::run(v, p, lds + xl, TX, tw);
+  };
+  auto put = [&](int slot, const C<T>* val) {
+    if (!xok) return;
+    C<T>* dst = base + long(slot) * g.s_comp;
+    PE::for_out(p, [&](int e, int k) { dst[long(k) * g.s_axis] = val[e]; });
+  };
+  auto kyv = [&](int k) { return T(kf) * T(freq(k, N)); };
+  // xx -> A_x = kx s_xx
+  fft(0);
+  PE::for_out(p, [&](int e, int) { acc[e] = scl<T>(v[e], kx); });
+  // xy -> A_x += ky s_xy (write slot 0); A_y = kx s_xy
+  fft(3);
+  PE::for_out(p, [&](int e, int k) { acc[e] = add<T>(acc[e], scl<T>(v[e], kyv(k))); });
+  put(0, acc);
+  PE::for_out(p, [&](int e, int) { acc[e] = scl<T>(v[e], kx); });
+  // yy -> A_y += ky s_yy (slot 1)
+  fft(1);
+  PE::for_out(p, [&](int e, int k) { acc[e] = add<T>(acc[e], scl<T>(v[e], kyv(k))); });
+  put(1, acc);
+  // zz -> slot 2
+  fft(2);
+  put(2, v);
+  // zx -> slot 5; A_z = kx s_zx
+  fft(5);
+  put(5, v);
+  PE::for_out(p, [&](int e, int) { acc[e] = scl<T>(v[e], kx); });
+  // yz -> slot 4; A_z += ky s_yz (slot 3)
+  fft(4);
+  put(4, v);
+  PE::for_out(p, [&](int e, int k) { acc[e] = add<T>(acc[e], scl<T>(v[e], kyv(k))); });
+  put(3, acc);
+}
+
+// Final z-pass: du_i = -i (A_i + kz s_iz) - nu |k|^2 u_i (+ A_f u_i on the band).
+template <typename T, int N>
+__global__ void __launch_bounds__((wide_threads<T, N>()))
+k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw, StridedGeom g,
+            RhsParams rp) {
+  constexpr int TX = tile_x<T, N>();
+  constexpr int TPP = tpp_wide<N>();
+  using PE = Pencil<T, N, -1, TPP>;
+  constexpr int E = PE::PI::E_LAST;
+  __shared__ C<T> lds[TX * N];
+  const int nb = g.ntile * g.nother;
+  const int b = xcd_remap(blockIdx.x, nb);
+  const int tile = b % g.ntile;
+  const int oi = b / g.ntile;  // y index
+  const int t = threadIdx.x;
+  const int xl = t % TX, p = t / TX;
+  const int x = tile * TX + xl;
+  const bool xok = x < g.xs;
+  const long off = long(oi) * g.s_other + x;
+  const T kf = T(rp.kf);
+  const int kyi = freq(oi, g.n_other);
+  const T kx = kf * T(x), ky = kf * T(kyi);
+  const T nu = T(rp.nu);
+  T fA = T(0);
+  if (rp.forceA) fA = T(*rp.forceA);
+  C<T> v[PE::MAXE];
+  C<T> acc[E];
+  auto fft = [&](int slot) {
+    const C<T>* src = buf + long(slot) * g.s_comp + off;
+    PE::load(v, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
+    PE::run(v, p, lds + xl, TX, tw);
+  };
+  constexpr int SA[3] = {0, 1, 3};  // A_x, A_y, A_z
+  constexpr int SZ[3] = {5, 4, 2};  // s_zx, s_yz, s_zz
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    fft(SZ[i]);
+    PE::for_out(p, [&](int e, int k) { acc[e] = scl<T>(v[e], kf * T(freq(k, N))); });
+    fft(SA[i]);
+    if (xok) {
+      const C<T>* ui = u + long(i) * g.s_comp + off;
+      C<T>* di = du + long(i) * g.s_comp + off;
+      PE::for_out(p, [&](int e, int k) {
+        const int kzi = freq(k, N);
+        const T kz = kf * T(kzi);
+        C<T> a = add<T>(acc[e], v[e]);
+        C<T> uu = ui[long(k) * g.s_axis];
+        const T k2 = kx * kx + ky * ky + kz * kz;
+        C<T> r = mk<T>(a.y - nu * k2 * uu.x, -a.x - nu * k2 * uu.y);  // -i a - nu k^2 u
+        if (rp.forceA) {
+          int k2i = x * x + kyi * kyi + kzi * kzi;
+          if (k2i > 0 && k2i <= rp.kband2) r = add<T>(r, scl<T>(uu, fA));
+        }
+        di[long(k) * g.s_axis] = r;
+      });
+    }
+  }
+}
+
+// 2D final y-pass over the packed stress (xx, yy, xy).
+template <typename T, int N>
+__global__ void __launch_bounds__((wide_threads<T, N>()))
+k_y_fwd_rhs2d(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw, StridedGeom g,
+              RhsParams rp) {
+  constexpr int TX = tile_x<T, N>();
+  constexpr int TPP = tpp_wide<N>();
+  using PE = Pencil<T, N, -1, TPP>;
+  constexpr int E = PE::PI::E_LAST;
+  __shared__ C<T> lds[TX * N];
+  const int b = xcd_remap(blockIdx.x, g.ntile);
+  const int t = threadIdx.x;
+  const int xl = t % TX, p = t / TX;
+  const int x = b * TX + xl;
+  const bool xok = x < g.xs;
+  const T kf = T(rp.kf);
+  const T kx = kf * T(x);
+  const T nu = T(rp.nu);
+  T fA = T(0);
+  if (rp.forceA) fA = T(*rp.forceA);
+  C<T> v[PE::MAXE];
+  C<T> sxy[E];
+  auto fft = [&](int slot) {
+    const C<T>* src = buf + long(slot) * g.s_comp + x;
+    PE::load(v, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
+    PE::run(v, p, lds + xl, TX, tw);
+  };
+  fft(2);
+  PE::for_out(p, [&](int e, int) { sxy[e] = v[e]; });
+#pragma unroll 1
+  for (int i = 0; i < 2; ++i) {
+    fft(i);
+    if (xok) {
+      const C<T>* ui = u + long(i) * g.s_comp + x;
+      C<T>* di = du + long(i) * g.s_comp + x;
+      PE::for_out(p, [&](int e, int k) {
+        const int kyi = freq(k, N);
+        const T ky = kf * T(kyi);
+        // i = 0: kx s_xx + ky s_xy ; i = 1: kx s_xy + ky s_yy
+        C<T> a = i == 0 ? add<T>(scl<T>(v[e], kx), scl<T>(sxy[e], ky)) : add<T>(scl<T>(sxy[e], kx), scl<T>(v[e], ky));
+        C<T> uu = ui[long(k) * g.s_axis];
+        const T k2 = kx * kx + ky * ky;
+        C<T> r = mk<T>(a.y - nu * k2 * uu.x, -a.x - nu * k2 * uu.y);
+        if (rp.forceA) {
+          int k2i = x * x + kyi * kyi;
+          if (k2i > 0 && k2i <= rp.kband2) r = add<T>(r, scl<T>(uu, fA));
+        }
+        di[long(k) * g.s_axis] = r;
+      });
+    }
+  }
+}
+
+// --------------------------------------------------------------- CFL dt
+static __global__ void k_cfl_dt(const unsigned long long* vmax_bits, double cfl, double h, int D, double nu, double dtmax,
+                         double* dt) {
+  double vmax = __longlong_as_double((long long)*vmax_bits);
+  double dconv = h / vmax;
+  double ddiff = h * h / D / 2 / nu;
+  double d = cfl * fmin(dconv, ddiff);
+  *dt = fmin(d, dtmax);
+}
+static __global__ void k_set_f64(double* p, double v) { *p = v; }
+static __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+}  // namespace sc

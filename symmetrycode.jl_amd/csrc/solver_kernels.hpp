@@ -279,13 +279,17 @@ __global__ void k_inverse_cutoff(C<T>* u, const C<T>* ub, SpecGeom gb, SpecGeom 
 
 // ------------------------------------------------------------- RK updates
 // wray3! stage update (src/solver.jl:997-1008):
-//   u = P(src + a dt du);  if (bdt != 0) ustart = src + b dt du
+//   u = P(src + a dt du);  if (write_ustart) ustart = src + b dt du
 // src is u itself on the first stage (the reference copies u to ustart).
+// dt is read from device memory (set by the host or by the fused CFL step);
+// the coefficients are a[i] and b[i] rounded to T, as in the reference.
 template <typename T, int D>
-__global__ void k_wray3_update(C<T>* u, C<T>* ustart, const C<T>* src, const C<T>* du, T adt, T bdt,
-                               int write_ustart, SpecGeom g) {
+__global__ void k_wray3_update(C<T>* u, C<T>* ustart, const C<T>* src, const C<T>* du, const double* dtp, T a,
+                               T b, int write_ustart, SpecGeom g) {
   Mode m;
   if (!mode_of(gid(), g, m)) return;
+  const double dt = *dtp;
+  const T adt = T(double(a) * dt), bdt = T(double(b) * dt);
   C<T> s[D], d[D], v[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) {
@@ -303,11 +307,13 @@ __global__ void k_wray3_update(C<T>* u, C<T>* ustart, const C<T>* src, const C<T
 }
 
 // RK4 stage update.  stage 0: acc = k;  1,2: acc += 2k;  3: acc += k.
-// stages 0-2 write ust = P(u0 + cdt k); stage 3 writes u0 = P(u0 + dt/6 acc).
+// stages 0-2 write ust = P(u0 + frac dt k); stage 3 writes u0 = P(u0 + dt/6 acc).
 template <typename T, int D>
-__global__ void k_rk4_update(C<T>* u0, C<T>* ust, C<T>* acc, const C<T>* k, T cdt, int stage, SpecGeom g) {
+__global__ void k_rk4_update(C<T>* u0, C<T>* ust, C<T>* acc, const C<T>* k, const double* dtp, double frac, int stage,
+                             SpecGeom g) {
   Mode m;
   if (!mode_of(gid(), g, m)) return;
+  const T cdt = T(*dtp * frac);
   C<T> v[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) {

@@ -10,6 +10,6 @@ from . import solver, filtering, closures  # noqa: F401
 from .solver import *  # noqa: F401,F403
 from .filtering import cutoff, gaussianfilter, sfs, inverse_cutoff  # noqa: F401
 from .closures import (create_clark, create_smagorinsky, create_dynamic_smagorinsky,  # noqa: F401
-                       les_rhs, les_wray3, solve_les, gradient_phys)
+                       les_rhs, les_wray3, les_wray3_cfl, solve_les, gradient_phys)
 
 lib()  # fail loudly at import if the HIP library is missing

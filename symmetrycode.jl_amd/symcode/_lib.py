@@ -50,6 +50,9 @@ SIGNATURES = {
     "sc_wray3": [P, P, c_double, c_double],
     "sc_rk4": [P, P, c_double, c_double],
     "sc_propose_timestep": [P, P, c_double, DP],
+    "sc_rk4_cfl": [P, P, c_double, c_double, c_double, DP],
+    "sc_wray3_cfl": [P, P, c_double, c_double, c_double, DP],
+    "sc_les_wray3_cfl": [P, P, c_double, c_double, c_double, c_int, c_double, c_double, DP],
     "sc_set_band_forcing": [P, c_double, c_int],
     "sc_energy": [P, P, c_int, DP],
     "sc_spectraldot": [P, P, P, c_int, DP, DP],

@@ -11,8 +11,7 @@ import math
 
 from ._lib import lib
 from .grid import Grid, _p
-from .solver import (_c, energy, energy_shells, maintain_shell_energy, propose_timestep, twothirds,
-                     wray3)
+from .solver import _c, energy, energy_shells, maintain_shell_energy, twothirds, wray3_cfl
 
 
 class Closure:
@@ -107,6 +106,14 @@ def les_wray3(u, dt, g: Grid, model, visc):
     return u
 
 
+def les_wray3_cfl(u, g: Grid, model, visc, cfl, dtmax):
+    """wray3!(les!) with Δt = min(cfl · propose_timestep(u), dtmax); returns Δt."""
+    dt = ctypes.c_double()
+    _c(lib().sc_les_wray3_cfl(g.ctx, _p(u), cfl, visc, dtmax, model.kind, model.p1, model.p2,
+                              ctypes.byref(dt)), g)
+    return dt.value
+
+
 def solve_les(u, times, g: Grid, visc, model, cfl, forced=True, onsnapshot=None):
     """solve_les! (src/les.jl:188-247): CFL-limited Wray3 rollout with the
     shell-energy clamp and the blow-up guard.  Returns the number of save
@@ -116,13 +123,11 @@ def solve_les(u, times, g: Grid, visc, model, cfl, forced=True, onsnapshot=None)
     for i, tstop in enumerate(times):
         if i > 0:
             while t < tstop:
-                dt = cfl * propose_timestep(u, g, visc)
-                dt = min(dt, tstop - t)
-                t += dt
                 if model is None:
-                    wray3(u, dt, g, visc)
+                    dt = wray3_cfl(u, g, visc, cfl, tstop - t)
                 else:
-                    les_wray3(u, dt, g, model, visc)
+                    dt = les_wray3_cfl(u, g, model, visc, cfl, tstop - t)
+                t += dt
                 if forced:
                     maintain_shell_energy(u, shells, g)
                 e = energy(u, g)

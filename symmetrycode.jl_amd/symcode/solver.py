@@ -147,6 +147,22 @@ def rk4(u, dt, g: Grid, visc):
     return u
 
 
+def rk4_cfl(u, g: Grid, visc, cfl, dtmax=float("inf")):
+    """One RK4 step with Δt = min(cfl · propose_timestep(u), dtmax); the
+    velocity maximum comes from the first stage's own C2R.  Returns Δt."""
+    dt = _d()
+    _c(lib().sc_rk4_cfl(g.ctx, _p(u), cfl, visc, min(dtmax, 1e300), ctypes.byref(dt)), g)
+    return dt.value
+
+
+def wray3_cfl(u, g: Grid, visc, cfl, dtmax=float("inf")):
+    """wray3! with Δt = min(cfl · propose_timestep(u), dtmax) (the callers'
+    step-size rule, src/data.jl:61-62).  Returns Δt."""
+    dt = _d()
+    _c(lib().sc_wray3_cfl(g.ctx, _p(u), cfl, visc, min(dtmax, 1e300), ctypes.byref(dt)), g)
+    return dt.value
+
+
 def propose_timestep(u, g: Grid, visc):
     """propose_timestep (src/solver.jl:931-947)."""
     dt = _d()

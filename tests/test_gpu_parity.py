@@ -267,3 +267,72 @@ def test_fp32_rhs(D, n):
     x = rand_phys(go.phys_shape, 2)
     s = sc.to_spec(gg.spec(1), gg.upload_physical(x), gg)
     assert rel(gg.download_spectral(s)[0], S.to_spec(x, go)) < 1e-6
+
+
+# ------------------------------------------------------- CFL-driven steppers
+@pytest.mark.parametrize("D,n", [(3, 16), (3, 30), (2, 64)])
+def test_cfl_steppers(D, n):
+    """dt = min(cfl * propose_timestep(u), dtmax) computed on the device from
+    the first stage, with band forcing inside the fused RHS."""
+    import symcode as sc
+
+    print_case("cfl", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 14, trunc=True)
+    nu, cfl = 0.01, 0.3
+    sc.set_band_forcing(gg, 0.1, 2)
+    f = lambda v: S.convectiondiffusion(v, go, nu, forcing=(0.1, 2))
+    for ostep, gstep in ((S.rk4, sc.rk4_cfl), (S.wray3, sc.wray3_cfl)):
+        ud = gg.upload_spectral(u)
+        ref = u
+        for k in range(3):
+            dto = cfl * S.propose_timestep(ref, go, nu)
+            dtg = gstep(ud, gg, nu, cfl)
+            assert dtg == pytest.approx(dto, rel=1e-12)
+            ref = ostep(f, ref, dto, go)
+            assert rel(gg.download_spectral(ud), ref) < TOL64 * (k + 1)
+    assert sc.rk4_cfl(ud, gg, nu, cfl, dtmax=1e-4) == 1e-4
+    sc.set_band_forcing(gg, 0.0)
+
+
+@pytest.mark.parametrize("kind", ["none", "clark", "smag", "dynsmag"])
+def test_les_rollout(kind):
+    """solve_les! (src/les.jl:188-247): CFL Wray3 with the closure, the shell
+    clamp and save-time truncation, against the oracle's loop."""
+    import symcode as sc
+
+    print_case("les", kind)
+    D, n = 3, 16
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 15, trunc=True)
+    nu, cfl = 0.005, 0.35
+    delta = 2 * np.pi / n * 2
+    models = {
+        "none": (None, None),
+        "clark": (sc.create_clark(delta, gg), lambda u_, G: OC.clark(G, delta, D)),
+        "smag": (sc.create_smagorinsky(0.1, delta, gg), lambda u_, G: OC.smagorinsky(G, 0.1, delta, D)),
+        "dynsmag": (sc.create_dynamic_smagorinsky(delta, gg), lambda u_, G: OC.dynamic_smagorinsky(u_, delta, go)),
+    }
+    gm, om = models[kind]
+    times = [0.0, 0.05, 0.1]
+    ud = gg.upload_spectral(u)
+    snaps = []
+    sc.solve_les(ud, times, gg, nu, gm, cfl, forced=True, onsnapshot=lambda i, v: snaps.append(gg.download_spectral(v)))
+    # oracle loop
+    ref = u.copy()
+    shells = S.energy_shells(ref, go)
+    f = (lambda v: S.convectiondiffusion(v, go, nu)) if om is None else (lambda v: OC.les_rhs(v, go, nu, om))
+    t = times[0]
+    osnaps = []
+    for i, tstop in enumerate(times):
+        if i > 0:
+            while t < tstop:
+                dt = min(cfl * S.propose_timestep(ref, go, nu), tstop - t)
+                t += dt
+                ref = S.wray3(f, ref, dt, go)
+                ref = S.maintain_shell_energy(ref, go, shells)
+        ref = np.stack([S.twothirds(c, go) for c in ref])
+        osnaps.append(ref)
+    assert len(snaps) == len(osnaps)
+    for a, b in zip(snaps, osnaps):
+        assert rel(a, b) < 1e-10
