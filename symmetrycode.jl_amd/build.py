@@ -26,14 +26,17 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(jobs=8, verbose=False):
-    bdir = os.path.join(HERE, "_build")
+def build(jobs=8, verbose=False, defines=(), out=None, bdir=None):
+    """Compile and link.  ``defines`` are extra -D tuning knobs (tile sizes,
+    waves-per-SIMD hints); ``out`` / ``bdir`` redirect the library and objects."""
+    bdir = bdir or os.path.join(HERE, "_build")
+    out = out or OUT
     os.makedirs(bdir, exist_ok=True)
     cc = hipcc()
 
     def one(src):
         obj = os.path.join(bdir, src + ".o")
-        cmd = [cc, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,18 +46,21 @@ def build(jobs=8, verbose=False):
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(one, SOURCES))
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=8)
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("-D", action="append", default=[], help="extra define, e.g. SC_WPE_X=3")
+    ap.add_argument("-o", default=None, help="output library path")
+    ap.add_argument("--bdir", default=None)
     a = ap.parse_args()
-    print(build(a.j, a.v))
+    print(build(a.j, a.v, a.D, a.o, a.bdir))

@@ -131,6 +131,12 @@ template <int S, typename T> struct Dft<16, S, T> : DftPQ<4, 4, S, T> {};
 // ------------------------------------------------------------ plans
 // radices and threads-per-pencil (TPP) for each supported length
 template <int N> struct Plan;
+#ifndef SC_TPP810
+#define SC_TPP810 90
+#endif
+#ifndef SC_TPP405
+#define SC_TPP405 45
+#endif
 #define SC_PLAN(N_, TPP_, ...)                                   \
   template <> struct Plan<N_> {                                  \
     static constexpr int tpp = TPP_;                             \
@@ -155,9 +161,9 @@ SC_PLAN(64, 8, 8, 8)
 SC_PLAN(96, 16, 6, 4, 4)
 SC_PLAN(128, 16, 8, 4, 4)
 SC_PLAN(256, 32, 4, 8, 8)
-SC_PLAN(405, 45, 5, 9, 9)
+SC_PLAN(405, SC_TPP405, 5, 9, 9)
 SC_PLAN(512, 64, 8, 8, 8)
-SC_PLAN(810, 45, 10, 9, 9)
+SC_PLAN(810, SC_TPP810, 10, 9, 9)
 #undef SC_PLAN
 
 template <int N, int TPP_ = Plan<N>::tpp> struct PlanInfo {
@@ -181,11 +187,27 @@ template <int N, int TPP_ = Plan<N>::tpp> struct PlanInfo {
 // Pencil engine: TPP threads (index p) cooperatively transform one pencil of
 // length N.  LDS pencil storage is element i at lds[i * LS] (LS = pencil
 // interleave stride).
-template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp> struct Pencil {
+// Barrier between the LDS write and read of an exchange: the whole block, or
+// (WAVE) only the calling wavefront when one wavefront owns the pencil.  LDS
+// instructions of one wavefront execute in order, so a wavefront-scope fence
+// that keeps the compiler from reordering them is sufficient there.
+template <bool WAVE> DEV void xsync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp, bool WAVE = false> struct Pencil {
   using PI = PlanInfo<N, TPP_>;
   static constexpr int TPP = PI::TPP;
   static constexpr int NS = PI::NS;
   static constexpr int MAXE = PI::MAXE;
+  static constexpr int E0 = PI::NB(0) * PI::R(0);   // stage-0 input slots
+  static constexpr int EL = PI::E_LAST;             // last-stage output slots
 
   // positions read by stage s, slot (m, r)
   template <int s> static DEV int in_pos(int p, int m, int r) {
@@ -291,9 +313,9 @@ template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp> struct Pencil {
   template <int s> static DEV void run_from(C<T>* v, int p, C<T>* lds, int LS, const C<T>* tw) {
     compute<s>(v, p, tw);
     if constexpr (s + 1 < NS) {
-      __syncthreads();  // previous users of lds are done
+      xsync<WAVE>();  // previous users of lds are done
       lds_write<s>(v, p, lds, LS);
-      __syncthreads();
+      xsync<WAVE>();
       lds_read<s + 1>(v, p, lds, LS);
       run_from<s + 1>(v, p, lds, LS, tw);
     }
@@ -304,14 +326,14 @@ template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp> struct Pencil {
   // Move the final-stage outputs (natural order) to the stage-0 input
   // layout of another transform of the same length, through LDS.
   static DEV void out_to_in(C<T>* v, int p, C<T>* lds, int LS) {
-    __syncthreads();
+    xsync<WAVE>();
     lds_write<NS - 1>(v, p, lds, LS);
-    __syncthreads();
+    xsync<WAVE>();
     lds_read<0>(v, p, lds, LS);
   }
   // Write final-stage outputs into LDS in natural order (caller syncs after)
   static DEV void out_to_lds(const C<T>* v, int p, C<T>* lds, int LS) {
-    __syncthreads();
+    xsync<WAVE>();
     lds_write<NS - 1>(v, p, lds, LS);
   }
 };

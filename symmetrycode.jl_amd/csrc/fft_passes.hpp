@@ -21,8 +21,11 @@ DEV int xcd_remap(int b, int nb) {
 
 // pencils per block along x for the strided passes: a 128-byte row segment,
 // halved until the LDS tile fits in 64 KiB
+#ifndef SC_TILE_BYTES
+#define SC_TILE_BYTES 128
+#endif
 template <typename T, int N> constexpr int tile_x() {
-  int tx = 128 / int(sizeof(C<T>));
+  int tx = SC_TILE_BYTES / int(sizeof(C<T>));
   while (tx > 1 && tx * N * int(sizeof(C<T>)) > 65536) tx /= 2;
   return tx;
 }
@@ -46,8 +49,34 @@ struct StridedGeom {
   double scale;       // output scale
 };
 
+// minimum waves per SIMD requested from the register allocator (0 = none)
+#ifndef SC_WPE_C2C
+#define SC_WPE_C2C 0
+#endif
+#ifndef SC_WPE_X
+#define SC_WPE_X 0
+#endif
+#ifndef SC_WPE_WIDE
+#define SC_WPE_WIDE 0
+#endif
+#if SC_WPE_C2C
+#define SC_LB_C2C(n) __launch_bounds__((n), SC_WPE_C2C)
+#else
+#define SC_LB_C2C(n) __launch_bounds__((n))
+#endif
+#if SC_WPE_X
+#define SC_LB_X(n) __launch_bounds__((n), SC_WPE_X)
+#else
+#define SC_LB_X(n) __launch_bounds__((n))
+#endif
+#if SC_WPE_WIDE
+#define SC_LB_WIDE(n) __launch_bounds__((n), SC_WPE_WIDE)
+#else
+#define SC_LB_WIDE(n) __launch_bounds__((n))
+#endif
+
 template <typename T, int N, int SGN>
-__global__ void __launch_bounds__((tile_x<T, N>() * Plan<N>::tpp))
+__global__ void SC_LB_C2C((tile_x<T, N>() * Plan<N>::tpp))
 k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw, StridedGeom g) {
   constexpr int TX = tile_x<T, N>();
   using PE = Pencil<T, N, SGN>;
@@ -103,9 +132,12 @@ constexpr int pow2_ceil(int v) {
   return p;
 }
 
+#ifndef SC_ROWS_THREADS
+#define SC_ROWS_THREADS 512
+#endif
 template <int M> constexpr int rows_per_block() {
   int tpp = Plan<M>::tpp;
-  int r = 256 / tpp;
+  int r = SC_ROWS_THREADS / tpp;
   return r < 1 ? 1 : r;
 }
 
@@ -195,7 +227,7 @@ k_x_r2c(const T* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm,
 // products v_i v_j, and R2C of each product (times scale).  Also the
 // row-wise max of |v| for the CFL time step.
 template <typename T, int M, int D>
-__global__ void __launch_bounds__((rows_per_block<M>() * Plan<M>::tpp))
+__global__ void SC_LB_X((rows_per_block<M>() * Plan<M>::tpp))
 k_x_products(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm,
              const C<T>* __restrict__ twn, RowGeom g, unsigned long long* vmax_bits) {
   constexpr int TY = rows_per_block<M>();
@@ -263,6 +295,95 @@ k_x_products(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw
     if (rok) {
       C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
       for (int k = p; k <= M; k += PI::TPP) dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn), s);
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// The same fused x-pass with one wavefront per row: every LDS exchange is
+// wavefront-local (no block barrier after the twiddle staging), so the
+// wavefronts of a CU progress independently and hide each other's latency.
+// Lanes beyond the pencil's thread count duplicate lane 0's work and store
+// nothing.
+#ifndef SC_XW_WAVES
+#define SC_XW_WAVES 4
+#endif
+template <typename T, int M, int D>
+__global__ void __launch_bounds__(64 * SC_XW_WAVES)
+k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm,
+               const C<T>* __restrict__ twn, RowGeom g, unsigned long long* vmax_bits) {
+  constexpr int TPP = Plan<M>::tpp;
+  static_assert(TPP <= 64, "one wavefront per row");
+  constexpr int NW = SC_XW_WAVES;
+  constexpr int NP = D == 3 ? 6 : 3;
+  using PI = Pencil<T, M, +1, TPP, true>;
+  using PF = Pencil<T, M, -1, TPP, true>;
+  constexpr int E = PlanInfo<M>::E_LAST;
+  __shared__ C<T> lds[NW * M];
+  __shared__ C<T> twm_l[M];
+  __shared__ C<T> twn_l[M + 1];
+  __shared__ double red[NW];
+  for (int i = threadIdx.x; i < M; i += blockDim.x) twm_l[i] = twm[i];
+  for (int i = threadIdx.x; i <= M; i += blockDim.x) twn_l[i] = twn[i];
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool act = lane < TPP;
+  const int p = act ? lane : 0;
+  const long row = long(blockIdx.x) * NW + w;
+  const bool rok = row < g.nrows;
+  C<T>* ll = lds + w * M;
+  C<T> u[D][E];
+  C<T> v[PlanInfo<M>::MAXE];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const C<T>* src = in + long(c) * g.s_comp_spec + row * g.nhp;
+    PI::load(v, p, [&](int k) {
+      if (!rok) return mk<T>(T(0), T(0));
+      return c2r_pre<T, M>(k, twn_l, [&](int kk) { return kk <= g.kin ? src[kk] : mk<T>(T(0), T(0)); });
+    });
+    PI::run(v, p, ll, 1, twm_l);
+#pragma unroll
+    for (int e = 0; e < E; ++e) u[c][e] = v[e];
+  }
+  if (vmax_bits != nullptr) {
+    double vm = 0.0;
+    PI::for_out(p, [&](int e, int) {
+      double a = 0, b = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        a += double(u[c][e].x) * double(u[c][e].x);
+        b += double(u[c][e].y) * double(u[c][e].y);
+      }
+      vm = fmax(vm, fmax(a, b));
+    });
+    if (!rok) vm = 0.0;
+    for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o));
+    if (lane == 0) red[w] = vm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int i = 1; i < NW; ++i) m = fmax(m, red[i]);
+      atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(sqrt(m)));
+    }
+  }
+  constexpr int PA[6] = {0, 1, 2, 0, 1, 2};
+  constexpr int PB[6] = {0, 1, 2, 1, 2, 0};
+  constexpr int QA[3] = {0, 1, 0};
+  constexpr int QB[3] = {0, 1, 1};
+  const T s = T(g.scale);
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int i = D == 3 ? PA[q] : QA[q];
+    const int j = D == 3 ? PB[q] : QB[q];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
+    PI::out_to_in(v, p, ll, 1);
+    PF::run(v, p, ll, 1, twm_l);
+    PF::out_to_lds(v, p, ll, 1);
+    xsync<true>();
+    if (rok) {
+      C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
+      for (int k = lane; k <= M; k += 64) dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn_l), s);
     }
   }
 }

@@ -159,16 +159,37 @@ static void x_r2c(Ctx& c, C<T>* out, const T* in, int ncomp, double scale) {
   SC_HIP(hipGetLastError());
 }
 
+#ifndef SC_XPROD_WAVE
+#define SC_XPROD_WAVE 1
+#endif
 template <typename T>
 static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale, unsigned long long* vmax) {
   const int M = c.n / 2;
   RowGeom g = row_geom<T>(c, kin, scale);
-  const int TY = rows_rt(M);
-  dim3 grid(unsigned((g.nrows + TY - 1) / TY), 1);
   auto twm = reinterpret_cast<const C<T>*>(c.tw_m);
   auto twn = reinterpret_cast<const C<T>*>(c.tw_n);
   c.prof_begin("x_products");
-  switch (M) {
+  if (SC_XPROD_WAVE && tpp_rt(M) <= 64) {
+    constexpr int NW = SC_XW_WAVES;
+    dim3 grid(unsigned((g.nrows + NW - 1) / NW), 1);
+    switch (M) {
+#define X(MM)                                                                                            \
+  case MM:                                                                                               \
+    if constexpr (Plan<MM>::tpp <= 64) {                                                                 \
+      if (c.D == 3)                                                                                      \
+        hipLaunchKernelGGL((k_x_products_w<T, MM, 3>), grid, dim3(64 * NW), 0, c.stream, in, out, twm, twn, g, vmax); \
+      else                                                                                               \
+        hipLaunchKernelGGL((k_x_products_w<T, MM, 2>), grid, dim3(64 * NW), 0, c.stream, in, out, twm, twn, g, vmax); \
+    }                                                                                                    \
+    break;
+      SC_FOR_M(X)
+#undef X
+      default: throw Error(2, "unsupported grid size");
+    }
+  } else {
+    const int TY = rows_rt(M);
+    dim3 grid(unsigned((g.nrows + TY - 1) / TY), 1);
+    switch (M) {
 #define X(MM)                                                                                                 \
   case MM:                                                                                                    \
     if (c.D == 3)                                                                                             \
@@ -178,9 +199,10 @@ static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale,
       hipLaunchKernelGGL((k_x_products<T, MM, 2>), grid, dim3(rows_per_block<MM>() * Plan<MM>::tpp), 0,       \
                          c.stream, in, out, twm, twn, g, vmax);                                               \
     break;
-    SC_FOR_M(X)
+      SC_FOR_M(X)
 #undef X
-    default: throw Error(2, "unsupported grid size");
+      default: throw Error(2, "unsupported grid size");
+    }
   }
   c.prof_end();
   SC_HIP(hipGetLastError());

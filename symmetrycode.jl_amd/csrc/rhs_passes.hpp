@@ -14,7 +14,11 @@
 namespace sc {
 
 // threads per pencil giving one butterfly per thread in the largest stage
+#ifndef SC_WIDE
+#define SC_WIDE 1
+#endif
 template <int N> constexpr int tpp_wide() {
+  if (!SC_WIDE) return Plan<N>::tpp;
   int m = 1;
   for (int s = 0; s < Plan<N>::ns; ++s) {
     int b = N / Plan<N>::r[s];
@@ -31,18 +35,30 @@ struct RhsParams {
   int kband2;
 };
 
+// Copy an n-entry twiddle table into LDS (all threads participate).
+template <typename T>
+DEV void stage_twiddles(C<T>* dst, const C<T>* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+
 // Output slots of the y-pass: 0 A_x, 1 A_y, 2 s_zz, 3 A_z, 4 s_yz, 5 s_zx.
 // Input slots (packed stress): 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  The
 // processing order below only ever writes slots whose input this block has
-// already consumed, so the pass runs in place.
+// already consumed, so the pass runs in place.  The global loads of the next
+// component are issued before the current component's FFT (software
+// pipelining); twiddles are read from LDS so that waiting for them never
+// waits for the prefetch.
 template <typename T, int N>
-__global__ void __launch_bounds__((wide_threads<T, N>()))
+__global__ void SC_LB_WIDE((wide_threads<T, N>()))
 k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
   constexpr int TX = tile_x<T, N>();
   constexpr int TPP = tpp_wide<N>();
   using PE = Pencil<T, N, -1, TPP>;
-  constexpr int E = PE::PI::E_LAST;
+  constexpr int E = PE::EL;
   __shared__ C<T> lds[TX * N];
+  __shared__ C<T> twl[N];
+  stage_twiddles<T>(twl, tw, N);
   const int nb = g.ntile * g.nother;
   const int b = xcd_remap(blockIdx.x, nb);
   const int tile = b % g.ntile;
@@ -54,11 +70,18 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
   C<T>* base = buf + long(oi) * g.s_other + x;
   const T kx = T(kf) * T(x);
   C<T> v[PE::MAXE];
+  C<T> nx[PE::E0];
   C<T> acc[E];
-  auto fft = [&](int comp) {
+  auto fetch = [&](int comp) {
     const C<T>* src = base + long(comp) * g.s_comp;
-    PE::load(v, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
-    PE::run(v, p, lds + xl, TX, tw);
+    PE::load(nx, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
+  };
+  // run the FFT of the prefetched component, prefetching `next` (-1: none)
+  auto fft = [&](int next) {
+#pragma unroll
+    for (int e = 0; e < PE::E0; ++e) v[e] = nx[e];
+    if (next >= 0) fetch(next);
+    PE::run(v, p, lds + xl, TX, twl);
   };
   auto put = [&](int slot, const C<T>* val) {
     if (!xok) return;
@@ -66,27 +89,28 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
     PE::for_out(p, [&](int e, int k) { dst[long(k) * g.s_axis] = val[e]; });
   };
   auto kyv = [&](int k) { return T(kf) * T(freq(k, N)); };
+  fetch(0);
   // xx -> A_x = kx s_xx
-  fft(0);
+  fft(3);
   PE::for_out(p, [&](int e, int) { acc[e] = scl<T>(v[e], kx); });
   // xy -> A_x += ky s_xy (write slot 0); A_y = kx s_xy
-  fft(3);
+  fft(1);
   PE::for_out(p, [&](int e, int k) { acc[e] = add<T>(acc[e], scl<T>(v[e], kyv(k))); });
   put(0, acc);
   PE::for_out(p, [&](int e, int) { acc[e] = scl<T>(v[e], kx); });
   // yy -> A_y += ky s_yy (slot 1)
-  fft(1);
+  fft(2);
   PE::for_out(p, [&](int e, int k) { acc[e] = add<T>(acc[e], scl<T>(v[e], kyv(k))); });
   put(1, acc);
   // zz -> slot 2
-  fft(2);
+  fft(5);
   put(2, v);
   // zx -> slot 5; A_z = kx s_zx
-  fft(5);
+  fft(4);
   put(5, v);
   PE::for_out(p, [&](int e, int) { acc[e] = scl<T>(v[e], kx); });
   // yz -> slot 4; A_z += ky s_yz (slot 3)
-  fft(4);
+  fft(-1);
   put(4, v);
   PE::for_out(p, [&](int e, int k) { acc[e] = add<T>(acc[e], scl<T>(v[e], kyv(k))); });
   put(3, acc);
@@ -94,14 +118,17 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
 
 // Final z-pass: du_i = -i (A_i + kz s_iz) - nu |k|^2 u_i (+ A_f u_i on the band).
 template <typename T, int N>
-__global__ void __launch_bounds__((wide_threads<T, N>()))
+__global__ void SC_LB_WIDE((wide_threads<T, N>()))
 k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw, StridedGeom g,
             RhsParams rp) {
   constexpr int TX = tile_x<T, N>();
   constexpr int TPP = tpp_wide<N>();
   using PE = Pencil<T, N, -1, TPP>;
-  constexpr int E = PE::PI::E_LAST;
+  constexpr int E = PE::EL;
   __shared__ C<T> lds[TX * N];
+  __shared__ C<T> twl[N];
+  __shared__ C<T> accl[TX * N];  // kz * FFT(s_iz), natural order (each thread reads back its own)
+  stage_twiddles<T>(twl, tw, N);
   const int nb = g.ntile * g.nother;
   const int b = xcd_remap(blockIdx.x, nb);
   const int tile = b % g.ntile;
@@ -118,26 +145,32 @@ k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* _
   T fA = T(0);
   if (rp.forceA) fA = T(*rp.forceA);
   C<T> v[PE::MAXE];
-  C<T> acc[E];
-  auto fft = [&](int slot) {
+  C<T> nx[PE::E0];
+  auto fetch = [&](int slot) {
     const C<T>* src = buf + long(slot) * g.s_comp + off;
-    PE::load(v, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
-    PE::run(v, p, lds + xl, TX, tw);
+    PE::load(nx, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
+  };
+  auto fft = [&](int next) {
+#pragma unroll
+    for (int e = 0; e < PE::E0; ++e) v[e] = nx[e];
+    if (next >= 0) fetch(next);
+    PE::run(v, p, lds + xl, TX, twl);
   };
   constexpr int SA[3] = {0, 1, 3};  // A_x, A_y, A_z
   constexpr int SZ[3] = {5, 4, 2};  // s_zx, s_yz, s_zz
+  fetch(SZ[0]);
 #pragma unroll 1
   for (int i = 0; i < 3; ++i) {
-    fft(SZ[i]);
-    PE::for_out(p, [&](int e, int k) { acc[e] = scl<T>(v[e], kf * T(freq(k, N))); });
     fft(SA[i]);
+    PE::for_out(p, [&](int e, int k) { accl[k * TX + xl] = scl<T>(v[e], kf * T(freq(k, N))); });
+    fft(i < 2 ? SZ[i + 1] : -1);
     if (xok) {
       const C<T>* ui = u + long(i) * g.s_comp + off;
       C<T>* di = du + long(i) * g.s_comp + off;
       PE::for_out(p, [&](int e, int k) {
         const int kzi = freq(k, N);
         const T kz = kf * T(kzi);
-        C<T> a = add<T>(acc[e], v[e]);
+        C<T> a = add<T>(accl[k * TX + xl], v[e]);
         C<T> uu = ui[long(k) * g.s_axis];
         const T k2 = kx * kx + ky * ky + kz * kz;
         C<T> r = mk<T>(a.y - nu * k2 * uu.x, -a.x - nu * k2 * uu.y);  // -i a - nu k^2 u
@@ -153,7 +186,7 @@ k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* _
 
 // 2D final y-pass over the packed stress (xx, yy, xy).
 template <typename T, int N>
-__global__ void __launch_bounds__((wide_threads<T, N>()))
+__global__ void SC_LB_WIDE((wide_threads<T, N>()))
 k_y_fwd_rhs2d(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw, StridedGeom g,
               RhsParams rp) {
   constexpr int TX = tile_x<T, N>();
@@ -161,6 +194,8 @@ k_y_fwd_rhs2d(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>*
   using PE = Pencil<T, N, -1, TPP>;
   constexpr int E = PE::PI::E_LAST;
   __shared__ C<T> lds[TX * N];
+  __shared__ C<T> twl[N];
+  stage_twiddles<T>(twl, tw, N);
   const int b = xcd_remap(blockIdx.x, g.ntile);
   const int t = threadIdx.x;
   const int xl = t % TX, p = t / TX;
@@ -176,7 +211,7 @@ k_y_fwd_rhs2d(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>*
   auto fft = [&](int slot) {
     const C<T>* src = buf + long(slot) * g.s_comp + x;
     PE::load(v, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
-    PE::run(v, p, lds + xl, TX, tw);
+    PE::run(v, p, lds + xl, TX, twl);
   };
   fft(2);
   PE::for_out(p, [&](int e, int) { sxy[e] = v[e]; });

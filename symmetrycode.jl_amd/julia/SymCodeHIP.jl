@@ -1,0 +1,224 @@
+# SymCodeHIP.jl -- Julia binding of libsymcode_hip.so for SymmetryCode.jl.
+#
+# Adds methods of the reference's own functions for a new backend tag, so the
+# existing call sites (create_dns, create_data, solve_les!, sfs!) run on an
+# MI355X through the C ABI in include/symcode.h.  Fields are `HIPField`s:
+# device buffers in the library's device layout, created by the allocators
+# below (the Julia GC owns them through a finalizer).
+#
+# Usage (see INTEGRATION.md):
+#     using SymmetryCode, SymCodeHIP
+#     g = SymmetryCode.Grid{3}(; l = 2π, n = 810, backend = SymCodeHIP.HIPBackend())
+#     u = SymmetryCode.randomfield(SymmetryCode.peak_profile, g; kpeak = 4)
+#     cache = SymmetryCode.getcache(g)
+#     SymmetryCode.wray3!(SymmetryCode.convectiondiffusion!, u, Δt, g, cache; visc)
+module SymCodeHIP
+
+using SymmetryCode: SymmetryCode as S
+
+const libsymcode = joinpath(@__DIR__, "..", "symcode", "libsymcode_hip.so")
+
+"Backend tag for `Grid(; backend = HIPBackend())`."
+struct HIPBackend end
+
+# ------------------------------------------------------------------ context
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    D::Int
+    n::Int
+    nh::Int
+    nhp::Int
+    spec_elems::Int
+    phys_elems::Int
+    prec::Int
+end
+
+const CONTEXTS = Dict{Tuple{Int,Int,Float64,DataType},Context}()
+
+function check(rc, ctx)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:sc_last_error, libsymcode), Cstring, (Ptr{Cvoid},), ctx))
+    error("symcode error $rc: $msg")
+end
+
+"One library context per (D, n, l, T), created on first use."
+function context(g::S.Grid{D,T,HIPBackend}) where {D,T}
+    get!(CONTEXTS, (D, g.n, Float64(g.l), T)) do
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        prec = T === Float64 ? 64 : 32
+        check(ccall((:sc_create, libsymcode), Cint, (Cint, Cint, Cdouble, Cint, Cint, Ref{Ptr{Cvoid}}),
+                    D, g.n, g.l, prec, 0, out), C_NULL)
+        info = zeros(Int64, 8)
+        check(ccall((:sc_info, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Int64}), out[], info), out[])
+        ctx = Context(out[], D, g.n, info[3], info[4], info[6], info[7], prec)
+        finalizer(c -> ccall((:sc_destroy, libsymcode), Cint, (Ptr{Cvoid},), c.ptr), ctx)
+        ctx
+    end
+end
+
+# -------------------------------------------------------------------- fields
+"A device field of `ncomp` spectral (complex) or physical (real) scalars."
+mutable struct HIPField{T}
+    ptr::Ptr{Cvoid}
+    ncomp::Int
+    spectral::Bool
+    ctx::Context
+end
+
+function HIPField{T}(ctx::Context, ncomp, spectral) where {T}
+    elems = spectral ? ctx.spec_elems : ctx.phys_elems
+    bytes = ncomp * elems * (spectral ? 2 : 1) * sizeof(T)
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    # hipMalloc through the HIP runtime the library links against
+    rc = ccall((:hipMalloc, "libamdhip64"), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, bytes)
+    rc == 0 || error("hipMalloc failed ($rc)")
+    ccall((:hipMemset, "libamdhip64"), Cint, (Ptr{Cvoid}, Cint, Csize_t), p[], 0, bytes)
+    f = HIPField{T}(p[], ncomp, spectral, ctx)
+    finalizer(x -> ccall((:hipFree, "libamdhip64"), Cint, (Ptr{Cvoid},), x.ptr), f)
+    f
+end
+
+const HIPGrid{D,T} = S.Grid{D,T,HIPBackend}
+
+# the reference's allocators (src/solver.jl:246-317) return one HIPField per
+# vector/tensor field instead of a NamedTuple of arrays
+S.scalarfield(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), 1, true)
+S.vectorfield(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), D, true)
+S.tensorfield(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), D == 2 ? 3 : 6, true)
+S.tensorfield_nonsym(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), D * D, true)
+S.spacescalarfield(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), 1, false)
+S.spacevectorfield(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), D, false)
+S.spacetensorfield(g::HIPGrid{D,T}) where {D,T} = HIPField{T}(context(g), D == 2 ? 3 : 6, false)
+
+"Copy a NamedTuple of host RFFT arrays (the reference layout) to the device."
+function upload!(f::HIPField{T}, u::NamedTuple) where {T}
+    host = stack(map(Array, values(u)))
+    check(ccall((:sc_upload_spectral, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                f.ctx.ptr, f.ptr, host, f.ncomp), f.ctx.ptr)
+    f
+end
+"Device field -> host array of size (n/2+1, n, n, ncomp)."
+function download(f::HIPField{T}) where {T}
+    c = f.ctx
+    host = Array{Complex{T}}(undef, c.nh, c.n, c.D == 3 ? c.n : 1, f.ncomp)
+    check(ccall((:sc_download_spectral, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                c.ptr, host, f.ptr, f.ncomp), c.ptr)
+    host
+end
+
+# ------------------------------------------------------------------ solver
+# cache: the library owns all scratch; getcache only binds the context
+S.getcache(g::HIPGrid) = (; ctx = context(g))
+S.statscache(g::HIPGrid) = (; ctx = context(g))
+
+S.apply!(::typeof(S.twothirds!), g::HIPGrid, (u, _)) =
+    check(ccall((:sc_twothirds, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), u.ctx.ptr, u.ptr, u.ncomp), u.ctx.ptr)
+S.apply!(::typeof(S.project!), g::HIPGrid, (u, _)) =
+    check(ccall((:sc_project, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), u.ctx.ptr, u.ptr), u.ctx.ptr)
+S.apply!(::typeof(S.gaussianfilter!), g::HIPGrid, (u, Δ, _)) =
+    check(ccall((:sc_gaussianfilter, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble),
+                u.ctx.ptr, u.ptr, u.ncomp, Δ), u.ctx.ptr)
+S.apply!(::typeof(S.cutoff!), g::HIPGrid, (ubar, u)) =
+    check(ccall((:sc_cutoff, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                ubar.ctx.ptr, u.ctx.ptr, ubar.ptr, u.ptr, u.ncomp), ubar.ctx.ptr)
+
+S.to_phys!(phys::HIPField, spec::HIPField, plan, g::HIPGrid) =
+    (check(ccall((:sc_to_phys, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                 spec.ctx.ptr, phys.ptr, spec.ptr, spec.ncomp), spec.ctx.ptr); phys)
+S.to_spec!(spec::HIPField, phys::HIPField, plan, g::HIPGrid) =
+    (check(ccall((:sc_to_spec, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                 spec.ctx.ptr, spec.ptr, phys.ptr, phys.ncomp), spec.ctx.ptr); spec)
+
+S.nonlinearity!(σ::HIPField, vi_vj, v, u::HIPField, plan, g::HIPGrid) =
+    check(ccall((:sc_nonlinearity, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                u.ctx.ptr, σ.ptr, u.ptr), u.ctx.ptr)
+
+function S.convectiondiffusion!(du::HIPField, u::HIPField, grid::HIPGrid, cache; visc)
+    check(ccall((:sc_convectiondiffusion, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble),
+                u.ctx.ptr, du.ptr, u.ptr, visc), u.ctx.ptr)
+end
+
+function S.wray3!(::typeof(S.convectiondiffusion!), u::HIPField, Δt, grid::HIPGrid, cache; visc)
+    check(ccall((:sc_wray3, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble),
+                u.ctx.ptr, u.ptr, Δt, visc), u.ctx.ptr)
+    u
+end
+
+function S.propose_timestep(u::HIPField, grid::HIPGrid, visc, cache)
+    dt = Ref{Cdouble}(0)
+    check(ccall((:sc_propose_timestep, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ref{Cdouble}),
+                u.ctx.ptr, u.ptr, visc, dt), u.ctx.ptr)
+    dt[]
+end
+
+function S.energy(u::HIPField)
+    e = Ref{Cdouble}(0)
+    check(ccall((:sc_energy, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Cdouble}),
+                u.ctx.ptr, u.ptr, u.ncomp, e), u.ctx.ptr)
+    e[]
+end
+
+function S.get_dissipation!(diss, u::HIPField, visc, g::HIPGrid)
+    e = Ref{Cdouble}(0)
+    check(ccall((:sc_dissipation, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ref{Cdouble}),
+                u.ctx.ptr, u.ptr, visc, e), u.ctx.ptr)
+    e[]
+end
+
+function S.spectrum(u::HIPField, grid::HIPGrid, stuff = nothing)
+    kcut = div(2 * div(grid.n, 2), 3)
+    s = zeros(kcut)
+    check(ccall((:sc_spectrum, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}),
+                u.ctx.ptr, u.ptr, u.ncomp, kcut, s), u.ctx.ptr)
+    (; k = 2π / grid.l * (1:kcut), s)
+end
+
+# shell clamp (src/solver.jl:1135-1152): shells are 1..m
+function S.energy_shells(grid::HIPGrid, shells, u::HIPField)
+    m = length(shells)
+    e = zeros(m + 1)
+    check(ccall((:sc_shell_energies, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}),
+                u.ctx.ptr, u.ptr, u.ncomp, m + 1, e), u.ctx.ptr)
+    map(i -> (; shell = i, eref = e[i+1]), shells)
+end
+function S.maintain_shell_energy!(u::HIPField, shells)
+    eref = [s.eref for s in shells]
+    check(ccall((:sc_maintain_shell_energy, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cdouble}),
+                u.ctx.ptr, u.ptr, length(eref), eref), u.ctx.ptr)
+    u
+end
+
+function S.randomfield(profile, grid::HIPGrid; totalenergy = 1, rng = nothing, seed = 0, kwargs...)
+    kdiag = floor(Int, sqrt(S.dim(grid)) * div(grid.n, 2))
+    prof = [profile(k; kwargs...) for k in 0:kdiag]
+    E0 = totalenergy .* prof ./ sum(prof)
+    u = S.vectorfield(grid)
+    check(ccall((:sc_randomfield, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, UInt64, Cint, Ptr{Cdouble}),
+                u.ctx.ptr, u.ptr, seed, kdiag + 1, E0), u.ctx.ptr)
+    u
+end
+
+# ------------------------------------------------------------------ closures
+struct BuiltinClosure
+    kind::Cint
+    p1::Cdouble
+    p2::Cdouble
+end
+S.create_clark(Δ, g::HIPGrid) = BuiltinClosure(1, Δ, 0)
+S.create_smagorinsky(CS, Δ, g::HIPGrid) = BuiltinClosure(2, CS, Δ)
+S.create_dynamic_smagorinsky(Δ, g::HIPGrid) = BuiltinClosure(3, Δ, 0)
+
+function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::BuiltinClosure, visc)
+    check(ccall((:sc_les_rhs, libsymcode), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
+                u.ctx.ptr, du.ptr, u.ptr, visc, model.kind, model.p1, model.p2), u.ctx.ptr)
+end
+
+function S.wray3!(::typeof(S.les!), u::HIPField, Δt, grid::HIPGrid, cache; model::BuiltinClosure, visc)
+    check(ccall((:sc_les_wray3, libsymcode), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cdouble, Cdouble),
+                u.ctx.ptr, u.ptr, Δt, visc, model.kind, model.p1, model.p2), u.ctx.ptr)
+    u
+end
+
+end # module

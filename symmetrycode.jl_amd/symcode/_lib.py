@@ -3,7 +3,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "libsymcode_hip.so")
+# SYMCODE_LIB selects an alternative build (e.g. a tuning variant)
+LIBPATH = os.environ.get("SYMCODE_LIB") or os.path.join(_HERE, "libsymcode_hip.so")
 
 _lib = None
 
