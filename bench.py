@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--prec", type=int, default=64, choices=(32, 64))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--copy-calib-gb", type=float, default=0.0,
+                    help="also run one device-to-device copy of this many GB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU slab/pencil sharding is "
@@ -93,6 +95,13 @@ def main():
 
     for _ in range(args.warmup):
         step()
+    if args.copy_calib_gb > 0:
+        nb = int(args.copy_calib_gb * 1e9) // 16
+        a = torch.empty(nb, dtype=torch.complex128, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     torch.cuda.synchronize()
     L.sc_profile(g.ctx, 1)
     stream = torch.cuda.current_stream()

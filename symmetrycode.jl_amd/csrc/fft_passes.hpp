@@ -75,12 +75,18 @@ struct StridedGeom {
 #define SC_LB_WIDE(n) __launch_bounds__((n))
 #endif
 
+// All components of a tile are handled by one block: the global loads of
+// component c+1 are issued before the FFT of component c, and twiddles are
+// read from LDS, so the memory pipeline stays busy during the transforms.
 template <typename T, int N, int SGN>
 __global__ void SC_LB_C2C((tile_x<T, N>() * Plan<N>::tpp))
-k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw, StridedGeom g) {
+k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw, StridedGeom g, int ncomp) {
   constexpr int TX = tile_x<T, N>();
   using PE = Pencil<T, N, SGN>;
   __shared__ C<T> lds[TX * N];
+  __shared__ C<T> twl[N];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+  __syncthreads();
   const int nb = g.ntile * g.nother;
   const int b = xcd_remap(blockIdx.x, nb);
   const int tile = b % g.ntile;
@@ -90,9 +96,7 @@ k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ t
   const int xl = t % TX, p = t / TX;
   const int x = tile * TX + xl;
   const bool xok = x < g.xs;
-  const long base = long(blockIdx.y) * g.s_comp + long(oi) * g.s_other + x;
-  const C<T>* src = in + base;
-  C<T>* dst = out + base;
+  const long base = long(oi) * g.s_other + x;
   bool xmask = true;
   bool zero_o = false;
   if (g.mask23) {
@@ -101,16 +105,28 @@ k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ t
     zero_o = (x == 0) && (ko == 0);
   }
   C<T> v[PE::MAXE];
-  PE::load(v, p, [&](int j) {
-    int kj = freq(j, N);
-    bool ok = xok && xmask && kj <= g.kin && kj >= -g.kin;
-    if (zero_o && j == 0) ok = false;
-    return ok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0));
-  });
-  PE::run(v, p, lds + xl, TX, tw);
-  if (xok) {
-    const T sc_ = T(g.scale);
-    PE::store(v, p, [&](int k, C<T> val) { dst[long(k) * g.s_axis] = scl<T>(val, sc_); });
+  C<T> nx[PE::E0];
+  auto fetch = [&](int c) {
+    const C<T>* src = in + long(c) * g.s_comp + base;
+    PE::load(nx, p, [&](int j) {
+      int kj = freq(j, N);
+      bool ok = xok && xmask && kj <= g.kin && kj >= -g.kin;
+      if (zero_o && j == 0) ok = false;
+      return ok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0));
+    });
+  };
+  const T sc_ = T(g.scale);
+  fetch(0);
+#pragma unroll 1
+  for (int c = 0; c < ncomp; ++c) {
+#pragma unroll
+    for (int e = 0; e < PE::E0; ++e) v[e] = nx[e];
+    if (c + 1 < ncomp) fetch(c + 1);
+    PE::run(v, p, lds + xl, TX, twl);
+    if (xok) {
+      C<T>* dst = out + long(c) * g.s_comp + base;
+      PE::store(v, p, [&](int k, C<T> val) { dst[long(k) * g.s_axis] = scl<T>(val, sc_); });
+    }
   }
 }
 
@@ -306,7 +322,7 @@ k_x_products(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw
 // Lanes beyond the pencil's thread count duplicate lane 0's work and store
 // nothing.
 #ifndef SC_XW_WAVES
-#define SC_XW_WAVES 4
+#define SC_XW_WAVES 2
 #endif
 template <typename T, int M, int D>
 __global__ void __launch_bounds__(64 * SC_XW_WAVES)

@@ -78,7 +78,7 @@ static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int 
   g.mask23 = mask23 ? 1 : 0;
   g.kcut = kc;
   g.scale = scale;
-  dim3 grid(unsigned(g.ntile) * unsigned(g.nother), unsigned(ncomp));
+  dim3 grid(unsigned(g.ntile) * unsigned(g.nother), 1);
   const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
   static const char* names[2][2] = {{"c2c_y_fwd", "c2c_y_inv"}, {"c2c_z_fwd", "c2c_z_inv"}};
   c.prof_begin(names[axis - 1][sign > 0]);
@@ -87,10 +87,10 @@ static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int 
   case NN:                                                                                           \
     if (sign < 0)                                                                                    \
       hipLaunchKernelGGL((k_c2c_strided<T, NN, -1>), grid, dim3(tile_x<T, NN>() * Plan<NN>::tpp), 0, \
-                         c.stream, in, out, tw, g);                                                  \
+                         c.stream, in, out, tw, g, ncomp);                                           \
     else                                                                                             \
       hipLaunchKernelGGL((k_c2c_strided<T, NN, 1>), grid, dim3(tile_x<T, NN>() * Plan<NN>::tpp), 0,  \
-                         c.stream, in, out, tw, g);                                                  \
+                         c.stream, in, out, tw, g, ncomp);                                           \
     break;
     SC_FOR_N(X)
 #undef X
