@@ -149,6 +149,15 @@ int sc_closure_dynsmag(sc_ctx* ctx, void* tau, const void* u, double delta, doub
 int sc_les_rhs(sc_ctx* ctx, void* du, const void* u, double visc, int closure, double p1, double p2);
 int sc_les_wray3(sc_ctx* ctx, void* u, double dt, double visc, int closure, double p1, double p2);
 
+/* adjoint (no reference counterpart: the reference trains a-priori only).
+ * Inner product: the RFFT-weighted one of spectraldot (src/solver.jl:708).
+ * sc_rhs_vjp: out = J(u)^T w for convectiondiffusion! (+ band forcing).
+ * sc_rk4_vjp / sc_wray3_vjp: out = (d step / d u0)^T ubar for one step of
+ * fixed dt, recomputing the stage values from the checkpoint u0. */
+int sc_rhs_vjp(sc_ctx* ctx, void* out, const void* u, const void* w, double visc);
+int sc_rk4_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc);
+int sc_wray3_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc);
+
 #ifdef __cplusplus
 }
 #endif

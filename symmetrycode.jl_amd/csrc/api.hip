@@ -92,6 +92,9 @@ int sc_destroy(sc_ctx* c) {
     if (b) (void)hipFree(b);
   for (void* b : c->ws)
     if (b) (void)hipFree(b);
+  for (void* b : c->adjw)
+    if (b) (void)hipFree(b);
+  if (c->bandadj) (void)hipFree(c->bandadj);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -405,6 +408,24 @@ int sc_les_wray3(sc_ctx* c, void* u, double dt, double nu, int closure, double p
     need_fft(c);
     need(closure >= 0 && closure <= 3, "closure kind");
     TYPED(c, sc::Ops<T>::les_wray3(*c, (CT*)u, dt, nu, closure, p1, p2));
+  });
+}
+
+int sc_rhs_vjp(sc_ctx* c, void* out, const void* u, const void* w, double nu) {
+  return guard(c, [&] { need_fft(c); TYPED(c, sc::Ops<T>::rhs_vjp(*c, (CT*)out, (const CT*)u, (const CT*)w, nu)); });
+}
+int sc_rk4_vjp(sc_ctx* c, void* out, const void* u0, const void* ubar, double dt, double nu) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(out != u0 && out != ubar, "out must not alias u0 or ubar");
+    TYPED(c, sc::Ops<T>::rk4_vjp(*c, (CT*)out, (const CT*)u0, (const CT*)ubar, dt, nu));
+  });
+}
+int sc_wray3_vjp(sc_ctx* c, void* out, const void* u0, const void* ubar, double dt, double nu) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(out != u0 && out != ubar, "out must not alias u0 or ubar");
+    TYPED(c, sc::Ops<T>::wray3_vjp(*c, (CT*)out, (const CT*)u0, (const CT*)ubar, dt, nu));
   });
 }
 

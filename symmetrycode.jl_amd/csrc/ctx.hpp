@@ -42,8 +42,11 @@ struct Ctx {
   unsigned long long* vmax = nullptr;
   double* forceA = nullptr;
   double* dtdev = nullptr;  // time step of the current step (device)
-  void* ws[16] = {};       // named workspaces (closures, LES)
+  void* ws[16] = {};       // named workspaces (closures, LES, adjoint RHS)
   size_t ws_bytes[16] = {};
+  void* adjw[8] = {};      // adjoint step workspaces
+  size_t adjw_bytes[8] = {};
+  double* bandadj = nullptr;
   double* hbuf = nullptr;  // pinned host scratch
   long hbuf_n = 0;
   // forcing applied inside the RHS: 0 none, 1 linear band
@@ -112,6 +115,16 @@ struct Ctx {
       ws_bytes[slot] = bytes;
     }
     return ws[slot];
+  }
+  void* adj(int slot, size_t bytes) {
+    if (adjw_bytes[slot] < bytes) {
+      if (adjw[slot]) SC_HIP(hipFree(adjw[slot]));
+      adjw[slot] = nullptr;
+      SC_HIP(hipMalloc(&adjw[slot], bytes));
+      SC_HIP(hipMemsetAsync(adjw[slot], 0, bytes, stream));
+      adjw_bytes[slot] = bytes;
+    }
+    return adjw[slot];
   }
   double* dscratch(long n_) {
     if (dscr_n < n_) {

@@ -36,6 +36,11 @@ struct Ops {
   static double wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure, double p1, double p2);
   static double rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl);
   static double read_dt(Ctx& c);
+  // adjoint
+  static void lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double b, int ncomp);
+  static void rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu);
+  static void rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu);
+  static void wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu);
   // reductions (return to host)
   static double sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu);
   static void dot_modes(Ctx& c, const CT* u, const CT* v, int ncomp, double* re, double* im);

@@ -777,4 +777,125 @@ void Ops<T>::les_wray3(Ctx& c, CT* u, double dt, double nu, int closure, double 
   wray3_dev(c, u, dt, nu, 0.0, closure, p1, p2);
 }
 
+// ------------------------------------------------------------------ adjoint
+template <typename T>
+void Ops<T>::lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double b, int ncomp) {
+  SPEC_LAUNCH(k_lincomb<T>, dst, x, T(a), y, T(b), g_, ncomp);
+}
+
+// J(u)^T w of the DNS right-hand side (convectiondiffusion! plus the band
+// forcing when enabled) under the RFFT-weighted inner product:
+//   g_ij = C2R(i k_j w_i), v = C2R(2/3 u), a_m = sum_j g_mj v_j + sum_i g_im v_i,
+//   out = 2/3 R2C(a)/n^D - nu (|k|^2 w + k (k.w)) + forcing transpose.
+template <typename T>
+void Ops<T>::rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu) {
+  const int D = c.D, ng = D * D;
+  const long np = c.phys_elems;
+  T* v = reinterpret_cast<T*>(c.work(13, size_t(D) * np * sizeof(T)));
+  T* gr = reinterpret_cast<T*>(c.work(14, size_t(ng) * np * sizeof(T)));
+  CT* G = reinterpret_cast<CT*>(c.work(15, size_t(ng) * c.spec_elems * sizeof(CT)));
+  c2r(c, v, u, D, 1.0, true);
+  vectorgradient(c, G, w);
+  c2r(c, gr, G, ng, 1.0, false);
+  if (D == 3) hipLaunchKernelGGL((k_adj_products<T, 3>), dim3(nblk(np)), dim3(256), 0, c.stream, v, gr, np);
+  else hipLaunchKernelGGL((k_adj_products<T, 2>), dim3(nblk(np)), dim3(256), 0, c.stream, v, gr, np);
+  SC_HIP(hipGetLastError());
+  r2c(c, out, v, D, 1.0 / std::pow(double(c.n), D));
+  twothirds(c, out, D);
+  const double* bandp = nullptr;
+  if (c.force_kind == 1) {
+    if (!c.bandadj) SC_HIP(hipMalloc(&c.bandadj, 2 * sizeof(double)));
+    hipLaunchKernelGGL(k_band_adj<T>, dim3(1), dim3(256), 0, c.stream, u, w, D, c.geom(), c.kband, c.injection,
+                       c.bandadj);
+    SC_HIP(hipGetLastError());
+    bandp = c.bandadj;
+  }
+  if (D == 3) SPEC_LAUNCH((k_adj_visc_force<T, 3>), out, w, u, nu, g_, bandp, c.kband * c.kband);
+  else SPEC_LAUNCH((k_adj_visc_force<T, 2>), out, w, u, nu, g_, bandp, c.kband * c.kband);
+}
+
+// Transposed Jacobian of one RK4 step (fixed dt) applied to ubar; the stage
+// values are recomputed from the checkpoint u0.
+template <typename T>
+void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu) {
+  const int D = c.D;
+  const size_t fb = size_t(D) * c.spec_elems * sizeof(CT);
+  CT* k = reinterpret_cast<CT*>(c.adj(0, fb));
+  CT* U1 = reinterpret_cast<CT*>(c.adj(1, fb));
+  CT* U2 = reinterpret_cast<CT*>(c.adj(2, fb));
+  CT* U3 = reinterpret_cast<CT*>(c.adj(3, fb));
+  CT* Sb = reinterpret_cast<CT*>(c.adj(4, fb));
+  CT* KB = reinterpret_cast<CT*>(c.adj(5, fb));
+  CT* Tt = reinterpret_cast<CT*>(c.adj(6, fb));
+  CT* acc = reinterpret_cast<CT*>(c.adj(7, fb));
+  // forward recompute of the stage values
+  rhs_fused(c, k, u0, nu, nullptr);
+  lincomb(c, U1, u0, 1.0, k, dt / 2, D);
+  project(c, U1);
+  rhs_fused(c, k, U1, nu, nullptr);
+  lincomb(c, U2, u0, 1.0, k, dt / 2, D);
+  project(c, U2);
+  rhs_fused(c, k, U2, nu, nullptr);
+  lincomb(c, U3, u0, 1.0, k, dt, D);
+  project(c, U3);
+  // reverse sweep
+  SC_HIP(hipMemcpyAsync(Sb, ubar, fb, hipMemcpyDeviceToDevice, c.stream));
+  project(c, Sb);
+  SC_HIP(hipMemcpyAsync(acc, Sb, fb, hipMemcpyDeviceToDevice, c.stream));
+  lincomb(c, KB, Sb, dt / 6, Sb, 0.0, D);
+  rhs_vjp(c, Tt, U3, KB, nu);
+  project(c, Tt);
+  axpy(c, acc, Tt, 1.0, D);
+  lincomb(c, KB, Sb, dt / 3, Tt, dt, D);
+  rhs_vjp(c, Tt, U2, KB, nu);
+  project(c, Tt);
+  axpy(c, acc, Tt, 1.0, D);
+  lincomb(c, KB, Sb, dt / 3, Tt, dt / 2, D);
+  rhs_vjp(c, Tt, U1, KB, nu);
+  project(c, Tt);
+  axpy(c, acc, Tt, 1.0, D);
+  lincomb(c, KB, Sb, dt / 6, Tt, dt / 2, D);
+  rhs_vjp(c, Tt, u0, KB, nu);
+  lincomb(c, out, acc, 1.0, Tt, 1.0, D);
+}
+
+// Transposed Jacobian of one Wray3 step (fixed dt) applied to ubar.
+template <typename T>
+void Ops<T>::wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu) {
+  const int D = c.D;
+  const size_t fb = size_t(D) * c.spec_elems * sizeof(CT);
+  const double a1 = double(T(8.0 / 15.0)), a2 = double(T(5.0 / 12.0)), a3 = double(T(3.0 / 4.0));
+  const double b1 = double(T(1.0 / 4.0));
+  CT* k = reinterpret_cast<CT*>(c.adj(0, fb));
+  CT* U1 = reinterpret_cast<CT*>(c.adj(1, fb));
+  CT* S1 = reinterpret_cast<CT*>(c.adj(2, fb));
+  CT* U2 = reinterpret_cast<CT*>(c.adj(3, fb));
+  CT* Q = reinterpret_cast<CT*>(c.adj(4, fb));
+  CT* S1b = reinterpret_cast<CT*>(c.adj(5, fb));
+  CT* Tt = reinterpret_cast<CT*>(c.adj(6, fb));
+  CT* KB = reinterpret_cast<CT*>(c.adj(7, fb));
+  rhs_fused(c, k, u0, nu, nullptr);
+  lincomb(c, U1, u0, 1.0, k, a1 * dt, D);
+  project(c, U1);
+  lincomb(c, S1, u0, 1.0, k, b1 * dt, D);
+  rhs_fused(c, k, U1, nu, nullptr);
+  lincomb(c, U2, S1, 1.0, k, a2 * dt, D);
+  project(c, U2);
+  // reverse
+  SC_HIP(hipMemcpyAsync(Q, ubar, fb, hipMemcpyDeviceToDevice, c.stream));
+  project(c, Q);                                   // q3
+  SC_HIP(hipMemcpyAsync(S1b, Q, fb, hipMemcpyDeviceToDevice, c.stream));
+  lincomb(c, KB, Q, a3 * dt, Q, 0.0, D);
+  rhs_vjp(c, Tt, U2, KB, nu);                      // u2bar
+  project(c, Tt);                                  // q2
+  axpy(c, S1b, Tt, 1.0, D);
+  lincomb(c, KB, Tt, a2 * dt, Tt, 0.0, D);
+  rhs_vjp(c, Q, U1, KB, nu);                       // u1bar
+  project(c, Q);                                   // q1
+  lincomb(c, KB, Q, a1 * dt, S1b, b1 * dt, D);     // k1bar
+  rhs_vjp(c, Tt, u0, KB, nu);
+  lincomb(c, out, Q, 1.0, S1b, 1.0, D);
+  axpy(c, out, Tt, 1.0, D);
+}
+
 }  // namespace sc

@@ -344,6 +344,120 @@ __global__ void k_axpy(C<T>* y, const C<T>* x, T a, SpecGeom g, int ncomp) {
   }
 }
 
+// dst = a*x + b*y over ncomp spectral components (dst may alias x or y)
+template <typename T>
+__global__ void k_lincomb(C<T>* dst, const C<T>* x, T a, const C<T>* y, T b, SpecGeom g, int ncomp) {
+  Mode m;
+  if (!mode_of(gid(), g, m)) return;
+  for (int c = 0; c < ncomp; ++c) {
+    long o = c * g.s_comp + m.off;
+    dst[o] = add<T>(scl<T>(x[o], a), scl<T>(y[o], b));
+  }
+}
+
+// --------------------------------------------------------------- adjoint
+// Physical part of the RHS transpose: a_m = sum_j g_mj v_j + sum_i g_im v_i
+// with g the nonsym physical gradient of the cotangent (index i + D*j holds
+// d w_i / d x_j) and v the truncated velocity.  a overwrites v.
+template <typename T, int D>
+__global__ void k_adj_products(T* v, const T* gr, long np) {
+  long id = gid();
+  if (id >= np) return;
+  T vv[D], a[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) vv[c] = v[c * np + id];
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += gr[(m + D * j) * np + id] * vv[j];
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += gr[(i + D * m) * np + id] * vv[i];
+    a[m] = s;
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) v[c * np + id] = a[c];
+}
+
+// out += -nu (|k|^2 w + k (k.w)) + A B w - coefB B u, the viscous part of
+// the transpose (self-adjoint) and the band-forcing part; bandp = {A, coefB}
+// or null when no forcing.
+template <typename T, int D>
+__global__ void k_adj_visc_force(C<T>* out, const C<T>* w, const C<T>* u, double nu, SpecGeom g,
+                                 const double* bandp, int kband2) {
+  Mode m;
+  if (!mode_of(gid(), g, m)) return;
+  double k[3] = {g.kf * m.kx, g.kf * m.ky, g.kf * m.kz};
+  double k2 = 0, kwr = 0, kwi = 0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    C<T> ww = w[c * g.s_comp + m.off];
+    k2 += k[c] * k[c];
+    kwr += k[c] * double(ww.x);
+    kwi += k[c] * double(ww.y);
+  }
+  const int k2i = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+  const bool inband = bandp != nullptr && k2i > 0 && k2i <= kband2;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const long o = c * g.s_comp + m.off;
+    C<T> ww = w[o];
+    double rx = -nu * (k2 * double(ww.x) + k[c] * kwr);
+    double ry = -nu * (k2 * double(ww.y) + k[c] * kwi);
+    if (inband) {
+      C<T> uu = u[o];
+      rx += bandp[0] * double(ww.x) - bandp[1] * double(uu.x);
+      ry += bandp[0] * double(ww.y) - bandp[1] * double(uu.y);
+    }
+    C<T> r = out[o];
+    out[o] = mk<T>(T(double(r.x) + rx), T(double(r.y) + ry));
+  }
+}
+
+// Band sums for the forcing transpose: S = sum_band w|u|^2, c = sum_band w Re(conj(u) w)
+// -> bandp = {A, 2 A c / S} with A = injection / S
+template <typename T>
+__global__ void k_band_adj(const C<T>* u, const C<T>* wv, int ncomp, SpecGeom g, int kband, double injection,
+                           double* bandp) {
+  __shared__ double s1[256], s2[256];
+  double a = 0.0, b = 0.0;
+  const int kb2 = kband * kband;
+  const int w1 = 2 * kband + 1;
+  const int nz = g.D == 3 ? w1 : 1;
+  const int total = (kband + 1) * w1 * nz;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int kx = i % (kband + 1);
+    int ky = (i / (kband + 1)) % w1 - kband;
+    int kz = g.D == 3 ? (i / ((kband + 1) * w1)) - kband : 0;
+    int k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 == 0 || k2 > kb2) continue;
+    int y = ky < 0 ? ky + g.n : ky;
+    int z = kz < 0 ? kz + g.n : kz;
+    long off = kx + long(g.nhp) * (y + long(g.n) * z);
+    double wt = (kx >= 1 && kx <= g.nh - 2) ? 2.0 : 1.0;
+    for (int c = 0; c < ncomp; ++c) {
+      C<T> uu = u[c * g.s_comp + off], ww = wv[c * g.s_comp + off];
+      a += wt * (double(uu.x) * uu.x + double(uu.y) * uu.y);
+      b += wt * (double(uu.x) * ww.x + double(uu.y) * ww.y);
+    }
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double A = s1[0] > 0 ? injection / s1[0] : 0.0;
+    bandp[0] = A;
+    bandp[1] = s1[0] > 0 ? 2 * A * s2[0] / s1[0] : 0.0;
+  }
+}
+
 // ------------------------------------------------------------- reductions
 // Block-partial FP64 sums of an RFFT-weighted mode function; the final sum
 // over blocks is done by k_finish_sum (fixed order: deterministic).

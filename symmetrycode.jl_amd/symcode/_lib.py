@@ -72,6 +72,9 @@ SIGNATURES = {
     "sc_closure_dynsmag": [P, P, P, c_double, DP],
     "sc_les_rhs": [P, P, P, c_double, c_int, c_double, c_double],
     "sc_les_wray3": [P, P, c_double, c_double, c_int, c_double, c_double],
+    "sc_rhs_vjp": [P, P, P, P, c_double],
+    "sc_rk4_vjp": [P, P, P, P, c_double, c_double],
+    "sc_wray3_vjp": [P, P, P, P, c_double, c_double],
 }
 
 

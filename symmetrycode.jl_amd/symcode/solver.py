@@ -307,3 +307,25 @@ def taylorgreen(g: Grid, V0=1.0, doproject=True):
     u = g.vectorfield()
     _c(lib().sc_taylorgreen(g.ctx, _p(u), V0, 1 if doproject else 0), g)
     return u
+
+
+# ------------------------------------------------------------------ adjoint
+def rhs_vjp(out, u, w, g: Grid, visc):
+    """out = J(u)^T w for convectiondiffusion! (plus band forcing when set),
+    under the RFFT-weighted inner product of spectraldot."""
+    _c(lib().sc_rhs_vjp(g.ctx, _p(out), _p(u), _p(w), visc), g)
+    return out
+
+
+def rk4_vjp(u0, ubar, dt, g: Grid, visc):
+    """(d rk4 step / d u0)^T ubar for one step of fixed dt; returns a new field."""
+    out = g.vectorfield()
+    _c(lib().sc_rk4_vjp(g.ctx, _p(out), _p(u0), _p(ubar), dt, visc), g)
+    return out
+
+
+def wray3_vjp(u0, ubar, dt, g: Grid, visc):
+    """(d wray3! step / d u0)^T ubar for one step of fixed dt."""
+    out = g.vectorfield()
+    _c(lib().sc_wray3_vjp(g.ctx, _p(out), _p(u0), _p(ubar), dt, visc), g)
+    return out

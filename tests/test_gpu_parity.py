@@ -336,3 +336,82 @@ def test_les_rollout(kind):
     assert len(snaps) == len(osnaps)
     for a, b in zip(snaps, osnaps):
         assert rel(a, b) < 1e-10
+
+
+# ------------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("D,n,forced", [(3, 16, False), (3, 16, True), (3, 30, False), (2, 64, False)])
+def test_rhs_vjp_parity(D, n, forced):
+    import symcode as sc
+    from oracle import adjoint as OA
+
+    print_case("rhs_vjp", D, n, forced)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 16, trunc=True)
+    w = oracle_solenoidal(go, 17)
+    nu = 0.02
+    forcing = (0.1, 2) if forced else None
+    if forced:
+        sc.set_band_forcing(gg, 0.1, 2)
+    out = sc.rhs_vjp(gg.vectorfield(), gg.upload_spectral(u), gg.upload_spectral(w), gg, nu)
+    assert rel(gg.download_spectral(out), OA.rhs_vjp(u, w, go, nu, forcing)) < TOL64
+    sc.set_band_forcing(gg, 0.0)
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "wray3"])
+def test_step_vjp_parity_and_dot_product(scheme):
+    """GPU step adjoint vs the oracle adjoint, and a dot-product test against
+    central differences of the GPU step itself."""
+    import symcode as sc
+    from oracle import adjoint as OA
+
+    print_case("step_vjp", scheme)
+    D, n = 3, 16
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 18, trunc=True)
+    ub = oracle_solenoidal(go, 19)
+    du = oracle_solenoidal(go, 20, trunc=True)
+    nu, dt = 0.02, 0.05
+    vjp = sc.rk4_vjp if scheme == "rk4" else sc.wray3_vjp
+    ovjp = OA.rk4_vjp if scheme == "rk4" else OA.wray3_vjp
+    step = sc.rk4 if scheme == "rk4" else sc.wray3
+    got = gg.download_spectral(vjp(gg.upload_spectral(u), gg.upload_spectral(ub), dt, gg, nu))
+    assert rel(got, ovjp(u, ub, dt, go, nu)) < 1e-11
+    eps = 1e-4
+    a = gg.upload_spectral(u + eps * du)
+    b = gg.upload_spectral(u - eps * du)
+    step(a, dt, gg, nu)
+    step(b, dt, gg, nu)
+    Jdu = (gg.download_spectral(a) - gg.download_spectral(b)) / (2 * eps)
+    lhs = sum(S.spectraldot(ub[i], Jdu[i], go) for i in range(D)).real
+    rhs_ = sum(S.spectraldot(got[i], du[i], go) for i in range(D)).real
+    assert lhs == pytest.approx(rhs_, rel=1e-7)
+
+
+def test_autograd_step_matches_finite_differences():
+    """torch.autograd through symcode.autograd.rk4_step (two steps) against
+    central differences along 4 random directions."""
+    import symcode as sc
+
+    print_case("autograd")
+    D, n = 3, 16
+    go, gg = grids(D, n)
+    u = gg.upload_spectral(oracle_solenoidal(go, 21, trunc=True))
+    target = gg.upload_spectral(oracle_solenoidal(go, 22, trunc=True))
+    nu, dt = 0.02, 0.04
+
+    def loss(v):
+        for _ in range(2):
+            v = sc.autograd.rk4_step(v, dt, gg, nu)
+        return torch.sum(torch.abs(v - target) ** 2)
+
+    x = u.clone().requires_grad_(True)
+    L = loss(x)
+    (grad,) = torch.autograd.grad(L, x)
+    rng = np.random.default_rng(0)
+    for _ in range(4):
+        d = gg.upload_spectral(oracle_solenoidal(go, int(rng.integers(1000)), trunc=True))
+        eps = 1e-4
+        with torch.no_grad():
+            fd = (loss(u + eps * d) - loss(u - eps * d)) / (2 * eps)
+        an = torch.sum((torch.conj(grad) * d).real)
+        assert float(an) == pytest.approx(float(fd), rel=1e-6)
