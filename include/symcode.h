@@ -158,6 +158,40 @@ int sc_rhs_vjp(sc_ctx* ctx, void* out, const void* u, const void* w, double visc
 int sc_rk4_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc);
 int sc_wray3_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc);
 
+/* ------------------------------------------------------------------------
+ * Learned closure: 3x3x3 circular-padding CNN (cin -> 64 -> 64 -> 64 -> cout,
+ * tanh-GELU after the hidden layers, no bias on the last), BF16 MFMA with FP32
+ * accumulation and FP32 master weights, trained with Adam.  No reference
+ * counterpart: the reference's learned closures are pointwise (1x1x1) nets
+ * (PointwiseConv / mlp, src/nets.jl:37-81,247-298), which this net reduces to
+ * when every non-centre tap is zero.  n in {64, 128}; activations are
+ * channels-last bf16 [batch][n^3][cin_p] (cin_p = cout_p = 32).
+ */
+typedef struct sc_cnn sc_cnn;
+int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t seed, sc_cnn** out);
+int sc_cnn_destroy(sc_cnn* net);
+const char* sc_cnn_last_error(const sc_cnn* net);
+/* info[0..7] = n, batch_max, cin, width, cout, cin_p, cout_p, nparams */
+int sc_cnn_info(const sc_cnn* net, int64_t* info);
+int sc_cnn_set_stream(sc_cnn* net, void* hip_stream);
+/* FP32 master parameters, host copies: conv weights [27][cout_p][cin_p] per
+ * layer (tap t = (dz+1)*9 + (dy+1)*3 + (dx+1)), then the 3 hidden biases */
+int sc_cnn_get_params(sc_cnn* net, float* host);
+int sc_cnn_set_params(sc_cnn* net, const float* host);
+int sc_cnn_get_grads(sc_cnn* net, float* host);
+int sc_cnn_zero_grad(sc_cnn* net);
+/* ncomp physical fields (component stride cstride, batch stride bstride
+ * elements, fp64 != 0 for double) -> bf16 input; channel ncomp = extra when
+ * has_extra (the constant Re_Delta feature) */
+int sc_cnn_pack_input(sc_cnn* net, const void* fields, int fp64, int ncomp, long cstride, long bstride, int batch,
+                      double scale, int has_extra, double extra, void* x);
+int sc_cnn_forward(sc_cnn* net, const void* x, void* y, int batch);
+/* MSE on the first cout channels; dy = d loss / d y as bf16 */
+int sc_cnn_mse(sc_cnn* net, const void* y, const void* target, void* dy, int batch, double* loss);
+/* accumulate parameter gradients of the last forward; dx (nullable) = d loss / d x (fp32) */
+int sc_cnn_backward(sc_cnn* net, const void* x, const void* dy, void* dx, int batch);
+int sc_cnn_adam(sc_cnn* net, double lr, double beta1, double beta2, double eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "symcode", "libsymcode_hip.so")
-SOURCES = ["api.hip", "lib_f64.hip", "lib_f32.hip"]
+SOURCES = ["api.hip", "lib_f64.hip", "lib_f32.hip", "cnn.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are disabled for all device code

@@ -6,7 +6,7 @@ importing raises if the library has not been built.
 """
 from ._lib import LIBPATH, SymcodeError, lib  # noqa: F401
 from .grid import Grid  # noqa: F401
-from . import solver, filtering, closures, autograd  # noqa: F401
+from . import solver, filtering, closures, autograd, cnn  # noqa: F401
 from .solver import *  # noqa: F401,F403
 from .filtering import cutoff, gaussianfilter, sfs, inverse_cutoff  # noqa: F401
 from .closures import (create_clark, create_smagorinsky, create_dynamic_smagorinsky,  # noqa: F401

@@ -75,6 +75,19 @@ SIGNATURES = {
     "sc_rhs_vjp": [P, P, P, P, c_double],
     "sc_rk4_vjp": [P, P, P, P, c_double, c_double],
     "sc_wray3_vjp": [P, P, P, P, c_double, c_double],
+    "sc_cnn_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.POINTER(P)],
+    "sc_cnn_destroy": [P],
+    "sc_cnn_info": [P, ctypes.POINTER(c_int64)],
+    "sc_cnn_set_stream": [P, P],
+    "sc_cnn_get_params": [P, P],
+    "sc_cnn_set_params": [P, P],
+    "sc_cnn_get_grads": [P, P],
+    "sc_cnn_zero_grad": [P],
+    "sc_cnn_pack_input": [P, P, c_int, c_int, ctypes.c_long, ctypes.c_long, c_int, c_double, c_int, c_double, P],
+    "sc_cnn_forward": [P, P, P, c_int],
+    "sc_cnn_mse": [P, P, P, P, c_int, DP],
+    "sc_cnn_backward": [P, P, P, P, c_int],
+    "sc_cnn_adam": [P, c_double, c_double, c_double, c_double],
 }
 
 
@@ -97,8 +110,16 @@ def lib():
             f.restype = c_int
         L.sc_last_error.argtypes = [P]
         L.sc_last_error.restype = ctypes.c_char_p
+        L.sc_cnn_last_error.argtypes = [P]
+        L.sc_cnn_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
+
+
+def check_cnn(rc, net):
+    if rc != 0:
+        msg = lib().sc_cnn_last_error(net)
+        raise SymcodeError(f"symcode cnn error {rc}: {msg.decode() if msg else ''}")
 
 
 def check(rc, ctx=None):

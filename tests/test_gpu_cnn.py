@@ -1,0 +1,99 @@
+"""GPU parity of the BF16-MFMA CNN closure against the torch-CPU oracle that
+rounds at the same BF16 storage points.  Tolerances: forward 1e-2 and
+gradients 3e-2 relative L2 (BF16 storage of activations and data gradients,
+FP32 accumulation in a different order)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import print_case, rel
+from oracle import cnn as OC
+
+pytestmark = pytest.mark.gpu
+
+
+def make(n=64, batch=1, seed=0):
+    import symcode as sc
+
+    return sc.cnn.ConvClosureNet(n, batch_max=batch, cin=4, width=64, cout=6, seed=seed)
+
+
+def rand_input(net, batch, seed):
+    r = np.random.default_rng(seed)
+    x = np.zeros((batch, net.nvox, net.cin_p), np.float32)
+    x[..., :4] = r.standard_normal((batch, net.nvox, 4))
+    return x
+
+
+def test_cnn_forward_backward_parity():
+    print_case("cnn parity")
+    net = make()
+    n, B = net.n, 1
+    h = net.get_params()
+    Ws, bs = net.unflatten(h)
+    bs = [np.random.default_rng(9 + i).standard_normal(64).astype(np.float32) * 0.05 for i in range(3)]
+    h[-192:] = np.concatenate(bs)
+    net.set_params(h)
+    x = rand_input(net, B, 1)
+    xt = torch.tensor(x).to(torch.bfloat16).cuda()
+    y = net.forward(xt)
+    yo, store = OC.forward(Ws, bs, torch.tensor(x), n)
+    assert rel(y.cpu().numpy(), yo.numpy()) < 1e-2
+    tgt = np.random.default_rng(2).standard_normal((B, net.nvox, 6)).astype(np.float32) * 0.1
+    loss, dy = net.mse(y, torch.tensor(tgt).cuda())
+    lo, dyo = OC.mse(yo, tgt, 6)
+    assert loss == pytest.approx(lo, rel=1e-2)
+    net.zero_grad()
+    dx = net.backward(xt, dy, want_dx=True)
+    gW, gb, dxo = OC.backward(store, dyo, n)
+    g = net.get_grads()
+    GW, GB = net.unflatten(g)
+    for l in range(4):
+        assert rel(GW[l], gW[l].numpy()) < 3e-2, l
+    for l in range(3):
+        assert rel(GB[l], gb[l].numpy()) < 3e-2, l
+    assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
+
+
+def test_cnn_pointwise_special_case():
+    """only the centre tap non-zero: the network is the reference's pointwise
+    MLP (pointwise_apply + GELU, src/nets.jl:63-81)"""
+    print_case("cnn pointwise")
+    net = make()
+    h = net.get_params()
+    Ws, bs = net.unflatten(h)
+    for W in Ws:
+        W[np.arange(27) != 13] = 0.0
+    net.set_params(h)
+    x = rand_input(net, 1, 3)
+    xb = torch.tensor(x).to(torch.bfloat16)
+    y = net.forward(xb.cuda()).cpu().numpy()
+    Wr = [torch.tensor(W).to(torch.bfloat16).float().numpy() for W in Ws]
+    ref = OC.pointwise_mlp(Wr, bs, xb.float().numpy())
+    assert rel(y, ref) < 1e-2
+
+
+def test_cnn_training_reduces_loss():
+    print_case("cnn train")
+    net = make(seed=4)
+    x = torch.tensor(rand_input(net, 1, 5)).to(torch.bfloat16).cuda()
+    # a smooth target: a fixed random conv net's output
+    teacher = make(seed=77)
+    tgt = teacher.forward(x)[..., :6].contiguous()
+    losses = [net.train_step(x, tgt, lr=1e-3) for _ in range(30)]
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+
+
+def test_cnn_pack_input():
+    import symcode as sc
+
+    print_case("cnn pack")
+    net = make()
+    n = net.n
+    f = torch.randn(3, n, n, n, dtype=torch.float64, device="cuda")
+    x = net.pack_input(f, scale=2.0, extra=0.75)
+    xc = x.float().cpu().numpy()[0]
+    ref = (2.0 * f).reshape(3, -1).T.cpu().numpy()
+    assert np.allclose(xc[:, :3], torch.tensor(ref).to(torch.bfloat16).float().numpy())
+    assert np.all(xc[:, 3] == np.float32(torch.tensor(0.75).to(torch.bfloat16).float()))
+    assert np.all(xc[:, 4:] == 0)
