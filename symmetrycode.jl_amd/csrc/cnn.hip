@@ -332,6 +332,7 @@ struct sc_cnn {
   int n = 0, bmax = 0, cin = 0, width = 0, cout = 0;
   int cin_p = 32, cout_p = 32;
   hipStream_t stream = nullptr;
+  bool own_stream = false;  // created here (destroyed by sc_cnn_destroy) or the caller's
   std::string err;
   // master weights / grads / Adam state, one flat buffer each; offsets per layer
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
@@ -349,6 +350,7 @@ struct sc_cnn {
 
 namespace {
 using namespace cnn;
+thread_local std::string g_cnn_create_err;
 
 int layer_cin(const sc_cnn* c, int l) { return l == 0 ? c->cin_p : c->width; }
 int layer_cout(const sc_cnn* c, int l) { return l == 3 ? c->cout_p : c->width; }
@@ -358,6 +360,7 @@ int real_cout(const sc_cnn* c, int l) { return l == 3 ? c->cout : c->width; }
 template <class F>
 int cguard(sc_cnn* c, F&& f) {
   if (!c) return 1;
+  (void)hipGetLastError();  // drop an error state left by unrelated HIP calls of the caller
   try {
     f();
     CNN_HIP(hipStreamSynchronize(c->stream));
@@ -424,10 +427,13 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
   *out = nullptr;
   if (n % VPB != 0 || n > 128 || batch_max < 1 || cin < 1 || cin > 32 || width != 64 || cout < 1 || cout > 32)
     return 1;
+  g_cnn_create_err.clear();
+  (void)hipGetLastError();
   sc_cnn* c = new sc_cnn();
   c->n = n; c->bmax = batch_max; c->cin = cin; c->width = width; c->cout = cout;
   try {
     CNN_HIP(hipStreamCreate(&c->stream));
+    c->own_stream = true;
     long off = 0;
     for (int l = 0; l < 4; ++l) {
       c->woff[l] = off;
@@ -483,7 +489,8 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
     pack_all(c);
     CNN_HIP(hipStreamSynchronize(c->stream));
   } catch (std::exception& e) {
-    delete c;
+    g_cnn_create_err = e.what();
+    sc_cnn_destroy(c);
     return 3;
   }
   *out = c;
@@ -502,12 +509,12 @@ int sc_cnn_destroy(sc_cnn* c) {
     if (c->dz[l]) (void)hipFree(c->dz[l]);
   }
   if (c->hostd) (void)hipHostFree(c->hostd);
-  (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
 
-const char* sc_cnn_last_error(const sc_cnn* c) { return c ? c->err.c_str() : ""; }
+const char* sc_cnn_last_error(const sc_cnn* c) { return c ? c->err.c_str() : g_cnn_create_err.c_str(); }
 
 int sc_cnn_info(const sc_cnn* c, int64_t* info) {
   if (!c || !info) return 1;
@@ -627,8 +634,9 @@ int sc_cnn_pack_input(sc_cnn* c, const void* f, int fp64, int ncomp, long cstrid
 int sc_cnn_set_stream(sc_cnn* c, void* s) {
   return cguard(c, [&] {
     CNN_HIP(hipStreamSynchronize(c->stream));
-    CNN_HIP(hipStreamDestroy(c->stream));
+    if (c->own_stream) CNN_HIP(hipStreamDestroy(c->stream));
     c->stream = (hipStream_t)s;
+    c->own_stream = false;
   });
 }
 

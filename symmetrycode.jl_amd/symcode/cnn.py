@@ -24,7 +24,9 @@ class ConvClosureNet:
         net = ctypes.c_void_p()
         rc = lib().sc_cnn_create(n, batch_max, cin, width, cout, seed, ctypes.byref(net))
         if rc != 0:
-            raise ValueError(f"sc_cnn_create failed ({rc}): n must be 64 or 128, width 64, cin/cout <= 32")
+            if rc == 1:
+                raise ValueError("sc_cnn_create: n must be a multiple of 64 and <= 128, width 64, cin/cout <= 32")
+            raise RuntimeError("sc_cnn_create failed: " + lib().sc_cnn_last_error(None).decode())
         self.net = net
         info = (ctypes.c_int64 * 8)()
         check_cnn(lib().sc_cnn_info(net, info), net)
