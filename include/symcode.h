@@ -145,9 +145,15 @@ int sc_closure_smagorinsky(sc_ctx* ctx, void* tau, const void* GG, double cs, do
 int sc_closure_dynsmag(sc_ctx* ctx, void* tau, const void* u, double delta, double* coeff);
 /* les!(du, u, grid, cache; model, visc) (src/les.jl:4-33) with a built-in
  * closure: 0 none, 1 Clark(p1 = Delta), 2 Smagorinsky(p1 = C_S, p2 = Delta),
- * 3 dynamic Smagorinsky(p1 = Delta) */
+ * 3 dynamic Smagorinsky(p1 = Delta), 4 the learned closure of
+ * sc_set_cnn_closure; the band forcing is added when enabled */
 int sc_les_rhs(sc_ctx* ctx, void* du, const void* u, double visc, int closure, double p1, double p2);
 int sc_les_wray3(sc_ctx* ctx, void* u, double dt, double visc, int closure, double p1, double p2);
+/* solve_les! with RK4 in place of wray3 (src/les.jl:188-247; RK4 has no
+ * reference counterpart), fixed dt or CFL-limited */
+int sc_les_rk4(sc_ctx* ctx, void* u, double dt, double visc, int closure, double p1, double p2);
+int sc_les_rk4_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, int closure, double p1, double p2,
+                   double* dt);
 
 /* adjoint (no reference counterpart: the reference trains a-priori only).
  * Inner product: the RFFT-weighted one of spectraldot (src/solver.jl:708).
@@ -157,6 +163,14 @@ int sc_les_wray3(sc_ctx* ctx, void* u, double dt, double visc, int closure, doub
 int sc_rhs_vjp(sc_ctx* ctx, void* out, const void* u, const void* w, double visc);
 int sc_rk4_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc);
 int sc_wray3_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc);
+/* the same for les! with closure 0 (none) or 4 (the attached learned closure);
+ * with closure 4 the network is re-run on each stage value and its parameter
+ * gradients accumulate (sc_cnn_zero_grad clears them) */
+int sc_les_rhs_vjp(sc_ctx* ctx, void* out, const void* u, const void* w, double visc, int closure);
+int sc_les_rk4_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc, int closure);
+int sc_les_wray3_vjp(sc_ctx* ctx, void* out, const void* u0, const void* ubar, double dt, double visc, int closure);
+/* dst = a x + b y over ncomp spectral components (dst may alias x or y) */
+int sc_lincomb(sc_ctx* ctx, void* dst, const void* x, double a, const void* y, double b, int ncomp);
 
 /* ------------------------------------------------------------------------
  * Learned closure: 3x3x3 circular-padding CNN (cin -> 64 -> 64 -> 64 -> cout,
@@ -191,6 +205,15 @@ int sc_cnn_mse(sc_cnn* net, const void* y, const void* target, void* dy, int bat
 /* accumulate parameter gradients of the last forward; dx (nullable) = d loss / d x (fp32) */
 int sc_cnn_backward(sc_cnn* net, const void* x, const void* dy, void* dx, int batch);
 int sc_cnn_adam(sc_cnn* net, double lr, double beta1, double beta2, double eps);
+/* attach net as closure kind 4 of sc_les_*: tau = s_out * net(C2R(2/3 u) / s_in,
+ * extra) on the channels (xx, yy, zz, xy, yz, zx); the net must have the
+ * context's n, cin 3 or 4 (channel 3 = extra), cout 6, and outlive the
+ * attachment.  net = NULL detaches.  (The reference's closure hook is the
+ * `model(u, grad)` of les!, src/les.jl:4-33, built by fullchain,
+ * src/nets.jl:311-334.) */
+int sc_set_cnn_closure(sc_ctx* ctx, sc_cnn* net, double s_in, double s_out, double extra);
+/* the attached learned closure's tau (packed, physical) for the spectral u */
+int sc_closure_cnn(sc_ctx* ctx, void* tau, const void* u);
 
 #ifdef __cplusplus
 }

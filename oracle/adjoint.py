@@ -54,9 +54,11 @@ def rhs_vjp(u, w, g: S.Grid, visc, forcing=None):
     return out.astype(u.dtype)
 
 
-def rk4_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None):
-    """d(rk4 step)/du0 transposed, applied to ubar (stages recomputed)."""
-    f = lambda v: S.convectiondiffusion(v, g, visc, forcing=forcing)
+def rk4_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None, rhs=None, vjp=None):
+    """d(rk4 step)/du0 transposed, applied to ubar (stages recomputed).
+    ``rhs(v)`` / ``vjp(v, w)`` replace the DNS right-hand side and its
+    transpose (the LES with a learned closure, oracle/les_cnn.py)."""
+    f = rhs or (lambda v: S.convectiondiffusion(v, g, visc, forcing=forcing))
     P = lambda v: S.project(v, g)
     k1 = f(u0)
     u1 = P(u0 + (dt / 2) * k1)
@@ -64,7 +66,7 @@ def rk4_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None):
     u2 = P(u0 + (dt / 2) * k2)
     k3 = f(u2)
     u3 = P(u0 + dt * k3)
-    J = lambda v, ww: rhs_vjp(v, ww, g, visc, forcing)
+    J = vjp or (lambda v, ww: rhs_vjp(v, ww, g, visc, forcing))
     s = P(ubar)
     t3 = P(J(u3, (dt / 6) * s))
     t2 = P(J(u2, (dt / 3) * s + dt * t3))
@@ -72,7 +74,7 @@ def rk4_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None):
     return s + t3 + t2 + t1 + J(u0, (dt / 6) * s + (dt / 2) * t1)
 
 
-def wray3_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None):
+def wray3_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None, rhs=None, vjp=None):
     """d(wray3 step)/du0 transposed, applied to ubar.
 
     Forward (src/solver.jl:968-1012): k1 = f(u0); u1 = P(u0 + a1 h k1);
@@ -80,14 +82,14 @@ def wray3_vjp(u0, ubar, dt, g: S.Grid, visc, forcing=None):
     k3 = f(u2); u3 = P(s2 + a3 h k3)."""
     a1, a2, a3 = S.WRAY3_A
     b1, _ = S.WRAY3_B
-    f = lambda v: S.convectiondiffusion(v, g, visc, forcing=forcing)
+    f = rhs or (lambda v: S.convectiondiffusion(v, g, visc, forcing=forcing))
     P = lambda v: S.project(v, g)
     k1 = f(u0)
     u1 = P(u0 + a1 * dt * k1)
     s1 = u0 + b1 * dt * k1
     k2 = f(u1)
     u2 = P(s1 + a2 * dt * k2)
-    J = lambda v, ww: rhs_vjp(v, ww, g, visc, forcing)
+    J = vjp or (lambda v, ww: rhs_vjp(v, ww, g, visc, forcing))
     # u3 = P(s1 + a3 h k3), k3 = f(u2)
     q3 = P(ubar)                      # cotangent of the pre-projection sum
     s1bar = q3

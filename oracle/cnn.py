@@ -23,7 +23,7 @@ C_GELU = 0.7978845608028654
 
 
 def bf(t, on=True):
-    return t.to(torch.bfloat16).to(torch.float32) if on else t
+    return t.to(torch.bfloat16).to(t.dtype) if on else t
 
 
 def gelu(z):
@@ -38,7 +38,9 @@ def gelu_grad(z):
 
 def torch_weight(W):
     """(27, cout, cin) -> conv3d weight (cout, cin, 3, 3, 3)"""
-    W = torch.as_tensor(np.asarray(W, np.float32))
+    W = torch.as_tensor(np.asarray(W))
+    if W.dtype not in (torch.float32, torch.float64):
+        W = W.to(torch.float32)
     co, ci = W.shape[1], W.shape[2]
     return W.reshape(3, 3, 3, co, ci).permute(3, 4, 0, 1, 2).contiguous()
 
@@ -75,14 +77,15 @@ def split_params(h, shapes, width):
     return Ws, bs
 
 
-def forward(Ws, bs, x, n, emulate=True):
+def forward(Ws, bs, x, n, emulate=True, dtype=torch.float32):
     """x: (B, n^3, cin_p) float tensor.  Returns y (B, n^3, cout_p) and the
-    stored tensors needed by backward."""
-    Wt = [bf(torch_weight(W), emulate) for W in Ws]
-    a = to_ncdhw(bf(torch.as_tensor(x, dtype=torch.float32), emulate), n)
+    stored tensors needed by backward.  ``dtype=torch.float64`` (with
+    emulate=False) gives the smooth FP64 network for finite differences."""
+    Wt = [bf(torch_weight(W).to(dtype), emulate) for W in Ws]
+    a = to_ncdhw(bf(torch.as_tensor(x, dtype=dtype), emulate), n)
     acts, zs = [a], []
     for l in range(3):
-        z = conv(a, Wt[l]) + torch.as_tensor(np.asarray(bs[l], np.float32)).view(1, -1, 1, 1, 1)
+        z = conv(a, Wt[l]) + torch.as_tensor(np.asarray(bs[l]), dtype=dtype).view(1, -1, 1, 1, 1)
         zs.append(bf(z, emulate))
         a = bf(gelu(z), emulate)
         acts.append(a)
@@ -94,7 +97,7 @@ def backward(store, dy, n, emulate=True):
     """dy: (B, n^3, cout_p).  Returns (weight grads [(27,cout,cin)], bias
     grads, dx (B, n^3, cin_p))."""
     Wt, acts, zs = store["Wt"], store["acts"], store["zs"]
-    d = to_ncdhw(bf(torch.as_tensor(dy, dtype=torch.float32), emulate), n)
+    d = to_ncdhw(bf(torch.as_tensor(dy, dtype=Wt[0].dtype), emulate), n)
     gW, gb = [None] * 4, [None] * 3
     gW[3] = conv_weight_grad(acts[3], d, Wt[3].shape)
     d = bf(conv_data_grad(d, Wt[3]) * gelu_grad(zs[2]), emulate)

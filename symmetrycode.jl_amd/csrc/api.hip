@@ -4,6 +4,7 @@
 #include <string>
 #include "../../include/symcode.h"
 #include "ops.hpp"
+#include "cnn_link.hpp"
 
 struct sc_ctx : sc::Ctx {};
 
@@ -291,9 +292,27 @@ int sc_les_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, in
   return guard(c, [&] {
     need_fft(c);
     need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
-    need(closure >= 0 && closure <= 3, "closure kind");
+    need(closure >= 0 && closure <= 4, "closure kind");
     double d = 0;
     TYPED(c, d = sc::Ops<T>::wray3_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
+    if (dt) *dt = d;
+  });
+}
+int sc_les_rk4(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(closure >= 0 && closure <= 4, "closure kind");
+    TYPED(c, sc::Ops<T>::rk4_dev(*c, (CT*)u, dt, nu, 0.0, closure, p1, p2));
+  });
+}
+int sc_les_rk4_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, int closure, double p1, double p2,
+                   double* dt) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
+    need(closure >= 0 && closure <= 4, "closure kind");
+    double d = 0;
+    TYPED(c, d = sc::Ops<T>::rk4_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
     if (dt) *dt = d;
   });
 }
@@ -399,14 +418,14 @@ int sc_closure_dynsmag(sc_ctx* c, void* tau, const void* u, double delta, double
 int sc_les_rhs(sc_ctx* c, void* du, const void* u, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 3, "closure kind");
+    need(closure >= 0 && closure <= 4, "closure kind");
     TYPED(c, sc::Ops<T>::les_rhs(*c, (CT*)du, (const CT*)u, nu, closure, p1, p2));
   });
 }
 int sc_les_wray3(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 3, "closure kind");
+    need(closure >= 0 && closure <= 4, "closure kind");
     TYPED(c, sc::Ops<T>::les_wray3(*c, (CT*)u, dt, nu, closure, p1, p2));
   });
 }
@@ -426,6 +445,60 @@ int sc_wray3_vjp(sc_ctx* c, void* out, const void* u0, const void* ubar, double 
     need_fft(c);
     need(out != u0 && out != ubar, "out must not alias u0 or ubar");
     TYPED(c, sc::Ops<T>::wray3_vjp(*c, (CT*)out, (const CT*)u0, (const CT*)ubar, dt, nu));
+  });
+}
+
+int sc_set_cnn_closure(sc_ctx* c, sc_cnn* net, double s_in, double s_out, double extra) {
+  return guard(c, [&] {
+    if (!net) {
+      c->cnn = nullptr;
+      return;
+    }
+    need(c->D == 3, "the learned closure is three-dimensional");
+    need(cnnlink::grid_n(net) == c->n, "network grid size differs from the context's");
+    need(cnnlink::out_channels(net) == 6, "the network must output the 6 packed stress components");
+    need(cnnlink::in_channels(net) == 3 || cnnlink::in_channels(net) == 4,
+         "the network input is the 3 velocity components (+ 1 constant channel)");
+    need(s_in > 0 && s_out != 0, "scales: s_in > 0, s_out != 0");
+    c->cnn = net;
+    c->cnn_sin = s_in;
+    c->cnn_sout = s_out;
+    c->cnn_extra = extra;
+  });
+}
+int sc_closure_cnn(sc_ctx* c, void* tau, const void* u) {
+  return guard(c, [&] {
+    need_fft(c);
+    TYPED(c, sc::Ops<T>::cnn_tau(*c, (T*)tau, (const CT*)u));
+  });
+}
+int sc_les_rhs_vjp(sc_ctx* c, void* out, const void* u, const void* w, double nu, int closure) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(closure == 0 || closure == 4, "the transpose exists for closure kinds 0 and 4");
+    TYPED(c, sc::Ops<T>::rhs_vjp(*c, (CT*)out, (const CT*)u, (const CT*)w, nu, closure));
+  });
+}
+int sc_les_rk4_vjp(sc_ctx* c, void* out, const void* u0, const void* ubar, double dt, double nu, int closure) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(out != u0 && out != ubar, "out must not alias u0 or ubar");
+    need(closure == 0 || closure == 4, "the transpose exists for closure kinds 0 and 4");
+    TYPED(c, sc::Ops<T>::rk4_vjp(*c, (CT*)out, (const CT*)u0, (const CT*)ubar, dt, nu, closure));
+  });
+}
+int sc_les_wray3_vjp(sc_ctx* c, void* out, const void* u0, const void* ubar, double dt, double nu, int closure) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(out != u0 && out != ubar, "out must not alias u0 or ubar");
+    need(closure == 0 || closure == 4, "the transpose exists for closure kinds 0 and 4");
+    TYPED(c, sc::Ops<T>::wray3_vjp(*c, (CT*)out, (const CT*)u0, (const CT*)ubar, dt, nu, closure));
+  });
+}
+int sc_lincomb(sc_ctx* c, void* dst, const void* x, double a, const void* y, double b, int ncomp) {
+  return guard(c, [&] {
+    need(ncomp >= 1 && ncomp <= 9, "ncomp");
+    TYPED(c, sc::Ops<T>::lincomb(*c, (CT*)dst, (const CT*)x, a, (const CT*)y, b, ncomp));
   });
 }
 

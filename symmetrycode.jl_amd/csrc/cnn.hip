@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 #include "../../include/symcode.h"
+#include "cnn_link.hpp"
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -325,6 +326,26 @@ __global__ void k_unpack(const float* y, long nvox, int ncomp, int cp, double sc
   for (int c = 0; c < ncomp; ++c) out[c * stride + v] = T(double(y[v * cp + c]) * scale);
 }
 
+// cotangent of the six packed stress channels (xx, yy, zz, xy, yz, zx) from
+// the nine physical components g (index i + 3 j): a diagonal entry takes g_ii,
+// an off-diagonal entry appears twice in the divergence and takes g_ij + g_ji
+template <typename T>
+__global__ void k_pack_tau_cot(const T* g, long nvox, long stride, double scale, bf16* dy, int cp) {
+  long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  auto G = [&](int i, int j) { return double(g[long(i + 3 * j) * stride + v]); };
+  const double t[6] = {G(0, 0), G(1, 1), G(2, 2), G(0, 1) + G(1, 0), G(1, 2) + G(2, 1), G(2, 0) + G(0, 2)};
+  for (int c = 0; c < cp; ++c) dy[v * cp + c] = bf16(c < 6 ? float(scale * t[c]) : 0.f);
+}
+
+// a_m += scale * dx[v][m] for the three velocity channels
+template <typename T>
+__global__ void k_add_vel_cot(const float* dx, long nvox, int cp, double scale, T* a, long stride) {
+  long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  for (int m = 0; m < 3; ++m) a[m * stride + v] += T(scale * double(dx[v * cp + m]));
+}
+
 }  // namespace cnn
 
 // ------------------------------------------------------------------ host side
@@ -339,8 +360,11 @@ struct sc_cnn {
   long wlen = 0;
   long woff[4] = {}, wsz[4] = {}, boff[3] = {};
   bf16 *wf[4] = {}, *wb[4] = {};
-  // activations for the largest batch: a0 (input), z1..z3, a1..a3, y
-  bf16 *a0 = nullptr, *z[3] = {}, *a[3] = {}, *dz[3] = {}, *dyb = nullptr;
+  // activations for the largest batch: z1..z3, a1..a3 and their gradients
+  bf16 *z[3] = {}, *a[3] = {}, *dz[3] = {};
+  // one-cube input / output / cotangent buffers of the LES closure (allocated
+  // on first use by the solver)
+  bf16 *a0 = nullptr, *dyb = nullptr;
   float *y = nullptr, *dx = nullptr, *part = nullptr;
   long part_len = 0;
   double* lossp = nullptr;
@@ -420,6 +444,106 @@ void pack_all(sc_cnn* c) {
 }
 }  // namespace
 
+// ----------------------------------------------- interface used by the solver
+namespace cnnlink {
+using namespace cnn;
+
+void forward(sc_cnn* c, const void* x, float* y, int batch) {
+  if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
+  const bf16* in = (const bf16*)x;
+  for (int l = 0; l < 3; ++l) {
+    launch_conv(c, layer_cin(c, l), c->width, HIDDEN, in, c->wf[l], c->w + c->boff[l], c->a[l], c->z[l], nullptr,
+                nullptr, batch);
+    in = c->a[l];
+  }
+  launch_conv(c, c->width, c->cout_p, LAST, in, c->wf[3], nullptr, nullptr, nullptr, y, nullptr, batch);
+}
+
+void backward(sc_cnn* c, const void* x, const void* dy, float* dx, int batch) {
+  if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
+  const long nv = long(batch) * c->n * c->n * c->n;
+  const bf16* dY = (const bf16*)dy;
+  // last layer: dW4, then the data gradient into dz[2] with gelu'(z3)
+  launch_wgrad(c, c->width, c->cout_p, c->a[2], dY, c->g + c->woff[3], batch);
+  launch_conv(c, c->cout_p, c->width, BWD_HIDDEN, dY, c->wb[3], nullptr, c->dz[2], nullptr, nullptr, c->z[2], batch);
+  for (int l = 2; l >= 0; --l) {
+    const bf16* inl = l == 0 ? (const bf16*)x : c->a[l - 1];
+    launch_wgrad(c, layer_cin(c, l), c->width, inl, c->dz[l], c->g + c->woff[l], batch);
+    hipLaunchKernelGGL(k_bias_grad<64>, dim3(1024), dim3(256), 0, c->stream, c->dz[l], c->g + c->boff[l], nv);
+    CNN_HIP(hipGetLastError());
+    if (l > 0)
+      launch_conv(c, c->width, c->width, BWD_HIDDEN, c->dz[l], c->wb[l], nullptr, c->dz[l - 1], nullptr, nullptr,
+                  c->z[l - 1], batch);
+    else if (dx)
+      launch_conv(c, c->width, c->cin_p, BWD_INPUT, c->dz[0], c->wb[0], nullptr, nullptr, nullptr, dx, nullptr, batch);
+  }
+}
+
+int grid_n(const sc_cnn* c) { return c->n; }
+int in_channels(const sc_cnn* c) { return c->cin; }
+int out_channels(const sc_cnn* c) { return c->cout; }
+
+Bufs buffers(sc_cnn* c) {
+  const long nv = long(c->n) * c->n * c->n;
+  if (!c->a0) {
+    CNN_HIP(hipMalloc(&c->a0, nv * c->cin_p * 2));
+    CNN_HIP(hipMalloc(&c->y, nv * c->cout_p * 4));
+    CNN_HIP(hipMalloc(&c->dyb, nv * c->cout_p * 2));
+    CNN_HIP(hipMalloc(&c->dx, nv * c->cin_p * 4));
+  }
+  return Bufs{c->a0, c->y, c->dyb, c->dx};
+}
+
+hipStream_t set_stream(sc_cnn* c, hipStream_t s) {
+  hipStream_t old = c->stream;
+  c->stream = s;
+  return old;
+}
+
+static unsigned nb1(long nv) { return unsigned((nv + 255) / 256); }
+
+template <typename T>
+void pack_velocity(sc_cnn* c, const T* v, long stride, double scale, double extra, void* x) {
+  const long nv = long(c->n) * c->n * c->n;
+  const int has_extra = c->cin > 3 ? 1 : 0;
+  hipLaunchKernelGGL(k_pack_input<T>, dim3(nb1(nv)), dim3(256), 0, c->stream, v, nv, 3, scale, has_extra,
+                     float(extra), (bf16*)x, c->cin_p, stride);
+  CNN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void unpack_tau(sc_cnn* c, const float* y, double scale, T* tau, long stride) {
+  const long nv = long(c->n) * c->n * c->n;
+  hipLaunchKernelGGL(k_unpack<T>, dim3(nb1(nv)), dim3(256), 0, c->stream, y, nv, 6, c->cout_p, scale, tau, stride);
+  CNN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void pack_tau_cotangent(sc_cnn* c, const T* g9, long stride, double scale, void* dy) {
+  const long nv = long(c->n) * c->n * c->n;
+  hipLaunchKernelGGL(k_pack_tau_cot<T>, dim3(nb1(nv)), dim3(256), 0, c->stream, g9, nv, stride, scale, (bf16*)dy,
+                     c->cout_p);
+  CNN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void add_velocity_cotangent(sc_cnn* c, const float* dx, double scale, T* a, long stride) {
+  const long nv = long(c->n) * c->n * c->n;
+  hipLaunchKernelGGL(k_add_vel_cot<T>, dim3(nb1(nv)), dim3(256), 0, c->stream, dx, nv, c->cin_p, scale, a, stride);
+  CNN_HIP(hipGetLastError());
+}
+
+template void pack_velocity<float>(sc_cnn*, const float*, long, double, double, void*);
+template void pack_velocity<double>(sc_cnn*, const double*, long, double, double, void*);
+template void unpack_tau<float>(sc_cnn*, const float*, double, float*, long);
+template void unpack_tau<double>(sc_cnn*, const float*, double, double*, long);
+template void pack_tau_cotangent<float>(sc_cnn*, const float*, long, double, void*);
+template void pack_tau_cotangent<double>(sc_cnn*, const double*, long, double, void*);
+template void add_velocity_cotangent<float>(sc_cnn*, const float*, double, float*, long);
+template void add_velocity_cotangent<double>(sc_cnn*, const float*, double, double*, long);
+
+}  // namespace cnnlink
+
 extern "C" {
 
 int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t seed, sc_cnn** out) {
@@ -475,15 +599,11 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
       CNN_HIP(hipMalloc(&c->wb[l], len * 2));
     }
     const long nv = long(batch_max) * n * n * n;
-    CNN_HIP(hipMalloc(&c->a0, nv * c->cin_p * 2));
     for (int l = 0; l < 3; ++l) {
       CNN_HIP(hipMalloc(&c->z[l], nv * width * 2));
       CNN_HIP(hipMalloc(&c->a[l], nv * width * 2));
       CNN_HIP(hipMalloc(&c->dz[l], nv * width * 2));
     }
-    CNN_HIP(hipMalloc(&c->y, nv * c->cout_p * 4));
-    CNN_HIP(hipMalloc(&c->dyb, nv * c->cout_p * 2));
-    CNN_HIP(hipMalloc(&c->dx, nv * c->cin_p * 4));
     CNN_HIP(hipMalloc(&c->lossp, ((nv + 255) / 256) * sizeof(double) + 16));
     CNN_HIP(hipHostMalloc(&c->hostd, 64));
     pack_all(c);
@@ -544,41 +664,13 @@ int sc_cnn_zero_grad(sc_cnn* c) {
 // input: device bf16 [batch][n^3][cin_p] channels-last (see sc_cnn_pack_input)
 // output: device fp32 [batch][n^3][cout_p]
 int sc_cnn_forward(sc_cnn* c, const void* x, void* y, int batch) {
-  return cguard(c, [&] {
-    if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
-    const bf16* in = (const bf16*)x;
-    for (int l = 0; l < 3; ++l) {
-      launch_conv(c, layer_cin(c, l), c->width, HIDDEN, in, c->wf[l], c->w + c->boff[l], c->a[l], c->z[l], nullptr,
-                  nullptr, batch);
-      in = c->a[l];
-    }
-    launch_conv(c, c->width, c->cout_p, LAST, in, c->wf[3], nullptr, nullptr, nullptr, (float*)y, nullptr, batch);
-  });
+  return cguard(c, [&] { cnnlink::forward(c, x, (float*)y, batch); });
 }
 
 // backward of the last sc_cnn_forward: dy device bf16 [batch][n^3][cout_p];
 // accumulates parameter gradients; dx (nullable) receives fp32 [batch][n^3][cin_p]
 int sc_cnn_backward(sc_cnn* c, const void* x, const void* dy, void* dx, int batch) {
-  return cguard(c, [&] {
-    if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
-    const long nv = long(batch) * c->n * c->n * c->n;
-    const bf16* dY = (const bf16*)dy;
-    // layer 3 (last): dW4, then data gradient into dz[2] with gelu'(z3)
-    launch_wgrad(c, c->width, c->cout_p, c->a[2], dY, c->g + c->woff[3], batch);
-    launch_conv(c, c->cout_p, c->width, BWD_HIDDEN, dY, c->wb[3], nullptr, c->dz[2], nullptr, nullptr, c->z[2], batch);
-    for (int l = 2; l >= 0; --l) {
-      const bf16* inl = l == 0 ? (const bf16*)x : c->a[l - 1];
-      launch_wgrad(c, layer_cin(c, l), c->width, inl, c->dz[l], c->g + c->woff[l], batch);
-      hipLaunchKernelGGL(k_bias_grad<64>, dim3(1024), dim3(256), 0, c->stream, c->dz[l], c->g + c->boff[l], nv);
-      CNN_HIP(hipGetLastError());
-      if (l > 0)
-        launch_conv(c, c->width, c->width, BWD_HIDDEN, c->dz[l], c->wb[l], nullptr, c->dz[l - 1], nullptr, nullptr,
-                    c->z[l - 1], batch);
-      else if (dx)
-        launch_conv(c, c->width, c->cin_p, BWD_INPUT, c->dz[0], c->wb[0], nullptr, nullptr, nullptr, (float*)dx, nullptr,
-                    batch);
-    }
-  });
+  return cguard(c, [&] { cnnlink::backward(c, x, dy, (float*)dx, batch); });
 }
 
 // MSE loss of y (fp32 [batch][n^3][cout_p]) against tgt (fp32 [batch][n^3][cout]);

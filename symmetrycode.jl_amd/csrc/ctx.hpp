@@ -6,6 +6,8 @@
 #include <stdexcept>
 #include "solver_kernels.hpp"
 
+struct sc_cnn;  // learned closure (cnn.hip)
+
 namespace sc {
 
 struct Error : std::runtime_error {
@@ -49,6 +51,9 @@ struct Ctx {
   double* bandadj = nullptr;
   double* hbuf = nullptr;  // pinned host scratch
   long hbuf_n = 0;
+  // learned closure of les_rhs (closure kind 4): tau = cnn_sout * net(v / cnn_sin, cnn_extra)
+  ::sc_cnn* cnn = nullptr;
+  double cnn_sin = 1.0, cnn_sout = 1.0, cnn_extra = 0.0;
   // forcing applied inside the RHS: 0 none, 1 linear band
   int force_kind = 0;
   double injection = 0.1;

@@ -34,13 +34,16 @@ struct Ops {
   static void max_velocity_into(Ctx& c, const CT* u);
   static void rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax);
   static double wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure, double p1, double p2);
-  static double rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl);
+  static double rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure = -1, double p1 = 0,
+                        double p2 = 0);
   static double read_dt(Ctx& c);
   // adjoint
   static void lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double b, int ncomp);
-  static void rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu);
-  static void rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu);
-  static void wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu);
+  // closure < 0: the DNS right-hand side; 0 or 4: les_rhs with that closure
+  static void rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int closure = -1);
+  static void rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu, int closure = -1);
+  static void wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu, int closure = -1);
+  static void stage_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure);
   // reductions (return to host)
   static double sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu);
   static void dot_modes(Ctx& c, const CT* u, const CT* v, int ncomp, double* re, double* im);
@@ -59,6 +62,8 @@ struct Ops {
   static void gradient_phys(Ctx& c, T* GG, const CT* u);
   static double dynsmag(Ctx& c, T* tau, const CT* u, double delta);
   static void les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2);
+  static void cnn_tau(Ctx& c, T* tau, const CT* u);
+  static const double* band_coeff(Ctx& c, const CT* u);
   static void phys_tracefree(Ctx& c, T* tau);
   static void les_wray3(Ctx& c, CT* u, double dt, double nu, int closure, double p1, double p2);
   static void launch_phys(Ctx& c, T* y, const T* x, T a, T b, long n);

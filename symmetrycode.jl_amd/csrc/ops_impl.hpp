@@ -6,6 +6,7 @@
 #include "ops.hpp"
 #include "fft_passes.hpp"
 #include "rhs_passes.hpp"
+#include "cnn_link.hpp"
 
 namespace sc {
 
@@ -386,16 +387,20 @@ template <typename T>
 void Ops<T>::rhs(Ctx& c, CT* du, const CT* u, double nu) {
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
   nonlinearity(c, B, u, nullptr);
-  const double* fA = nullptr;
-  if (c.force_kind == 1) {
-    if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
-    hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, u, c.D, c.geom(), c.kband, c.injection,
-                       c.forceA);
-    SC_HIP(hipGetLastError());
-    fA = c.forceA;
-  }
+  const double* fA = band_coeff(c, u);
   if (c.D == 3) SPEC_LAUNCH((k_div_visc<T, 3>), du, B, u, nu, g_, fA, c.kband * c.kband);
   else SPEC_LAUNCH((k_div_visc<T, 2>), du, B, u, nu, g_, fA, c.kband * c.kband);
+}
+
+// device coefficient A of the linear band forcing (null when forcing is off)
+template <typename T>
+const double* Ops<T>::band_coeff(Ctx& c, const CT* u) {
+  if (c.force_kind != 1) return nullptr;
+  if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
+  hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, u, c.D, c.geom(), c.kband, c.injection,
+                     c.forceA);
+  SC_HIP(hipGetLastError());
+  return c.forceA;
 }
 
 // Right-hand side of the time steppers: the five-pass pipeline with the
@@ -484,7 +489,7 @@ double Ops<T>::wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int le
 
 // classic RK4 with a projection of every stage value
 template <typename T>
-double Ops<T>::rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl) {
+double Ops<T>::rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure, double p1, double p2) {
   CT* ust = reinterpret_cast<CT*>(c.ensure_spec(c.ust, c.D));
   CT* acc = reinterpret_cast<CT*>(c.ensure_spec(c.acc, c.D));
   CT* du = reinterpret_cast<CT*>(c.ensure_spec(c.du, c.D));
@@ -493,7 +498,12 @@ double Ops<T>::rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl) {
   if (cfl > 0) zero_vmax<T>(c);
   else hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c.stream, dtp, dt);
   for (int s = 0; s < 4; ++s) {
-    rhs_fused(c, du, s == 0 ? u : ust, nu, (cfl > 0 && s == 0) ? c.vmax : nullptr);
+    if (les_closure >= 0) {
+      if (cfl > 0 && s == 0) max_velocity_into(c, u);
+      les_rhs(c, du, s == 0 ? u : ust, nu, les_closure, p1, p2);
+    } else {
+      rhs_fused(c, du, s == 0 ? u : ust, nu, (cfl > 0 && s == 0) ? c.vmax : nullptr);
+    }
     if (cfl > 0 && s == 0) cfl_dt<T>(c, cfl, nu, dt);
     if (c.D == 3) SPEC_LAUNCH((k_rk4_update<T, 3>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
     else SPEC_LAUNCH((k_rk4_update<T, 2>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
@@ -656,18 +666,21 @@ template <typename T> void Ops<T>::gradient_phys(Ctx& c, T* GG, const CT* u) {
 
 // les! (src/les.jl:4-33) for the built-in closures:
 // closure 1 = Clark(Delta=p1), 2 = Smagorinsky(C_S=p1, Delta=p2),
-// 3 = dynamic Smagorinsky(Delta=p1), 0 = none.
+// 3 = dynamic Smagorinsky(Delta=p1), 4 = the attached learned closure, 0 = none.
+// The band forcing is added when it is enabled on the context.
 template <typename T>
 void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2) {
   const int nt = c.tdim();
   const int ng = c.D * c.D;
+  if (closure < 0 || closure > 4) throw Error(1, "unknown closure kind");
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, nt));
   nonlinearity(c, B, u, nullptr);
-  viscosity(c, B, u, nu);
   if (closure != 0) {
     T* tau = reinterpret_cast<T*>(c.work(0, size_t(nt) * c.phys_elems * sizeof(T)));
     if (closure == 3) {
       dynsmag(c, tau, u, p1);
+    } else if (closure == 4) {
+      cnn_tau(c, tau, u);
     } else {
       T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));
       gradient_phys(c, GG, u);
@@ -679,7 +692,34 @@ void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double
     r2c(c, th, tau, nt, 1.0 / std::pow(double(c.n), c.D));
     axpy(c, B, th, 1.0, nt);
   }
-  tensordivergence(c, du, B);
+  // viscosity! + tensordivergence! (+ forcing) in one pass
+  const double* fA = band_coeff(c, u);
+  if (c.D == 3) SPEC_LAUNCH((k_div_visc<T, 3>), du, B, u, nu, g_, fA, c.kband * c.kband);
+  else SPEC_LAUNCH((k_div_visc<T, 2>), du, B, u, nu, g_, fA, c.kband * c.kband);
+}
+
+namespace {
+// runs the network on the solver's stream for the lifetime of the object
+struct OnStream {
+  ::sc_cnn* net;
+  hipStream_t old;
+  OnStream(::sc_cnn* n, hipStream_t s) : net(n), old(cnnlink::set_stream(n, s)) {}
+  ~OnStream() { cnnlink::set_stream(net, old); }
+};
+}  // namespace
+
+// learned closure: tau = s_out * net(C2R(2/3 u) / s_in, extra), the six packed
+// components xx, yy, zz, xy, yz, zx in physical space
+template <typename T>
+void Ops<T>::cnn_tau(Ctx& c, T* tau, const CT* u) {
+  if (!c.cnn) throw Error(1, "closure 4 needs a network attached with sc_set_cnn_closure");
+  T* v = reinterpret_cast<T*>(c.work(1, size_t(c.D) * c.phys_elems * sizeof(T)));
+  c2r(c, v, u, c.D, 1.0, true);
+  OnStream on(c.cnn, c.stream);
+  const cnnlink::Bufs b = cnnlink::buffers(c.cnn);
+  cnnlink::pack_velocity<T>(c.cnn, v, c.phys_elems, 1.0 / c.cnn_sin, c.cnn_extra, b.x);
+  cnnlink::forward(c.cnn, b.x, b.y, 1);
+  cnnlink::unpack_tau<T>(c.cnn, b.y, c.cnn_sout, tau, c.phys_elems);
 }
 
 // create_dynamic_smagorinsky's model (src/closures.jl:188-271), with the
@@ -788,18 +828,38 @@ void Ops<T>::lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double
 //   g_ij = C2R(i k_j w_i), v = C2R(2/3 u), a_m = sum_j g_mj v_j + sum_i g_im v_i,
 //   out = 2/3 R2C(a)/n^D - nu (|k|^2 w + k (k.w)) + forcing transpose.
 template <typename T>
-void Ops<T>::rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu) {
+void Ops<T>::rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int closure) {
   const int D = c.D, ng = D * D;
   const long np = c.phys_elems;
+  if (closure > 0 && closure != 4) throw Error(1, "the transpose exists for closure kinds 0 and 4");
+  if (closure == 4 && !c.cnn) throw Error(1, "closure 4 needs a network attached with sc_set_cnn_closure");
   T* v = reinterpret_cast<T*>(c.work(13, size_t(D) * np * sizeof(T)));
   T* gr = reinterpret_cast<T*>(c.work(14, size_t(ng) * np * sizeof(T)));
   CT* G = reinterpret_cast<CT*>(c.work(15, size_t(ng) * c.spec_elems * sizeof(CT)));
   c2r(c, v, u, D, 1.0, true);
   vectorgradient(c, G, w);
   c2r(c, gr, G, ng, 1.0, false);
+  // learned closure: recompute the network on v, then push the stress
+  // cotangent tau_bar = sym(g) through it (parameter gradients accumulate).
+  // Cotangents under sums over grid points: y_bar = s_out tau_bar / n^D, and
+  // the velocity part a += n^D x_bar / s_in.
+  const double nD = std::pow(double(c.n), D);
+  cnnlink::Bufs cb{};
+  if (closure == 4) {
+    OnStream on(c.cnn, c.stream);
+    cb = cnnlink::buffers(c.cnn);
+    cnnlink::pack_velocity<T>(c.cnn, v, np, 1.0 / c.cnn_sin, c.cnn_extra, cb.x);
+    cnnlink::forward(c.cnn, cb.x, cb.y, 1);
+    cnnlink::pack_tau_cotangent<T>(c.cnn, gr, np, c.cnn_sout / nD, cb.dy);
+    cnnlink::backward(c.cnn, cb.x, cb.dy, cb.dx, 1);
+  }
   if (D == 3) hipLaunchKernelGGL((k_adj_products<T, 3>), dim3(nblk(np)), dim3(256), 0, c.stream, v, gr, np);
   else hipLaunchKernelGGL((k_adj_products<T, 2>), dim3(nblk(np)), dim3(256), 0, c.stream, v, gr, np);
   SC_HIP(hipGetLastError());
+  if (closure == 4) {
+    OnStream on(c.cnn, c.stream);
+    cnnlink::add_velocity_cotangent<T>(c.cnn, cb.dx, nD / c.cnn_sin, v, np);
+  }
   r2c(c, out, v, D, 1.0 / std::pow(double(c.n), D));
   twothirds(c, out, D);
   const double* bandp = nullptr;
@@ -817,7 +877,13 @@ void Ops<T>::rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu) {
 // Transposed Jacobian of one RK4 step (fixed dt) applied to ubar; the stage
 // values are recomputed from the checkpoint u0.
 template <typename T>
-void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu) {
+void Ops<T>::stage_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure) {
+  if (closure < 0) rhs_fused(c, du, u, nu, nullptr);
+  else les_rhs(c, du, u, nu, closure, 0.0, 0.0);
+}
+
+template <typename T>
+void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu, int closure) {
   const int D = c.D;
   const size_t fb = size_t(D) * c.spec_elems * sizeof(CT);
   CT* k = reinterpret_cast<CT*>(c.adj(0, fb));
@@ -829,13 +895,13 @@ void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, d
   CT* Tt = reinterpret_cast<CT*>(c.adj(6, fb));
   CT* acc = reinterpret_cast<CT*>(c.adj(7, fb));
   // forward recompute of the stage values
-  rhs_fused(c, k, u0, nu, nullptr);
+  stage_rhs(c, k, u0, nu, closure);
   lincomb(c, U1, u0, 1.0, k, dt / 2, D);
   project(c, U1);
-  rhs_fused(c, k, U1, nu, nullptr);
+  stage_rhs(c, k, U1, nu, closure);
   lincomb(c, U2, u0, 1.0, k, dt / 2, D);
   project(c, U2);
-  rhs_fused(c, k, U2, nu, nullptr);
+  stage_rhs(c, k, U2, nu, closure);
   lincomb(c, U3, u0, 1.0, k, dt, D);
   project(c, U3);
   // reverse sweep
@@ -843,25 +909,25 @@ void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, d
   project(c, Sb);
   SC_HIP(hipMemcpyAsync(acc, Sb, fb, hipMemcpyDeviceToDevice, c.stream));
   lincomb(c, KB, Sb, dt / 6, Sb, 0.0, D);
-  rhs_vjp(c, Tt, U3, KB, nu);
+  rhs_vjp(c, Tt, U3, KB, nu, closure);
   project(c, Tt);
   axpy(c, acc, Tt, 1.0, D);
   lincomb(c, KB, Sb, dt / 3, Tt, dt, D);
-  rhs_vjp(c, Tt, U2, KB, nu);
+  rhs_vjp(c, Tt, U2, KB, nu, closure);
   project(c, Tt);
   axpy(c, acc, Tt, 1.0, D);
   lincomb(c, KB, Sb, dt / 3, Tt, dt / 2, D);
-  rhs_vjp(c, Tt, U1, KB, nu);
+  rhs_vjp(c, Tt, U1, KB, nu, closure);
   project(c, Tt);
   axpy(c, acc, Tt, 1.0, D);
   lincomb(c, KB, Sb, dt / 6, Tt, dt / 2, D);
-  rhs_vjp(c, Tt, u0, KB, nu);
+  rhs_vjp(c, Tt, u0, KB, nu, closure);
   lincomb(c, out, acc, 1.0, Tt, 1.0, D);
 }
 
 // Transposed Jacobian of one Wray3 step (fixed dt) applied to ubar.
 template <typename T>
-void Ops<T>::wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu) {
+void Ops<T>::wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu, int closure) {
   const int D = c.D;
   const size_t fb = size_t(D) * c.spec_elems * sizeof(CT);
   const double a1 = double(T(8.0 / 15.0)), a2 = double(T(5.0 / 12.0)), a3 = double(T(3.0 / 4.0));
@@ -874,11 +940,11 @@ void Ops<T>::wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt,
   CT* S1b = reinterpret_cast<CT*>(c.adj(5, fb));
   CT* Tt = reinterpret_cast<CT*>(c.adj(6, fb));
   CT* KB = reinterpret_cast<CT*>(c.adj(7, fb));
-  rhs_fused(c, k, u0, nu, nullptr);
+  stage_rhs(c, k, u0, nu, closure);
   lincomb(c, U1, u0, 1.0, k, a1 * dt, D);
   project(c, U1);
   lincomb(c, S1, u0, 1.0, k, b1 * dt, D);
-  rhs_fused(c, k, U1, nu, nullptr);
+  stage_rhs(c, k, U1, nu, closure);
   lincomb(c, U2, S1, 1.0, k, a2 * dt, D);
   project(c, U2);
   // reverse
@@ -886,14 +952,14 @@ void Ops<T>::wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt,
   project(c, Q);                                   // q3
   SC_HIP(hipMemcpyAsync(S1b, Q, fb, hipMemcpyDeviceToDevice, c.stream));
   lincomb(c, KB, Q, a3 * dt, Q, 0.0, D);
-  rhs_vjp(c, Tt, U2, KB, nu);                      // u2bar
+  rhs_vjp(c, Tt, U2, KB, nu, closure);                      // u2bar
   project(c, Tt);                                  // q2
   axpy(c, S1b, Tt, 1.0, D);
   lincomb(c, KB, Tt, a2 * dt, Tt, 0.0, D);
-  rhs_vjp(c, Q, U1, KB, nu);                       // u1bar
+  rhs_vjp(c, Q, U1, KB, nu, closure);                       // u1bar
   project(c, Q);                                   // q1
   lincomb(c, KB, Q, a1 * dt, S1b, b1 * dt, D);     // k1bar
-  rhs_vjp(c, Tt, u0, KB, nu);
+  rhs_vjp(c, Tt, u0, KB, nu, closure);
   lincomb(c, out, Q, 1.0, S1b, 1.0, D);
   axpy(c, out, Tt, 1.0, D);
 }

@@ -88,6 +88,14 @@ SIGNATURES = {
     "sc_cnn_mse": [P, P, P, P, c_int, DP],
     "sc_cnn_backward": [P, P, P, P, c_int],
     "sc_cnn_adam": [P, c_double, c_double, c_double, c_double],
+    "sc_set_cnn_closure": [P, P, c_double, c_double, c_double],
+    "sc_closure_cnn": [P, P, P],
+    "sc_les_rk4": [P, P, c_double, c_double, c_int, c_double, c_double],
+    "sc_les_rk4_cfl": [P, P, c_double, c_double, c_double, c_int, c_double, c_double, DP],
+    "sc_les_rhs_vjp": [P, P, P, P, c_double, c_int],
+    "sc_les_rk4_vjp": [P, P, P, P, c_double, c_double, c_int],
+    "sc_les_wray3_vjp": [P, P, P, P, c_double, c_double, c_int],
+    "sc_lincomb": [P, P, P, c_double, P, c_double, c_int],
 }
 
 

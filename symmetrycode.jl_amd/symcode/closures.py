@@ -69,6 +69,33 @@ class DynamicSmagorinsky(Closure):
         return tau
 
 
+class LearnedClosure(Closure):
+    """The convolutional closure as an LES model: τ = s_out · net(ū / s_in,
+    extra) with ū = C2R(2/3 û) (the solver-facing wrapper, like fullchain,
+    src/nets.jl:311-334, but with global scales: s_in a velocity scale, s_out
+    = Δ²⟨|∇ū|²⟩, extra the standardised log Re_Δ channel).  ``net`` is a
+    ``cnn.ConvClosureNet`` with the grid's n, cin 3 or 4 and cout 6; it
+    stays attached to the grid until another model is attached."""
+    kind = 4
+
+    def __init__(self, net, g, s_in, s_out, extra=0.0):
+        super().__init__(g)
+        self.net, self.s_in, self.s_out, self.extra = net, float(s_in), float(s_out), float(extra)
+        self.attach()
+
+    def attach(self):
+        _c(lib().sc_set_cnn_closure(self.g.ctx, self.net.net, self.s_in, self.s_out, self.extra), self.g)
+
+    def __call__(self, u, GG=None):
+        tau = self.g.spacetensorfield()
+        _c(lib().sc_closure_cnn(self.g.ctx, _p(tau), _p(u)), self.g)
+        return tau
+
+
+def create_learned_closure(net, g, s_in, s_out, extra=0.0):
+    return LearnedClosure(net, g, s_in, s_out, extra)
+
+
 def create_clark(delta, g):
     return Clark(delta, g)
 
@@ -90,27 +117,69 @@ def gradient_phys(u, g: Grid):
 
 def les_rhs(du, u, g: Grid, model, visc):
     """les!(du, u, grid, cache; model, visc) (src/les.jl:4-33)."""
-    kind = 0 if model is None else model.kind
-    p1 = 0.0 if model is None else model.p1
-    p2 = 0.0 if model is None else model.p2
+    kind, p1, p2 = _kind(model)
     _c(lib().sc_les_rhs(g.ctx, _p(du), _p(u), visc, kind, p1, p2), g)
     return du
 
 
 def les_wray3(u, dt, g: Grid, model, visc):
     """wray3!(les!, u, Δt, grid, cache; model, visc)."""
-    kind = 0 if model is None else model.kind
-    p1 = 0.0 if model is None else model.p1
-    p2 = 0.0 if model is None else model.p2
+    kind, p1, p2 = _kind(model)
     _c(lib().sc_les_wray3(g.ctx, _p(u), dt, visc, kind, p1, p2), g)
     return u
 
 
+def _kind(model):
+    if model is None:
+        return 0, 0.0, 0.0
+    if isinstance(model, LearnedClosure):
+        model.attach()
+    return model.kind, model.p1, model.p2
+
+
+def les_rk4(u, dt, g: Grid, model, visc):
+    """One RK4 step of les! (RK4 has no reference counterpart)."""
+    kind, p1, p2 = _kind(model)
+    _c(lib().sc_les_rk4(g.ctx, _p(u), dt, visc, kind, p1, p2), g)
+    return u
+
+
+def les_rk4_cfl(u, g: Grid, model, visc, cfl, dtmax):
+    """RK4 step of les! with Δt = min(cfl · propose_timestep(u), dtmax); returns Δt."""
+    kind, p1, p2 = _kind(model)
+    dt = ctypes.c_double()
+    _c(lib().sc_les_rk4_cfl(g.ctx, _p(u), cfl, visc, dtmax, kind, p1, p2, ctypes.byref(dt)), g)
+    return dt.value
+
+
+def les_rhs_vjp(out, u, w, g: Grid, model, visc):
+    """out = J(u)^T w for les! with no closure or a LearnedClosure (whose
+    parameter gradients accumulate), RFFT-weighted inner product."""
+    kind, _, _ = _kind(model)
+    _c(lib().sc_les_rhs_vjp(g.ctx, _p(out), _p(u), _p(w), visc, kind), g)
+    return out
+
+
+def les_rk4_vjp(u0, ubar, dt, g: Grid, model, visc, out=None):
+    """(d les_rk4 step / d u0)^T ubar, stages recomputed from u0."""
+    kind, _, _ = _kind(model)
+    out = g.vectorfield() if out is None else out
+    _c(lib().sc_les_rk4_vjp(g.ctx, _p(out), _p(u0), _p(ubar), dt, visc, kind), g)
+    return out
+
+
+def les_wray3_vjp(u0, ubar, dt, g: Grid, model, visc, out=None):
+    kind, _, _ = _kind(model)
+    out = g.vectorfield() if out is None else out
+    _c(lib().sc_les_wray3_vjp(g.ctx, _p(out), _p(u0), _p(ubar), dt, visc, kind), g)
+    return out
+
+
 def les_wray3_cfl(u, g: Grid, model, visc, cfl, dtmax):
     """wray3!(les!) with Δt = min(cfl · propose_timestep(u), dtmax); returns Δt."""
+    kind, p1, p2 = _kind(model)
     dt = ctypes.c_double()
-    _c(lib().sc_les_wray3_cfl(g.ctx, _p(u), cfl, visc, dtmax, model.kind, model.p1, model.p2,
-                              ctypes.byref(dt)), g)
+    _c(lib().sc_les_wray3_cfl(g.ctx, _p(u), cfl, visc, dtmax, kind, p1, p2, ctypes.byref(dt)), g)
     return dt.value
 
 

@@ -329,3 +329,9 @@ def wray3_vjp(u0, ubar, dt, g: Grid, visc):
     out = g.vectorfield()
     _c(lib().sc_wray3_vjp(g.ctx, _p(out), _p(u0), _p(ubar), dt, visc), g)
     return out
+
+
+def lincomb(dst, x, a, y, b, g: Grid):
+    """dst = a x + b y over all components (spectral; dst may alias x or y)."""
+    _c(lib().sc_lincomb(g.ctx, _p(dst), _p(x), a, _p(y), b, ncomp_of(x)), g)
+    return dst

@@ -1,0 +1,131 @@
+"""GPU parity of les! with the learned closure (closure kind 4) and of its
+transposes, against oracle/les_cnn.py with the network's BF16 storage points
+emulated.  The closure's share is compared at BF16 tolerance: the norm of
+(GPU - oracle) against the norm of the closure's contribution (oracle with
+closure - oracle without).  The closure-free paths keep the FP64 bar (1e-12)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle_solenoidal, print_case, rel
+from oracle import adjoint as A
+from oracle import les_cnn as L
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+N = 64
+S_IN, S_OUT, EXTRA = 1.0, 0.5, 0.25
+TOL_BF16 = 2e-2
+
+
+def share_err(got, ref, ref0):
+    return np.linalg.norm(got - ref) / np.linalg.norm(ref - ref0)
+
+
+@pytest.fixture(scope="module")
+def env():
+    import symcode as sc
+
+    go = S.Grid(3, N)
+    gg = sc.Grid(3, N, dtype=torch.float64)
+    net = sc.cnn.ConvClosureNet(N, batch_max=1, cin=4, width=64, cout=6, seed=3)
+    h = net.get_params()
+    h[-192:] = np.random.default_rng(4).standard_normal(192).astype(np.float32) * 0.05
+    net.set_params(h)
+    Ws, bs = net.unflatten(net.get_params())
+    onet = L.Net([W.copy() for W in Ws], [b.copy() for b in bs], 4, S_IN, S_OUT, EXTRA, emulate=True)
+    model = sc.create_learned_closure(net, gg, S_IN, S_OUT, EXTRA)
+    u = oracle_solenoidal(go, 21, trunc=True)
+    return sc, go, gg, net, onet, model, u
+
+
+def grads_of(net):
+    GW, GB = net.unflatten(net.get_grads())
+    return GW, GB
+
+
+def check_grads(net, og, tol=3e-2):
+    GW, GB = grads_of(net)
+    for l in range(4):
+        assert rel(GW[l], og[0][l]) < tol, ("W", l, rel(GW[l], og[0][l]))
+    for l in range(3):
+        assert rel(GB[l], og[1][l]) < tol, ("b", l, rel(GB[l], og[1][l]))
+
+
+def test_closure_tau(env):
+    sc, go, gg, net, onet, model, u = env
+    print_case("cnn closure tau", N)
+    tau = gg.download_physical(model(gg.upload_spectral(u)))
+    otau, _ = L.cnn_tau(u, go, onet)
+    assert rel(tau, otau) < 1e-2
+
+
+def test_les_rhs_closure4(env):
+    sc, go, gg, net, onet, model, u = env
+    print_case("les_rhs closure 4", N)
+    nu = 0.01
+    du = gg.download_spectral(sc.les_rhs(gg.vectorfield(), gg.upload_spectral(u), gg, model, nu))
+    ref = L.les_rhs(u, go, nu, onet)
+    ref0 = S.convectiondiffusion(u, go, nu)
+    assert share_err(du, ref, ref0) < TOL_BF16
+
+
+def test_les_rk4_closure4(env):
+    sc, go, gg, net, onet, model, u = env
+    print_case("les_rk4 closure 4", N)
+    nu, dt = 0.01, 0.01
+    ud = gg.upload_spectral(u)
+    sc.les_rk4(ud, dt, gg, model, nu)
+    ref = L.rk4(u, dt, go, nu, onet)
+    ref0 = S.rk4(lambda v: S.convectiondiffusion(v, go, nu), u, dt, go)
+    assert share_err(gg.download_spectral(ud), ref, ref0) < TOL_BF16
+
+
+def test_les_rhs_vjp_closure4(env):
+    sc, go, gg, net, onet, model, u = env
+    print_case("les_rhs_vjp closure 4", N)
+    nu = 0.01
+    w = oracle_solenoidal(go, 22)
+    net.zero_grad()
+    out = sc.les_rhs_vjp(gg.vectorfield(), gg.upload_spectral(u), gg.upload_spectral(w), gg, model, nu)
+    og = L.zero_grads(onet)
+    ref = L.les_rhs_vjp(u, w, go, nu, onet, grads=og)
+    ref0 = A.rhs_vjp(u, w, go, nu)
+    assert share_err(gg.download_spectral(out), ref, ref0) < 3e-2
+    check_grads(net, og)
+
+
+def test_les_rk4_vjp_closure4(env):
+    sc, go, gg, net, onet, model, u = env
+    print_case("les_rk4_vjp closure 4", N)
+    nu, dt = 0.01, 0.01
+    ub = oracle_solenoidal(go, 23)
+    net.zero_grad()
+    out = sc.les_rk4_vjp(gg.upload_spectral(u), gg.upload_spectral(ub), dt, gg, model, nu)
+    og = L.zero_grads(onet)
+    ref = L.rk4_vjp(u, ub, dt, go, nu, onet, grads=og)
+    ref0 = A.rk4_vjp(u, ub, dt, go, nu)
+    assert share_err(gg.download_spectral(out), ref, ref0) < 3e-2
+    check_grads(net, og)
+
+
+@pytest.mark.parametrize("D,n", [(3, 16), (2, 32)])
+def test_les_rk4_and_vjp_without_closure(D, n):
+    """closure 0: the LES RK4 step and its transpose equal the DNS ones
+    (band forcing on)"""
+    import symcode as sc
+
+    print_case("les_rk4 closure 0", D, n)
+    go = S.Grid(D, n)
+    gg = sc.Grid(D, n, dtype=torch.float64)
+    sc.set_band_forcing(gg, 0.1, 2)
+    u = oracle_solenoidal(go, 24, trunc=True)
+    nu, dt = 0.01, 0.02
+    f = lambda v: S.convectiondiffusion(v, go, nu, forcing=(0.1, 2))
+    ud = gg.upload_spectral(u)
+    sc.les_rk4(ud, dt, gg, None, nu)
+    assert rel(gg.download_spectral(ud), S.rk4(f, u, dt, go)) < 1e-12
+    ub = oracle_solenoidal(go, 25)
+    out = sc.les_rk4_vjp(gg.upload_spectral(u), gg.upload_spectral(ub), dt, gg, None, nu)
+    assert rel(gg.download_spectral(out), A.rk4_vjp(u, ub, dt, go, nu, (0.1, 2))) < 1e-12
