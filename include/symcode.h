@@ -199,6 +199,9 @@ int sc_cnn_zero_grad(sc_cnn* net);
  * has_extra (the constant Re_Delta feature) */
 int sc_cnn_pack_input(sc_cnn* net, const void* fields, int fp64, int ncomp, long cstride, long bstride, int batch,
                       double scale, int has_extra, double extra, void* x);
+/* cout physical fields -> fp32 target [batch][n^3][cout] (times scale) */
+int sc_cnn_pack_target(sc_cnn* net, const void* fields, int fp64, long cstride, long bstride, int batch,
+                       double scale, void* target);
 int sc_cnn_forward(sc_cnn* net, const void* x, void* y, int batch);
 /* MSE on the first cout channels; dy = d loss / d y as bf16 */
 int sc_cnn_mse(sc_cnn* net, const void* y, const void* target, void* dy, int batch, double* loss);

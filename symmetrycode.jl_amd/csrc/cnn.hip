@@ -318,6 +318,14 @@ __global__ void k_pack_input(const T* f, long nvox, int ncomp, double scale, int
   }
 }
 
+// pack ncomp component-major fields into channels-last fp32 [nvox][ncomp] * scale
+template <typename T>
+__global__ void k_pack_target(const T* f, long nvox, int ncomp, double scale, float* out, long cstride) {
+  long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  for (int c = 0; c < ncomp; ++c) out[v * ncomp + c] = float(double(f[c * cstride + v]) * scale);
+}
+
 // unpack channels-last fp32 [nvox][cp] -> component-major T (first ncomp)
 template <typename T>
 __global__ void k_unpack(const float* y, long nvox, int ncomp, int cp, double scale, T* out, long stride) {
@@ -718,6 +726,23 @@ int sc_cnn_pack_input(sc_cnn* c, const void* f, int fp64, int ncomp, long cstrid
         hipLaunchKernelGGL(k_pack_input<float>, dim3(unsigned((nv + 255) / 256)), dim3(256), 0, c->stream,
                            (const float*)f + b * bstride, nv, ncomp, scale, has_extra, float(extra), out, c->cin_p,
                            cstride);
+      CNN_HIP(hipGetLastError());
+    }
+  });
+}
+
+int sc_cnn_pack_target(sc_cnn* c, const void* f, int fp64, long cstride, long bstride, int batch, double scale,
+                       void* out) {
+  return cguard(c, [&] {
+    const long nv = long(c->n) * c->n * c->n;
+    for (int b = 0; b < batch; ++b) {
+      float* o = (float*)out + long(b) * nv * c->cout;
+      if (fp64)
+        hipLaunchKernelGGL(k_pack_target<double>, dim3(unsigned((nv + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)f + b * bstride, nv, c->cout, scale, o, cstride);
+      else
+        hipLaunchKernelGGL(k_pack_target<float>, dim3(unsigned((nv + 255) / 256)), dim3(256), 0, c->stream,
+                           (const float*)f + b * bstride, nv, c->cout, scale, o, cstride);
       CNN_HIP(hipGetLastError());
     }
   });

@@ -97,6 +97,21 @@ class ConvClosureNet:
                                           _p(x)), self.net)
         return x
 
+    def pack_target(self, fields, scale=1.0, out=None):
+        """fields: (batch, cout, n, n, n) or (cout, n, n, n) physical -> fp32
+        target (batch, n^3, cout) times scale."""
+        if fields.dim() == 4:
+            fields = fields.unsqueeze(0)
+        fields = fields.contiguous()
+        batch = fields.shape[0]
+        if fields.shape[1] != self.cout:
+            raise ValueError(f"expected {self.cout} components")
+        out = out if out is not None else torch.empty((batch, self.nvox, self.cout), dtype=torch.float32,
+                                                      device=self.device)
+        check_cnn(lib().sc_cnn_pack_target(self.net, _p(fields), 1 if fields.dtype == torch.float64 else 0,
+                                           self.nvox, self.cout * self.nvox, batch, scale, _p(out)), self.net)
+        return out
+
     def forward(self, x, y=None):
         batch = x.shape[0]
         y = y if y is not None else self.new_output(batch)
