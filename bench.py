@@ -62,6 +62,129 @@ def cpu_baseline(n_sample=256, threads=None):
     }
 
 
+PEAK_BF16 = 2.5e15  # FLOP/s, MI355X dense BF16 MFMA
+
+
+def spectrum_error(u, ref, g):
+    """relative L2 distance of the shell spectra over the resolved shells"""
+    import numpy as np
+
+    import symcode as sc
+
+    _, a = sc.spectrum(u, g)
+    _, b = sc.spectrum(ref, g)
+    m = min(len(a), len(b), g.n // 3)
+    a, b = np.asarray(a[:m]), np.asarray(b[:m])
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, log=print):
+    """The learned-closure workloads.
+
+    Data: a 512^3 FP64 forced DNS (nu = 2e-3, band forcing 0.1 on |k| <= 2,
+    E(k) ~ k^4 exp(-2(k/4)^2)) filtered with Gaussian Delta in {2, 4} h_LES and
+    cut to 128^3 at 8 save times: 16 (u_bar, tau) pairs.
+    Training: one Adam step on the batch of 16 = forward, MSE, backward
+    (data and weight gradients of all four 3x3x3 layers), Adam.
+    A-posteriori: 128^3 FP32 LES with the learned closure, 20 RK4 steps from
+    the filtered DNS, trajectory MSE against the filtered DNS; one gradient
+    step = rollout + hand-written adjoint (stages recomputed) + Adam."""
+    import torch
+
+    import symcode as sc
+    from symcode import training as T
+
+    visc, cfl = 2e-3, 0.3
+    t_setup = time.perf_counter()
+    g_dns = sc.Grid(3, n_dns, dtype=torch.float64)
+    g_les = sc.Grid(3, n_les, dtype=torch.float64)
+    u = sc.randomfield(sc.peak_profile, g_dns, seed=1, totalenergy=0.2, kpeak=4)
+    sc.set_band_forcing(g_dns, 0.1, 2)
+    T.create_data(u, g_dns, g_les, visc, [2], 1, 0.2, cfl=cfl)  # spin-up
+    data = T.create_data(u, g_dns, g_les, visc, [2, 4], 8, 0.02, cfl=cfl)
+    pairs = data[2] + data[4]
+    nrm = T.Normalization.of(pairs, g_les)
+    net = sc.cnn.ConvClosureNet(n_les, batch_max=16, cin=4, width=64, cout=6, seed=0)
+    x, y = T.pack_pairs(net, g_les, pairs, nrm)
+    log(f"closure data ready ({time.perf_counter() - t_setup:.1f} s)")
+    # ---- a-priori training step, batch 16
+    net.train_step(x, y)
+    torch.cuda.synchronize()
+    net.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(train_steps):
+        loss = net.train_step(x, y)
+    torch.cuda.synchronize()
+    t_train = (time.perf_counter() - t0) / train_steps
+    rep = net.profile_report()
+    net.profile(False)
+    conv = {k: v for k, v in rep.items() if k.startswith("conv_")}
+    conv_ms = sum(v[1] for v in conv.values()) / train_steps
+    conv_flop = sum(v[2] for v in conv.values()) / train_steps
+    nvox = 16 * n_les**3
+    useful = 3 * 2.0 * nvox * 27 * (4 * 64 + 64 * 64 * 2 + 64 * 6)  # fwd + data grad + weight grad
+    cnn_train = {
+        "value": 1.0 / t_train, "unit": "training steps/s", "ms_per_step": t_train * 1e3,
+        "workload": f"3x3x3 circular CNN 4->64->64->64->6 (GELU), batch 16 x {n_les}^3 filtered "
+                    f"{n_dns}^3 DNS pairs, BF16 MFMA / FP32 accumulate, Adam lr 1e-3, MSE",
+        "loss": loss,
+        "conv_ms_per_step": conv_ms,
+        "conv_tflops_executed": conv_flop / (conv_ms / 1e3) / 1e12,
+        "mfma_util_executed": conv_flop / (conv_ms / 1e3) / PEAK_BF16,
+        "mfma_util_useful": useful / (conv_ms / 1e3) / PEAK_BF16,
+        "mfma_peak_tflops": PEAK_BF16 / 1e12,
+        "kernels_ms_per_step": {k: v[1] / train_steps for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
+    }
+    log(f"cnn train: {cnn_train['ms_per_step']:.1f} ms/step")
+    # ---- a-posteriori gradient steps, 128^3 FP32
+    g32 = sc.Grid(3, n_les, dtype=torch.float32)
+    delta = 2 * g_les.l / n_les
+    # fixed LES step: half the CFL-0.3 step of the DNS scaled to the 4x coarser grid
+    dt = 0.5 * cfl * sc.propose_timestep(u, g_dns, visc) * (n_dns / n_les)
+    ub, _ = sc.sfs(u, g_dns, g_les, delta)
+    u0 = T.regrid(ub, g_les, g32)
+    re0 = T.filter_reynolds(ub, g_les, visc, delta)
+    refs = []
+    for _ in range(nsteps):
+        # the DNS advances with sub-steps of its own CFL limit to each LES time
+        tt = 0.0
+        while tt < dt * (1 - 1e-12):
+            tt += sc.rk4_cfl(u, g_dns, visc, cfl, dt - tt)
+        refs.append(T.regrid(sc.sfs(u, g_dns, g_les, delta)[0], g_les, g32))
+    del g_dns, u
+    model = nrm.closure(net, g32, delta, re0)
+    T.aposteriori_step(net, u0, refs, dt, g32, visc, model)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(grad_steps):
+        Lc = T.aposteriori_step(net, u0, refs, dt, g32, visc, model)
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / grad_steps
+    rollouts = {}
+    ref_norm = sum(sc.spectraldot(r, r, g32).real for r in refs) / (2 * nsteps)
+    for name, m in (("learned", model), ("dynsmag", sc.create_dynamic_smagorinsky(delta, g32)), ("none", None)):
+        T.rollout_loss(u0, refs, dt, g32, visc, m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        states = []
+        Lm = T.rollout_loss(u0, refs, dt, g32, visc, m, states)
+        torch.cuda.synchronize()
+        rollouts[name] = {"rollouts_per_s": 1.0 / (time.perf_counter() - t0),
+                          "rel_traj_mse": Lm / ref_norm,
+                          "spectrum_err_final": spectrum_error(states[-1], refs[-1], g32)}
+    les_grad = {
+        "value": 1.0 / t_grad, "unit": "gradient steps/s", "ms_per_grad_step": t_grad * 1e3,
+        "workload": f"{n_les}^3 FP32 LES with the learned closure, {nsteps} RK4 steps (dt={dt:.3e}) from "
+                    f"filtered {n_dns}^3 DNS, trajectory MSE, hand-written adjoint with per-step "
+                    "checkpoints and stage recompute, Adam",
+        "loss": Lc,
+        "rollouts": rollouts,
+        "grad_steps_vs_dynsmag_rollouts": (1.0 / t_grad) / rollouts["dynsmag"]["rollouts_per_s"],
+    }
+    log(f"les grad: {les_grad['ms_per_grad_step']:.1f} ms/step")
+    return cnn_train, les_grad
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +194,8 @@ def main():
     ap.add_argument("--prec", type=int, default=64, choices=(32, 64))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the learned-closure training and a-posteriori gradient benchmarks")
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (counter calibration)")
     args = ap.parse_args()
@@ -161,6 +286,10 @@ def main():
         "energy_after": e,
         "wall_s": wall,
     }
+    if not args.no_closure:
+        del u, g
+        torch.cuda.empty_cache()
+        out["cnn_train"], out["les_grad"] = bench_closure(log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)

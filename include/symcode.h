@@ -178,8 +178,9 @@ int sc_lincomb(sc_ctx* ctx, void* dst, const void* x, double a, const void* y, d
  * accumulation and FP32 master weights, trained with Adam.  No reference
  * counterpart: the reference's learned closures are pointwise (1x1x1) nets
  * (PointwiseConv / mlp, src/nets.jl:37-81,247-298), which this net reduces to
- * when every non-centre tap is zero.  n in {64, 128}; activations are
- * channels-last bf16 [batch][n^3][cin_p] (cin_p = cout_p = 32).
+ * when every non-centre tap is zero.  n in {64, 128}, cin <= 8, cout <= 16;
+ * tensors are channels-last: input bf16 [batch][n^3][cin_p] (cin_p = 8),
+ * output fp32 [batch][n^3][cout_p] (cout_p = 16).
  */
 typedef struct sc_cnn sc_cnn;
 int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t seed, sc_cnn** out);
@@ -208,6 +209,10 @@ int sc_cnn_mse(sc_cnn* net, const void* y, const void* target, void* dy, int bat
 /* accumulate parameter gradients of the last forward; dx (nullable) = d loss / d x (fp32) */
 int sc_cnn_backward(sc_cnn* net, const void* x, const void* dy, void* dx, int batch);
 int sc_cnn_adam(sc_cnn* net, double lr, double beta1, double beta2, double eps);
+/* per-launch HIP-event timing: enable (clears the record), then a report of
+ * lines "name count total_ms executed_flop" (flop = 2 * voxels * 27 * cin_p * cout_p) */
+int sc_cnn_profile(sc_cnn* net, int enable);
+int sc_cnn_profile_report(sc_cnn* net, char* buf, int buflen);
 /* attach net as closure kind 4 of sc_les_*: tau = s_out * net(C2R(2/3 u) / s_in,
  * extra) on the channels (xx, yy, zz, xy, yz, zx); the net must have the
  * context's n, cin 3 or 4 (channel 3 = extra), cout 6, and outlive the

@@ -18,8 +18,10 @@
 // rows of a 64-voxel output segment (with one halo voxel each side) are
 // staged in LDS; weights stream from L2.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -29,6 +31,7 @@
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 namespace cnn {
 
@@ -37,6 +40,9 @@ namespace cnn {
     hipError_t e_ = (call);                                                             \
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// periodic index for v in [-n, 2n)
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ __forceinline__ float gelu(float z) {
   const float c = 0.7978845608028654f;  // sqrt(2/pi)
@@ -52,174 +58,311 @@ __device__ __forceinline__ float gelu_grad(float z) {
 
 enum Mode { HIDDEN = 0, LAST = 1, BWD_HIDDEN = 2, BWD_INPUT = 3 };
 
-constexpr int VPB = 64;   // output voxels per block (one x segment)
+constexpr int VPB = 64;   // output voxels per x segment (n must be a multiple)
+constexpr int TROWS = 4;  // output y rows per block (one per wavefront)
 constexpr int PADC = 8;   // LDS channel padding (16 bytes) against bank conflicts
 
-// One convolution layer (or its data gradient).  n must be a multiple of 64.
+constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
+
+// One convolution layer (or its data gradient) as an implicit GEMM with the
+// reduction index k = t * CIN + c (tap-major; narrow layers pack several taps
+// into one 32-wide MFMA step).  Block: 4 wavefronts over 4 consecutive y rows
+// of one z plane, 64 x voxels each; a wavefront holds its 64-voxel x COUTT
+// output tile in 4 x (COUTT/16) MFMA accumulators, so every B fragment feeds
+// 4 MFMAs and every A fragment COUTT/16.  Inputs are staged in LDS: one input
+// plane (6 rows x 66 voxels) at a time for CIN = 64, all three planes for the
+// narrow layers.  Weights are pre-arranged per (32-wide k step, 16 output
+// channels) as the 64 lanes' B fragments (1 KB contiguous, see
+// k_pack_weights), read through L1.  The next input plane is fetched into
+// registers while the current one's MFMAs run.
 //  HIDDEN:     z = conv + bias; out_z = z, out_a = gelu(z)            (bf16)
 //  LAST:       out_f = conv                                           (fp32)
 //  BWD_HIDDEN: out_a = conv * gelu'(zprev)                            (bf16)
-//  BWD_INPUT:  out_f = conv                                           (fp32)
-template <int CIN, int COUT, int MODE>
-__global__ void __launch_bounds__(256) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
-                                              const float* __restrict__ bias, bf16* out_a, bf16* out_z,
-                                              float* out_f, const bf16* __restrict__ zprev, int n) {
+//  BWD_INPUT:  out_f = conv (first COUTS channels)                    (fp32)
+template <int CIN, int COUTT, int COUTS, int MODE>
+__global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
+                                                 const float* __restrict__ bias, bf16* out_a, bf16* out_z,
+                                                 float* out_f, const bf16* __restrict__ zprev, int n) {
   constexpr int CL = CIN + PADC;
-  __shared__ __attribute__((aligned(16))) bf16 win[9 * (VPB + 2) * CL];
-  const int segs = n / VPB;
-  const long row = blockIdx.x / segs;  // (b*n + z)*n + y
-  const int x0 = (blockIdx.x % segs) * VPB;
-  const int y = int(row % n);
-  const long bz = row / n;
-  const int z = int(bz % n);
-  const long b = bz / n;
-  // stage the 9 neighbour rows, 16 bytes (8 channels) per load
+  constexpr int RW = TROWS + 2, XW = VPB + 2;
+  constexpr bool BYPLANE = CIN >= 32;
+  constexpr int NPL = BYPLANE ? 1 : 3;
+  constexpr int NKC = kpad(CIN) / 32;
+  constexpr int NT = COUTT / 16;
   constexpr int CH8 = CIN / 8;
-  for (int i = threadIdx.x; i < 9 * (VPB + 2) * CH8; i += blockDim.x) {
-    const int c8 = i % CH8;
-    const int xx = (i / CH8) % (VPB + 2);
-    const int rr = i / (CH8 * (VPB + 2));
-    const int dz = rr / 3 - 1, dy = rr % 3 - 1;
-    const int zs = (z + dz + n) % n, ys = (y + dy + n) % n, xs = (x0 + xx - 1 + n) % n;
-    const long src = (((b * n + zs) * n + ys) * n + xs) * CIN + c8 * 8;
-    *reinterpret_cast<bf16x8*>(&win[(rr * (VPB + 2) + xx) * CL + c8 * 8]) =
-        *reinterpret_cast<const bf16x8*>(&in[src]);
-  }
-  __syncthreads();
+  constexpr int MT = 4;
+  __shared__ __attribute__((aligned(16))) bf16 win[NPL * RW * XW * CL];
+  const int segs = n / VPB, ytiles = n / TROWS;
+  long bid = blockIdx.x;
+  const int xs = int(bid % segs);
+  bid /= segs;
+  const int yt = int(bid % ytiles);
+  bid /= ytiles;
+  const int z = int(bid % n);
+  const long b = bid / n;
+  const int x0 = xs * VPB, y0 = yt * TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
-  floatx4 acc[COUT / 16];
+  floatx4 acc[MT][NT];
 #pragma unroll
-  for (int nb = 0; nb < COUT / 16; ++nb) acc[nb] = floatx4{0, 0, 0, 0};
-#pragma unroll 1
-  for (int t = 0; t < 27; ++t) {
-    const int rr = t / 3, dx = t % 3 - 1;
-    const bf16* arow = &win[(rr * (VPB + 2) + 1 + w * 16 + li + dx) * CL + 8 * lk];
-    const bf16* wt = W + long(t) * COUT * CIN + 8 * lk;
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int kc = 0; kc < CIN / 32; ++kc) {
-      bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kc * 32);
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0, 0, 0, 0};
+  // A fragment of M tile mt (row w, x 16 mt + li) at k step kc
+  auto load_a = [&](int kc, bf16x8* a) {
+    const int k0 = kc * 32 + 8 * lk;
+    const int t = k0 / CIN, c0 = k0 % CIN;
+    if (t < 27) {
+      const int pl = BYPLANE ? 0 : t / 9;
+      const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const bf16* base = &win[((pl * RW + w + 1 + dy) * XW + 1 + li + dx) * CL + c0];
 #pragma unroll
-      for (int nb = 0; nb < COUT / 16; ++nb) {
-        bf16x8 bv = *reinterpret_cast<const bf16x8*>(wt + long(nb * 16 + li) * CIN + kc * 32);
-        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[nb], 0, 0, 0);
-      }
+      for (int mt = 0; mt < MT; ++mt)
+        a[mt] = *reinterpret_cast<const bf16x8*>(base + 16 * mt * CL);
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[mt][e] = bf16(0.f);
+    }
+  };
+  auto load_b = [&](int kc, bf16x8* bv) {
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      bv[nt] = *reinterpret_cast<const bf16x8*>(&W[(long(kc * NT + nt) * 64 + l) * 8]);
+  };
+  // staging: all of a phase's 16-byte chunks are fetched into registers at
+  // once (one memory latency per phase), the next phase's during this one's
+  // MFMAs, and written to LDS between two barriers
+  constexpr int NCH = NPL * RW * XW * CH8;
+  constexpr int NPT = (NCH + 255) / 256;
+  constexpr int NPH = BYPLANE ? 3 : 1;
+  u32x4 pre[NPT];
+#define CONV_FETCH(PH)                                                                                      \
+  _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                                       \
+    const int i = min(int(threadIdx.x) + 256 * j, NCH - 1);                                                 \
+    const int c8 = i % CH8;                                                                                 \
+    int r = i / CH8;                                                                                        \
+    const int xi = r % XW;                                                                                  \
+    r /= XW;                                                                                                \
+    const int row = r % RW, pl = r / RW;                                                                    \
+    const int zs = wrap(z + (BYPLANE ? (PH) : pl) - 1, n), ys = wrap(y0 - 1 + row, n);                      \
+    const int xx = wrap(x0 - 1 + xi, n);                                                                    \
+    pre[j] = *reinterpret_cast<const u32x4*>(&in[(((b * n + zs) * n + ys) * n + xx) * CIN + c8 * 8]);       \
+  }
+#define CONV_COMMIT()                                                                                       \
+  _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                                       \
+    const int i = threadIdx.x + 256 * j;                                                                    \
+    if (i < NCH) *reinterpret_cast<u32x4*>(&win[(i / CH8) * CL + (i % CH8) * 8]) = pre[j];                  \
+  }
+  CONV_FETCH(0)
+  CONV_COMMIT()
+  __syncthreads();
+#pragma unroll
+  for (int ph = 0; ph < NPH; ++ph) {
+    if (ph + 1 < NPH) {
+      CONV_FETCH(ph + 1)
+    }
+    const int kc0 = BYPLANE ? ph * (NKC / 3) : 0;
+    const int kc1 = BYPLANE ? kc0 + NKC / 3 : NKC;
+#pragma unroll 2
+    for (int kc = kc0; kc < kc1; ++kc) {
+      bf16x8 a[MT], bv[NT];
+      load_a(kc, a);
+      load_b(kc, bv);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv[nt], acc[mt][nt], 0, 0, 0);
+    }
+    if (ph + 1 < NPH) {
+      __syncthreads();
+      CONV_COMMIT()
+      __syncthreads();
     }
   }
-  // epilogue: lane holds D[voxel = 4*lk + r][co = li + 16 nb]
-  const long vbase = row * n + x0 + w * 16;
+#undef CONV_FETCH
+#undef CONV_COMMIT
+  // epilogue: lane holds D[voxel = 16 mt + 4 lk + r of row w][co = 16 nt + li]
 #pragma unroll
-  for (int nb = 0; nb < COUT / 16; ++nb) {
-    const int co = nb * 16 + li;
+  for (int nt = 0; nt < NT; ++nt) {
+    const int co = nt * 16 + li;
+    if (co >= COUTS) continue;
+    float bco = 0.f;
+    if constexpr (MODE == HIDDEN) bco = bias[co];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const long v = vbase + 4 * lk + r;
-      float d = acc[nb][r];
-      if constexpr (MODE == HIDDEN) {
-        d += bias[co];
-        out_z[v * COUT + co] = bf16(d);
-        out_a[v * COUT + co] = bf16(gelu(d));
-      } else if constexpr (MODE == LAST) {
-        out_f[v * COUT + co] = d;
-      } else if constexpr (MODE == BWD_HIDDEN) {
-        out_a[v * COUT + co] = bf16(d * gelu_grad(float(zprev[v * COUT + co])));
-      } else {
-        out_f[v * COUT + co] = d;
+    for (int mt = 0; mt < MT; ++mt) {
+      const long vrow = ((b * n + z) * n + y0 + w) * long(n) + x0 + 16 * mt + 4 * lk;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long v = vrow + r;
+        const float d = acc[mt][nt][r];
+        if constexpr (MODE == HIDDEN) {
+          out_z[v * COUTS + co] = bf16(d + bco);
+          out_a[v * COUTS + co] = bf16(gelu(d + bco));
+        } else if constexpr (MODE == BWD_HIDDEN) {
+          out_a[v * COUTS + co] = bf16(d * gelu_grad(float(zprev[v * COUTS + co])));
+        } else {
+          out_f[v * COUTS + co] = d;
+        }
       }
     }
   }
 }
 
-// Weight gradient of one layer for the three taps sharing (dz, dy) = blockIdx.y:
+// 8 x 8 transpose of 16-bit elements held as 8 rows of 4 dwords:
+// out[c] (dwords 0..3 = columns 2j, 2j+1 of row pair) = column c of in
+__device__ __forceinline__ void transpose8x8(const uint32_t in[8][4], uint32_t out[8][4]) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t a = in[2 * j][c >> 1], bb = in[2 * j + 1][c >> 1];
+      // v_perm_b32 selector: bytes of {bb, a}; low half of a|b or high half
+      out[c][j] = (c & 1) ? __builtin_amdgcn_perm(bb, a, 0x07060302u) : __builtin_amdgcn_perm(bb, a, 0x05040100u);
+    }
+}
+
+// Weight gradient of one layer:
 //   dW[t][co][ci] += sum_v dout[v][co] * in[v + (dz,dy,dx)][ci]
-// over the rows [blockIdx.x * rpb, +rpb).  Partial sums go to
-// part[blockIdx.x][t][co][ci]; k_reduce_partials adds them up.
+// as GEMMs with M = co, N = ci, K = the voxels of an x row.  Persistent
+// blocks: blockIdx.y = dz + 1, and block p takes the (batch, z) planes p,
+// p + gridDim.x, ... whole, walking y with a 3-row sliding window of the
+// input plane z + dz, so each block keeps all nine (dy, dx) taps of its dz in
+// registers (wavefront w owns input channels 16 w .. 16 w + 15).  Rows are
+// staged transposed ([channel][x], 8 x 8 register transposes); the input row
+// carries an 8-voxel periodic halo on both sides, and the x shifts dx = -1, +1
+// are formed in registers from three aligned 16-byte reads.  Block sums go to
+// part[p][t][co][ci] (k_reduce_partials adds them up); with db set, the
+// dz = 0 blocks also add the bias gradient sum_v dout[v][co].
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(256) k_conv_wgrad(const bf16* __restrict__ in, const bf16* __restrict__ dout,
-                                                    float* part, int n, long nrows, int rpb) {
-  constexpr int XL = 136;  // 128 voxels + halo, padded (x contiguous in LDS)
+__global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ in, const bf16* __restrict__ dout,
+                                                       float* part, float* db, int n, long nplanes) {
+  constexpr int XL = 128 + 16;                 // row length (n <= 128) + halo
+  constexpr int CP = CIN < 16 ? 16 : CIN;      // staged input channels (zero-padded)
+  constexpr int MT = COUT / 16;
+  constexpr int NW = CP / 16;                  // wavefronts with input channels
   __shared__ __attribute__((aligned(16))) bf16 dT[COUT * XL];
-  __shared__ __attribute__((aligned(16))) bf16 iT[CIN * XL];
-  const int rr = blockIdx.y;
-  const int dz = rr / 3 - 1, dy = rr % 3 - 1;
+  __shared__ __attribute__((aligned(16))) bf16 xT[3 * CP * XL];
+  const int dz = int(blockIdx.y) - 1;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
-  // wave w owns output-channel blocks cb = w, w+4, ... and all input blocks
-  constexpr int NCB = COUT / 16, NIB = CIN / 16;
-  constexpr int MYCB = (NCB + 3) / 4;
-  floatx4 acc[MYCB][NIB][3];
+  floatx4 acc[3][3][MT];
 #pragma unroll
-  for (int a = 0; a < MYCB; ++a)
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int i = 0; i < NIB; ++i)
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) acc[a][i][d] = floatx4{0, 0, 0, 0};
-  const long r0 = long(blockIdx.x) * rpb;
-#pragma unroll 1
-  for (long row = r0; row < r0 + rpb && row < nrows; ++row) {
-    const int y = int(row % n);
-    const long bz = row / n;
-    const int z = int(bz % n);
-    const long b = bz / n;
-    const long srow = ((b * n + (z + dz + n) % n) * n + (y + dy + n) % n) * n;
-    __syncthreads();
-    // stage dout[row][x][co] transposed to dT[co][x], and the shifted input
-    // row in[srow][x-1 .. x+n][ci] transposed to iT[ci][x]
-    for (int i = threadIdx.x; i < n * (COUT / 8); i += blockDim.x) {
-      const int c8 = i % (COUT / 8), x = i / (COUT / 8);
-      bf16x8 v8 = *reinterpret_cast<const bf16x8*>(&dout[(row * n + x) * COUT + c8 * 8]);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) dT[(c8 * 8 + e) * XL + x] = v8[e];
+      for (int m = 0; m < MT; ++m) acc[a][d][m] = floatx4{0, 0, 0, 0};
+  if (CIN < CP) {  // zero channel rows that are never loaded
+    for (int i = threadIdx.x; i < 3 * (CP - CIN) * XL; i += 256) {
+      const int s = i / ((CP - CIN) * XL), rem = i % ((CP - CIN) * XL);
+      xT[(s * CP + CIN) * XL + rem] = bf16(0.f);
     }
-    for (int i = threadIdx.x; i < (n + 2) * (CIN / 8); i += blockDim.x) {
-      const int c8 = i % (CIN / 8), xx = i / (CIN / 8);
-      const int xs = (xx - 1 + n) % n;
-      bf16x8 v8 = *reinterpret_cast<const bf16x8*>(&in[(srow + xs) * CIN + c8 * 8]);
+  }
+  float bsum = 0.f;
+  const bool dobias = db != nullptr && dz == 0;
+  const int nb8 = n / 8;
+  // transposed staging of one row: dout row (ro >= 0) and/or input row yi into slot
+  auto stage = [&](long drow, long xrow, int slot) {
+    const int nd = drow >= 0 ? nb8 * (COUT / 8) : 0;
+    const int nx = xrow >= 0 ? (nb8 + 2) * (CIN / 8) : 0;
+    for (int blk = threadIdx.x; blk < nd + nx; blk += 256) {
+      uint32_t v[8][4], o[8][4];
+      bf16* dst;
+      if (blk < nd) {
+        const int xb = blk % nb8, cb = blk / nb8;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) iT[(c8 * 8 + e) * XL + xx] = v8[e];
+        for (int e = 0; e < 8; ++e) {
+          const uint4 q = *reinterpret_cast<const uint4*>(&dout[(drow * n + xb * 8 + e) * COUT + cb * 8]);
+          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
+        }
+        dst = &dT[(cb * 8) * XL + xb * 8];
+      } else {
+        const int j = blk - nd;
+        const int xb = j % (nb8 + 2), cb = j / (nb8 + 2);  // x from 8 (xb - 1)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int xs = (8 * (xb - 1) + e + n) % n;
+          const uint4 q = *reinterpret_cast<const uint4*>(&in[(xrow * n + xs) * CIN + cb * 8]);
+          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
+        }
+        dst = &xT[(slot * CP + cb * 8) * XL + xb * 8];
+      }
+      transpose8x8(v, o);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        *reinterpret_cast<uint4*>(dst + c * XL) = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
     }
-    __syncthreads();
-    // K = voxels x in [0, n), 32 per MFMA
+  };
 #pragma unroll 1
-    for (int k0 = 0; k0 < n; k0 += 32) {
+  for (long q = blockIdx.x; q < nplanes; q += gridDim.x) {
+    const int z = int(q % n);
+    const long b = q / n;
+    const long pin = b * n + (z + dz + n) % n;  // input plane
+    __syncthreads();
+    stage(-1, pin * n + (n - 1), 2);  // row y = -1 -> slot 2
+    stage(-1, pin * n + 0, 0);        // row y = 0  -> slot 0
+#pragma unroll 1
+    for (int y = 0; y < n; ++y) {
+      __syncthreads();
+      stage(q * n + y, pin * n + (y + 1) % n, (y + 1) % 3);
+      __syncthreads();
+      if (dobias && threadIdx.x < COUT)
+        for (int x = 0; x < n; ++x) bsum += float(dT[threadIdx.x * XL + x]);
+      if (w >= NW) continue;
+      const int sl[3] = {(y + 2) % 3, y % 3, (y + 1) % 3};
+#pragma unroll 1
+      for (int k0 = 0; k0 < n; k0 += 32) {
+        const int x0 = k0 + 8 * lk;
+        bf16x8 av[MT];
 #pragma unroll
-      for (int a = 0; a < MYCB; ++a) {
-        const int cb = w + 4 * a;
-        if (cb >= NCB) continue;
-        bf16x8 av;
+        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8*>(&dT[(m * 16 + li) * XL + x0]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) av[e] = dT[(cb * 16 + li) * XL + k0 + 8 * lk + e];
-#pragma unroll
-        for (int ib = 0; ib < NIB; ++ib) {
+        for (int a = 0; a < 3; ++a) {
+          const bf16* xr = &xT[(sl[a] * CP + 16 * w + li) * XL + x0];
+          // dwords: m = (x0 - 2, x0 - 1), L0 = x0 .. x0 + 7, p = (x0 + 8, x0 + 9)
+          const uint32_t m = *reinterpret_cast<const uint32_t*>(xr + 6);
+          const uint4 L0 = *reinterpret_cast<const uint4*>(xr + 8);
+          const uint32_t p = *reinterpret_cast<const uint32_t*>(xr + 16);
+          uint4 S[3];
+          // dx = -1: x0 - 1 .. x0 + 6;  dx = 0: x0 .. x0 + 7;  dx = +1: x0 + 1 .. x0 + 8
+          S[0] = uint4{__builtin_amdgcn_alignbyte(L0.x, m, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
+                       __builtin_amdgcn_alignbyte(L0.z, L0.y, 2), __builtin_amdgcn_alignbyte(L0.w, L0.z, 2)};
+          S[1] = L0;
+          S[2] = uint4{__builtin_amdgcn_alignbyte(L0.y, L0.x, 2), __builtin_amdgcn_alignbyte(L0.z, L0.y, 2),
+                       __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(p, L0.w, 2)};
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            // output voxel x reads input x + dx -> iT column x + dx + 1
             bf16x8 bv;
+            __builtin_memcpy(&bv, &S[d], 16);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) bv[e] = iT[(ib * 16 + li) * XL + k0 + 8 * lk + e + d];
-            acc[a][ib][d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[a][ib][d], 0, 0, 0);
+            for (int m = 0; m < MT; ++m)
+              acc[a][d][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[a][d][m], 0, 0, 0);
           }
         }
       }
     }
   }
-  // D[i = co in block][j = ci in block]: lane holds rows 4*lk + r, column li
+  if (dobias && threadIdx.x < COUT) atomicAdd(&db[threadIdx.x], bsum);
+  if (w >= NW) return;
+  // D[co = 16 m + 4 lk + r][ci = 16 w + li]
+  const int ci = 16 * w + li;
+  if (ci >= CIN) return;
 #pragma unroll
-  for (int a = 0; a < MYCB; ++a) {
-    const int cb = w + 4 * a;
-    if (cb >= NCB) continue;
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int ib = 0; ib < NIB; ++ib)
+    for (int d = 0; d < 3; ++d) {
+      const int t = (dz + 1) * 9 + a * 3 + d;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const int t = rr * 3 + d;
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int co = cb * 16 + 4 * lk + r, ci = ib * 16 + li;
-          part[((long(blockIdx.x) * 27 + t) * COUT + co) * CIN + ci] = acc[a][ib][d][r];
+          const int co = m * 16 + 4 * lk + r;
+          part[((long(blockIdx.x) * 27 + t) * COUT + co) * CIN + ci] = acc[a][d][m][r];
         }
-      }
-  }
+    }
 }
 
 __global__ void k_reduce_partials(const float* part, float* out, long nparts, long len) {
@@ -230,23 +373,6 @@ __global__ void k_reduce_partials(const float* part, float* out, long nparts, lo
   out[i] += s;
 }
 
-// bias gradient: db[c] += sum_v d[v][c] (block partials then atomic add)
-template <int C>
-__global__ void k_bias_grad(const bf16* d, float* db, long nvox) {
-  __shared__ float sm[256];
-  const int c = threadIdx.x % C;
-  const int lane = threadIdx.x / C;
-  const int lanes = 256 / C;
-  float s = 0.f;
-  for (long v = long(blockIdx.x) * lanes + lane; v < nvox; v += long(gridDim.x) * lanes) s += float(d[v * C + c]);
-  sm[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x < C) {
-    float t = 0.f;
-    for (int i = 0; i < lanes; ++i) t += sm[i * C + threadIdx.x];
-    atomicAdd(&db[threadIdx.x], t);
-  }
-}
 
 // MSE loss over the first `nout` channels of y (fp32, channel pitch cp):
 // loss partials, and the gradient dy = 2 (y - t) / (nvox * nout) as bf16 with
@@ -290,16 +416,31 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, long len, f
   w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// master [27][cout][cin] -> forward bf16 [27][cop][cip] and mirrored
-// transposed bf16 [27][cip][cop] (tap t -> 26 - t)
-__global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cout, int cin, int cop, int cip) {
-  long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  long len = 27L * cop * cip;
-  if (i >= len) return;
-  int ci = int(i % cip), co = int((i / cip) % cop), t = int(i / (long(cip) * cop));
-  float val = (co < cout && ci < cin) ? w[(long(t) * cout + co) * cin + ci] : 0.f;
-  wf[i] = bf16(val);
-  wb[(long(26 - t) * cip + ci) * cop + co] = bf16(val);
+// master [27][cop][cip] -> bf16 B fragments of the forward convolution,
+// W[co][k = t * cip + ci] (k padded to kpad(cip), co to a multiple of 16), and
+// of the backward-data convolution, W'[ci][k = t' * cop + co] = W[26 - t'][co][ci]
+// (taps mirrored).  Fragment layout: for k step kc and channel tile nt, lane
+// L = 16 j + i holds entries (row 16 nt + i, k = 32 kc + 8 j .. + 7), stored at
+// ((kc * NT + nt) * 64 + L) * 8; padding entries are zero.
+__device__ __forceinline__ long frag_index(int row, int k, int nrows) {
+  const int kc = k >> 5, j = (k >> 3) & 3, e = k & 7, nt = row >> 4, i = row & 15;
+  return ((long(kc) * (nrows / 16) + nt) * 64 + 16 * j + i) * 8 + e;
+}
+__global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cop, int cip) {
+  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int kpf = kpad(cip), kpb = kpad(cop);
+  const int rf = (cop + 15) / 16 * 16, rb = (cip + 15) / 16 * 16;
+  const long nf = long(rf) * kpf, nb = long(rb) * kpb;
+  if (i < nf) {
+    const int co = int(i / kpf), k = int(i % kpf);
+    const int t = k / cip, ci = k % cip;
+    wf[frag_index(co, k, rf)] = bf16(t < 27 && co < cop ? w[(long(t) * cop + co) * cip + ci] : 0.f);
+  } else if (i < nf + nb) {
+    const long j = i - nf;
+    const int ci = int(j / kpb), k = int(j % kpb);
+    const int tp = k / cop, co = k % cop;
+    wb[frag_index(ci, k, rb)] = bf16(tp < 27 && ci < cip ? w[(long(26 - tp) * cop + co) * cip + ci] : 0.f);
+  }
 }
 
 // pack physical fields (component-major, ncomp x nvox, T) into channels-last
@@ -359,7 +500,7 @@ __global__ void k_add_vel_cot(const float* dx, long nvox, int cp, double scale, 
 // ------------------------------------------------------------------ host side
 struct sc_cnn {
   int n = 0, bmax = 0, cin = 0, width = 0, cout = 0;
-  int cin_p = 32, cout_p = 32;
+  int cin_p = 8, cout_p = 16;
   hipStream_t stream = nullptr;
   bool own_stream = false;  // created here (destroyed by sc_cnn_destroy) or the caller's
   std::string err;
@@ -378,6 +519,30 @@ struct sc_cnn {
   double* lossp = nullptr;
   double* hostd = nullptr;
   int adam_t = 0;
+  int ncu = 256;  // compute units of the device (persistent wgrad grid)
+  // per-launch HIP-event timing (sc_cnn_profile): name, events, executed flops
+  struct Prof { const char* name; hipEvent_t a, b; double flop; };
+  bool prof_on = false;
+  std::vector<Prof> prof;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      CNN_HIP(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  void pbeg(const char* name, double flop) {
+    if (!prof_on) return;
+    Prof r{name, next_event(), next_event(), flop};
+    CNN_HIP(hipEventRecord(r.a, stream));
+    prof.push_back(r);
+  }
+  void pend() {
+    if (prof_on) CNN_HIP(hipEventRecord(prof.back().b, stream));
+  }
 };
 
 namespace {
@@ -406,47 +571,59 @@ int cguard(sc_cnn* c, F&& f) {
 void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const bf16* W, const float* bias,
                  bf16* oa, bf16* oz, float* of, const bf16* zp, int batch) {
   const long rows = long(batch) * c->n * c->n;
-  dim3 grid(unsigned(rows * (c->n / VPB)));
-#define CONV(CI, CO, MO)                                                                                   \
-  if (cinp == CI && coutp == CO && mode == MO) {                                                          \
-    hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, dim3(256), 0, c->stream, in, W, bias, oa, oz, of, zp, c->n); \
+  dim3 grid(unsigned(long(batch) * c->n * (c->n / TROWS) * (c->n / VPB)));
+  static const char* names[4] = {"conv_fwd", "conv_last", "conv_bwd_data", "conv_bwd_input"};
+  const double flop = 2.0 * double(rows) * c->n * kpad(cinp) * ((coutp + 15) / 16 * 16);
+#define CONV(CI, COT, COS, MO)                                                                             \
+  if (cinp == CI && coutp == COS && mode == MO) {                                                         \
+    c->pbeg(names[MO], flop);                                                                             \
+    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(256), 0, c->stream, in, W, bias, oa, oz, of, zp, \
+                       c->n);                                                                             \
     CNN_HIP(hipGetLastError());                                                                           \
+    c->pend();                                                                                            \
     return;                                                                                               \
   }
-  CONV(32, 64, HIDDEN) CONV(64, 64, HIDDEN) CONV(64, 32, LAST)
-  CONV(32, 64, BWD_HIDDEN) CONV(64, 64, BWD_HIDDEN) CONV(64, 32, BWD_INPUT)
+  CONV(8, 64, 64, HIDDEN) CONV(64, 64, 64, HIDDEN) CONV(64, 16, 16, LAST)
+  CONV(16, 64, 64, BWD_HIDDEN) CONV(64, 64, 64, BWD_HIDDEN) CONV(64, 16, 8, BWD_INPUT)
 #undef CONV
   throw std::runtime_error("unsupported conv shape");
 }
 
-void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* dout, float* grad, int batch) {
-  const long rows = long(batch) * c->n * c->n;
-  const int rpb = 64;
-  const long nchunks = (rows + rpb - 1) / rpb;
+void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* dout, float* grad, float* db,
+                  int batch) {
+  const long nplanes = long(batch) * c->n;
+  const long G = std::min<long>(nplanes, std::max(1, 2 * c->ncu / 3));
   const long len = 27L * coutp * cinp;
-  if (c->part_len < nchunks * len) {
+  if (c->part_len < G * len) {
     if (c->part) CNN_HIP(hipFree(c->part));
-    CNN_HIP(hipMalloc(&c->part, nchunks * len * sizeof(float)));
-    c->part_len = nchunks * len;
+    CNN_HIP(hipMalloc(&c->part, G * len * sizeof(float)));
+    c->part_len = G * len;
   }
-  dim3 grid(unsigned(nchunks), 9);
+  dim3 grid(unsigned(G), 3);
+  c->pbeg("conv_wgrad", 2.0 * double(nplanes) * c->n * c->n * 27.0 * ((cinp + 15) / 16 * 16) * coutp);
 #define WG(CI, CO)                                                                                              \
   if (cinp == CI && coutp == CO) {                                                                               \
-    hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), grid, dim3(256), 0, c->stream, in, dout, c->part, c->n, rows, rpb); \
+    hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), grid, dim3(256), 0, c->stream, in, dout, c->part, db, c->n, nplanes); \
     CNN_HIP(hipGetLastError());                                                                                  \
   } else
-  WG(32, 64) WG(64, 64) WG(64, 32) { throw std::runtime_error("unsupported wgrad shape"); }
+  WG(8, 64) WG(64, 64) WG(64, 16) { throw std::runtime_error("unsupported wgrad shape"); }
 #undef WG
-  hipLaunchKernelGGL(k_reduce_partials, dim3(unsigned((len + 255) / 256)), dim3(256), 0, c->stream, c->part, grad,
-                     nchunks, len);
+  c->pend();
+  c->pbeg("wgrad_reduce", 0);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(unsigned((len + 255) / 256)), dim3(256), 0, c->stream, c->part, grad, G,
+                     len);
   CNN_HIP(hipGetLastError());
+  c->pend();
 }
+
+long wf_len(const sc_cnn* c, int l) { return long((layer_cout(c, l) + 15) / 16 * 16) * kpad(layer_cin(c, l)); }
+long wb_len(const sc_cnn* c, int l) { return long((layer_cin(c, l) + 15) / 16 * 16) * kpad(layer_cout(c, l)); }
 
 void pack_all(sc_cnn* c) {
   for (int l = 0; l < 4; ++l) {
-    long len = 27L * layer_cout(c, l) * layer_cin(c, l);
+    const long len = wf_len(c, l) + wb_len(c, l);
     hipLaunchKernelGGL(k_pack_weights, dim3(unsigned((len + 255) / 256)), dim3(256), 0, c->stream, c->w + c->woff[l],
-                       c->wf[l], c->wb[l], layer_cout(c, l), layer_cin(c, l), layer_cout(c, l), layer_cin(c, l));
+                       c->wf[l], c->wb[l], layer_cout(c, l), layer_cin(c, l));
     CNN_HIP(hipGetLastError());
   }
 }
@@ -469,16 +646,13 @@ void forward(sc_cnn* c, const void* x, float* y, int batch) {
 
 void backward(sc_cnn* c, const void* x, const void* dy, float* dx, int batch) {
   if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
-  const long nv = long(batch) * c->n * c->n * c->n;
   const bf16* dY = (const bf16*)dy;
   // last layer: dW4, then the data gradient into dz[2] with gelu'(z3)
-  launch_wgrad(c, c->width, c->cout_p, c->a[2], dY, c->g + c->woff[3], batch);
+  launch_wgrad(c, c->width, c->cout_p, c->a[2], dY, c->g + c->woff[3], nullptr, batch);
   launch_conv(c, c->cout_p, c->width, BWD_HIDDEN, dY, c->wb[3], nullptr, c->dz[2], nullptr, nullptr, c->z[2], batch);
   for (int l = 2; l >= 0; --l) {
     const bf16* inl = l == 0 ? (const bf16*)x : c->a[l - 1];
-    launch_wgrad(c, layer_cin(c, l), c->width, inl, c->dz[l], c->g + c->woff[l], batch);
-    hipLaunchKernelGGL(k_bias_grad<64>, dim3(1024), dim3(256), 0, c->stream, c->dz[l], c->g + c->boff[l], nv);
-    CNN_HIP(hipGetLastError());
+    launch_wgrad(c, layer_cin(c, l), c->width, inl, c->dz[l], c->g + c->woff[l], c->g + c->boff[l], batch);
     if (l > 0)
       launch_conv(c, c->width, c->width, BWD_HIDDEN, c->dz[l], c->wb[l], nullptr, c->dz[l - 1], nullptr, nullptr,
                   c->z[l - 1], batch);
@@ -557,7 +731,7 @@ extern "C" {
 int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t seed, sc_cnn** out) {
   if (!out) return 1;
   *out = nullptr;
-  if (n % VPB != 0 || n > 128 || batch_max < 1 || cin < 1 || cin > 32 || width != 64 || cout < 1 || cout > 32)
+  if (n % VPB != 0 || n > 128 || batch_max < 1 || cin < 1 || cin > 8 || width != 64 || cout < 1 || cout > 16)
     return 1;
   g_cnn_create_err.clear();
   (void)hipGetLastError();
@@ -566,6 +740,11 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
   try {
     CNN_HIP(hipStreamCreate(&c->stream));
     c->own_stream = true;
+    {
+      int dev = 0;
+      CNN_HIP(hipGetDevice(&dev));
+      CNN_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     long off = 0;
     for (int l = 0; l < 4; ++l) {
       c->woff[l] = off;
@@ -602,9 +781,8 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
     CNN_HIP(hipMemset(c->m, 0, c->wlen * 4));
     CNN_HIP(hipMemset(c->v, 0, c->wlen * 4));
     for (int l = 0; l < 4; ++l) {
-      long len = 27L * layer_cout(c, l) * layer_cin(c, l);
-      CNN_HIP(hipMalloc(&c->wf[l], len * 2));
-      CNN_HIP(hipMalloc(&c->wb[l], len * 2));
+      CNN_HIP(hipMalloc(&c->wf[l], wf_len(c, l) * 2));
+      CNN_HIP(hipMalloc(&c->wb[l], wb_len(c, l) * 2));
     }
     const long nv = long(batch_max) * n * n * n;
     for (int l = 0; l < 3; ++l) {
@@ -637,6 +815,7 @@ int sc_cnn_destroy(sc_cnn* c) {
     if (c->dz[l]) (void)hipFree(c->dz[l]);
   }
   if (c->hostd) (void)hipHostFree(c->hostd);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -687,9 +866,11 @@ int sc_cnn_mse(sc_cnn* c, const void* y, const void* tgt, void* dy, int batch, d
   return cguard(c, [&] {
     const long nv = long(batch) * c->n * c->n * c->n;
     const unsigned nb = unsigned((nv + 255) / 256);
+    c->pbeg("mse", 0);
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, c->stream, (const float*)y, (const float*)tgt, (bf16*)dy, nullptr,
                        c->lossp, nv, c->cout, c->cout_p, 1.0 / (double(nv) * c->cout));
     CNN_HIP(hipGetLastError());
+    c->pend();
     std::vector<double> h(nb);
     CNN_HIP(hipMemcpyAsync(h.data(), c->lossp, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CNN_HIP(hipStreamSynchronize(c->stream));
@@ -745,6 +926,43 @@ int sc_cnn_pack_target(sc_cnn* c, const void* f, int fp64, long cstride, long bs
                            (const float*)f + b * bstride, nv, c->cout, scale, o, cstride);
       CNN_HIP(hipGetLastError());
     }
+  });
+}
+
+int sc_cnn_profile(sc_cnn* c, int enable) {
+  return cguard(c, [&] {
+    c->prof.clear();
+    c->ev_used = 0;
+    c->prof_on = enable != 0;
+  });
+}
+
+// one line per launch name: "name count total_ms executed_flop"
+int sc_cnn_profile_report(sc_cnn* c, char* buf, int buflen) {
+  return cguard(c, [&] {
+    if (!buf || buflen < 1) throw std::runtime_error("buffer");
+    CNN_HIP(hipStreamSynchronize(c->stream));
+    std::vector<std::string> names;
+    std::vector<double> tot, flop;
+    std::vector<long> cnt;
+    for (auto& r : c->prof) {
+      float ms = 0;
+      CNN_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      size_t i = 0;
+      while (i < names.size() && names[i] != r.name) ++i;
+      if (i == names.size()) { names.push_back(r.name); tot.push_back(0); cnt.push_back(0); flop.push_back(0); }
+      tot[i] += ms;
+      cnt[i] += 1;
+      flop[i] += r.flop;
+    }
+    std::string out;
+    char line[256];
+    for (size_t i = 0; i < names.size(); ++i) {
+      std::snprintf(line, sizeof line, "%s %ld %.6f %.6e\n", names[i].c_str(), cnt[i], tot[i], flop[i]);
+      out += line;
+    }
+    std::strncpy(buf, out.c_str(), size_t(buflen) - 1);
+    buf[buflen - 1] = 0;
   });
 }
 

@@ -25,7 +25,7 @@ class ConvClosureNet:
         rc = lib().sc_cnn_create(n, batch_max, cin, width, cout, seed, ctypes.byref(net))
         if rc != 0:
             if rc == 1:
-                raise ValueError("sc_cnn_create: n must be a multiple of 64 and <= 128, width 64, cin/cout <= 32")
+                raise ValueError("sc_cnn_create: n must be a multiple of 64 and <= 128, width 64, cin <= 8, cout <= 16")
             raise RuntimeError("sc_cnn_create failed: " + lib().sc_cnn_last_error(None).decode())
         self.net = net
         info = (ctypes.c_int64 * 8)()
@@ -132,6 +132,20 @@ class ConvClosureNet:
         dx = torch.zeros((batch, self.nvox, self.cin_p), dtype=torch.float32, device=self.device) if want_dx else None
         check_cnn(lib().sc_cnn_backward(self.net, _p(x), _p(dy), None if dx is None else _p(dx), batch), self.net)
         return dx
+
+    def profile(self, enable=True):
+        """start (clearing) or stop per-launch HIP-event timing"""
+        check_cnn(lib().sc_cnn_profile(self.net, 1 if enable else 0), self.net)
+
+    def profile_report(self):
+        """{name: (launches, total ms, executed flop)} since profile(True)"""
+        buf = ctypes.create_string_buffer(1 << 14)
+        check_cnn(lib().sc_cnn_profile_report(self.net, buf, len(buf)), self.net)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms, flop = line.split()
+            out[name] = (int(cnt), float(ms), float(flop))
+        return out
 
     def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         check_cnn(lib().sc_cnn_adam(self.net, lr, beta1, beta2, eps), self.net)
