@@ -44,45 +44,51 @@ namespace cnn {
 // periodic index for v in [-n, 2n)
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-__device__ __forceinline__ float gelu(float z) {
-  const float c = 0.7978845608028654f;  // sqrt(2/pi)
-  float t = tanhf(c * (z + 0.044715f * z * z * z));
-  return 0.5f * z * (1.0f + t);
+// tanh-form GELU, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z + 0.044715 z^3),
+// evaluated as z * sigmoid(2u) (the same function) with the hardware exp2 and
+// reciprocal: relative error ~1e-7, far below the bf16 storage it feeds
+__device__ __forceinline__ float sig2u(float z) {
+  const float c2 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+  const float u = z * __builtin_fmaf(0.044715f * z, z, 1.0f);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(c2 * u));
 }
+__device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
+// d/dz: s + 2 z s (1 - s) sqrt(2/pi) (1 + 3 * 0.044715 z^2), s = sigmoid(2u)
 __device__ __forceinline__ float gelu_grad(float z) {
-  const float c = 0.7978845608028654f;
-  float u = c * (z + 0.044715f * z * z * z);
-  float t = tanhf(u);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
+  const float s = sig2u(z);
+  return s + 2.0f * 0.7978845608028654f * z * s * (1.0f - s) * __builtin_fmaf(0.134145f * z, z, 1.0f);
 }
 
 enum Mode { HIDDEN = 0, LAST = 1, BWD_HIDDEN = 2, BWD_INPUT = 3 };
 
 constexpr int VPB = 64;   // output voxels per x segment (n must be a multiple)
-constexpr int TROWS = 4;  // output y rows per block (one per wavefront)
-constexpr int PADC = 8;   // LDS channel padding (16 bytes) against bank conflicts
+constexpr int TROWS = 8;  // output y rows per block (one per wavefront)
+#ifndef SC_CNN_PADC
+#define SC_CNN_PADC 8
+#endif
+constexpr int PADC = SC_CNN_PADC;  // LDS channel padding (multiple of 8) against bank conflicts
 
 constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 
 // One convolution layer (or its data gradient) as an implicit GEMM with the
 // reduction index k = t * CIN + c (tap-major; narrow layers pack several taps
-// into one 32-wide MFMA step).  Block: 4 wavefronts over 4 consecutive y rows
+// into one 32-wide MFMA step).  Block: 8 wavefronts over 8 consecutive y rows
 // of one z plane, 64 x voxels each; a wavefront holds its 64-voxel x COUTT
 // output tile in 4 x (COUTT/16) MFMA accumulators, so every B fragment feeds
 // 4 MFMAs and every A fragment COUTT/16.  Inputs are staged in LDS: one input
-// plane (6 rows x 66 voxels) at a time for CIN = 64, all three planes for the
+// plane (10 rows x 66 voxels) at a time for CIN = 64, all three planes for the
 // narrow layers.  Weights are pre-arranged per (32-wide k step, 16 output
 // channels) as the 64 lanes' B fragments (1 KB contiguous, see
-// k_pack_weights), read through L1.  The next input plane is fetched into
-// registers while the current one's MFMAs run.
+// k_pack_weights), read through L1, two k steps ahead.  The next input plane
+// is fetched into registers while the current one's MFMAs run.
 //  HIDDEN:     z = conv + bias; out_z = z, out_a = gelu(z)            (bf16)
 //  LAST:       out_f = conv                                           (fp32)
 //  BWD_HIDDEN: out_a = conv * gelu'(zprev)                            (bf16)
 //  BWD_INPUT:  out_f = conv (first COUTS channels)                    (fp32)
 template <int CIN, int COUTT, int COUTS, int MODE>
-__global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
+__global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
                                                  const float* __restrict__ bias, bf16* out_a, bf16* out_z,
-                                                 float* out_f, const bf16* __restrict__ zprev, int n) {
+                                                 float* out_f, const bf16* __restrict__ zprev, int n, long ntiles) {
   constexpr int CL = CIN + PADC;
   constexpr int RW = TROWS + 2, XW = VPB + 2;
   constexpr bool BYPLANE = CIN >= 32;
@@ -91,25 +97,40 @@ __global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, co
   constexpr int NT = COUTT / 16;
   constexpr int CH8 = CIN / 8;
   constexpr int MT = 4;
+  constexpr int NPH = BYPLANE ? 3 : 1;
+  constexpr int NCH = NPL * RW * XW * CH8;
+  constexpr int NPT = (NCH + 511) / 512;
   __shared__ __attribute__((aligned(16))) bf16 win[NPL * RW * XW * CL];
   const int segs = n / VPB, ytiles = n / TROWS;
-  long bid = blockIdx.x;
-  const int xs = int(bid % segs);
-  bid /= segs;
-  const int yt = int(bid % ytiles);
-  bid /= ytiles;
-  const int z = int(bid % n);
-  const long b = bid / n;
-  const int x0 = xs * VPB, y0 = yt * TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
+  struct Tile { long b; int z, y0, x0; };
+  auto decode = [&](long t) {
+    Tile T;
+    T.x0 = int(t % segs) * VPB;
+    t /= segs;
+    T.y0 = int(t % ytiles) * TROWS;
+    t /= ytiles;
+    T.z = int(t % n);
+    T.b = t / n;
+    return T;
+  };
   floatx4 acc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0, 0, 0, 0};
   // A fragment of M tile mt (row w, x 16 mt + li) at k step kc
+  const bf16* lane_a = &win[li * CL + 8 * lk];
   auto load_a = [&](int kc, bf16x8* a) {
+    if constexpr (CIN >= 32) {  // one tap per k step: the offsets are uniform
+      const int t = (kc * 32) / CIN, cb = (kc * 32) % CIN;
+      const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const bf16* base = lane_a + ((w + 1 + dy) * XW + 1 + dx) * CL + cb;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + 16 * mt * CL);
+      return;
+    }
     const int k0 = kc * 32 + 8 * lk;
     const int t = k0 / CIN, c0 = k0 % CIN;
     if (t < 27) {
@@ -117,8 +138,7 @@ __global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, co
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
       const bf16* base = &win[((pl * RW + w + 1 + dy) * XW + 1 + li + dx) * CL + c0];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-        a[mt] = *reinterpret_cast<const bf16x8*>(base + 16 * mt * CL);
+      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + 16 * mt * CL);
     } else {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
@@ -131,50 +151,64 @@ __global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, co
     for (int nt = 0; nt < NT; ++nt)
       bv[nt] = *reinterpret_cast<const bf16x8*>(&W[(long(kc * NT + nt) * 64 + l) * 8]);
   };
-  // staging: all of a phase's 16-byte chunks are fetched into registers at
-  // once (one memory latency per phase), the next phase's during this one's
-  // MFMAs, and written to LDS between two barriers
-  constexpr int NCH = NPL * RW * XW * CH8;
-  constexpr int NPT = (NCH + 255) / 256;
-  constexpr int NPH = BYPLANE ? 3 : 1;
+  // staging of (tile T, phase PH): all of a phase's 16-byte chunks are fetched
+  // into registers at once, and written to LDS between two barriers
   u32x4 pre[NPT];
-#define CONV_FETCH(PH)                                                                                      \
-  _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                                       \
-    const int i = min(int(threadIdx.x) + 256 * j, NCH - 1);                                                 \
+#define CONV_FETCH(T, PH)                                                                                   \
+  _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                        \
+    const int i = min(int(threadIdx.x) + 512 * j, NCH - 1);                                                 \
     const int c8 = i % CH8;                                                                                 \
     int r = i / CH8;                                                                                        \
     const int xi = r % XW;                                                                                  \
     r /= XW;                                                                                                \
     const int row = r % RW, pl = r / RW;                                                                    \
-    const int zs = wrap(z + (BYPLANE ? (PH) : pl) - 1, n), ys = wrap(y0 - 1 + row, n);                      \
-    const int xx = wrap(x0 - 1 + xi, n);                                                                    \
-    pre[j] = *reinterpret_cast<const u32x4*>(&in[(((b * n + zs) * n + ys) * n + xx) * CIN + c8 * 8]);       \
+    const int zs = wrap((T).z + (BYPLANE ? (PH) : pl) - 1, n), ys = wrap((T).y0 - 1 + row, n);              \
+    const int xx = wrap((T).x0 - 1 + xi, n);                                                                \
+    const unsigned vox = ((unsigned((T).b) * n + zs) * n + ys) * n + xx;                                   \
+    pre[j] = reinterpret_cast<const u32x4*>(in)[vox * CH8 + c8];                                            \
   }
 #define CONV_COMMIT()                                                                                       \
-  _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                                       \
-    const int i = threadIdx.x + 256 * j;                                                                    \
+  _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                        \
+    const int i = threadIdx.x + 512 * j;                                                                    \
     if (i < NCH) *reinterpret_cast<u32x4*>(&win[(i / CH8) * CL + (i % CH8) * 8]) = pre[j];                  \
   }
-  CONV_FETCH(0)
+  if (long(blockIdx.x) >= ntiles) return;
+  const Tile cur = decode(blockIdx.x);
+  CONV_FETCH(cur, 0)
   CONV_COMMIT()
   __syncthreads();
-#pragma unroll
+#pragma unroll 1
   for (int ph = 0; ph < NPH; ++ph) {
+    // prefetch this tile's next input plane while this one is used
     if (ph + 1 < NPH) {
-      CONV_FETCH(ph + 1)
+      CONV_FETCH(cur, ph + 1)
     }
     const int kc0 = BYPLANE ? ph * (NKC / 3) : 0;
     const int kc1 = BYPLANE ? kc0 + NKC / 3 : NKC;
-#pragma unroll 2
-    for (int kc = kc0; kc < kc1; ++kc) {
-      bf16x8 a[MT], bv[NT];
-      load_a(kc, a);
-      load_b(kc, bv);
+    // software pipeline over k steps, two at a time (E = even, O = odd):
+    // B fragments are requested two steps ahead
+    bf16x8 aE[MT], bE[NT], bO[NT];
+    load_b(kc0, bE);
+    if (kc0 + 1 < kc1) load_b(kc0 + 1, bO);
+#pragma unroll 1
+    for (int kc = kc0; kc < kc1; kc += 2) {
+      const bool h1 = kc + 1 < kc1, h2 = kc + 2 < kc1, h3 = kc + 3 < kc1;
+      load_a(kc, aE);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv[nt], acc[mt][nt], 0, 0, 0);
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aE[mt], bE[nt], acc[mt][nt], 0, 0, 0);
+      if (h2) load_b(kc + 2, bE);
+      if (h1) {
+        load_a(kc + 1, aE);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aE[mt], bO[nt], acc[mt][nt], 0, 0, 0);
+        if (h3) load_b(kc + 3, bO);
+      }
     }
     if (ph + 1 < NPH) {
       __syncthreads();
@@ -182,8 +216,6 @@ __global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, co
       __syncthreads();
     }
   }
-#undef CONV_FETCH
-#undef CONV_COMMIT
   // epilogue: lane holds D[voxel = 16 mt + 4 lk + r of row w][co = 16 nt + li]
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
@@ -193,10 +225,12 @@ __global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, co
     if constexpr (MODE == HIDDEN) bco = bias[co];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      const long vrow = ((b * n + z) * n + y0 + w) * long(n) + x0 + 16 * mt + 4 * lk;
+      // voxel and element offsets fit in 32 bits (at most 2^31 elements per tensor)
+      const unsigned vrow =
+          ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + w) * unsigned(n) + cur.x0 + 16 * mt + 4 * lk;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long v = vrow + r;
+        const unsigned v = vrow + r;
         const float d = acc[mt][nt][r];
         if constexpr (MODE == HIDDEN) {
           out_z[v * COUTS + co] = bf16(d + bco);
@@ -209,6 +243,8 @@ __global__ void __launch_bounds__(256, 2) k_conv(const bf16* __restrict__ in, co
       }
     }
   }
+#undef CONV_FETCH
+#undef CONV_COMMIT
 }
 
 // 8 x 8 transpose of 16-bit elements held as 8 rows of 4 dwords:
@@ -261,7 +297,7 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
       xT[(s * CP + CIN) * XL + rem] = bf16(0.f);
     }
   }
-  float bsum = 0.f;
+  float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bias partial sums of this thread's dout channel block
   const bool dobias = db != nullptr && dz == 0;
   const int nb8 = n / 8;
   // transposed staging of one row: dout row (ro >= 0) and/or input row yi into slot
@@ -279,6 +315,13 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
           v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
         }
         dst = &dT[(cb * 8) * XL + xb * 8];
+        if (dobias) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+              bs[c] += __uint_as_float((c & 1) ? (v[e][c >> 1] & 0xffff0000u) : (v[e][c >> 1] << 16));
+        }
       } else {
         const int j = blk - nd;
         const int xb = j % (nb8 + 2), cb = j / (nb8 + 2);  // x from 8 (xb - 1)
@@ -309,8 +352,6 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
       __syncthreads();
       stage(q * n + y, pin * n + (y + 1) % n, (y + 1) % 3);
       __syncthreads();
-      if (dobias && threadIdx.x < COUT)
-        for (int x = 0; x < n; ++x) bsum += float(dT[threadIdx.x * XL + x]);
       if (w >= NW) continue;
       const int sl[3] = {(y + 2) % 3, y % 3, (y + 1) % 3};
 #pragma unroll 1
@@ -345,7 +386,22 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
       }
     }
   }
-  if (dobias && threadIdx.x < COUT) atomicAdd(&db[threadIdx.x], bsum);
+  if (dobias) {
+    // the dout blocks are the first nb8 * COUT / 8 of every row, the same for
+    // each thread on every row: reduce the 8-channel sums over the x blocks
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(xT);  // >= 8 KB for every CIN
+    const int nd = nb8 * (COUT / 8);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) red[threadIdx.x * 8 + c] = threadIdx.x < nd ? bs[c] : 0.f;
+    __syncthreads();
+    if (threadIdx.x < COUT) {
+      const int cb = threadIdx.x / 8, c = threadIdx.x % 8;
+      float t = 0.f;
+      for (int xb = 0; xb < nb8; ++xb) t += red[(cb * nb8 + xb) * 8 + c];
+      atomicAdd(&db[threadIdx.x], t);
+    }
+  }
   if (w >= NW) return;
   // D[co = 16 m + 4 lk + r][ci = 16 w + li]
   const int ci = 16 * w + li;
@@ -571,14 +627,15 @@ int cguard(sc_cnn* c, F&& f) {
 void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const bf16* W, const float* bias,
                  bf16* oa, bf16* oz, float* of, const bf16* zp, int batch) {
   const long rows = long(batch) * c->n * c->n;
-  dim3 grid(unsigned(long(batch) * c->n * (c->n / TROWS) * (c->n / VPB)));
+  const long ntiles = long(batch) * c->n * (c->n / TROWS) * (c->n / VPB);
+  const dim3 grid{unsigned(ntiles)};
   static const char* names[4] = {"conv_fwd", "conv_last", "conv_bwd_data", "conv_bwd_input"};
   const double flop = 2.0 * double(rows) * c->n * kpad(cinp) * ((coutp + 15) / 16 * 16);
 #define CONV(CI, COT, COS, MO)                                                                             \
   if (cinp == CI && coutp == COS && mode == MO) {                                                         \
     c->pbeg(names[MO], flop);                                                                             \
-    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(256), 0, c->stream, in, W, bias, oa, oz, of, zp, \
-                       c->n);                                                                             \
+    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(512), 0, c->stream, in, W, bias, oa, oz, of, zp, \
+                       c->n, ntiles);                                                                     \
     CNN_HIP(hipGetLastError());                                                                           \
     c->pend();                                                                                            \
     return;                                                                                               \
