@@ -1,5 +1,6 @@
 // Host-side launchers of the typed operations (included once per precision).
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>

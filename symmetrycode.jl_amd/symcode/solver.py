@@ -335,3 +335,20 @@ def lincomb(dst, x, a, y, b, g: Grid):
     """dst = a x + b y over all components (spectral; dst may alias x or y)."""
     _c(lib().sc_lincomb(g.ctx, _p(dst), _p(x), a, _p(y), b, ncomp_of(x)), g)
     return dst
+
+
+def profile(g: Grid, enable=True):
+    """start (clearing the record) or stop per-launch HIP-event timing of the
+    library's kernels on this grid's context"""
+    _c(lib().sc_profile(g.ctx, 1 if enable else 0), g)
+
+
+def profile_report(g: Grid):
+    """{kernel: (launches, total ms)} recorded since profile(g, True)"""
+    buf = ctypes.create_string_buffer(1 << 16)
+    _c(lib().sc_profile_report(g.ctx, buf, len(buf)), g)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, tot = line.rsplit(" ", 2)
+        out[name] = (int(cnt), float(tot))
+    return out
