@@ -201,7 +201,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU slab/pencil sharding is "
-                                   "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "not built (one GPU per box)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch

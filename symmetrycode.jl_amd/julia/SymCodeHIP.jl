@@ -221,4 +221,96 @@ function S.wray3!(::typeof(S.les!), u::HIPField, Δt, grid::HIPGrid, cache; mode
     u
 end
 
+# ------------------------------------------------------------ learned closure
+"The BF16-MFMA 3x3x3 circular CNN (sc_cnn_*); parameters stay on the device."
+mutable struct CNN
+    ptr::Ptr{Cvoid}
+    n::Int
+end
+function CNN(n; batch = 1, cin = 4, width = 64, cout = 6, seed = 0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:sc_cnn_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Ref{Ptr{Cvoid}}),
+               n, batch, cin, width, cout, seed, out)
+    rc == 0 || error("sc_cnn_create failed ($rc)")
+    net = CNN(out[], n)
+    finalizer(x -> ccall((:sc_cnn_destroy, libsymcode), Cint, (Ptr{Cvoid},), x.ptr), net)
+    net
+end
+
+"""
+The network as an LES model, the counterpart of `fullchain`
+(src/nets.jl:311-334): tau = s_out * net(u_bar / s_in, z(Re_Delta)).
+`les!` and the RK4 steps take it like a built-in closure (kind 4).
+"""
+struct CNNClosure
+    net::CNN
+    s_in::Cdouble
+    s_out::Cdouble
+    extra::Cdouble
+end
+const LESModel = Union{BuiltinClosure,CNNClosure}
+kind(m::BuiltinClosure) = m.kind
+kind(::CNNClosure) = Cint(4)
+kind(::Nothing) = Cint(0)
+attach(m, ctx) = nothing
+attach(m::CNNClosure, ctx) = check(ccall((:sc_set_cnn_closure, libsymcode), Cint,
+                                         (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble),
+                                         ctx.ptr, m.net.ptr, m.s_in, m.s_out, m.extra), ctx.ptr)
+
+function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::CNNClosure, visc)
+    attach(model, u.ctx)
+    check(ccall((:sc_les_rhs, libsymcode), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
+                u.ctx.ptr, du.ptr, u.ptr, visc, 4, 0, 0), u.ctx.ptr)
+end
+
+# ------------------------------------------------------------ adjoint
+# Differentiable RK4 step for the training loop (src/training.jl:45-129):
+# out-of-place, fixed dt, optional closure (none or a CNNClosure).  The
+# pullback is the hand-written adjoint (sc_rk4_vjp / sc_les_rk4_vjp) under
+# the RFFT-weighted inner product of spectraldot (src/solver.jl:708); with a
+# CNNClosure the network's parameter gradient accumulates on the device
+# (sc_cnn_zero_grad / sc_cnn_get_grads / sc_cnn_adam).
+using ChainRulesCore: ChainRulesCore, NoTangent, unthunk
+
+function copyfield(u::HIPField{T}) where {T}
+    v = HIPField{T}(u.ctx, u.ncomp, u.spectral)
+    bytes = u.ncomp * (u.spectral ? 2 * u.ctx.spec_elems : u.ctx.phys_elems) * sizeof(T)
+    ccall((:hipMemcpy, "libamdhip64"), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), v.ptr, u.ptr, bytes, 3)
+    v
+end
+
+function rk4_step(u::HIPField, Δt, grid::HIPGrid; visc, model = nothing)
+    v = copyfield(u)
+    if model === nothing
+        check(ccall((:sc_rk4, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble),
+                    u.ctx.ptr, v.ptr, Δt, visc), u.ctx.ptr)
+    else
+        attach(model, u.ctx)
+        check(ccall((:sc_les_rk4, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cdouble, Cdouble),
+                    u.ctx.ptr, v.ptr, Δt, visc, kind(model), 0, 0), u.ctx.ptr)
+    end
+    v
+end
+
+function ChainRulesCore.rrule(::typeof(rk4_step), u::HIPField, Δt, grid::HIPGrid; visc, model = nothing)
+    v = rk4_step(u, Δt, grid; visc, model)
+    function rk4_step_pullback(v̄)
+        w = unthunk(v̄)
+        ū = HIPField{eltype(u)}(u.ctx, u.ncomp, true)
+        if model === nothing
+            check(ccall((:sc_rk4_vjp, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble),
+                        u.ctx.ptr, ū.ptr, u.ptr, w.ptr, Δt, visc), u.ctx.ptr)
+        else
+            attach(model, u.ctx)
+            check(ccall((:sc_les_rk4_vjp, libsymcode), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint),
+                        u.ctx.ptr, ū.ptr, u.ptr, w.ptr, Δt, visc, kind(model)), u.ctx.ptr)
+        end
+        return NoTangent(), ū, NoTangent(), NoTangent()
+    end
+    v, rk4_step_pullback
+end
+Base.eltype(::HIPField{T}) where {T} = T
+
 end # module
