@@ -32,6 +32,10 @@ ALG_C = {
     "c2c_y_fwd": 12, "c2c_z_fwd": 12, "(k_div_visc<T, 3>)": 12,
 }
 RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates)
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at
+# 810^3 FP64, calibrated against a 4 GB device copy (FETCH x 2, WRITE x 1);
+# profiles/rk4_810_fp64_counters.md holds the table and the commands
+MEASURED_GB_810_FP64 = {"x_products": 34.18, "y_fwd_div": 51.33, "z_fwd_rhs": 51.42}
 
 
 def cpu_baseline(n_sample=256, threads=None):
@@ -170,7 +174,7 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
         Lm = T.rollout_loss(u0, refs, dt, g32, visc, m, states)
         torch.cuda.synchronize()
         rollouts[name] = {"rollouts_per_s": 1.0 / (time.perf_counter() - t0),
-                          "rel_traj_mse": Lm / ref_norm,
+                          "rel_traj_mse": Lm / ref_norm if ref_norm > 0 else None,
                           "spectrum_err_final": spectrum_error(states[-1], refs[-1], g32)}
     les_grad = {
         "value": 1.0 / t_grad, "unit": "gradient steps/s", "ms_per_grad_step": t_grad * 1e3,
@@ -275,7 +279,9 @@ def main():
                                f"nu=2.7e-4", "grid": n, "scheme": "rk4", "precision": args.prec},
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM,
+            "traffic": MEASURED_GB_810_FP64.get(dom) if (n == 810 and args.prec == 64) else None,
+            "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "alg_bytes_per_launch": ALG_C[dom] * C, "avg_launch_ms": avg_s * 1e3,
         },
         "step_roofline": {
