@@ -30,6 +30,7 @@
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -69,6 +70,10 @@ constexpr int TROWS = 8;  // output y rows per block (one per wavefront)
 constexpr int PADC = SC_CNN_PADC;  // LDS channel padding (multiple of 8) against bank conflicts
 
 constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
+
+#ifndef SC_CNN_BDIST
+#define SC_CNN_BDIST 4  // k steps between a B-fragment request and its use (2 or 4)
+#endif
 
 // One convolution layer (or its data gradient) as an implicit GEMM with the
 // reduction index k = t * CIN + c (tap-major; narrow layers pack several taps
@@ -185,6 +190,32 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
     }
     const int kc0 = BYPLANE ? ph * (NKC / 3) : 0;
     const int kc1 = BYPLANE ? kc0 + NKC / 3 : NKC;
+#if SC_CNN_BDIST == 4
+    // software pipeline over k steps, four at a time: B fragments are
+    // requested four steps ahead, A fragments one step ahead
+    bf16x8 aE[MT], aO[MT], b0[NT], b1[NT], b2[NT], b3[NT];
+    load_b(kc0, b0);
+    if (kc0 + 1 < kc1) load_b(kc0 + 1, b1);
+    if (kc0 + 2 < kc1) load_b(kc0 + 2, b2);
+    if (kc0 + 3 < kc1) load_b(kc0 + 3, b3);
+    load_a(kc0, aE);
+#define CONV_STEP(A_CUR, A_NXT, B_CUR, J)                                                                   \
+    if (kc + J < kc1) {                                                                                       \
+      if (kc + J + 1 < kc1) load_a(kc + J + 1, A_NXT);                                                       \
+      _Pragma("unroll") for (int mt = 0; mt < MT; ++mt)                                                      \
+          _Pragma("unroll") for (int nt = 0; nt < NT; ++nt)                                                  \
+              acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B_CUR[nt], A_CUR[mt], acc[mt][nt], 0, 0, 0); \
+      if (kc + J + 4 < kc1) load_b(kc + J + 4, B_CUR);                                                       \
+    }
+#pragma unroll 1
+    for (int kc = kc0; kc < kc1; kc += 4) {
+      CONV_STEP(aE, aO, b0, 0)
+      CONV_STEP(aO, aE, b1, 1)
+      CONV_STEP(aE, aO, b2, 2)
+      CONV_STEP(aO, aE, b3, 3)
+    }
+#undef CONV_STEP
+#else
     // software pipeline over k steps, two at a time (E = even, O = odd):
     // B fragments are requested two steps ahead
     bf16x8 aE[MT], bE[NT], bO[NT];
@@ -198,7 +229,7 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
       for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aE[mt], bE[nt], acc[mt][nt], 0, 0, 0);
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bE[nt], aE[mt], acc[mt][nt], 0, 0, 0);
       if (h2) load_b(kc + 2, bE);
       if (h1) {
         load_a(kc + 1, aE);
@@ -206,40 +237,53 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
           for (int nt = 0; nt < NT; ++nt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aE[mt], bO[nt], acc[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bO[nt], aE[mt], acc[mt][nt], 0, 0, 0);
         if (h3) load_b(kc + 3, bO);
       }
     }
+#endif
     if (ph + 1 < NPH) {
       __syncthreads();
       CONV_COMMIT()
       __syncthreads();
     }
   }
-  // epilogue: lane holds D[voxel = 16 mt + 4 lk + r of row w][co = 16 nt + li]
+  // epilogue: with the weights as the MFMA's A operand, lane (li, lk) holds
+  // D[co = 16 nt + 4 lk + r][voxel = 16 mt + li]: four consecutive channels of
+  // one voxel, stored (and, for the data gradient, the pre-activation read)
+  // as one 8- or 16-byte access
+  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + w) * unsigned(n) + cur.x0 + li;
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
-    const int co = nt * 16 + li;
-    if (co >= COUTS) continue;
-    float bco = 0.f;
-    if constexpr (MODE == HIDDEN) bco = bias[co];
+    const int co0 = nt * 16 + 4 * lk;
+    if (co0 >= COUTS) continue;
+    float bco[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (MODE == HIDDEN) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bco[r] = bias[co0 + r];
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       // voxel and element offsets fit in 32 bits (at most 2^31 elements per tensor)
-      const unsigned vrow =
-          ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + w) * unsigned(n) + cur.x0 + 16 * mt + 4 * lk;
+      const unsigned v = vrow0 + 16 * mt;
+      const floatx4 d = acc[mt][nt];
+      if constexpr (MODE == HIDDEN) {
+        bf16x4 zz, aa;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const unsigned v = vrow + r;
-        const float d = acc[mt][nt][r];
-        if constexpr (MODE == HIDDEN) {
-          out_z[v * COUTS + co] = bf16(d + bco);
-          out_a[v * COUTS + co] = bf16(gelu(d + bco));
-        } else if constexpr (MODE == BWD_HIDDEN) {
-          out_a[v * COUTS + co] = bf16(d * gelu_grad(float(zprev[v * COUTS + co])));
-        } else {
-          out_f[v * COUTS + co] = d;
+        for (int r = 0; r < 4; ++r) {
+          zz[r] = bf16(d[r] + bco[r]);
+          aa[r] = bf16(gelu(d[r] + bco[r]));
         }
+        *reinterpret_cast<bf16x4*>(&out_z[v * COUTS + co0]) = zz;
+        *reinterpret_cast<bf16x4*>(&out_a[v * COUTS + co0]) = aa;
+      } else if constexpr (MODE == BWD_HIDDEN) {
+        const bf16x4 zp = *reinterpret_cast<const bf16x4*>(&zprev[v * COUTS + co0]);
+        bf16x4 o;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) o[r] = bf16(d[r] * gelu_grad(float(zp[r])));
+        *reinterpret_cast<bf16x4*>(&out_a[v * COUTS + co0]) = o;
+      } else {
+        *reinterpret_cast<floatx4*>(&out_f[v * COUTS + co0]) = d;
       }
     }
   }
