@@ -308,26 +308,32 @@ __device__ __forceinline__ void transpose8x8(const uint32_t in[8][4], uint32_t o
 //   dW[t][co][ci] += sum_v dout[v][co] * in[v + (dz,dy,dx)][ci]
 // as GEMMs with M = co, N = ci, K = the voxels of an x row.  Persistent
 // blocks: blockIdx.y = dz + 1, and block p takes the (batch, z) planes p,
-// p + gridDim.x, ... whole, walking y with a 3-row sliding window of the
-// input plane z + dz, so each block keeps all nine (dy, dx) taps of its dz in
-// registers (wavefront w owns input channels 16 w .. 16 w + 15).  Rows are
-// staged transposed ([channel][x], 8 x 8 register transposes); the input row
-// carries an 8-voxel periodic halo on both sides, and the x shifts dx = -1, +1
-// are formed in registers from three aligned 16-byte reads.  Block sums go to
-// part[p][t][co][ci] (k_reduce_partials adds them up); with db set, the
-// dz = 0 blocks also add the bias gradient sum_v dout[v][co].
+// p + gridDim.x, ... whole, walking y with a sliding window of the input
+// plane z + dz, so each block keeps all nine (dy, dx) taps of its dz in
+// registers.  Block = 8 wavefronts: wavefronts 0-3 compute (wavefront w owns
+// input channels 16 w .. 16 w + 15), wavefronts 4-7 stage the next dout row
+// and the next input row into the free LDS slots meanwhile (one barrier per
+// row).  Rows are staged transposed ([channel][x], 8 x 8 register
+// transposes); input rows carry an 8-voxel periodic halo on both sides, and
+// the x shifts dx = -1, +1 are formed in registers from aligned LDS reads.
+// Block sums go to part[p][t][co][ci] (k_reduce_partials adds them up); with
+// db set, the dz = 0 blocks also add the bias gradient sum_v dout[v][co].
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ in, const bf16* __restrict__ dout,
+__global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ in, const bf16* __restrict__ dout,
                                                        float* part, float* db, int n, long nplanes) {
-  constexpr int XL = 128 + 16;                 // row length (n <= 128) + halo
+  // row strides in bf16 elements: 136 (272 B) for dout rows (n <= 128) and
+  // 152 (304 B) for input rows with their halos
+  constexpr int XD = 128 + 8, XL = 128 + 16 + 8;
   constexpr int CP = CIN < 16 ? 16 : CIN;      // staged input channels (zero-padded)
   constexpr int MT = COUT / 16;
-  constexpr int NW = CP / 16;                  // wavefronts with input channels
-  __shared__ __attribute__((aligned(16))) bf16 dT[COUT * XL];
-  __shared__ __attribute__((aligned(16))) bf16 xT[3 * CP * XL];
+  constexpr int NW = CP / 16;                  // compute wavefronts with input channels
+  __shared__ __attribute__((aligned(16))) bf16 dT[2 * COUT * XD];  // dout rows y, y + 1
+  __shared__ __attribute__((aligned(16))) bf16 xT[4 * CP * XL];    // input rows y - 1 .. y + 2
   const int dz = int(blockIdx.y) - 1;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
+  const bool producer = w >= 4;
+  const int pt = threadIdx.x - 256;  // producer thread index
   floatx4 acc[3][3][MT];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -336,21 +342,24 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
 #pragma unroll
       for (int m = 0; m < MT; ++m) acc[a][d][m] = floatx4{0, 0, 0, 0};
   if (CIN < CP) {  // zero channel rows that are never loaded
-    for (int i = threadIdx.x; i < 3 * (CP - CIN) * XL; i += 256) {
+    for (int i = threadIdx.x; i < 4 * (CP - CIN) * XL; i += 512) {
       const int s = i / ((CP - CIN) * XL), rem = i % ((CP - CIN) * XL);
       xT[(s * CP + CIN) * XL + rem] = bf16(0.f);
     }
   }
-  float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bias partial sums of this thread's dout channel block
+  float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bias partial sums of this producer's dout channel block
   const bool dobias = db != nullptr && dz == 0;
   const int nb8 = n / 8;
-  // transposed staging of one row: dout row (ro >= 0) and/or input row yi into slot
-  auto stage = [&](long drow, long xrow, int slot) {
-    const int nd = drow >= 0 ? nb8 * (COUT / 8) : 0;
+  const int nd_row = nb8 * (COUT / 8);
+  // transposed staging by the producer threads: dout row drow (if >= 0) into
+  // dT slot ds, and input row xrow (if >= 0) into xT slot xs
+  auto stage = [&](long drow, int ds, long xrow, int xs) {
+    const int nd = drow >= 0 ? nd_row : 0;
     const int nx = xrow >= 0 ? (nb8 + 2) * (CIN / 8) : 0;
-    for (int blk = threadIdx.x; blk < nd + nx; blk += 256) {
+    for (int blk = pt; blk < nd + nx; blk += 256) {
       uint32_t v[8][4], o[8][4];
       bf16* dst;
+      int rs;
       if (blk < nd) {
         const int xb = blk % nb8, cb = blk / nb8;
 #pragma unroll
@@ -358,7 +367,8 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
           const uint4 q = *reinterpret_cast<const uint4*>(&dout[(drow * n + xb * 8 + e) * COUT + cb * 8]);
           v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
         }
-        dst = &dT[(cb * 8) * XL + xb * 8];
+        dst = &dT[(ds * COUT + cb * 8) * XD + xb * 8];
+        rs = XD;
         if (dobias) {
 #pragma unroll
           for (int e = 0; e < 8; ++e)
@@ -371,73 +381,83 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
         const int xb = j % (nb8 + 2), cb = j / (nb8 + 2);  // x from 8 (xb - 1)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const int xs = (8 * (xb - 1) + e + n) % n;
-          const uint4 q = *reinterpret_cast<const uint4*>(&in[(xrow * n + xs) * CIN + cb * 8]);
+          const int xsrc = wrap(8 * (xb - 1) + e, n);
+          const uint4 q = *reinterpret_cast<const uint4*>(&in[(xrow * n + xsrc) * CIN + cb * 8]);
           v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
         }
-        dst = &xT[(slot * CP + cb * 8) * XL + xb * 8];
+        dst = &xT[(xs * CP + cb * 8) * XL + xb * 8];
+        rs = XL;
       }
       transpose8x8(v, o);
 #pragma unroll
       for (int c = 0; c < 8; ++c)
-        *reinterpret_cast<uint4*>(dst + c * XL) = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+        *reinterpret_cast<uint4*>(dst + c * rs) = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
     }
   };
+  // slot of input row r (r in [-1, n]): (r + 1) & 3
 #pragma unroll 1
   for (long q = blockIdx.x; q < nplanes; q += gridDim.x) {
     const int z = int(q % n);
     const long b = q / n;
-    const long pin = b * n + (z + dz + n) % n;  // input plane
+    const long pin = b * n + wrap(z + dz, n);  // input plane
     __syncthreads();
-    stage(-1, pin * n + (n - 1), 2);  // row y = -1 -> slot 2
-    stage(-1, pin * n + 0, 0);        // row y = 0  -> slot 0
+    if (producer) {
+      stage(q * n + 0, 0, pin * n + (n - 1), 0);  // dout row 0; input row -1
+      stage(-1, 0, pin * n + 0, 1);               // input row 0
+      stage(-1, 0, pin * n + (1 % n), 2);         // input row 1
+    }
+    __syncthreads();
 #pragma unroll 1
     for (int y = 0; y < n; ++y) {
-      __syncthreads();
-      stage(q * n + y, pin * n + (y + 1) % n, (y + 1) % 3);
-      __syncthreads();
-      if (w >= NW) continue;
-      const int sl[3] = {(y + 2) % 3, y % 3, (y + 1) % 3};
+      if (producer) {
+        // next dout row and input row y + 2 into the slots not in use
+        if (y + 1 < n) stage(q * n + y + 1, (y + 1) & 1, pin * n + (y + 2) % n, (y + 3) & 3);
+      } else if (w < NW) {
+        const int sl[3] = {y & 3, (y + 1) & 3, (y + 2) & 3};  // input rows y - 1, y, y + 1
+        const bf16* drow = &dT[((y & 1) * COUT) * XD];
 #pragma unroll 1
-      for (int k0 = 0; k0 < n; k0 += 32) {
-        const int x0 = k0 + 8 * lk;
-        bf16x8 av[MT];
+        for (int k0 = 0; k0 < n; k0 += 32) {
+          const int x0 = k0 + 8 * lk;
+          bf16x8 av[MT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8*>(&dT[(m * 16 + li) * XL + x0]);
+          for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8*>(&drow[(m * 16 + li) * XD + x0]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const bf16* xr = &xT[(sl[a] * CP + 16 * w + li) * XL + x0];
-          // dwords: m = (x0 - 2, x0 - 1), L0 = x0 .. x0 + 7, p = (x0 + 8, x0 + 9)
-          const uint32_t m = *reinterpret_cast<const uint32_t*>(xr + 6);
-          const uint4 L0 = *reinterpret_cast<const uint4*>(xr + 8);
-          const uint32_t p = *reinterpret_cast<const uint32_t*>(xr + 16);
-          uint4 S[3];
-          // dx = -1: x0 - 1 .. x0 + 6;  dx = 0: x0 .. x0 + 7;  dx = +1: x0 + 1 .. x0 + 8
-          S[0] = uint4{__builtin_amdgcn_alignbyte(L0.x, m, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
-                       __builtin_amdgcn_alignbyte(L0.z, L0.y, 2), __builtin_amdgcn_alignbyte(L0.w, L0.z, 2)};
-          S[1] = L0;
-          S[2] = uint4{__builtin_amdgcn_alignbyte(L0.y, L0.x, 2), __builtin_amdgcn_alignbyte(L0.z, L0.y, 2),
-                       __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(p, L0.w, 2)};
+          for (int a = 0; a < 3; ++a) {
+            const bf16* xr = &xT[(sl[a] * CP + 16 * w + li) * XL + x0];
+            // dwords: m = (x0 - 2, x0 - 1), L0 = x0 .. x0 + 7, p = (x0 + 8, x0 + 9)
+            const uint32_t mm = *reinterpret_cast<const uint32_t*>(xr + 6);
+            const uint4 L0 = *reinterpret_cast<const uint4*>(xr + 8);
+            const uint32_t pp = *reinterpret_cast<const uint32_t*>(xr + 16);
+            uint4 S[3];
+            // dx = -1: x0 - 1 .. x0 + 6;  dx = 0: x0 .. x0 + 7;  dx = +1: x0 + 1 .. x0 + 8
+            S[0] = uint4{__builtin_amdgcn_alignbyte(L0.x, mm, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
+                         __builtin_amdgcn_alignbyte(L0.z, L0.y, 2), __builtin_amdgcn_alignbyte(L0.w, L0.z, 2)};
+            S[1] = L0;
+            S[2] = uint4{__builtin_amdgcn_alignbyte(L0.y, L0.x, 2), __builtin_amdgcn_alignbyte(L0.z, L0.y, 2),
+                         __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(pp, L0.w, 2)};
 #pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            bf16x8 bv;
-            __builtin_memcpy(&bv, &S[d], 16);
+            for (int d = 0; d < 3; ++d) {
+              bf16x8 bv;
+              __builtin_memcpy(&bv, &S[d], 16);
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
-              acc[a][d][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[a][d][m], 0, 0, 0);
+              for (int m = 0; m < MT; ++m)
+                acc[a][d][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[a][d][m], 0, 0, 0);
+            }
           }
         }
       }
+      __syncthreads();
     }
   }
   if (dobias) {
-    // the dout blocks are the first nb8 * COUT / 8 of every row, the same for
-    // each thread on every row: reduce the 8-channel sums over the x blocks
+    // a producer's dout block is the same on every row: reduce the 8-channel
+    // sums over the x blocks
     __syncthreads();
     float* red = reinterpret_cast<float*>(xT);  // >= 8 KB for every CIN
-    const int nd = nb8 * (COUT / 8);
+    if (producer) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) red[threadIdx.x * 8 + c] = threadIdx.x < nd ? bs[c] : 0.f;
+      for (int c = 0; c < 8; ++c) red[pt * 8 + c] = pt < nd_row ? bs[c] : 0.f;
+    }
     __syncthreads();
     if (threadIdx.x < COUT) {
       const int cb = threadIdx.x / 8, c = threadIdx.x % 8;
@@ -446,7 +466,7 @@ __global__ void __launch_bounds__(256, 2) k_conv_wgrad(const bf16* __restrict__ 
       atomicAdd(&db[threadIdx.x], t);
     }
   }
-  if (w >= NW) return;
+  if (producer || w >= NW) return;
   // D[co = 16 m + 4 lk + r][ci = 16 w + li]
   const int ci = 16 * w + li;
   if (ci >= CIN) return;
@@ -693,7 +713,7 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
 void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* dout, float* grad, float* db,
                   int batch) {
   const long nplanes = long(batch) * c->n;
-  const long G = std::min<long>(nplanes, std::max(1, 2 * c->ncu / 3));
+  const long G = std::min<long>(nplanes, std::max(1, c->ncu / 3));  // one block per CU
   const long len = 27L * coutp * cinp;
   if (c->part_len < G * len) {
     if (c->part) CNN_HIP(hipFree(c->part));
@@ -704,7 +724,7 @@ void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* do
   c->pbeg("conv_wgrad", 2.0 * double(nplanes) * c->n * c->n * 27.0 * ((cinp + 15) / 16 * 16) * coutp);
 #define WG(CI, CO)                                                                                              \
   if (cinp == CI && coutp == CO) {                                                                               \
-    hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), grid, dim3(256), 0, c->stream, in, dout, c->part, db, c->n, nplanes); \
+    hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), grid, dim3(512), 0, c->stream, in, dout, c->part, db, c->n, nplanes); \
     CNN_HIP(hipGetLastError());                                                                                  \
   } else
   WG(8, 64) WG(64, 64) WG(64, 16) { throw std::runtime_error("unsupported wgrad shape"); }
