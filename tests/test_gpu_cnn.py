@@ -98,3 +98,29 @@ def test_cnn_pack_input():
     assert np.allclose(xc[:, :3], torch.tensor(ref).to(torch.bfloat16).float().numpy())
     assert np.all(xc[:, 3] == np.float32(torch.tensor(0.75).to(torch.bfloat16).float()))
     assert np.all(xc[:, 4:] == 0)
+
+
+def test_cnn_parity_n128():
+    """two 64-voxel x segments per row: forward and gradients at n = 128"""
+    print_case("cnn parity n128")
+    net = make(n=128, seed=5)
+    n = net.n
+    h = net.get_params()
+    Ws, bs = net.unflatten(h)
+    x = rand_input(net, 1, 6)
+    xt = torch.tensor(x).to(torch.bfloat16).cuda()
+    y = net.forward(xt)
+    yo, store = OC.forward(Ws, bs, torch.tensor(x), n)
+    assert rel(y.cpu().numpy(), yo.numpy()) < 1e-2
+    tgt = np.random.default_rng(7).standard_normal((1, net.nvox, 6)).astype(np.float32) * 0.1
+    _, dy = net.mse(y, torch.tensor(tgt).cuda())
+    _, dyo = OC.mse(yo, tgt, 6)
+    net.zero_grad()
+    dx = net.backward(xt, dy, want_dx=True)
+    gW, gb, dxo = OC.backward(store, dyo, n)
+    GW, GB = net.unflatten(net.get_grads())
+    for l in range(4):
+        assert rel(GW[l], gW[l].numpy()) < 3e-2, l
+    for l in range(3):
+        assert rel(GB[l], gb[l].numpy()) < 3e-2, l
+    assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
