@@ -88,8 +88,9 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     Data: a 512^3 FP64 forced DNS (nu = 2e-3, band forcing 0.1 on |k| <= 2,
     E(k) ~ k^4 exp(-2(k/4)^2)) filtered with Gaussian Delta in {2, 4} h_LES and
     cut to 128^3 at 8 save times: 16 (u_bar, tau) pairs.
-    Training: one Adam step on the batch of 16 = forward, MSE, backward
-    (data and weight gradients of all four 3x3x3 layers), Adam.
+    Training: one optimiser step on the batch of 16 with the reference's
+    semantics = forward (trace-free output), relative loss, backward (data
+    and weight gradients of all four 3x3x3 layers), ClipNorm(1) + AdamW.
     A-posteriori: 128^3 FP32 LES with the learned closure, 20 RK4 steps from
     the filtered DNS, trajectory MSE against the filtered DNS; one gradient
     step = rollout + hand-written adjoint (stages recomputed) + Adam."""
@@ -108,16 +109,17 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     data = T.create_data(u, g_dns, g_les, visc, [2, 4], 8, 0.02, cfl=cfl)
     pairs = data[2] + data[4]
     nrm = T.Normalization.of(pairs, g_les)
-    net = sc.cnn.ConvClosureNet(n_les, batch_max=16, cin=4, width=64, cout=6, seed=0)
+    net = sc.cnn.ConvClosureNet(n_les, batch_max=16, cin=4, width=64, cout=6, seed=0, tracefree=True)
+    step = lambda: net.train_step(x, y, 1e-3, loss="relative", clip=1.0)
     x, y = T.pack_pairs(net, g_les, pairs, nrm)
     log(f"closure data ready ({time.perf_counter() - t_setup:.1f} s)")
     # ---- a-priori training step, batch 16
-    net.train_step(x, y)
+    step()
     torch.cuda.synchronize()
     net.profile(True)
     t0 = time.perf_counter()
     for _ in range(train_steps):
-        loss = net.train_step(x, y)
+        loss = step()
     torch.cuda.synchronize()
     t_train = (time.perf_counter() - t0) / train_steps
     rep = net.profile_report()
@@ -130,7 +132,8 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     cnn_train = {
         "value": 1.0 / t_train, "unit": "training steps/s", "ms_per_step": t_train * 1e3,
         "workload": f"3x3x3 circular CNN 4->64->64->64->6 (GELU), batch 16 x {n_les}^3 filtered "
-                    f"{n_dns}^3 DNS pairs, BF16 MFMA / FP32 accumulate, Adam lr 1e-3, MSE",
+                    f"{n_dns}^3 DNS pairs, BF16 MFMA / FP32 accumulate, trace-free output, relative loss, "
+                    "ClipNorm(1) + AdamW lr 1e-3",
         "loss": loss,
         "conv_ms_per_step": conv_ms,
         "conv_tflops_executed": conv_flop / (conv_ms / 1e3) / 1e12,
@@ -175,7 +178,8 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
         torch.cuda.synchronize()
         rollouts[name] = {"rollouts_per_s": 1.0 / (time.perf_counter() - t0),
                           "rel_traj_mse": Lm / ref_norm if ref_norm > 0 else None,
-                          "spectrum_err_final": spectrum_error(states[-1], refs[-1], g32)}
+                          "spectrum_err_final": spectrum_error(states[-1], refs[-1], g32),
+                          "e_post_final": T.e_post(states[-1], refs[-1], g32)}
     les_grad = {
         "value": 1.0 / t_grad, "unit": "gradient steps/s", "ms_per_grad_step": t_grad * 1e3,
         "workload": f"{n_les}^3 FP32 LES with the learned closure, {nsteps} RK4 steps (dt={dt:.3e}) from "

@@ -120,6 +120,10 @@ int sc_energy(sc_ctx* ctx, const void* u, int ncomp, double* e);          /* ene
 int sc_spectraldot(sc_ctx* ctx, const void* u, const void* v, int ncomp,
                    double* re, double* im);                                /* :708-709 */
 int sc_dissipation(sc_ctx* ctx, const void* u, double visc, double* eps);  /* get_dissipation!, :796-811 */
+/* sum over the stored half spectrum of |u|^2, no RFFT weights: the squared
+ * Julia norm of the rfft arrays, as e_post = norm(u - u_ref) / norm(u_ref)
+ * uses it (src/les.jl:97) */
+int sc_norm2(sc_ctx* ctx, const void* u, int ncomp, double* s);
 int sc_palinstrophy(sc_ctx* ctx, const void* u, double visc, double* p);   /* get_palinstrophy!, :813-827 */
 /* spectrum (:664-699): s[i-1] = shell i energy, i = 1..nshell */
 int sc_spectrum(sc_ctx* ctx, const void* u, int ncomp, int nshell, double* s);
@@ -206,9 +210,24 @@ int sc_cnn_pack_target(sc_cnn* net, const void* fields, int fp64, long cstride, 
 int sc_cnn_forward(sc_cnn* net, const void* x, void* y, int batch);
 /* MSE on the first cout channels; dy = d loss / d y as bf16 */
 int sc_cnn_mse(sc_cnn* net, const void* y, const void* target, void* dy, int batch, double* loss);
+/* loss of the first cout channels, dy = d loss / d y as bf16: kind 0 the MSE,
+ * kind 1 the reference's relative loss sum|y - t|^2 / (sum|t|^2 + eps(Float32))
+ * over the batch (create_loss, src/training.jl:3-9) */
+int sc_cnn_loss(sc_cnn* net, const void* y, const void* target, void* dy, int batch, int kind, double* loss);
 /* accumulate parameter gradients of the last forward; dx (nullable) = d loss / d x (fp32) */
 int sc_cnn_backward(sc_cnn* net, const void* x, const void* dy, void* dx, int batch);
 int sc_cnn_adam(sc_cnn* net, double lr, double beta1, double beta2, double eps);
+/* OptimiserChain(ClipNorm(clip), AdamW(lr, (beta1, beta2), weight_decay, eps))
+ * of the reference's train (src/training.jl:50; Optimisers.jl 0.4 rules): the
+ * gradient of each parameter array (4 conv weights, 3 biases) is scaled by
+ * min(1, clip / |g|) (clip <= 0: no clipping), then
+ * w -= lr (mhat / (sqrt(vhat) + eps) + weight_decay w).  The accumulated
+ * gradient is left as it is. */
+int sc_cnn_adamw(sc_cnn* net, double lr, double beta1, double beta2, double eps, double weight_decay, double clip);
+/* trace removal of the output (on != 0; cout must be 6): channels xx, yy, zz
+ * lose their mean, as the last layer of the reference's nets does
+ * (src/nets.jl:275-293); forward, backward and the closure follow it */
+int sc_cnn_set_tracefree(sc_cnn* net, int on);
 /* per-launch HIP-event timing: enable (clears the record), then a report of
  * lines "name count total_ms executed_flop" (flop = 2 * voxels * 27 * cin_p * cout_p) */
 int sc_cnn_profile(sc_cnn* net, int enable);

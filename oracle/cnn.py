@@ -77,10 +77,21 @@ def split_params(h, shapes, width):
     return Ws, bs
 
 
-def forward(Ws, bs, x, n, emulate=True, dtype=torch.float32):
+def tracefree(y, dim=1):
+    """trace removal of the packed stress channels (xx, yy, zz, xy, yz, zx):
+    the first three lose their mean (the `symm` layer, src/nets.jl:275-293).
+    The map is an orthogonal projection, so it is also its own transpose."""
+    y = y.clone()
+    d = y.narrow(dim, 0, 3)
+    d -= d.sum(dim=dim, keepdim=True) / 3
+    return y
+
+
+def forward(Ws, bs, x, n, emulate=True, dtype=torch.float32, tracefree_out=False):
     """x: (B, n^3, cin_p) float tensor.  Returns y (B, n^3, cout_p) and the
     stored tensors needed by backward.  ``dtype=torch.float64`` (with
-    emulate=False) gives the smooth FP64 network for finite differences."""
+    emulate=False) gives the smooth FP64 network for finite differences;
+    ``tracefree_out`` applies the trace removal to the output."""
     Wt = [bf(torch_weight(W).to(dtype), emulate) for W in Ws]
     a = to_ncdhw(bf(torch.as_tensor(x, dtype=dtype), emulate), n)
     acts, zs = [a], []
@@ -90,7 +101,9 @@ def forward(Ws, bs, x, n, emulate=True, dtype=torch.float32):
         a = bf(gelu(z), emulate)
         acts.append(a)
     y = conv(a, Wt[3])
-    return to_nvc(y), dict(Wt=Wt, acts=acts, zs=zs)
+    if tracefree_out:
+        y = tracefree(y)
+    return to_nvc(y), dict(Wt=Wt, acts=acts, zs=zs, tracefree=tracefree_out)
 
 
 def backward(store, dy, n, emulate=True):
@@ -98,6 +111,8 @@ def backward(store, dy, n, emulate=True):
     grads, dx (B, n^3, cin_p))."""
     Wt, acts, zs = store["Wt"], store["acts"], store["zs"]
     d = to_ncdhw(bf(torch.as_tensor(dy, dtype=Wt[0].dtype), emulate), n)
+    if store.get("tracefree"):
+        d = tracefree(d)
     gW, gb = [None] * 4, [None] * 3
     gW[3] = conv_weight_grad(acts[3], d, Wt[3].shape)
     d = bf(conv_data_grad(d, Wt[3]) * gelu_grad(zs[2]), emulate)
@@ -121,6 +136,19 @@ def mse(y, tgt, nout):
     dy = torch.zeros_like(y)
     dy[..., :nout] = 2 * e / N
     return float((e.double() ** 2).sum() / N), dy
+
+
+def rel_loss(y, tgt, nout):
+    """loss and dL/dy of the reference's relative loss over the batch,
+    sum|y - t|^2 / (sum|t|^2 + eps(Float32)) (create_loss,
+    src/training.jl:3-9), on the first nout channels"""
+    y = torch.as_tensor(y, dtype=torch.float32)
+    tgt = torch.as_tensor(tgt, dtype=torch.float32)
+    e = (y[..., :nout] - tgt).double()
+    den = float((tgt.double() ** 2).sum()) + float(np.finfo(np.float32).eps)
+    dy = torch.zeros_like(y)
+    dy[..., :nout] = (2 * e / den).float()
+    return float((e**2).sum() / den), dy
 
 
 def pointwise_mlp(Ws, bs, x):

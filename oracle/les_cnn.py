@@ -33,16 +33,16 @@ PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0))
 class Net:
     """the network's parameters and the closure's scales"""
 
-    def __init__(self, Ws, bs, cin, s_in, s_out, extra=0.0, emulate=True, dtype=torch.float32):
+    def __init__(self, Ws, bs, cin, s_in, s_out, extra=0.0, emulate=True, dtype=torch.float32, tracefree=False):
         self.Ws, self.bs, self.cin = Ws, bs, cin
         self.cin_p = Ws[0].shape[2]
         self.s_in, self.s_out, self.extra = s_in, s_out, extra
-        self.emulate, self.dtype = emulate, dtype
+        self.emulate, self.dtype, self.tracefree = emulate, dtype, tracefree
 
     def perturbed(self, dW, db, eps):
         Ws = [W + eps * d for W, d in zip(self.Ws, dW)]
         bs = [b + eps * d for b, d in zip(self.bs, db)]
-        return Net(Ws, bs, self.cin, self.s_in, self.s_out, self.extra, self.emulate, self.dtype)
+        return Net(Ws, bs, self.cin, self.s_in, self.s_out, self.extra, self.emulate, self.dtype, self.tracefree)
 
 
 def _input(u, g: S.Grid, net: Net):
@@ -59,7 +59,8 @@ def _input(u, g: S.Grid, net: Net):
 def cnn_tau(u, g: S.Grid, net: Net):
     """tau (6, n, n, n) and the network's stored forward state"""
     n = g.n
-    y, store = OC.forward(net.Ws, net.bs, torch.as_tensor(_input(u, g, net)), n, net.emulate, net.dtype)
+    y, store = OC.forward(net.Ws, net.bs, torch.as_tensor(_input(u, g, net)), n, net.emulate, net.dtype,
+                          tracefree_out=net.tracefree)
     y = y.double().numpy()
     tau = np.stack([net.s_out * y[0, :, c].reshape(n, n, n) for c in range(6)])
     return tau, store

@@ -344,6 +344,9 @@ int sc_energy(sc_ctx* c, const void* u, int ncomp, double* e) {
 int sc_spectraldot(sc_ctx* c, const void* u, const void* v, int ncomp, double* re, double* im) {
   return guard(c, [&] { TYPED(c, sc::Ops<T>::dot_modes(*c, (const CT*)u, (const CT*)v, ncomp, re, im)); });
 }
+int sc_norm2(sc_ctx* c, const void* u, int ncomp, double* s) {
+  return guard(c, [&] { TYPED(c, *s = sc::Ops<T>::sum_modes(*c, (const CT*)u, ncomp, 3, 0.0)); });
+}
 int sc_dissipation(sc_ctx* c, const void* u, double nu, double* eps) {
   return guard(c, [&] { TYPED(c, *eps = sc::Ops<T>::sum_modes(*c, (const CT*)u, c->D, 1, nu)); });
 }

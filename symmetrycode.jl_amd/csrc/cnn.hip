@@ -19,6 +19,7 @@
 // staged in LDS; weights stream from L2.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -87,13 +88,15 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 // k_pack_weights), read through L1, two k steps ahead.  The next input plane
 // is fetched into registers while the current one's MFMAs run.
 //  HIDDEN:     z = conv + bias; out_z = z, out_a = gelu(z)            (bf16)
-//  LAST:       out_f = conv                                           (fp32)
+//  LAST:       out_f = conv, with tracefree: channels 0-2 (xx, yy, zz)
+//              minus their mean                                       (fp32)
 //  BWD_HIDDEN: out_a = conv * gelu'(zprev)                            (bf16)
 //  BWD_INPUT:  out_f = conv (first COUTS channels)                    (fp32)
 template <int CIN, int COUTT, int COUTS, int MODE>
 __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
                                                  const float* __restrict__ bias, bf16* out_a, bf16* out_z,
-                                                 float* out_f, const bf16* __restrict__ zprev, int n, long ntiles) {
+                                                 float* out_f, const bf16* __restrict__ zprev, int n, long ntiles,
+                                                 int tracefree) {
   constexpr int CL = CIN + PADC;
   constexpr int RW = TROWS + 2, XW = VPB + 2;
   constexpr bool BYPLANE = CIN >= 32;
@@ -283,7 +286,17 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
         for (int r = 0; r < 4; ++r) o[r] = bf16(d[r] * gelu_grad(float(zp[r])));
         *reinterpret_cast<bf16x4*>(&out_a[v * COUTS + co0]) = o;
       } else {
-        *reinterpret_cast<floatx4*>(&out_f[v * COUTS + co0]) = d;
+        floatx4 o = d;
+        if constexpr (MODE == LAST) {
+          // trace removal: lane lk = 0 of tile nt = 0 holds xx, yy, zz of its voxel
+          if (tracefree && co0 == 0) {
+            const float t = (o[0] + o[1] + o[2]) / 3.f;
+            o[0] -= t;
+            o[1] -= t;
+            o[2] -= t;
+          }
+        }
+        *reinterpret_cast<floatx4*>(&out_f[v * COUTS + co0]) = o;
       }
     }
   }
@@ -524,16 +537,92 @@ __global__ void k_mse(const float* y, const float* tgt, bf16* dy, float* dyf, do
 }
 
 // Adam on FP32 master weights; refreshes the two BF16 copies.
-__global__ void k_adam(float* w, const float* g, float* m, float* v, long len, float lr, float b1, float b2,
-                       float eps, float c1, float c2) {
+// Boundaries of the seven parameter arrays in the flat parameter vector
+// (four conv weights, three biases): array a is [b[a], b[a + 1])
+struct Arrays { long b[8]; };
+
+// sum of squares of x[0, len) in block partials (FP64)
+__global__ void k_sumsq(const float* x, long len, double* part) {
+  __shared__ double sm[256];
+  double acc = 0;
+  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += long(gridDim.x) * blockDim.x) {
+    const double e = x[i];
+    acc += e * e;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// squared L2 norm of each parameter array's gradient: block (j, a) sums a
+// strided share of array a into part[a][j]; k_finish_norms adds the shares
+constexpr int NORM_BLOCKS = 32;
+__global__ void k_array_sumsq(const float* g, Arrays A, double* part) {
+  __shared__ double sm[256];
+  const int a = blockIdx.y;
+  double acc = 0;
+  for (long i = A.b[a] + long(blockIdx.x) * blockDim.x + threadIdx.x; i < A.b[a + 1];
+       i += long(gridDim.x) * blockDim.x) {
+    const double e = g[i];
+    acc += e * e;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[a * NORM_BLOCKS + blockIdx.x] = sm[0];
+}
+__global__ void k_finish_norms(const double* part, double* nrm2) {
+  const int a = threadIdx.x;
+  if (a >= 7) return;
+  double s = 0;
+  for (int j = 0; j < NORM_BLOCKS; ++j) s += part[a * NORM_BLOCKS + j];
+  nrm2[a] = s;
+}
+
+// ClipNorm(clip) per parameter array (gradient scaled by min(1, clip / |g_a|);
+// clip <= 0 disables it) followed by AdamW, its weight decay scaled by the
+// learning rate: w -= lr (mhat / (sqrt(vhat) + eps) + wd w)
+__global__ void k_adamw(float* w, const float* g, float* m, float* v, long len, float lr, float b1, float b2,
+                        float eps, float wd, float c1, float c2, const double* nrm2, Arrays A, double clip) {
   long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= len) return;
   float gi = g[i];
+  if (clip > 0) {
+    int a = 0;
+    while (a < 6 && i >= A.b[a + 1]) ++a;
+    const double nrm = sqrt(nrm2[a]);
+    if (nrm > clip) gi *= float(clip / nrm);
+  }
+  const float wi = w[i];
   float mi = b1 * m[i] + (1 - b1) * gi;
   float vi = b2 * v[i] + (1 - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi - lr * ((mi / c1) / (sqrtf(vi / c2) + eps) + wd * wi);
+}
+
+// trace removal on the last layer's weight gradient: for every (tap, input
+// channel), rows co = 0, 1, 2 (xx, yy, zz) lose their mean.  The map is the
+// projection the forward pass applies to the output, and it is its own
+// transpose, so this turns the gradient of the raw weights' output into the
+// gradient through the projected output.  Idempotent: earlier (projected)
+// accumulations are unchanged.
+__global__ void k_tracefree_rows(float* g, int cop, int cip) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 27 * cip) return;
+  const int t = i / cip, ci = i % cip;
+  float* r = g + long(t) * cop * cip + ci;
+  const float m = (r[0] + r[cip] + r[2 * cip]) / 3.f;
+  r[0] -= m;
+  r[cip] -= m;
+  r[2 * cip] -= m;
 }
 
 // master [27][cop][cip] -> bf16 B fragments of the forward convolution,
@@ -541,12 +630,15 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, long len, f
 // of the backward-data convolution, W'[ci][k = t' * cop + co] = W[26 - t'][co][ci]
 // (taps mirrored).  Fragment layout: for k step kc and channel tile nt, lane
 // L = 16 j + i holds entries (row 16 nt + i, k = 32 kc + 8 j .. + 7), stored at
-// ((kc * NT + nt) * 64 + L) * 8; padding entries are zero.
+// ((kc * NT + nt) * 64 + L) * 8; padding entries are zero.  With tracefree
+// (last layer only) the backward-data weights are those of the projected
+// output: rows co = 0, 1, 2 minus their mean, so the data gradient is the
+// raw weights' transpose applied to the projected cotangent.
 __device__ __forceinline__ long frag_index(int row, int k, int nrows) {
   const int kc = k >> 5, j = (k >> 3) & 3, e = k & 7, nt = row >> 4, i = row & 15;
   return ((long(kc) * (nrows / 16) + nt) * 64 + 16 * j + i) * 8 + e;
 }
-__global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cop, int cip) {
+__global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cop, int cip, int tracefree) {
   const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
   const int kpf = kpad(cip), kpb = kpad(cop);
   const int rf = (cop + 15) / 16 * 16, rb = (cip + 15) / 16 * 16;
@@ -559,7 +651,13 @@ __global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cop, int 
     const long j = i - nf;
     const int ci = int(j / kpb), k = int(j % kpb);
     const int tp = k / cop, co = k % cop;
-    wb[frag_index(ci, k, rb)] = bf16(tp < 27 && ci < cip ? w[(long(26 - tp) * cop + co) * cip + ci] : 0.f);
+    float val = 0.f;
+    if (tp < 27 && ci < cip) {
+      const float* r = w + long(26 - tp) * cop * cip + ci;
+      val = r[long(co) * cip];
+      if (tracefree && co < 3) val -= (r[0] + r[cip] + r[2 * cip]) / 3.f;
+    }
+    wb[frag_index(ci, k, rb)] = bf16(val);
   }
 }
 
@@ -638,7 +736,9 @@ struct sc_cnn {
   long part_len = 0;
   double* lossp = nullptr;
   double* hostd = nullptr;
+  double* gnorm = nullptr;  // per-array gradient norm partials [7][NORM_BLOCKS] then the 7 squared norms
   int adam_t = 0;
+  bool tracefree = false;  // output channels 0-2 (xx, yy, zz) lose their mean (cout 6)
   int ncu = 256;  // compute units of the device (persistent wgrad grid)
   // per-launch HIP-event timing (sc_cnn_profile): name, events, executed flops
   struct Prof { const char* name; hipEvent_t a, b; double flop; };
@@ -699,7 +799,7 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
   if (cinp == CI && coutp == COS && mode == MO) {                                                         \
     c->pbeg(names[MO], flop);                                                                             \
     hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(512), 0, c->stream, in, W, bias, oa, oz, of, zp, \
-                       c->n, ntiles);                                                                     \
+                       c->n, ntiles, c->tracefree ? 1 : 0);                                              \
     CNN_HIP(hipGetLastError());                                                                           \
     c->pend();                                                                                            \
     return;                                                                                               \
@@ -744,7 +844,7 @@ void pack_all(sc_cnn* c) {
   for (int l = 0; l < 4; ++l) {
     const long len = wf_len(c, l) + wb_len(c, l);
     hipLaunchKernelGGL(k_pack_weights, dim3(unsigned((len + 255) / 256)), dim3(256), 0, c->stream, c->w + c->woff[l],
-                       c->wf[l], c->wb[l], layer_cout(c, l), layer_cin(c, l));
+                       c->wf[l], c->wb[l], layer_cout(c, l), layer_cin(c, l), (l == 3 && c->tracefree) ? 1 : 0);
     CNN_HIP(hipGetLastError());
   }
 }
@@ -768,8 +868,14 @@ void forward(sc_cnn* c, const void* x, float* y, int batch) {
 void backward(sc_cnn* c, const void* x, const void* dy, float* dx, int batch) {
   if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
   const bf16* dY = (const bf16*)dy;
-  // last layer: dW4, then the data gradient into dz[2] with gelu'(z3)
+  // last layer: dW4 (projected when the output is trace-free), then the data
+  // gradient into dz[2] with gelu'(z3)
   launch_wgrad(c, c->width, c->cout_p, c->a[2], dY, c->g + c->woff[3], nullptr, batch);
+  if (c->tracefree) {
+    hipLaunchKernelGGL(k_tracefree_rows, dim3(unsigned((27 * c->width + 255) / 256)), dim3(256), 0, c->stream,
+                       c->g + c->woff[3], c->cout_p, c->width);
+    CNN_HIP(hipGetLastError());
+  }
   launch_conv(c, c->cout_p, c->width, BWD_HIDDEN, dY, c->wb[3], nullptr, c->dz[2], nullptr, nullptr, c->z[2], batch);
   for (int l = 2; l >= 0; --l) {
     const bf16* inl = l == 0 ? (const bf16*)x : c->a[l - 1];
@@ -912,6 +1018,7 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
       CNN_HIP(hipMalloc(&c->dz[l], nv * width * 2));
     }
     CNN_HIP(hipMalloc(&c->lossp, ((nv + 255) / 256) * sizeof(double) + 16));
+    CNN_HIP(hipMalloc(&c->gnorm, (7 * NORM_BLOCKS + 8) * sizeof(double)));
     CNN_HIP(hipHostMalloc(&c->hostd, 64));
     pack_all(c);
     CNN_HIP(hipStreamSynchronize(c->stream));
@@ -927,7 +1034,7 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
 int sc_cnn_destroy(sc_cnn* c) {
   if (!c) return 0;
   (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->w, c->g, c->m, c->v, c->a0, c->y, c->dyb, c->dx, c->part, c->lossp};
+  void* ptrs[] = {c->w, c->g, c->m, c->v, c->a0, c->y, c->dyb, c->dx, c->part, c->lossp, c->gnorm};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   for (int l = 0; l < 4; ++l) { if (c->wf[l]) (void)hipFree(c->wf[l]); if (c->wb[l]) (void)hipFree(c->wb[l]); }
   for (int l = 0; l < 3; ++l) {
@@ -981,15 +1088,32 @@ int sc_cnn_backward(sc_cnn* c, const void* x, const void* dy, void* dx, int batc
   return cguard(c, [&] { cnnlink::backward(c, x, dy, (float*)dx, batch); });
 }
 
-// MSE loss of y (fp32 [batch][n^3][cout_p]) against tgt (fp32 [batch][n^3][cout]);
-// writes dy (bf16 [batch][n^3][cout_p]) and returns the loss
-int sc_cnn_mse(sc_cnn* c, const void* y, const void* tgt, void* dy, int batch, double* loss) {
+// Loss of y (fp32 [batch][n^3][cout_p]) against tgt (fp32 [batch][n^3][cout])
+// over the first cout channels; writes dy = dL/dy (bf16 [batch][n^3][cout_p])
+// and returns the loss.  kind 0: mean squared error; kind 1: the relative
+// error sum|y - t|^2 / (sum|t|^2 + eps_f32) of the whole batch.
+int sc_cnn_loss(sc_cnn* c, const void* y, const void* tgt, void* dy, int batch, int kind, double* loss) {
   return cguard(c, [&] {
+    if (kind != 0 && kind != 1) throw std::runtime_error("loss kind must be 0 (mse) or 1 (relative)");
+    if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
     const long nv = long(batch) * c->n * c->n * c->n;
     const unsigned nb = unsigned((nv + 255) / 256);
-    c->pbeg("mse", 0);
+    double inv = 1.0 / (double(nv) * c->cout);
+    if (kind == 1) {
+      // sum of squares of the target first (it fixes the gradient's scale)
+      const unsigned ns = std::min(nb, 1024u);
+      hipLaunchKernelGGL(k_sumsq, dim3(ns), dim3(256), 0, c->stream, (const float*)tgt, nv * c->cout, c->lossp);
+      CNN_HIP(hipGetLastError());
+      std::vector<double> h(ns);
+      CNN_HIP(hipMemcpyAsync(h.data(), c->lossp, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      CNN_HIP(hipStreamSynchronize(c->stream));
+      double s = 0;
+      for (double v : h) s += v;
+      inv = 1.0 / (s + double(FLT_EPSILON));
+    }
+    c->pbeg("loss", 0);
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, c->stream, (const float*)y, (const float*)tgt, (bf16*)dy, nullptr,
-                       c->lossp, nv, c->cout, c->cout_p, 1.0 / (double(nv) * c->cout));
+                       c->lossp, nv, c->cout, c->cout_p, inv);
     CNN_HIP(hipGetLastError());
     c->pend();
     std::vector<double> h(nb);
@@ -997,17 +1121,48 @@ int sc_cnn_mse(sc_cnn* c, const void* y, const void* tgt, void* dy, int batch, d
     CNN_HIP(hipStreamSynchronize(c->stream));
     double s = 0;
     for (double v : h) s += v;
-    if (loss) *loss = s / (double(nv) * c->cout);
+    if (loss) *loss = s * inv;
+  });
+}
+
+int sc_cnn_mse(sc_cnn* c, const void* y, const void* tgt, void* dy, int batch, double* loss) {
+  return sc_cnn_loss(c, y, tgt, dy, batch, 0, loss);
+}
+
+// One optimiser step on the accumulated gradient: per-array ClipNorm(clip)
+// (clip <= 0: none), then AdamW (weight_decay 0: Adam); the gradient is kept.
+int sc_cnn_adamw(sc_cnn* c, double lr, double b1, double b2, double eps, double weight_decay, double clip) {
+  return cguard(c, [&] {
+    c->adam_t += 1;
+    const float c1 = float(1.0 - std::pow(b1, c->adam_t)), c2 = float(1.0 - std::pow(b2, c->adam_t));
+    Arrays A;
+    for (int l = 0; l < 4; ++l) A.b[l] = c->woff[l];
+    for (int l = 0; l < 3; ++l) A.b[4 + l] = c->boff[l];
+    A.b[7] = c->wlen;
+    double* nrm2 = c->gnorm + 7 * NORM_BLOCKS;
+    if (clip > 0) {
+      hipLaunchKernelGGL(k_array_sumsq, dim3(NORM_BLOCKS, 7), dim3(256), 0, c->stream, c->g, A, c->gnorm);
+      CNN_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c->stream, c->gnorm, nrm2);
+      CNN_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_adamw, dim3(unsigned((c->wlen + 255) / 256)), dim3(256), 0, c->stream, c->w, c->g, c->m, c->v,
+                       c->wlen, float(lr), float(b1), float(b2), float(eps), float(weight_decay), c1, c2, nrm2, A,
+                       clip);
+    CNN_HIP(hipGetLastError());
+    pack_all(c);
   });
 }
 
 int sc_cnn_adam(sc_cnn* c, double lr, double b1, double b2, double eps) {
+  return sc_cnn_adamw(c, lr, b1, b2, eps, 0.0, 0.0);
+}
+
+// Trace removal on the output (needs cout 6: xx, yy, zz, xy, yz, zx)
+int sc_cnn_set_tracefree(sc_cnn* c, int on) {
   return cguard(c, [&] {
-    c->adam_t += 1;
-    const float c1 = float(1.0 - std::pow(b1, c->adam_t)), c2 = float(1.0 - std::pow(b2, c->adam_t));
-    hipLaunchKernelGGL(k_adam, dim3(unsigned((c->wlen + 255) / 256)), dim3(256), 0, c->stream, c->w, c->g, c->m, c->v,
-                       c->wlen, float(lr), float(b1), float(b2), float(eps), c1, c2);
-    CNN_HIP(hipGetLastError());
+    if (on && c->cout != 6) throw std::runtime_error("trace removal needs cout = 6 (xx, yy, zz, xy, yz, zx)");
+    c->tracefree = on != 0;
     pack_all(c);
   });
 }

@@ -480,6 +480,7 @@ DEV double rfft_w(const Mode& m, const SpecGeom& g) { return (m.x >= 1 && m.x <=
 // kind 0: sum_c w |u_c|^2          (getenergy / energy)
 // kind 1: nu * sum_c w k^2 |u_c|^2 (get_dissipation!)
 // kind 2: nu * sum_c w k^4 |u_c|^2 (get_palinstrophy!)
+// kind 3: sum_c |u_c|^2 over the stored half spectrum, unweighted (LinearAlgebra.norm^2)
 template <typename T>
 __global__ void k_sum_modes(const C<T>* u, int ncomp, SpecGeom g, int kind, double nu, double* partial) {
   Mode m;
@@ -490,8 +491,8 @@ __global__ void k_sum_modes(const C<T>* u, int ncomp, SpecGeom g, int kind, doub
       C<T> v = u[c * g.s_comp + m.off];
       a += double(v.x) * double(v.x) + double(v.y) * double(v.y);
     }
-    double w = rfft_w(m, g);
-    if (kind >= 1) {
+    double w = kind == 3 ? 1.0 : rfft_w(m, g);
+    if (kind == 1 || kind == 2) {
       double kx = g.kf * m.kx, ky = g.kf * m.ky, kz = g.kf * m.kz;
       double k2 = kx * kx + ky * ky + kz * kz;
       a *= nu * (kind == 2 ? k2 * k2 : k2);

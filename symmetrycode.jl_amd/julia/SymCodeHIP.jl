@@ -227,15 +227,25 @@ mutable struct CNN
     ptr::Ptr{Cvoid}
     n::Int
 end
-function CNN(n; batch = 1, cin = 4, width = 64, cout = 6, seed = 0)
+function CNN(n; batch = 1, cin = 4, width = 64, cout = 6, seed = 0, tracefree = cout == 6)
     out = Ref{Ptr{Cvoid}}(C_NULL)
     rc = ccall((:sc_cnn_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Ref{Ptr{Cvoid}}),
                n, batch, cin, width, cout, seed, out)
     rc == 0 || error("sc_cnn_create failed ($rc)")
     net = CNN(out[], n)
     finalizer(x -> ccall((:sc_cnn_destroy, libsymcode), Cint, (Ptr{Cvoid},), x.ptr), net)
+    # the `symm` trace-removal layer of the reference's nets (src/nets.jl:275-293)
+    tracefree && ccall((:sc_cnn_set_tracefree, libsymcode), Cint, (Ptr{Cvoid}, Cint), net.ptr, 1) == 0 ||
+        !tracefree || error("sc_cnn_set_tracefree failed")
     net
 end
+
+# One step of the reference's optimiser, OptimiserChain(ClipNorm(clip),
+# AdamW(lr)) (src/training.jl:50), on the gradient accumulated on the device;
+# the caller supplies lr from the warm-up + cosine schedule (:64-70).
+adamw!(net::CNN, lr; clip = 1.0, β = (0.9, 0.999), ϵ = 1e-8, λ = 0.0) =
+    ccall((:sc_cnn_adamw, libsymcode), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble),
+          net.ptr, lr, β[1], β[2], ϵ, λ, clip) == 0 || error("sc_cnn_adamw failed")
 
 """
 The network as an LES model, the counterpart of `fullchain`
@@ -270,7 +280,7 @@ end
 # pullback is the hand-written adjoint (sc_rk4_vjp / sc_les_rk4_vjp) under
 # the RFFT-weighted inner product of spectraldot (src/solver.jl:708); with a
 # CNNClosure the network's parameter gradient accumulates on the device
-# (sc_cnn_zero_grad / sc_cnn_get_grads / sc_cnn_adam).
+# (sc_cnn_zero_grad / sc_cnn_get_grads, then adamw!).
 using ChainRulesCore: ChainRulesCore, NoTangent, unthunk
 
 function copyfield(u::HIPField{T}) where {T}

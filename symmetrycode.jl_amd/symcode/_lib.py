@@ -88,6 +88,10 @@ SIGNATURES = {
     "sc_cnn_mse": [P, P, P, P, c_int, DP],
     "sc_cnn_backward": [P, P, P, P, c_int],
     "sc_cnn_adam": [P, c_double, c_double, c_double, c_double],
+    "sc_norm2": [P, P, c_int, DP],
+    "sc_cnn_loss": [P, P, P, P, c_int, c_int, DP],
+    "sc_cnn_adamw": [P, c_double, c_double, c_double, c_double, c_double, c_double],
+    "sc_cnn_set_tracefree": [P, c_int],
     "sc_cnn_profile": [P, c_int],
     "sc_cnn_profile_report": [P, ctypes.c_char_p, c_int],
     "sc_cnn_pack_target": [P, P, c_int, ctypes.c_long, ctypes.c_long, c_int, c_double, P],

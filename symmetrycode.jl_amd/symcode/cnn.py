@@ -20,7 +20,9 @@ def _p(t):
 
 
 class ConvClosureNet:
-    def __init__(self, n, batch_max=1, cin=4, width=64, cout=6, seed=0, device=0):
+    def __init__(self, n, batch_max=1, cin=4, width=64, cout=6, seed=0, device=0, tracefree=False):
+        """tracefree: the output's xx, yy, zz channels lose their mean (the
+        reference nets' last layer, src/nets.jl:275-293; needs cout 6)"""
         net = ctypes.c_void_p()
         rc = lib().sc_cnn_create(n, batch_max, cin, width, cout, seed, ctypes.byref(net))
         if rc != 0:
@@ -35,6 +37,15 @@ class ConvClosureNet:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check_cnn(lib().sc_cnn_set_stream(net, ctypes.c_void_p(stream)), net)
         self.nvox = self.n**3
+        self.tracefree = False
+        if tracefree:
+            self.set_tracefree(True)
+
+    def set_tracefree(self, on=True):
+        if on and self.cout != 6:
+            raise ValueError("trace removal needs cout = 6 (xx, yy, zz, xy, yz, zx)")
+        check_cnn(lib().sc_cnn_set_tracefree(self.net, 1 if on else 0), self.net)
+        self.tracefree = bool(on)
 
     def __del__(self):
         try:
@@ -118,14 +129,24 @@ class ConvClosureNet:
         check_cnn(lib().sc_cnn_forward(self.net, _p(x), _p(y), batch), self.net)
         return y
 
-    def mse(self, y, target, dy=None):
+    LOSSES = {"mse": 0, "relative": 1}
+
+    def loss(self, y, target, kind="mse", dy=None):
+        """(loss, dy): "mse" the mean squared error, "relative" the reference's
+        sum|y - t|^2 / (sum|t|^2 + eps) over the batch (create_loss,
+        src/training.jl:3-9)"""
+        if kind not in self.LOSSES:
+            raise ValueError(f"loss kind must be one of {sorted(self.LOSSES)}")
         batch = y.shape[0]
         dy = dy if dy is not None else torch.zeros((batch, self.nvox, self.cout_p), dtype=torch.bfloat16,
                                                    device=self.device)
         loss = ctypes.c_double()
-        check_cnn(lib().sc_cnn_mse(self.net, _p(y), _p(target.contiguous()), _p(dy), batch, ctypes.byref(loss)),
-                  self.net)
+        check_cnn(lib().sc_cnn_loss(self.net, _p(y), _p(target.contiguous()), _p(dy), batch, self.LOSSES[kind],
+                                    ctypes.byref(loss)), self.net)
         return loss.value, dy
+
+    def mse(self, y, target, dy=None):
+        return self.loss(y, target, "mse", dy)
 
     def backward(self, x, dy, want_dx=False):
         batch = x.shape[0]
@@ -150,11 +171,16 @@ class ConvClosureNet:
     def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         check_cnn(lib().sc_cnn_adam(self.net, lr, beta1, beta2, eps), self.net)
 
-    def train_step(self, x, target, lr=1e-3):
-        """one Adam step on the MSE loss; returns the loss before the step"""
+    def adamw(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0, clip=0.0):
+        """ClipNorm(clip) per parameter array (clip <= 0: none), then AdamW
+        (src/training.jl:50)"""
+        check_cnn(lib().sc_cnn_adamw(self.net, lr, beta1, beta2, eps, weight_decay, clip), self.net)
+
+    def train_step(self, x, target, lr=1e-3, loss="mse", clip=0.0, weight_decay=0.0):
+        """one optimiser step on the loss; returns the loss before the step"""
         self.zero_grad()
         y = self.forward(x)
-        loss, dy = self.mse(y, target)
+        value, dy = self.loss(y, target, loss)
         self.backward(x, dy)
-        self.adam(lr)
-        return loss
+        self.adamw(lr, weight_decay=weight_decay, clip=clip)
+        return value

@@ -196,6 +196,14 @@ def spectraldot(u, v, g: Grid):
     return complex(re.value, im.value)
 
 
+def norm2(u, g: Grid):
+    """Σ |û|² over the stored half spectrum without RFFT weights: the squared
+    LinearAlgebra.norm of the reference's rfft arrays."""
+    e = _d()
+    _c(lib().sc_norm2(g.ctx, _p(u), ncomp_of(u), ctypes.byref(e)), g)
+    return e.value
+
+
 def get_dissipation(u, visc, g: Grid):
     """get_dissipation! (src/solver.jl:796-811)."""
     e = _d()

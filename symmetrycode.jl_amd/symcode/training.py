@@ -3,7 +3,9 @@
 A-priori (the reference's route): march the DNS, form the filtered pairs
 (ū, τ) at each save time and filter width (create_data, src/data.jl:110-220;
 sfs!, :625-642), normalise them into network batches (create_dataloader,
-:705-757) and fit with Adam (train, src/training.jl:45-129).
+:705-757) and fit them: ``reference_train`` has the reference's semantics
+(relative loss, ClipNorm + AdamW, warm-up + cosine; train,
+src/training.jl:3-9,45-109), ``train`` exposes each choice.
 
 A-posteriori (no reference counterpart): the loss of an LES rollout against
 the filtered-DNS trajectory, differentiated through the hand-written adjoint
@@ -115,15 +117,49 @@ def pack_pairs(net, g: Grid, pairs, nrm: Normalization, x=None, y=None):
     return x, y
 
 
-def train(net, x, y, nsteps, lr=1e-3, log_every=0):
-    """Fixed-budget Adam on the MSE of the normalised stress (the reference's
-    train loop with a constant learning rate); returns the loss history."""
+def lr_schedule(step, total, lr, warmup_frac=0.05):
+    """learning rate of step 1..total (train, src/training.jl:64-70): linear
+    warm-up over nwarm = max(1, round(warmup_frac * total)) steps, then
+    cosine decay to zero at step total"""
+    nwarm = max(1, round(warmup_frac * total))
+    if step < nwarm:
+        return lr * step / nwarm
+    return lr * (1 + math.cos(math.pi * (step - nwarm) / max(1, total - nwarm))) / 2
+
+
+def train(net, x, y, nsteps, lr=1e-3, log_every=0, loss="mse", clip=0.0, schedule="constant", warmup_frac=0.05,
+          weight_decay=0.0, batchsize=None):
+    """Fixed-budget training of the network on (x, y); returns the loss
+    history.  ``loss``: "mse" or the reference's "relative"; ``clip``: per-array
+    ClipNorm before AdamW (0: none); ``schedule``: "constant" or the
+    reference's "cosine" (lr_schedule).  ``batchsize`` cycles through
+    contiguous minibatches of the samples in order (None: the whole batch every
+    step).  The reference's settings (train, src/training.jl:45-109;
+    default_schedule, src/experiment.jl:23-36) are ``reference_train``."""
+    if schedule not in ("constant", "cosine"):
+        raise ValueError("schedule must be 'constant' or 'cosine'")
+    B = x.shape[0]
+    bs = B if batchsize is None else int(batchsize)
+    if not 1 <= bs <= B:
+        raise ValueError("batchsize out of range")
+    nb = (B + bs - 1) // bs
     losses = []
     for it in range(nsteps):
-        losses.append(net.train_step(x, y, lr))
+        eta = lr if schedule == "constant" else lr_schedule(it + 1, nsteps, lr, warmup_frac)
+        j = it % nb
+        xb, yb = (x, y) if nb == 1 else (x[j * bs:(j + 1) * bs], y[j * bs:(j + 1) * bs])
+        losses.append(net.train_step(xb, yb, eta, loss=loss, clip=clip, weight_decay=weight_decay))
         if log_every and (it % log_every == 0 or it == nsteps - 1):
-            print(f"train step {it}: loss {losses[-1]:.4e}", flush=True)
+            print(f"train step {it}: lr {eta:.3e} loss {losses[-1]:.4e}", flush=True)
     return losses
+
+
+def reference_train(net, x, y, nsteps, lr=1e-3, batchsize=None, log_every=0):
+    """train with the reference's semantics: relative loss, ClipNorm(1.0) then
+    AdamW(lr), 5 % linear warm-up then cosine decay (src/training.jl:3-9,
+    45-70; src/experiment.jl:27-30)"""
+    return train(net, x, y, nsteps, lr, log_every, loss="relative", clip=1.0, schedule="cosine", warmup_frac=0.05,
+                 batchsize=batchsize)
 
 
 def apriori_metrics(tau_model, tau_ref):
@@ -135,6 +171,17 @@ def apriori_metrics(tau_model, tau_ref):
     aa, bb = a - a.mean(), b - b.mean()
     crosscor = float(torch.dot(aa, bb) / torch.sqrt(torch.dot(aa, aa) * torch.dot(bb, bb)))
     return relerr, crosscor
+
+
+def e_post(u, u_ref, g: Grid):
+    """a-posteriori error ‖u − ū‖ / ‖ū‖ of an LES state against the filtered
+    DNS, ū 2/3-truncated first, with the plain norm of the stored half
+    spectra (reduce!, src/les.jl:94-97)"""
+    r = u_ref.clone()
+    S.twothirds(r, g)
+    e = g.vectorfield()
+    S.lincomb(e, u, 1.0, r, -1.0, g)
+    return math.sqrt(S.norm2(e, g) / S.norm2(r, g))
 
 
 # ------------------------------------------------------------ a-posteriori
@@ -173,11 +220,12 @@ def rollout_gradient(u0, refs, dt, g: Grid, visc, model):
     return L, ubar
 
 
-def aposteriori_step(net, u0, refs, dt, g: Grid, visc, model, lr=1e-3):
-    """one a-posteriori gradient step: zero grads, rollout + adjoint, Adam"""
+def aposteriori_step(net, u0, refs, dt, g: Grid, visc, model, lr=1e-3, clip=0.0):
+    """one a-posteriori gradient step: zero grads, rollout + adjoint, then
+    Adam (with per-array ClipNorm when clip > 0)"""
     net.zero_grad()
     L, _ = rollout_gradient(u0, refs, dt, g, visc, model)
-    net.adam(lr)
+    net.adamw(lr, clip=clip)
     return L
 
 

@@ -124,3 +124,81 @@ def test_cnn_parity_n128():
     for l in range(3):
         assert rel(GB[l], gb[l].numpy()) < 3e-2, l
     assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
+
+
+def test_cnn_tracefree_parity():
+    """trace removal on the output (the reference nets' last layer): zero
+    trace, forward and gradients against the oracle's projected network"""
+    import symcode as sc
+
+    print_case("cnn tracefree")
+    net = sc.cnn.ConvClosureNet(64, batch_max=1, cin=4, width=64, cout=6, seed=8, tracefree=True)
+    n = net.n
+    Ws, bs = net.unflatten(net.get_params())
+    x = rand_input(net, 1, 11)
+    xt = torch.tensor(x).to(torch.bfloat16).cuda()
+    y = net.forward(xt)
+    yc = y.cpu().numpy()
+    assert np.abs(yc[..., :3].sum(-1)).max() < 1e-5 * np.abs(yc[..., :3]).max()
+    yo, store = OC.forward(Ws, bs, torch.tensor(x), n, tracefree_out=True)
+    assert rel(yc, yo.numpy()) < 1e-2
+    # a target with a trace: the projection must reach the gradient
+    tgt = np.random.default_rng(12).standard_normal((1, net.nvox, 6)).astype(np.float32) * 0.1
+    _, dy = net.mse(y, torch.tensor(tgt).cuda())
+    _, dyo = OC.mse(yo, tgt, 6)
+    net.zero_grad()
+    dx = net.backward(xt, dy, want_dx=True)
+    gW, gb, dxo = OC.backward(store, dyo, n)
+    GW, GB = net.unflatten(net.get_grads())
+    for l in range(4):
+        assert rel(GW[l], gW[l].numpy()) < 3e-2, l
+    for l in range(3):
+        assert rel(GB[l], gb[l].numpy()) < 3e-2, l
+    assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
+    with pytest.raises(ValueError):
+        sc.cnn.ConvClosureNet(64, batch_max=1, cin=4, width=64, cout=5, tracefree=True)
+
+
+def test_cnn_relative_loss():
+    """the reference's relative loss and its gradient on the GPU output"""
+    print_case("cnn relative loss")
+    net = make(seed=13, batch=2)
+    x = torch.tensor(rand_input(net, 2, 14)).to(torch.bfloat16).cuda()
+    y = net.forward(x)
+    tgt = np.random.default_rng(15).standard_normal((2, net.nvox, 6)).astype(np.float32) * 0.3
+    loss, dy = net.loss(y, torch.tensor(tgt).cuda(), "relative")
+    lo, dyo = OC.rel_loss(y.cpu(), tgt, 6)
+    assert loss == pytest.approx(lo, rel=1e-9)
+    assert rel(dy.float().cpu().numpy(), dyo.numpy()) < 5e-3
+    with pytest.raises(ValueError):
+        net.loss(y, torch.tensor(tgt).cuda(), "l1")
+
+
+def test_cnn_adamw_clip_matches_oracle():
+    """three ClipNorm + AdamW steps (with decay) on the network's own
+    gradients equal the oracle's optimiser applied to the same gradients"""
+    from oracle import optim as O
+
+    print_case("cnn adamw clip")
+    net = make(seed=16)
+    x = torch.tensor(rand_input(net, 1, 17)).to(torch.bfloat16).cuda()
+    shapes = net.layer_shapes()
+    bounds = [0]
+    for co, ci in shapes:
+        bounds.append(bounds[-1] + 27 * co * ci)
+    for _ in range(3):
+        bounds.append(bounds[-1] + net.width)
+    st = O.AdamWState(net.nparams)
+    w = net.get_params().astype(np.float64)
+    for k, (clip, wd) in enumerate([(1.0, 0.0), (0.05, 0.01), (1e3, 0.0)]):
+        net.zero_grad()
+        y = net.forward(x)
+        tgt = torch.full((1, net.nvox, 6), 0.1 * (k + 1), device="cuda")
+        _, dy = net.loss(y, tgt, "relative")
+        net.backward(x, dy)
+        gr = net.get_grads().astype(np.float64)
+        net.adamw(1e-3, weight_decay=wd, clip=clip)
+        w = O.clipnorm_adamw(w, gr, st, bounds, 1e-3, weight_decay=wd, clip=clip)
+        got = net.get_params().astype(np.float64)
+        assert np.abs(got - w).max() < 1e-6, (k, np.abs(got - w).max())
+        assert np.array_equal(net.get_grads(), gr.astype(np.float32))  # gradient kept

@@ -129,3 +129,47 @@ def test_les_rk4_and_vjp_without_closure(D, n):
     ub = oracle_solenoidal(go, 25)
     out = sc.les_rk4_vjp(gg.upload_spectral(u), gg.upload_spectral(ub), dt, gg, None, nu)
     assert rel(gg.download_spectral(out), A.rk4_vjp(u, ub, dt, go, nu, (0.1, 2))) < 1e-12
+
+
+def test_e_post_and_norm2():
+    """e_post (reduce!, src/les.jl:94-97) and the unweighted half-spectrum
+    norm against the oracle"""
+    import symcode as sc
+    from symcode import training as T
+    from oracle import optim as O
+
+    print_case("e_post", 3, 16)
+    go = S.Grid(3, 16)
+    gg = sc.Grid(3, 16, dtype=torch.float64)
+    u = oracle_solenoidal(go, 31, trunc=True)
+    r = oracle_solenoidal(go, 32)
+    assert sc.solver.norm2(gg.upload_spectral(r), gg) == pytest.approx(np.linalg.norm(r.ravel()) ** 2, rel=1e-13)
+    got = T.e_post(gg.upload_spectral(u), gg.upload_spectral(r), gg)
+    ref = O.e_post(u, r, lambda c: S.twothirds(c, go))
+    assert got == pytest.approx(ref, rel=1e-12)
+
+
+def test_les_rk4_vjp_closure4_tracefree(env):
+    """the learned closure with trace removal on the output: the RK4 step's
+    transpose and the parameter gradient against the oracle"""
+    sc, go, gg, net, onet, model, u = env
+    print_case("les_rk4_vjp closure 4 tracefree", N)
+    Ws, bs = net.unflatten(net.get_params())
+    tnet = sc.cnn.ConvClosureNet(N, batch_max=1, cin=4, width=64, cout=6, seed=3, tracefree=True)
+    tnet.set_params(net.get_params())
+    tonet = L.Net([W.copy() for W in Ws], [b.copy() for b in bs], 4, S_IN, S_OUT, EXTRA, emulate=True,
+                  tracefree=True)
+    tmodel = sc.create_learned_closure(tnet, gg, S_IN, S_OUT, EXTRA)
+    tau = gg.download_physical(tmodel(gg.upload_spectral(u)))
+    otau, _ = L.cnn_tau(u, go, tonet)
+    assert rel(tau, otau) < 1e-2
+    assert np.abs(tau[0] + tau[1] + tau[2]).max() < 1e-5 * np.abs(tau[:3]).max()
+    nu, dt = 0.01, 0.01
+    ub = oracle_solenoidal(go, 26)
+    tnet.zero_grad()
+    out = sc.les_rk4_vjp(gg.upload_spectral(u), gg.upload_spectral(ub), dt, gg, tmodel, nu)
+    og = L.zero_grads(tonet)
+    ref = L.rk4_vjp(u, ub, dt, go, nu, tonet, grads=og)
+    ref0 = A.rk4_vjp(u, ub, dt, go, nu)
+    assert share_err(gg.download_spectral(out), ref, ref0) < 3e-2
+    check_grads(tnet, og)

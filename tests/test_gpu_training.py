@@ -58,6 +58,30 @@ def test_apriori_gain(data):
     assert m_cnn[1] > m_dyn[1]
 
 
+def test_apriori_gain_reference_semantics(data):
+    """the same gain with the reference's training semantics: trace-free
+    output, relative loss, ClipNorm(1) + AdamW, 5 % warm-up then cosine decay
+    (src/training.jl:3-9,45-70; src/nets.jl:275-293)"""
+    sc, T, g, visc, pairs = data
+    print_case("apriori gain reference semantics 128->64")
+    train, test = pairs[:5], pairs[5]
+    nrm = T.Normalization.of(train, g)
+    net = sc.cnn.ConvClosureNet(64, batch_max=5, cin=4, width=64, cout=6, seed=0, tracefree=True)
+    x, y = T.pack_pairs(net, g, train, nrm)
+    losses = T.reference_train(net, x, y, 400, lr=1e-3)
+    print("relative loss", losses[0], losses[-1], flush=True)
+    assert losses[-1] < 0.5 * losses[0]
+    tau_ref = sc.to_phys(g.spacetensorfield(), test.tau, g)
+    tau_cnn = nrm.closure(net, g, test.delta, test.redelta)(test.ubar)
+    tc = tau_cnn.double()
+    assert float((tc[0] + tc[1] + tc[2]).abs().max()) < 1e-5 * float(tc[:3].abs().max())
+    m_cnn = T.apriori_metrics(tau_cnn, tau_ref)
+    m_dyn = T.apriori_metrics(sc.create_dynamic_smagorinsky(test.delta, g)(test.ubar), tau_ref)
+    print("relerr/crosscor  cnn", m_cnn, " dynsmag", m_dyn, flush=True)
+    assert m_cnn[0] < 0.9
+    assert m_cnn[1] > m_dyn[1]
+
+
 def test_aposteriori_gradient_step_reduces_rollout_loss(data):
     """a few a-posteriori Adam steps on a 64³ FP64 LES rollout (8 RK4 steps)
     against the filtered DNS lower its loss"""
@@ -89,3 +113,10 @@ def test_aposteriori_gradient_step_reduces_rollout_loss(data):
         assert math.isfinite(L)
         L0 = L if L0 is None else L0
     assert L < L0
+    # a-posteriori error of the final state (e_post, src/les.jl:97)
+    u = u0.clone()
+    for _ in full:
+        sc.les_rk4(u, dt, g, model, visc)
+    e = T.e_post(u, full[-1], g)
+    print("e_post after", len(full), "steps", e, flush=True)
+    assert 0 < e < 1
