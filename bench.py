@@ -28,8 +28,10 @@ METRIC = "810³ DNS RK4 steps/s + %HBM roofline; 128³ a-posteriori LES grad ste
 # Algorithmic HBM bytes per launch, in units of C (one spectral scalar field,
 # 16·(n/2+1)·n² B in FP64), for the unpruned pass model of one RHS.
 ALG_C = {
-    "c2c_z_inv": 6, "c2c_y_inv": 6, "x_products": 9, "y_fwd_div": 12, "z_fwd_rhs": 12,
+    "c2c_z_inv": 6, "c2c_y_inv": 6, "x_products": 9, "y_fwd_div": 12, "z_fwd_rhs": 12, "z_fwd_div": 9,
     "c2c_y_fwd": 12, "c2c_z_fwd": 12, "(k_div_visc<T, 3>)": 12,
+    # RK4 update with the viscous and forcing terms: 12, 18, 18, 15 C over the stages
+    "(k_rk4_update_lin<T, 3>)": 15.75,
 }
 RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates)
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at

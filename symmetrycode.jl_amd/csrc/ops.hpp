@@ -32,7 +32,7 @@ struct Ops {
   static void rk4(Ctx& c, CT* u, double dt, double nu);
   static double max_velocity(Ctx& c, const CT* u);
   static void max_velocity_into(Ctx& c, const CT* u);
-  static void rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax);
+  static void rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax, bool lin = true);
   static double wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure, double p1, double p2);
   static double rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure = -1, double p1 = 0,
                         double p2 = 0);

@@ -253,17 +253,21 @@ static void y_fwd_div(Ctx& c, C<T>* buf) {
 }
 
 template <typename T>
-static void fwd_rhs_last(Ctx& c, const C<T>* buf, const C<T>* u, C<T>* du, RhsParams rp) {
+static void fwd_rhs_last(Ctx& c, const C<T>* buf, const C<T>* u, C<T>* du, RhsParams rp, bool lin = true) {
   const int axis = c.D == 3 ? 2 : 1;
   StridedGeom g = full_geom<T>(c, axis);
   dim3 grid(unsigned(g.ntile) * unsigned(g.nother));
   const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
-  c.prof_begin(c.D == 3 ? "z_fwd_rhs" : "y_fwd_rhs2d");
+  c.prof_begin(c.D == 3 ? (lin ? "z_fwd_rhs" : "z_fwd_div") : "y_fwd_rhs2d");
   switch (c.n) {
 #define X(NN)                                                                                                  \
   case NN:                                                                                                     \
-    if (c.D == 3)                                                                                              \
-      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, rp); \
+    if (c.D == 3 && lin)                                                                                       \
+      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN, true>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, \
+                         rp);                                                                                  \
+    else if (c.D == 3)                                                                                         \
+      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN, false>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, \
+                         g, rp);                                                                               \
     else                                                                                                       \
       hipLaunchKernelGGL((k_y_fwd_rhs2d<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, rp); \
     break;
@@ -408,9 +412,11 @@ const double* Ops<T>::band_coeff(Ctx& c, const CT* u) {
 // divergence, viscous term and forcing fused into the forward passes.  The
 // gradient part -nu k (k.u) of the viscous term is omitted (removed by the
 // projection that follows every stage).  With vmax non-null the stage-input
-// velocity maximum is accumulated for the CFL step.
+// velocity maximum is accumulated for the CFL step.  lin = false (3D): du is
+// the divergence alone; the caller adds -nu |k|^2 u and the band forcing
+// (whose coefficient is left in c.forceA).
 template <typename T>
-void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax) {
+void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax, bool lin) {
   CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, c.D));
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
   const int kc = c.n / 3;
@@ -436,7 +442,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     rp.forceA = c.forceA;
   }
   if (c.D == 3) y_fwd_div<T>(c, B);
-  fwd_rhs_last<T>(c, B, u, du, rp);
+  fwd_rhs_last<T>(c, B, u, du, rp, lin || c.D != 3);
 }
 
 template <typename T>
@@ -503,10 +509,14 @@ double Ops<T>::rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_
       if (cfl > 0 && s == 0) max_velocity_into(c, u);
       les_rhs(c, du, s == 0 ? u : ust, nu, les_closure, p1, p2);
     } else {
-      rhs_fused(c, du, s == 0 ? u : ust, nu, (cfl > 0 && s == 0) ? c.vmax : nullptr);
+      // 3D: the viscous and forcing terms join k in the contiguous update pass
+      rhs_fused(c, du, s == 0 ? u : ust, nu, (cfl > 0 && s == 0) ? c.vmax : nullptr, c.D != 3);
     }
     if (cfl > 0 && s == 0) cfl_dt<T>(c, cfl, nu, dt);
-    if (c.D == 3) SPEC_LAUNCH((k_rk4_update<T, 3>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
+    if (c.D == 3 && les_closure < 0)
+      SPEC_LAUNCH((k_rk4_update_lin<T, 3>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_, nu,
+                  (const double*)(c.force_kind == 1 ? c.forceA : nullptr), c.kband * c.kband);
+    else if (c.D == 3) SPEC_LAUNCH((k_rk4_update<T, 3>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
     else SPEC_LAUNCH((k_rk4_update<T, 2>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
   }
   return read_dt(c);

@@ -117,7 +117,10 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
 }
 
 // Final z-pass: du_i = -i (A_i + kz s_iz) - nu |k|^2 u_i (+ A_f u_i on the band).
-template <typename T, int N>
+// With LIN false only the divergence -i (A_i + kz s_iz) is written and u is
+// not read: the RK4 update adds the viscous and forcing terms in its
+// contiguous pass (k_rk4_update_lin), where u costs no strided access.
+template <typename T, int N, bool LIN>
 __global__ void SC_LB_WIDE((wide_threads<T, N>()))
 k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw, StridedGeom g,
             RhsParams rp) {
@@ -171,12 +174,15 @@ k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* _
         const int kzi = freq(k, N);
         const T kz = kf * T(kzi);
         C<T> a = add<T>(accl[k * TX + xl], v[e]);
-        C<T> uu = ui[long(k) * g.s_axis];
-        const T k2 = kx * kx + ky * ky + kz * kz;
-        C<T> r = mk<T>(a.y - nu * k2 * uu.x, -a.x - nu * k2 * uu.y);  // -i a - nu k^2 u
-        if (rp.forceA) {
-          int k2i = x * x + kyi * kyi + kzi * kzi;
-          if (k2i > 0 && k2i <= rp.kband2) r = add<T>(r, scl<T>(uu, fA));
+        C<T> r = mk<T>(a.y, -a.x);  // -i a
+        if constexpr (LIN) {
+          C<T> uu = ui[long(k) * g.s_axis];
+          const T k2 = kx * kx + ky * ky + kz * kz;
+          r = mk<T>(r.x - nu * k2 * uu.x, r.y - nu * k2 * uu.y);  // - nu k^2 u
+          if (rp.forceA) {
+            int k2i = x * x + kyi * kyi + kzi * kzi;
+            if (k2i > 0 && k2i <= rp.kband2) r = add<T>(r, scl<T>(uu, fA));
+          }
         }
         di[long(k) * g.s_axis] = r;
       });

@@ -333,6 +333,45 @@ __global__ void k_rk4_update(C<T>* u0, C<T>* ust, C<T>* acc, const C<T>* k, cons
   for (int c = 0; c < D; ++c) dst[c * g.s_comp + m.off] = v[c];
 }
 
+// RK4 stage update whose k arrives without its linear terms (the divergence
+// of k_z_fwd_rhs<.., false>): k = k_div - nu |k|^2 us (+ A_f us on the band
+// 0 < |k|^2 <= kband2, fA non-null), us the stage input (u0 at stage 0, ust
+// after; read before ust is written, so the update runs in place).  Then as
+// k_rk4_update.
+template <typename T, int D>
+__global__ void k_rk4_update_lin(C<T>* u0, C<T>* ust, C<T>* acc, const C<T>* k, const double* dtp, double frac,
+                                 int stage, SpecGeom g, double nu, const double* fA, int kband2) {
+  Mode m;
+  if (!mode_of(gid(), g, m)) return;
+  const T cdt = T(*dtp * frac);
+  const T kf = T(g.kf);
+  const T kx = kf * T(m.kx), ky = kf * T(m.ky), kz = kf * T(m.kz);
+  const T nk2 = T(nu) * (kx * kx + ky * ky + kz * kz);
+  const int k2i = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+  const T fa = (fA && k2i > 0 && k2i <= kband2) ? T(*fA) : T(0);
+  C<T> v[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const long o = c * g.s_comp + m.off;
+    const C<T> b = u0[o];
+    C<T> uu = b;
+    if (stage > 0) uu = ust[o];
+    C<T> kk = k[o];
+    kk = mk<T>(kk.x - nk2 * uu.x, kk.y - nk2 * uu.y);
+    if (fa != T(0)) kk = add<T>(kk, scl<T>(uu, fa));
+    C<T> a;
+    if (stage == 0) a = kk;
+    else if (stage == 3) a = add<T>(acc[o], kk);
+    else a = add<T>(acc[o], scl<T>(kk, T(2)));
+    if (stage < 3) acc[o] = a;
+    v[c] = add<T>(b, scl<T>(stage < 3 ? kk : a, cdt));
+  }
+  project_mode<T, D>(m, g, v);
+  C<T>* dst = stage < 3 ? ust : u0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) dst[c * g.s_comp + m.off] = v[c];
+}
+
 // y = a*x + y over ncomp spectral components
 template <typename T>
 __global__ void k_axpy(C<T>* y, const C<T>* x, T a, SpecGeom g, int ncomp) {

@@ -235,8 +235,10 @@ function CNN(n; batch = 1, cin = 4, width = 64, cout = 6, seed = 0, tracefree = 
     net = CNN(out[], n)
     finalizer(x -> ccall((:sc_cnn_destroy, libsymcode), Cint, (Ptr{Cvoid},), x.ptr), net)
     # the `symm` trace-removal layer of the reference's nets (src/nets.jl:275-293)
-    tracefree && ccall((:sc_cnn_set_tracefree, libsymcode), Cint, (Ptr{Cvoid}, Cint), net.ptr, 1) == 0 ||
-        !tracefree || error("sc_cnn_set_tracefree failed")
+    if tracefree
+        ccall((:sc_cnn_set_tracefree, libsymcode), Cint, (Ptr{Cvoid}, Cint), net.ptr, 1) == 0 ||
+            error("sc_cnn_set_tracefree failed")
+    end
     net
 end
 

@@ -26,11 +26,13 @@ from .grid import Grid
 from .training import apriori_metrics, filter_reynolds
 
 # model bytes per launch in units of one spectral scalar field C: the
-# unpruned pass model of the fused RHS pipeline (the update kernel moves its
-# 12-15 C); pruned_c gives what the passes actually move, which the rocprofv3
-# FETCH/WRITE counters confirm
-PASS_C = {"c2c_z_inv": 6, "c2c_y_inv": 6, "x_products": 9, "y_fwd_div": 12, "z_fwd_rhs": 12,
-          "(k_rk4_update<T, 3>)": 13.5}
+# unpruned pass model of the fused RHS pipeline.  In the 3D RK4 step the last
+# pass writes the divergence alone (z_fwd_div, 9 C) and the update adds the
+# viscous and forcing terms (12, 18, 18, 15 C over the stages); pruned_c gives
+# what the passes actually move, which the rocprofv3 FETCH/WRITE counters
+# confirm
+PASS_C = {"c2c_z_inv": 6, "c2c_y_inv": 6, "x_products": 9, "y_fwd_div": 12, "z_fwd_rhs": 12, "z_fwd_div": 9,
+          "(k_rk4_update<T, 3>)": 13.5, "(k_rk4_update_lin<T, 3>)": 15.75}
 
 
 def cast_spectral(u, g_from: Grid, g_to: Grid):
@@ -53,7 +55,8 @@ def pruned_c(n):
     kc, nh = n // 3, n // 2 + 1
     fx, fb = (kc + 1) / nh, (2 * kc + 1) / n
     return {"c2c_z_inv": 3 * fx * fb * fb + 3 * fx * fb, "c2c_y_inv": 3 * fx * fb + 3 * fx,
-            "x_products": 3 * fx + 6, "y_fwd_div": 12, "z_fwd_rhs": 12, "(k_rk4_update<T, 3>)": 13.5}
+            "x_products": 3 * fx + 6, "y_fwd_div": 12, "z_fwd_rhs": 12, "z_fwd_div": 9,
+            "(k_rk4_update<T, 3>)": 13.5, "(k_rk4_update_lin<T, 3>)": 15.75}
 
 
 def _rates(g: Grid, rep, steps):
@@ -65,7 +68,7 @@ def _rates(g: Grid, rep, steps):
             t = ms / cnt / 1e3
             out[k] = {"ms_per_launch": ms / cnt, "GB_per_s": PASS_C[k] * C / t / 1e9,
                       "GB_per_s_moved": pc[k] * C / t / 1e9}
-    rhs_keys = [k for k in ("c2c_z_inv", "c2c_y_inv", "x_products", "y_fwd_div", "z_fwd_rhs") if k in rep]
+    rhs_keys = [k for k in ("c2c_z_inv", "c2c_y_inv", "x_products", "y_fwd_div", "z_fwd_rhs", "z_fwd_div") if k in rep]
     rhs_ms = sum(rep[k][1] for k in rhs_keys) / (4 * steps)
     rhs_c = sum(PASS_C[k] for k in rhs_keys)
     out["rhs"] = {"ms": rhs_ms, "GB_per_s": rhs_c * C / (rhs_ms / 1e3) / 1e9,
