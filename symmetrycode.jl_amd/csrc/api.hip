@@ -87,7 +87,7 @@ int sc_destroy(sc_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->tw_n, c->tw_m, c->bufA, c->bufB, c->scr, c->ustart, c->du, c->acc, c->ust,
+  void* bufs[] = {c->tw_n, c->tw_m, c->bufA, c->bufB, c->bufT, c->scr, c->ustart, c->du, c->acc, c->ust,
                   c->dscr, c->vmax, c->forceA, c->dtdev};
   for (void* b : bufs)
     if (b) (void)hipFree(b);

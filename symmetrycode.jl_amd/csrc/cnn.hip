@@ -64,7 +64,15 @@ __device__ __forceinline__ float gelu_grad(float z) {
 enum Mode { HIDDEN = 0, LAST = 1, BWD_HIDDEN = 2, BWD_INPUT = 3 };
 
 constexpr int VPB = 64;   // output voxels per x segment (n must be a multiple)
-constexpr int TROWS = 8;  // output y rows per block (one per wavefront)
+constexpr int TROWS = 8;  // output y rows per block
+// y rows per wavefront.  RPW = 2 halves the weight-fragment traffic per flop
+// but leaves one wavefront per SIMD (the LDS tile allows one block per CU):
+// batch-16 x 128^3 training steps took 87.8 ms against 80.3 ms with RPW = 1
+#ifndef SC_CNN_RPW
+#define SC_CNN_RPW 1
+#endif
+constexpr int RPW = SC_CNN_RPW;
+constexpr int CONV_THREADS = 64 * (TROWS / RPW);      // threads per convolution block
 #ifndef SC_CNN_PADC
 #define SC_CNN_PADC 8
 #endif
@@ -78,10 +86,12 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 
 // One convolution layer (or its data gradient) as an implicit GEMM with the
 // reduction index k = t * CIN + c (tap-major; narrow layers pack several taps
-// into one 32-wide MFMA step).  Block: 8 wavefronts over 8 consecutive y rows
-// of one z plane, 64 x voxels each; a wavefront holds its 64-voxel x COUTT
-// output tile in 4 x (COUTT/16) MFMA accumulators, so every B fragment feeds
-// 4 MFMAs and every A fragment COUTT/16.  Inputs are staged in LDS: one input
+// into one 32-wide MFMA step).  Block: 8 consecutive y rows of one z plane,
+// 64 x voxels each, RPW rows per wavefront; a wavefront holds its
+// (64 RPW)-voxel x COUTT output tile in 4 RPW x (COUTT/16) MFMA accumulators,
+// so every B fragment feeds 4 RPW MFMAs and every A fragment COUTT/16 (with
+// RPW = 2 the block's weight-fragment traffic through L1 is half that of one
+// row per wavefront, for the same LDS tile).  Inputs are staged in LDS: one input
 // plane (10 rows x 66 voxels) at a time for CIN = 64, all three planes for the
 // narrow layers.  Weights are pre-arranged per (32-wide k step, 16 output
 // channels) as the 64 lanes' B fragments (1 KB contiguous, see
@@ -93,7 +103,7 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 //  BWD_HIDDEN: out_a = conv * gelu'(zprev)                            (bf16)
 //  BWD_INPUT:  out_f = conv (first COUTS channels)                    (fp32)
 template <int CIN, int COUTT, int COUTS, int MODE>
-__global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
+__global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
                                                  const float* __restrict__ bias, bf16* out_a, bf16* out_z,
                                                  float* out_f, const bf16* __restrict__ zprev, int n, long ntiles,
                                                  int tracefree) {
@@ -104,10 +114,10 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
   constexpr int NKC = kpad(CIN) / 32;
   constexpr int NT = COUTT / 16;
   constexpr int CH8 = CIN / 8;
-  constexpr int MT = 4;
+  constexpr int MT = 4 * RPW;  // M tiles of 16 voxels: 4 per row
   constexpr int NPH = BYPLANE ? 3 : 1;
   constexpr int NCH = NPL * RW * XW * CH8;
-  constexpr int NPT = (NCH + 511) / 512;
+  constexpr int NPT = (NCH + CONV_THREADS - 1) / CONV_THREADS;
   __shared__ __attribute__((aligned(16))) bf16 win[NPL * RW * XW * CL];
   const int segs = n / VPB, ytiles = n / TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -128,15 +138,16 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0, 0, 0, 0};
-  // A fragment of M tile mt (row w, x 16 mt + li) at k step kc
+  // A fragment of M tile mt (row w RPW + mt / 4, x 16 (mt % 4) + li) at k step kc
   const bf16* lane_a = &win[li * CL + 8 * lk];
+  auto mt_off = [](int mt) { return ((mt / 4) * XW + 16 * (mt % 4)) * CL; };
   auto load_a = [&](int kc, bf16x8* a) {
     if constexpr (CIN >= 32) {  // one tap per k step: the offsets are uniform
       const int t = (kc * 32) / CIN, cb = (kc * 32) % CIN;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16* base = lane_a + ((w + 1 + dy) * XW + 1 + dx) * CL + cb;
+      const bf16* base = lane_a + ((w * RPW + 1 + dy) * XW + 1 + dx) * CL + cb;
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + 16 * mt * CL);
+      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
       return;
     }
     const int k0 = kc * 32 + 8 * lk;
@@ -144,9 +155,9 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
     if (t < 27) {
       const int pl = BYPLANE ? 0 : t / 9;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16* base = &win[((pl * RW + w + 1 + dy) * XW + 1 + li + dx) * CL + c0];
+      const bf16* base = &win[((pl * RW + w * RPW + 1 + dy) * XW + 1 + li + dx) * CL + c0];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + 16 * mt * CL);
+      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
     } else {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
@@ -164,7 +175,7 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
   u32x4 pre[NPT];
 #define CONV_FETCH(T, PH)                                                                                   \
   _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                        \
-    const int i = min(int(threadIdx.x) + 512 * j, NCH - 1);                                                 \
+    const int i = min(int(threadIdx.x) + CONV_THREADS * j, NCH - 1);                                        \
     const int c8 = i % CH8;                                                                                 \
     int r = i / CH8;                                                                                        \
     const int xi = r % XW;                                                                                  \
@@ -177,7 +188,7 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
   }
 #define CONV_COMMIT()                                                                                       \
   _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                        \
-    const int i = threadIdx.x + 512 * j;                                                                    \
+    const int i = threadIdx.x + CONV_THREADS * j;                                                           \
     if (i < NCH) *reinterpret_cast<u32x4*>(&win[(i / CH8) * CL + (i % CH8) * 8]) = pre[j];                  \
   }
   if (long(blockIdx.x) >= ntiles) return;
@@ -255,7 +266,7 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
   // D[co = 16 nt + 4 lk + r][voxel = 16 mt + li]: four consecutive channels of
   // one voxel, stored (and, for the data gradient, the pre-activation read)
   // as one 8- or 16-byte access
-  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + w) * unsigned(n) + cur.x0 + li;
+  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + w * RPW) * unsigned(n) + cur.x0 + li;
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
     const int co0 = nt * 16 + 4 * lk;
@@ -268,7 +279,7 @@ __global__ void __launch_bounds__(512, 1) k_conv(const bf16* __restrict__ in, co
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       // voxel and element offsets fit in 32 bits (at most 2^31 elements per tensor)
-      const unsigned v = vrow0 + 16 * mt;
+      const unsigned v = vrow0 + (mt / 4) * unsigned(n) + 16 * (mt % 4);
       const floatx4 d = acc[mt][nt];
       if constexpr (MODE == HIDDEN) {
         bf16x4 zz, aa;
@@ -798,7 +809,7 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
 #define CONV(CI, COT, COS, MO)                                                                             \
   if (cinp == CI && coutp == COS && mode == MO) {                                                         \
     c->pbeg(names[MO], flop);                                                                             \
-    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(512), 0, c->stream, in, W, bias, oa, oz, of, zp, \
+    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(CONV_THREADS), 0, c->stream, in, W, bias, oa, oz, of, zp, \
                        c->n, ntiles, c->tracefree ? 1 : 0);                                              \
     CNN_HIP(hipGetLastError());                                                                           \
     c->pend();                                                                                            \

@@ -34,6 +34,7 @@ struct Ctx {
   // lazily allocated scratch (device layout)
   void* bufA = nullptr;   // D spectral comps: inverse pipeline
   void* bufB = nullptr;   // 6 spectral comps: forward pipeline / stress
+  void* bufT = nullptr;   // 6 slots, z-pencil tiled: forward y-pass output
   void* scr = nullptr;    // general spectral scratch
   int scr_comp = 0;
   void* ustart = nullptr; // RK storage (D comps each)

@@ -142,6 +142,7 @@ template <int N> struct Plan;
     static constexpr int tpp = TPP_;                             \
     static constexpr int r[] = {__VA_ARGS__};                    \
     static constexpr int ns = sizeof(r) / sizeof(int);           \
+    static constexpr int rad(int s) { return r[s]; }             \
   };
 SC_PLAN(2, 1, 2)
 SC_PLAN(3, 1, 3)
@@ -166,12 +167,21 @@ SC_PLAN(512, 64, 8, 8, 8)
 SC_PLAN(810, SC_TPP810, 10, 9, 9)
 #undef SC_PLAN
 
-template <int N, int TPP_ = Plan<N>::tpp> struct PlanInfo {
-  using P = Plan<N>;
+// the same radices in reverse order: a transform with this plan reads its
+// stage-0 input in the layout a Plan<N> transform leaves after its last stage
+// when N / (last radix) == TPP (both are positions p + r N / R)
+template <int N> struct PlanRev {
+  static constexpr int tpp = Plan<N>::tpp;
+  static constexpr int ns = Plan<N>::ns;
+  static constexpr int rad(int s) { return Plan<N>::r[ns - 1 - s]; }
+};
+
+template <int N, int TPP_ = Plan<N>::tpp, class PL = Plan<N>> struct PlanInfo {
+  using P = PL;
   static constexpr int TPP = TPP_;
   static constexpr int NS = P::ns;
-  static constexpr int R(int s) { return P::r[s]; }
-  static constexpr int Ns(int s) { int p = 1; for (int i = 0; i < s; ++i) p *= P::r[i]; return p; }
+  static constexpr int R(int s) { return P::rad(s); }
+  static constexpr int Ns(int s) { int p = 1; for (int i = 0; i < s; ++i) p *= P::rad(i); return p; }
   static constexpr int NB(int s) { return (N / R(s) + TPP - 1) / TPP; }
   static constexpr int maxE() {
     int m = 0;
@@ -201,8 +211,8 @@ template <bool WAVE> DEV void xsync() {
   }
 }
 
-template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp, bool WAVE = false> struct Pencil {
-  using PI = PlanInfo<N, TPP_>;
+template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp, bool WAVE = false, class PL = Plan<N>> struct Pencil {
+  using PI = PlanInfo<N, TPP_, PL>;
   static constexpr int TPP = PI::TPP;
   static constexpr int NS = PI::NS;
   static constexpr int MAXE = PI::MAXE;

@@ -334,6 +334,11 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   constexpr int NP = D == 3 ? 6 : 3;
   using PI = Pencil<T, M, +1, TPP, true>;
   using PF = Pencil<T, M, -1, TPP, true>;
+  // forward transforms with the radices reversed read the products in the
+  // registers where the inverse transforms left the velocities: no LDS
+  // reordering between the pointwise product and the forward FFT
+  using PFR = Pencil<T, M, -1, TPP, true, PlanRev<M>>;
+  constexpr bool DIRECT = M / PlanInfo<M>::R(PlanInfo<M>::NS - 1) == TPP;
   constexpr int E = PlanInfo<M>::E_LAST;
   __shared__ C<T> lds[NW * M];
   __shared__ C<T> twm_l[M];
@@ -349,7 +354,7 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   const bool rok = row < g.nrows;
   C<T>* ll = lds + w * M;
   C<T> u[D][E];
-  C<T> v[PlanInfo<M>::MAXE];
+  C<T> v[PlanInfo<M>::MAXE > PFR::MAXE ? PlanInfo<M>::MAXE : PFR::MAXE];
 #pragma unroll
   for (int c = 0; c < D; ++c) {
     const C<T>* src = in + long(c) * g.s_comp_spec + row * g.nhp;
@@ -393,9 +398,15 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
     const int j = D == 3 ? PB[q] : QB[q];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
-    PI::out_to_in(v, p, ll, 1);
-    PF::run(v, p, ll, 1, twm_l);
-    PF::out_to_lds(v, p, ll, 1);
+    if constexpr (DIRECT) {
+      static_assert(PFR::E0 == E, "reversed plan reads the inverse's output slots");
+      PFR::run(v, p, ll, 1, twm_l);
+      PFR::out_to_lds(v, p, ll, 1);
+    } else {
+      PI::out_to_in(v, p, ll, 1);
+      PF::run(v, p, ll, 1, twm_l);
+      PF::out_to_lds(v, p, ll, 1);
+    }
     xsync<true>();
     if (rok) {
       C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;

@@ -232,8 +232,15 @@ static StridedGeom full_geom(const Ctx& c, int axis) {
   return g;
 }
 
+// slot stride of the z-pencil-tiled buffer between the forward y and z passes
 template <typename T>
-static void y_fwd_div(Ctx& c, C<T>* buf) {
+static long tiled_slot(const Ctx& c) {
+  const int TX = tile_x_rt<T>(c.n);
+  return long((c.nh + TX - 1) / TX) * TX * c.n * c.n;
+}
+
+template <typename T>
+static void y_fwd_div(Ctx& c, const C<T>* buf, C<T>* outT) {
   StridedGeom g = full_geom<T>(c, 1);
   dim3 grid(unsigned(g.ntile) * unsigned(g.nother));
   const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
@@ -242,7 +249,8 @@ static void y_fwd_div(Ctx& c, C<T>* buf) {
   switch (c.n) {
 #define X(NN)                                                                                          \
   case NN:                                                                                             \
-    hipLaunchKernelGGL((k_y_fwd_div<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, tw, g, kf); \
+    hipLaunchKernelGGL((k_y_fwd_div<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, outT, \
+                       tiled_slot<T>(c), tw, g, kf);                                                  \
     break;
     SC_FOR_N(X)
 #undef X
@@ -263,11 +271,11 @@ static void fwd_rhs_last(Ctx& c, const C<T>* buf, const C<T>* u, C<T>* du, RhsPa
 #define X(NN)                                                                                                  \
   case NN:                                                                                                     \
     if (c.D == 3 && lin)                                                                                       \
-      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN, true>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, \
-                         rp);                                                                                  \
+      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN, true>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf,         \
+                         tiled_slot<T>(c), u, du, tw, g, rp);                                                  \
     else if (c.D == 3)                                                                                         \
-      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN, false>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, \
-                         g, rp);                                                                               \
+      hipLaunchKernelGGL((k_z_fwd_rhs<T, NN, false>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf,        \
+                         tiled_slot<T>(c), u, du, tw, g, rp);                                                  \
     else                                                                                                       \
       hipLaunchKernelGGL((k_y_fwd_rhs2d<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, buf, u, du, tw, g, rp); \
     break;
@@ -441,8 +449,14 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     SC_HIP(hipGetLastError());
     rp.forceA = c.forceA;
   }
-  if (c.D == 3) y_fwd_div<T>(c, B);
-  fwd_rhs_last<T>(c, B, u, du, rp, lin || c.D != 3);
+  if (c.D == 3) {
+    if (!c.bufT) SC_HIP(hipMalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT)));
+    CT* BT = reinterpret_cast<CT*>(c.bufT);
+    y_fwd_div<T>(c, B, BT);
+    fwd_rhs_last<T>(c, BT, u, du, rp, lin);
+  } else {
+    fwd_rhs_last<T>(c, B, u, du, rp, true);
+  }
 }
 
 template <typename T>

@@ -44,14 +44,17 @@ DEV void stage_twiddles(C<T>* dst, const C<T>* __restrict__ src, int n) {
 
 // Output slots of the y-pass: 0 A_x, 1 A_y, 2 s_zz, 3 A_z, 4 s_yz, 5 s_zx.
 // Input slots (packed stress): 0 xx, 1 yy, 2 zz, 3 xy, 4 yz, 5 zx.  The
-// processing order below only ever writes slots whose input this block has
-// already consumed, so the pass runs in place.  The global loads of the next
+// output goes to a z-pencil-tiled buffer, element (x, y, z) of a slot at
+// ((tile * n + y) * n + z) * TX + x % TX with tile = x / TX: the z-pass then
+// reads each of its pencil tiles as one contiguous run, instead of TX
+// elements every 16 (n/2+1) n bytes, which misses the address-translation
+// cache on almost every access.  The global loads of the next
 // component are issued before the current component's FFT (software
 // pipelining); twiddles are read from LDS so that waiting for them never
 // waits for the prefetch.
 template <typename T, int N>
 __global__ void SC_LB_WIDE((wide_threads<T, N>()))
-k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
+k_y_fwd_div(const C<T>* buf, C<T>* outT, long s_slot, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
   constexpr int TX = tile_x<T, N>();
   constexpr int TPP = tpp_wide<N>();
   using PE = Pencil<T, N, -1, TPP>;
@@ -67,7 +70,8 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
   const int xl = t % TX, p = t / TX;
   const int x = tile * TX + xl;
   const bool xok = x < g.xs;
-  C<T>* base = buf + long(oi) * g.s_other + x;
+  const C<T>* base = buf + long(oi) * g.s_other + x;
+  C<T>* tbase = outT + (long(tile) * N * N + oi) * TX + xl;  // (tile, y = 0, z = oi)
   const T kx = T(kf) * T(x);
   C<T> v[PE::MAXE];
   C<T> nx[PE::E0];
@@ -85,8 +89,8 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
   };
   auto put = [&](int slot, const C<T>* val) {
     if (!xok) return;
-    C<T>* dst = base + long(slot) * g.s_comp;
-    PE::for_out(p, [&](int e, int k) { dst[long(k) * g.s_axis] = val[e]; });
+    C<T>* dst = tbase + long(slot) * s_slot;
+    PE::for_out(p, [&](int e, int k) { dst[long(k) * (N * TX)] = val[e]; });
   };
   auto kyv = [&](int k) { return T(kf) * T(freq(k, N)); };
   fetch(0);
@@ -119,11 +123,12 @@ k_y_fwd_div(C<T>* buf, const C<T>* __restrict__ tw, StridedGeom g, double kf) {
 // Final z-pass: du_i = -i (A_i + kz s_iz) - nu |k|^2 u_i (+ A_f u_i on the band).
 // With LIN false only the divergence -i (A_i + kz s_iz) is written and u is
 // not read: the RK4 update adds the viscous and forcing terms in its
-// contiguous pass (k_rk4_update_lin), where u costs no strided access.
+// contiguous pass (k_rk4_update_lin), where u costs no strided access.  The
+// six slots come from the y-pass's z-pencil-tiled buffer (slot stride s_slot).
 template <typename T, int N, bool LIN>
 __global__ void SC_LB_WIDE((wide_threads<T, N>()))
-k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw, StridedGeom g,
-            RhsParams rp) {
+k_z_fwd_rhs(const C<T>* bufT, long s_slot, const C<T>* __restrict__ u, C<T>* du, const C<T>* __restrict__ tw,
+            StridedGeom g, RhsParams rp) {
   constexpr int TX = tile_x<T, N>();
   constexpr int TPP = tpp_wide<N>();
   using PE = Pencil<T, N, -1, TPP>;
@@ -149,9 +154,10 @@ k_z_fwd_rhs(const C<T>* buf, const C<T>* __restrict__ u, C<T>* du, const C<T>* _
   if (rp.forceA) fA = T(*rp.forceA);
   C<T> v[PE::MAXE];
   C<T> nx[PE::E0];
+  const C<T>* tsrc = bufT + (long(tile) * N + oi) * N * TX + xl;  // (tile, y = oi, z = 0)
   auto fetch = [&](int slot) {
-    const C<T>* src = buf + long(slot) * g.s_comp + off;
-    PE::load(nx, p, [&](int j) { return xok ? src[long(j) * g.s_axis] : mk<T>(T(0), T(0)); });
+    const C<T>* src = tsrc + long(slot) * s_slot;
+    PE::load(nx, p, [&](int j) { return xok ? src[j * TX] : mk<T>(T(0), T(0)); });
   };
   auto fft = [&](int next) {
 #pragma unroll
