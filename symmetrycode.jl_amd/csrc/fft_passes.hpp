@@ -43,6 +43,10 @@ struct StridedGeom {
   long s_axis;        // element stride along the transform axis
   long s_other;       // element stride along the other axis
   long s_comp;        // component stride
+  long s_tile;        // stride of an x tile (TX for the x-contiguous layout)
+  // output strides (k_c2c_strided; equal to the input's unless the output
+  // goes to another layout)
+  long o_axis, o_other, o_comp, o_tile;
   int kin;            // read only axis positions with |freq| <= kin
   int mask23;         // apply the 2/3 box mask (and zero mode) on load
   int kcut;           // n / 3, for mask23
@@ -96,7 +100,8 @@ k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ t
   const int xl = t % TX, p = t / TX;
   const int x = tile * TX + xl;
   const bool xok = x < g.xs;
-  const long base = long(oi) * g.s_other + x;
+  const long base = long(oi) * g.s_other + tile * g.s_tile + xl;
+  const long obase = long(oi) * g.o_other + tile * g.o_tile + xl;
   bool xmask = true;
   bool zero_o = false;
   if (g.mask23) {
@@ -124,8 +129,8 @@ k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ t
     if (c + 1 < ncomp) fetch(c + 1);
     PE::run(v, p, lds + xl, TX, twl);
     if (xok) {
-      C<T>* dst = out + long(c) * g.s_comp + base;
-      PE::store(v, p, [&](int k, C<T> val) { dst[long(k) * g.s_axis] = scl<T>(val, sc_); });
+      C<T>* dst = out + long(c) * g.o_comp + obase;
+      PE::store(v, p, [&](int k, C<T> val) { dst[long(k) * g.o_axis] = scl<T>(val, sc_); });
     }
   }
 }

@@ -52,10 +52,10 @@ bool Ops<T>::supported(int n) {
   return false;
 }
 
-// strided complex pass along axis (1 = y, 2 = z) of length n
+// geometry of a strided complex pass along axis (1 = y, 2 = z) of length n,
+// input and output in the x-contiguous spectral layout
 template <typename T>
-static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int ncomp, int xs, bool prune_other,
-                     int kin, bool mask23, double scale) {
+static StridedGeom axis_geom(const Ctx& c, int axis, int xs, bool prune_other, int kin, bool mask23, double scale) {
   const int n = c.n;
   const int TX = tile_x_rt<T>(n);
   StridedGeom g;
@@ -76,10 +76,21 @@ static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int 
   g.s_axis = axis == 1 ? long(c.nhp) : long(c.nhp) * n;
   g.s_other = axis == 1 ? long(c.nhp) * n : long(c.nhp);
   g.s_comp = c.spec_elems;
+  g.s_tile = TX;
+  g.o_axis = g.s_axis;
+  g.o_other = g.s_other;
+  g.o_comp = g.s_comp;
+  g.o_tile = g.s_tile;
   g.kin = kin;
   g.mask23 = mask23 ? 1 : 0;
   g.kcut = kc;
   g.scale = scale;
+  return g;
+}
+
+template <typename T>
+static void c2c_launch(Ctx& c, const StridedGeom& g, int axis, int sign, const C<T>* in, C<T>* out, int ncomp) {
+  const int n = c.n;
   dim3 grid(unsigned(g.ntile) * unsigned(g.nother), 1);
   const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
   static const char* names[2][2] = {{"c2c_y_fwd", "c2c_y_inv"}, {"c2c_z_fwd", "c2c_z_inv"}};
@@ -100,6 +111,13 @@ static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int 
   }
   c.prof_end();
   SC_HIP(hipGetLastError());
+}
+
+// strided complex pass along axis (1 = y, 2 = z) of length n
+template <typename T>
+static void c2c_axis(Ctx& c, int axis, int sign, const C<T>* in, C<T>* out, int ncomp, int xs, bool prune_other,
+                     int kin, bool mask23, double scale) {
+  c2c_launch<T>(c, axis_geom<T>(c, axis, xs, prune_other, kin, mask23, scale), axis, sign, in, out, ncomp);
 }
 
 template <typename T>
@@ -225,6 +243,11 @@ static StridedGeom full_geom(const Ctx& c, int axis) {
   g.s_axis = axis == 1 ? long(c.nhp) : long(c.nhp) * n;
   g.s_other = axis == 1 ? long(c.nhp) * n : long(c.nhp);
   g.s_comp = c.spec_elems;
+  g.s_tile = TX;
+  g.o_axis = g.s_axis;
+  g.o_other = g.s_other;
+  g.o_comp = g.s_comp;
+  g.o_tile = g.s_tile;
   g.kin = n;
   g.mask23 = 0;
   g.kcut = n / 3;
@@ -232,7 +255,8 @@ static StridedGeom full_geom(const Ctx& c, int axis) {
   return g;
 }
 
-// slot stride of the z-pencil-tiled buffer between the forward y and z passes
+// component stride of the tiled buffers between strided passes (x tiles of
+// TX, each holding n x n pencil positions)
 template <typename T>
 static long tiled_slot(const Ctx& c) {
   const int TX = tile_x_rt<T>(c.n);
@@ -429,8 +453,25 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
   const int kc = c.n / 3;
   if (c.D == 3) {
-    c2c_axis<T>(c, 2, +1, u, A, 3, kc + 1, true, kc, true, 1.0);
-    c2c_axis<T>(c, 1, +1, A, A, 3, kc + 1, false, kc, false, 1.0);
+    // the inverse z-pass writes y-pencil tiles, element (x, y, z) of a
+    // component at ((tile * n + z) * n + y) * TX + x % TX (in the buffer the
+    // forward passes use later): both its writes and the y-pass's reads then
+    // stay within few address-translation pages
+    if (!c.bufT) SC_HIP(hipMalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT)));
+    CT* AT = reinterpret_cast<CT*>(c.bufT);
+    const long n = c.n, TX = tile_x_rt<T>(c.n), ts = tiled_slot<T>(c);
+    StridedGeom gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
+    gz.o_tile = n * n * TX;
+    gz.o_axis = n * TX;
+    gz.o_other = TX;
+    gz.o_comp = ts;
+    c2c_launch<T>(c, gz, 2, +1, u, AT, 3);
+    StridedGeom gy = axis_geom<T>(c, 1, kc + 1, false, kc, false, 1.0);
+    gy.s_tile = n * n * TX;
+    gy.s_axis = TX;
+    gy.s_other = n * TX;
+    gy.s_comp = ts;
+    c2c_launch<T>(c, gy, 1, +1, AT, A, 3);
   } else {
     c2c_axis<T>(c, 1, +1, u, A, 2, kc + 1, false, kc, true, 1.0);
   }
@@ -450,7 +491,6 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     rp.forceA = c.forceA;
   }
   if (c.D == 3) {
-    if (!c.bufT) SC_HIP(hipMalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT)));
     CT* BT = reinterpret_cast<CT*>(c.bufT);
     y_fwd_div<T>(c, B, BT);
     fwd_rhs_last<T>(c, BT, u, du, rp, lin);
