@@ -147,10 +147,32 @@ int sc_closure_clark(sc_ctx* ctx, void* tau, const void* GG, double delta);     
 int sc_closure_smagorinsky(sc_ctx* ctx, void* tau, const void* GG, double cs, double delta); /* :44-51 */
 /* create_dynamic_smagorinsky model (:188-271); coeff receives c */
 int sc_closure_dynsmag(sc_ctx* ctx, void* tau, const void* u, double delta, double* coeff);
+/* create_verstappen (:53-62,68-71), the Q-R model, 3D only */
+int sc_closure_verstappen(sc_ctx* ctx, void* tau, const void* GG, double c, double delta);
+/* create_bardina (:73-115): trace-free scale similarity with test filter s * Delta */
+int sc_closure_bardina(sc_ctx* ctx, void* tau, const void* u, double s, double delta);
+/* LES statistics (solve_les's reduce!, src/les.jl:94-139; compute_sfs_stats,
+ * src/analysis.jl:41-137), on device fields of the context's precision:
+ * eps = -tau:S pointwise (contract_dissipation, src/solver.jl:746-759, with
+ * S = sym(GG)); */
+int sc_sfs_dissipation(sc_ctx* ctx, void* eps, const void* tau, const void* GG);
+/* the shell-resolved transfer sum over shell i of w Re(conj(u) . clo),
+ * clo = tensordivergence(2/3 to_spec(tau)), out[i-1] for shells i = 1..nshell
+ * (the shells of spectrum) */
+int sc_shell_transfer(sc_ctx* ctx, const void* u, const void* tau, int nshell, double* out);
+/* FP64 mean of n device samples (the SFS drain -mean(tau:S), src/les.jl:121) */
+int sc_sample_mean(sc_ctx* ctx, const void* x, long n, double* mean);
+/* moments of n device samples (src/analysis.jl:9-18) and the backscatter
+ * fraction: out = mean, median, std (n - 1), skewness, fraction < 0 */
+int sc_moments(sc_ctx* ctx, const void* x, long n, double* out);
+/* relerr = |b - a| / |a| and crosscor (mean-removed) of n stacked entries,
+ * b the model, a the reference (src/analysis.jl:116-121) */
+int sc_apriori_metrics(sc_ctx* ctx, const void* model, const void* ref, long n, double* out);
 /* les!(du, u, grid, cache; model, visc) (src/les.jl:4-33) with a built-in
  * closure: 0 none, 1 Clark(p1 = Delta), 2 Smagorinsky(p1 = C_S, p2 = Delta),
  * 3 dynamic Smagorinsky(p1 = Delta), 4 the learned closure of
- * sc_set_cnn_closure; the band forcing is added when enabled */
+ * sc_set_cnn_closure, 5 Verstappen(p1 = C, p2 = Delta), 6 Bardina(p1 = s,
+ * p2 = Delta); the band forcing is added when enabled */
 int sc_les_rhs(sc_ctx* ctx, void* du, const void* u, double visc, int closure, double p1, double p2);
 int sc_les_wray3(sc_ctx* ctx, void* u, double dt, double visc, int closure, double p1, double p2);
 /* solve_les! with RK4 in place of wray3 (src/les.jl:188-247; RK4 has no

@@ -55,6 +55,31 @@ def smagorinsky(GG, cs, delta, D):
     return pack_sym(-2 * nu * S, D)
 
 
+def verstappen(GG, c, delta):
+    """verstappen_kernel! (src/closures.jl:53-62), 3D: q = tr(S S)/2,
+    r = tr(S S S)/3, ν = C²Δ²|r|/q, τ = -2νS."""
+    G = grad_matrix(GG, 3)
+    S = (G + np.swapaxes(G, 0, 1)) / 2
+    SS = np.einsum("ik...,kj...->ij...", S, S)
+    q = np.einsum("ii...->...", SS) / 2
+    r = np.einsum("ik...,ki...->...", SS, S) / 3
+    nu = c**2 * delta**2 * np.abs(r) / q
+    return pack_sym(-2 * nu * S, 3)
+
+
+def bardina(u, s, delta, g: Grid):
+    """create_bardina's model (src/closures.jl:73-115): τ = G̃∗(u_i u_j) −
+    (G̃∗u_i)(G̃∗u_j), test filter width sΔ, every filtered field 2/3-truncated,
+    then make_tracefree! (physical)."""
+    dt = s * delta
+    ut = np.stack([twothirds(gaussianfilter(u[i], dt, g), g) for i in range(g.D)])
+    sig = nonlinearity(u, g)
+    sigt = nonlinearity(ut, g)
+    sig = np.stack([twothirds(gaussianfilter(x, dt, g), g) for x in sig])
+    tau = np.stack([to_phys(sig[i], g) - to_phys(sigt[i], g) for i in range(g.tensordim)])
+    return make_tracefree(tau, g.D)
+
+
 def smagorinsky_tensor(S, delta, D):
     """smagorinsky_tensor! (src/closures.jl:117-150); S packed."""
     if D == 2:

@@ -292,7 +292,7 @@ int sc_les_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, in
   return guard(c, [&] {
     need_fft(c);
     need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
-    need(closure >= 0 && closure <= 4, "closure kind");
+    need(closure >= 0 && closure <= 6, "closure kind");
     double d = 0;
     TYPED(c, d = sc::Ops<T>::wray3_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
     if (dt) *dt = d;
@@ -301,7 +301,7 @@ int sc_les_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, in
 int sc_les_rk4(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 4, "closure kind");
+    need(closure >= 0 && closure <= 6, "closure kind");
     TYPED(c, sc::Ops<T>::rk4_dev(*c, (CT*)u, dt, nu, 0.0, closure, p1, p2));
   });
 }
@@ -310,7 +310,7 @@ int sc_les_rk4_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, int 
   return guard(c, [&] {
     need_fft(c);
     need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
-    need(closure >= 0 && closure <= 4, "closure kind");
+    need(closure >= 0 && closure <= 6, "closure kind");
     double d = 0;
     TYPED(c, d = sc::Ops<T>::rk4_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
     if (dt) *dt = d;
@@ -410,6 +410,41 @@ int sc_closure_clark(sc_ctx* c, void* tau, const void* GG, double delta) {
 int sc_closure_smagorinsky(sc_ctx* c, void* tau, const void* GG, double cs, double delta) {
   return guard(c, [&] { TYPED(c, sc::Ops<T>::smagorinsky(*c, (T*)tau, (const T*)GG, cs, delta)); });
 }
+int sc_sfs_dissipation(sc_ctx* c, void* eps, const void* tau, const void* GG) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::sfs_dissipation(*c, (T*)eps, (const T*)tau, (const T*)GG)); });
+}
+int sc_shell_transfer(sc_ctx* c, const void* u, const void* tau, int nshell, double* out) {
+  return guard(c, [&] {
+    need(nshell > 0 && out, "nshell / output");
+    std::vector<double> h(nshell + 1);
+    TYPED(c, sc::Ops<T>::shell_transfer(*c, (const CT*)u, (const T*)tau, nshell + 1, h.data()));
+    for (int i = 0; i < nshell; ++i) out[i] = h[i + 1];
+  });
+}
+int sc_sample_mean(sc_ctx* c, const void* x, long n, double* mean) {
+  return guard(c, [&] {
+    need(mean != nullptr && n > 0, "output / samples");
+    TYPED(c, *mean = sc::Ops<T>::sample_mean(*c, (const T*)x, n));
+  });
+}
+int sc_moments(sc_ctx* c, const void* x, long n, double* out) {
+  return guard(c, [&] {
+    need(out != nullptr, "output");
+    TYPED(c, sc::Ops<T>::sample_moments(*c, (const T*)x, n, out));
+  });
+}
+int sc_apriori_metrics(sc_ctx* c, const void* model, const void* ref, long n, double* out) {
+  return guard(c, [&] {
+    need(out != nullptr, "output");
+    TYPED(c, sc::Ops<T>::apriori(*c, (const T*)model, (const T*)ref, n, out));
+  });
+}
+int sc_closure_verstappen(sc_ctx* c, void* tau, const void* GG, double cq, double delta) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::verstappen(*c, (T*)tau, (const T*)GG, cq, delta)); });
+}
+int sc_closure_bardina(sc_ctx* c, void* tau, const void* u, double s, double delta) {
+  return guard(c, [&] { TYPED(c, sc::Ops<T>::bardina(*c, (T*)tau, (const CT*)u, s, delta)); });
+}
 int sc_closure_dynsmag(sc_ctx* c, void* tau, const void* u, double delta, double* coeff) {
   return guard(c, [&] {
     need_fft(c);
@@ -421,14 +456,14 @@ int sc_closure_dynsmag(sc_ctx* c, void* tau, const void* u, double delta, double
 int sc_les_rhs(sc_ctx* c, void* du, const void* u, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 4, "closure kind");
+    need(closure >= 0 && closure <= 6, "closure kind");
     TYPED(c, sc::Ops<T>::les_rhs(*c, (CT*)du, (const CT*)u, nu, closure, p1, p2));
   });
 }
 int sc_les_wray3(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 4, "closure kind");
+    need(closure >= 0 && closure <= 6, "closure kind");
     TYPED(c, sc::Ops<T>::les_wray3(*c, (CT*)u, dt, nu, closure, p1, p2));
   });
 }

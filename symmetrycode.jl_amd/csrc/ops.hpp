@@ -59,6 +59,13 @@ struct Ops {
   // closures (physical packed tensors)
   static void clark(Ctx& c, T* tau, const T* GG, double delta);
   static void smagorinsky(Ctx& c, T* tau, const T* GG, double cs, double delta);
+  static void verstappen(Ctx& c, T* tau, const T* GG, double cq, double delta);
+  static void shell_transfer(Ctx& c, const CT* u, const T* tau, int nshell, double* host_out);
+  static void sfs_dissipation(Ctx& c, T* eps, const T* tau, const T* GG);
+  static void sample_moments(Ctx& c, const T* x, long n, double* out);
+  static double sample_mean(Ctx& c, const T* x, long n);
+  static void apriori(Ctx& c, const T* model, const T* ref, long n, double* out);
+  static void bardina(Ctx& c, T* tau, const CT* u, double s, double delta);
   static void gradient_phys(Ctx& c, T* GG, const CT* u);
   static double dynsmag(Ctx& c, T* tau, const CT* u, double delta);
   static void les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2);

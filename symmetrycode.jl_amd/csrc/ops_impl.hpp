@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#include <hipcub/hipcub.hpp>
 #include "ops.hpp"
 #include "fft_passes.hpp"
 #include "rhs_passes.hpp"
@@ -660,6 +661,102 @@ void Ops<T>::shell_energy(Ctx& c, const CT* u, int ncomp, int nshell, double* ho
   std::memcpy(host_out, h, nshell * sizeof(double));
 }
 
+// solve_les's shell-resolved transfer (src/les.jl:125-139) of the physical
+// packed stress tau against the spectral u: shells 0..nshell-1 into host_out
+template <typename T>
+void Ops<T>::shell_transfer(Ctx& c, const CT* u, const T* tau, int nshell, double* host_out) {
+  const int nt = c.tdim();
+  CT* th = reinterpret_cast<CT*>(c.work(2, size_t(nt) * c.spec_elems * sizeof(CT)));
+  r2c(c, th, tau, nt, 1.0 / std::pow(double(c.n), c.D));
+  twothirds(c, th, nt);
+  SpecGeom g = c.geom();
+  double* d = c.dscratch(nshell);
+  SC_HIP(hipMemsetAsync(d, 0, nshell * sizeof(double), c.stream));
+  if (c.D == 3)
+    hipLaunchKernelGGL((k_shell_transfer<T, 3>), dim3(nblk(g.total)), dim3(256), nshell * sizeof(double), c.stream, u,
+                       th, g, nshell, d);
+  else
+    hipLaunchKernelGGL((k_shell_transfer<T, 2>), dim3(nblk(g.total)), dim3(256), nshell * sizeof(double), c.stream, u,
+                       th, g, nshell, d);
+  SC_HIP(hipGetLastError());
+  double* h = c.hostbuf(nshell);
+  SC_HIP(hipMemcpyAsync(h, d, nshell * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(host_out, h, nshell * sizeof(double));
+}
+
+template <typename T>
+void Ops<T>::sfs_dissipation(Ctx& c, T* eps, const T* tau, const T* GG) {
+  if (c.D == 3) hipLaunchKernelGGL((k_sfs_dissipation<T, 3>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, eps, tau, GG, c.phys_elems);
+  else hipLaunchKernelGGL((k_sfs_dissipation<T, 2>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, eps, tau, GG, c.phys_elems);
+  SC_HIP(hipGetLastError());
+}
+
+// FP64 sums of k_sample_stats (nv values) into host out
+template <typename T>
+static void sample_sums(Ctx& c, const T* x, const T* y, long n, int kind, double ma, double mb, int nv, double* out) {
+  const unsigned nb = unsigned(std::min<long>(1024, std::max<long>(1, (n + RB - 1) / RB)));
+  double* part = c.dscratch(long(nv) * (nb + 1));
+  hipLaunchKernelGGL(k_sample_stats<T>, dim3(nb), dim3(RB), 0, c.stream, x, y, n, kind, ma, mb, part);
+  SC_HIP(hipGetLastError());
+  for (int q = 0; q < nv; ++q)
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part + long(q) * nb, long(nb),
+                       part + long(nv) * nb + q);
+  SC_HIP(hipGetLastError());
+  double* h = c.hostbuf(nv);
+  SC_HIP(hipMemcpyAsync(h, part + long(nv) * nb, nv * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(out, h, nv * sizeof(double));
+}
+
+template <typename T>
+double Ops<T>::sample_mean(Ctx& c, const T* x, long n) {
+  double sum = 0;
+  sample_sums<T>(c, x, nullptr, n, 0, 0, 0, 1, &sum);
+  return sum / double(n);
+}
+
+// moments (src/analysis.jl:9-18) and the backscatter fraction
+// (:132) of the device samples x[0, n): mean, median, std (n - 1), skewness
+// (third standardised moment, 0 when std is 0), fraction of x < 0.  The
+// median comes from a device radix sort of a copy.
+template <typename T>
+void Ops<T>::sample_moments(Ctx& c, const T* x, long n, double* out) {
+  if (n < 2) throw Error(1, "moments need at least two samples");
+  double sum = 0, cm[3];
+  sample_sums<T>(c, x, nullptr, n, 0, 0, 0, 1, &sum);
+  const double m = sum / double(n);
+  sample_sums<T>(c, x, nullptr, n, 1, m, 0, 3, cm);
+  const double sd = std::sqrt(cm[0] / double(n - 1));
+  T* sorted = reinterpret_cast<T*>(c.work(6, size_t(n) * sizeof(T)));
+  size_t tb = 0;
+  SC_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, x, sorted, int(n), 0, int(8 * sizeof(T)), c.stream));
+  void* tmp = c.work(7, tb);
+  SC_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tb, x, sorted, int(n), 0, int(8 * sizeof(T)), c.stream));
+  T mid[2];
+  const long i0 = (n - 1) / 2;
+  SC_HIP(hipMemcpyAsync(mid, sorted + i0, 2 * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  out[0] = m;
+  out[1] = n % 2 ? double(mid[0]) : (double(mid[0]) + double(mid[1])) / 2;
+  out[2] = sd;
+  out[3] = sd == 0.0 ? 0.0 : cm[1] / double(n) / (sd * sd * sd);
+  out[4] = cm[2] / double(n);
+}
+
+// relerr and crosscor of compute_sfs_stats (src/analysis.jl:116-121) over
+// the n stacked entries of model (b) and reference (a)
+template <typename T>
+void Ops<T>::apriori(Ctx& c, const T* model, const T* ref, long n, double* out) {
+  if (n < 1) throw Error(1, "empty tensors");
+  double s1[6], s2[6];
+  sample_sums<T>(c, ref, model, n, 2, 0, 0, 6, s1);
+  const double ma = s1[0] / double(n), mb = s1[1] / double(n);
+  sample_sums<T>(c, ref, model, n, 2, ma, mb, 6, s2);
+  out[0] = std::sqrt(s2[5]) / std::sqrt(s1[2]);
+  out[1] = s2[4] / std::sqrt(s2[3] * s2[2]);
+}
+
 template <typename T>
 void Ops<T>::shell_scale(Ctx& c, CT* u, int ncomp, int nshell, const double* host_factor) {
   double* d = c.dscratch(nshell);
@@ -714,6 +811,11 @@ template <typename T> void Ops<T>::smagorinsky(Ctx& c, T* tau, const T* GG, doub
   else hipLaunchKernelGGL((k_smagorinsky<T, 2>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, GG, cs, delta, c.phys_elems);
   SC_HIP(hipGetLastError());
 }
+template <typename T> void Ops<T>::verstappen(Ctx& c, T* tau, const T* GG, double cq, double delta) {
+  if (c.D != 3) throw Error(1, "the Q-R model is only defined in 3D");
+  hipLaunchKernelGGL(k_verstappen<T>, dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, GG, cq, delta, c.phys_elems);
+  SC_HIP(hipGetLastError());
+}
 template <typename T> void Ops<T>::phys_tracefree(Ctx& c, T* tau) {
   hipLaunchKernelGGL(k_phys_tracefree<T>, dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, tau, c.D, c.phys_elems);
   SC_HIP(hipGetLastError());
@@ -731,13 +833,14 @@ template <typename T> void Ops<T>::gradient_phys(Ctx& c, T* GG, const CT* u) {
 
 // les! (src/les.jl:4-33) for the built-in closures:
 // closure 1 = Clark(Delta=p1), 2 = Smagorinsky(C_S=p1, Delta=p2),
-// 3 = dynamic Smagorinsky(Delta=p1), 4 = the attached learned closure, 0 = none.
+// 3 = dynamic Smagorinsky(Delta=p1), 4 = the attached learned closure,
+// 5 = Verstappen Q-R(C=p1, Delta=p2), 6 = Bardina(s=p1, Delta=p2), 0 = none.
 // The band forcing is added when it is enabled on the context.
 template <typename T>
 void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2) {
   const int nt = c.tdim();
   const int ng = c.D * c.D;
-  if (closure < 0 || closure > 4) throw Error(1, "unknown closure kind");
+  if (closure < 0 || closure > 6) throw Error(1, "unknown closure kind");
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, nt));
   nonlinearity(c, B, u, nullptr);
   if (closure != 0) {
@@ -746,11 +849,14 @@ void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double
       dynsmag(c, tau, u, p1);
     } else if (closure == 4) {
       cnn_tau(c, tau, u);
+    } else if (closure == 6) {
+      bardina(c, tau, u, p1, p2);
     } else {
       T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));
       gradient_phys(c, GG, u);
       if (closure == 1) clark(c, tau, GG, p1);
-      else smagorinsky(c, tau, GG, p1, p2);
+      else if (closure == 2) smagorinsky(c, tau, GG, p1, p2);
+      else verstappen(c, tau, GG, p1, p2);
     }
     // sigma += to_spec!(tau) per component (the closure stress is not truncated)
     CT* th = reinterpret_cast<CT*>(c.work(2, size_t(nt) * c.spec_elems * sizeof(CT)));
@@ -785,6 +891,32 @@ void Ops<T>::cnn_tau(Ctx& c, T* tau, const CT* u) {
   cnnlink::pack_velocity<T>(c.cnn, v, c.phys_elems, 1.0 / c.cnn_sin, c.cnn_extra, b.x);
   cnnlink::forward(c.cnn, b.x, b.y, 1);
   cnnlink::unpack_tau<T>(c.cnn, b.y, c.cnn_sout, tau, c.phys_elems);
+}
+
+// create_bardina's model (src/closures.jl:68-115): tau = tracefree(
+// G~ * (u_i u_j) - (G~ * u_i)(G~ * u_j)), test filter G~ Gaussian of width
+// s Delta, each filtered field 2/3-truncated, products of the truncated
+// velocities (nonlinearity!)
+template <typename T>
+void Ops<T>::bardina(Ctx& c, T* tau, const CT* u, double s, double delta) {
+  const int D = c.D, nt = c.tdim();
+  const long sp = c.spec_elems, np = c.phys_elems;
+  const double dt = s * delta;
+  CT* UT = reinterpret_cast<CT*>(c.work(3, size_t(D) * sp * sizeof(CT)));
+  CT* SIG = reinterpret_cast<CT*>(c.work(4, size_t(nt) * sp * sizeof(CT)));
+  CT* SIGT = reinterpret_cast<CT*>(c.work(5, size_t(nt) * sp * sizeof(CT)));
+  T* P = reinterpret_cast<T*>(c.work(8, size_t(nt) * np * sizeof(T)));
+  SC_HIP(hipMemcpyAsync(UT, u, size_t(D) * sp * sizeof(CT), hipMemcpyDeviceToDevice, c.stream));
+  gaussian(c, UT, D, dt);
+  twothirds(c, UT, D);
+  nonlinearity(c, SIG, u, nullptr);
+  nonlinearity(c, SIGT, UT, nullptr);
+  gaussian(c, SIG, nt, dt);
+  twothirds(c, SIG, nt);
+  c2r(c, tau, SIG, nt, 1.0, false);
+  c2r(c, P, SIGT, nt, 1.0, false);
+  launch_phys(c, tau, P, T(-1), T(1), long(nt) * np);
+  phys_tracefree(c, tau);
 }
 
 // create_dynamic_smagorinsky's model (src/closures.jl:188-271), with the

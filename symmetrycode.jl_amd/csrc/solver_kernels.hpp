@@ -605,6 +605,41 @@ __global__ void k_shell_energy(const C<T>* u, int ncomp, SpecGeom g, int nshell,
     if (hist[i] != 0.0) atomicAdd(&out[i], hist[i]);
 }
 
+// Shell-resolved SFS transfer of solve_les's reduce! (src/les.jl:125-139):
+// per mode clo_i = -i k_j tau_ij (tensordivergence!) of the packed spectral
+// stress th, T = w Re(sum_i conj(u_i) clo_i) with the RFFT weight w,
+// histogrammed into shells 0..nshell-1 (out zeroed, FP64 atomics)
+template <typename T, int D>
+__global__ void k_shell_transfer(const C<T>* u, const C<T>* th, SpecGeom g, int nshell, double* out) {
+  constexpr int NT = D == 3 ? 6 : 3;
+  extern __shared__ double hist[];
+  for (int i = threadIdx.x; i < nshell; i += blockDim.x) hist[i] = 0.0;
+  __syncthreads();
+  Mode m;
+  if (mode_of(gid(), g, m)) {
+    int k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+    int sh = isqrt_shell(k2);
+    if (sh < nshell) {
+      T k[3];
+      kvec<T>(m, g, k);
+      C<T> s[NT], clo[D];
+#pragma unroll
+      for (int c = 0; c < NT; ++c) s[c] = th[c * g.s_comp + m.off];
+      divergence_mode<T, D>(k, s, clo);
+      double a = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const C<T> v = u[i * g.s_comp + m.off];
+        a += double(v.x) * double(clo[i].x) + double(v.y) * double(clo[i].y);
+      }
+      atomicAdd(&hist[sh], rfft_w(m, g) * a);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nshell; i += blockDim.x)
+    if (hist[i] != 0.0) atomicAdd(&out[i], hist[i]);
+}
+
 // u *= factor[shell] for shells < nshell (others untouched)
 template <typename T>
 __global__ void k_shell_scale(C<T>* u, int ncomp, SpecGeom g, int nshell, const double* factor) {
@@ -802,6 +837,94 @@ __global__ void k_smagorinsky(T* tau, const T* GG, double cs, double delta, long
 #pragma unroll
     for (int j = 0; j < 3; ++j) S[i][j] = -2 * nu * S[i][j];
   store_sym<T, D>(tau, np, id, S);
+}
+
+// verstappen_kernel! (src/closures.jl:53-62), the Q-R model (3D):
+// q = tr(S S) / 2, r = tr(S S S) / 3, nu = C^2 Delta^2 |r| / q, tau = -2 nu S
+template <typename T>
+__global__ void k_verstappen(T* tau, const T* GG, double cq, double delta, long np) {
+  long id = gid();
+  if (id >= np) return;
+  T G[3][3], S[3][3], SS[3][3];
+  load_grad<T, 3>(GG, np, id, G);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S[i][j] = (G[i][j] + G[j][i]) / 2;
+  T q = 0, r = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      SS[i][j] = S[i][0] * S[0][j] + S[i][1] * S[1][j] + S[i][2] * S[2][j];
+      q += S[i][j] * S[j][i];
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r += SS[i][j] * S[j][i];
+  q /= 2;
+  r /= 3;
+  const T nu = T(cq * cq * delta * delta) * fabs(r) / q;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S[i][j] = -2 * nu * S[i][j];
+  store_sym<T, 3>(tau, np, id, S);
+}
+
+// pointwise SFS dissipation eps = -tau:S (contract_dissipation,
+// src/solver.jl:746-759; strain_from_gradient, :742-744) from the packed
+// stress and the physical velocity gradient
+template <typename T, int D>
+__global__ void k_sfs_dissipation(T* eps, const T* tau, const T* GG, long np) {
+  long id = gid();
+  if (id >= np) return;
+  T G[3][3];
+  load_grad<T, D>(GG, np, id, G);
+  auto S = [&](int i, int j) { return (G[i][j] + G[j][i]) / 2; };
+  T d;
+  if constexpr (D == 2) {
+    d = tau[id] * S(0, 0) + tau[np + id] * S(1, 1) + 2 * tau[2 * np + id] * S(0, 1);
+  } else {
+    d = tau[id] * S(0, 0) + tau[np + id] * S(1, 1) + tau[2 * np + id] * S(2, 2) +
+        2 * (tau[3 * np + id] * S(0, 1) + tau[4 * np + id] * S(1, 2) + tau[5 * np + id] * S(2, 0));
+  }
+  eps[id] = -d;
+}
+
+// FP64 block partials of sample statistics over x[0, n): kind 0 the sum;
+// kind 1 sum (x - m)^2, sum (x - m)^3 and the count of x < 0 (three partial
+// arrays of gridDim.x); kind 2 for the pair (a, b): sums of a, b, a^2
+// (relative to ma), b^2 (relative to mb), (a - ma)(b - mb), (b - a)^2
+template <typename T>
+__global__ void k_sample_stats(const T* x, const T* y, long n, int kind, double ma, double mb, double* partial) {
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (long i = gid(); i < n; i += long(gridDim.x) * blockDim.x) {
+    const double a = double(x[i]);
+    if (kind == 0) {
+      v[0] += a;
+    } else if (kind == 1) {
+      const double e = a - ma;
+      v[0] += e * e;
+      v[1] += e * e * e;
+      v[2] += a < 0 ? 1.0 : 0.0;
+    } else {
+      const double b = double(y[i]);
+      const double ea = a - ma, eb = b - mb;
+      v[0] += a;
+      v[1] += b;
+      v[2] += ea * ea;
+      v[3] += eb * eb;
+      v[4] += ea * eb;
+      v[5] += (b - a) * (b - a);
+    }
+  }
+  const int nv = kind == 0 ? 1 : (kind == 1 ? 3 : 6);
+  for (int q = 0; q < nv; ++q) {
+    block_sum_store<int>(v[q], partial + q * gridDim.x);
+    __syncthreads();
+  }
 }
 
 // smagorinsky_tensor! (src/closures.jl:117-150), optional output scale c

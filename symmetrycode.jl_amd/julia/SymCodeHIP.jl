@@ -207,6 +207,11 @@ end
 S.create_clark(Δ, g::HIPGrid) = BuiltinClosure(1, Δ, 0)
 S.create_smagorinsky(CS, Δ, g::HIPGrid) = BuiltinClosure(2, CS, Δ)
 S.create_dynamic_smagorinsky(Δ, g::HIPGrid) = BuiltinClosure(3, Δ, 0)
+function S.create_verstappen(C, Δ, g::HIPGrid{D}) where {D}
+    D == 3 || throw(AssertionError("Q-R model is only defined in 3D"))
+    BuiltinClosure(5, C, Δ)
+end
+S.create_bardina(s, Δ, g::HIPGrid) = BuiltinClosure(6, s, Δ)
 
 function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::BuiltinClosure, visc)
     check(ccall((:sc_les_rhs, libsymcode), Cint,

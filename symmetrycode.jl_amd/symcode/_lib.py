@@ -70,6 +70,13 @@ SIGNATURES = {
     "sc_closure_clark": [P, P, P, c_double],
     "sc_closure_smagorinsky": [P, P, P, c_double, c_double],
     "sc_closure_dynsmag": [P, P, P, c_double, DP],
+    "sc_closure_verstappen": [P, P, P, c_double, c_double],
+    "sc_sfs_dissipation": [P, P, P, P],
+    "sc_shell_transfer": [P, P, P, c_int, DP],
+    "sc_moments": [P, P, ctypes.c_long, DP],
+    "sc_sample_mean": [P, P, ctypes.c_long, DP],
+    "sc_apriori_metrics": [P, P, P, ctypes.c_long, DP],
+    "sc_closure_bardina": [P, P, P, c_double, c_double],
     "sc_les_rhs": [P, P, P, c_double, c_int, c_double, c_double],
     "sc_les_wray3": [P, P, c_double, c_double, c_int, c_double, c_double],
     "sc_rhs_vjp": [P, P, P, P, c_double],

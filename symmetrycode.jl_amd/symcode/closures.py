@@ -69,6 +69,38 @@ class DynamicSmagorinsky(Closure):
         return tau
 
 
+class Verstappen(Closure):
+    """create_verstappen (src/closures.jl:53-62,68-71): the Q-R model,
+    ν = C²Δ²|r|/q with q = tr(S²)/2, r = tr(S³)/3; 3D only."""
+    kind = 5
+
+    def __init__(self, c, delta, g):
+        if g.D != 3:
+            raise ValueError("the Q-R model is only defined in 3D")
+        super().__init__(g)
+        self.p1, self.p2 = float(c), float(delta)
+
+    def __call__(self, u, GG):
+        tau = self.g.spacetensorfield()
+        _c(lib().sc_closure_verstappen(self.g.ctx, _p(tau), _p(GG), self.p1, self.p2), self.g)
+        return tau
+
+
+class Bardina(Closure):
+    """create_bardina (src/closures.jl:73-115): τ = G̃∗(u_i u_j) −
+    (G̃∗u_i)(G̃∗u_j) made trace-free, test filter width s·Δ; works from u."""
+    kind = 6
+
+    def __init__(self, s, delta, g):
+        super().__init__(g)
+        self.p1, self.p2 = float(s), float(delta)
+
+    def __call__(self, u, GG=None):
+        tau = self.g.spacetensorfield()
+        _c(lib().sc_closure_bardina(self.g.ctx, _p(tau), _p(u), self.p1, self.p2), self.g)
+        return tau
+
+
 class LearnedClosure(Closure):
     """The convolutional closure as an LES model: τ = s_out · net(ū / s_in,
     extra) with ū = C2R(2/3 û) (the solver-facing wrapper, like fullchain,
@@ -106,6 +138,14 @@ def create_smagorinsky(cs, delta, g):
 
 def create_dynamic_smagorinsky(delta, g):
     return DynamicSmagorinsky(delta, g)
+
+
+def create_verstappen(c, delta, g):
+    return Verstappen(c, delta, g)
+
+
+def create_bardina(s, delta, g):
+    return Bardina(s, delta, g)
 
 
 def gradient_phys(u, g: Grid):

@@ -254,6 +254,38 @@ def test_closures_parity(D, n):
         assert rel(gg.download_spectral(du), OC.les_rhs(u, go, nu, omodel)) < 1e-11, type(model).__name__
 
 
+@pytest.mark.parametrize("D,n", [(3, 16), (2, 32)])
+def test_verstappen_bardina_parity(D, n):
+    """create_verstappen (3D only) and create_bardina: tau and les! with
+    them (closure ids 5 and 6) against the oracle, plus one LES RK4 step"""
+    import symcode as sc
+
+    print_case("verstappen bardina", D, n)
+    go, gg = grids(D, n)
+    u = oracle_solenoidal(go, 13, trunc=True)
+    ud = gg.upload_spectral(u)
+    delta = 2 * np.pi / n * 2
+    GG = sc.gradient_phys(ud, gg)
+    oGG = OC.gradient_phys(u, go)
+    nu = 0.01
+    models = [(sc.create_bardina(2.0, delta, gg), lambda u_, G_: OC.bardina(u_, 2.0, delta, go))]
+    if D == 3:
+        models.append((sc.create_verstappen(1.0, delta, gg), lambda u_, G_: OC.verstappen(G_, 1.0, delta)))
+    else:
+        with pytest.raises(ValueError):
+            sc.create_verstappen(1.0, delta, gg)
+    for model, omodel in models:
+        name = type(model).__name__
+        tau = model(ud, GG)
+        assert rel(gg.download_physical(tau), omodel(u, oGG)) < TOL64, name
+        du = sc.les_rhs(gg.vectorfield(), ud, gg, model, nu)
+        assert rel(gg.download_spectral(du), OC.les_rhs(u, go, nu, omodel)) < 1e-11, name
+        v = ud.clone()
+        sc.les_rk4(v, 0.01, gg, model, nu)
+        ref = S.rk4(lambda w: OC.les_rhs(w, go, nu, omodel), u, 0.01, go)
+        assert rel(gg.download_spectral(v), ref) < 1e-11, name
+
+
 # ----------------------------------------------------------------- FP32 path
 @pytest.mark.parametrize("D,n", [(3, 16), (3, 30), (2, 810)])
 def test_fp32_rhs(D, n):

@@ -393,3 +393,78 @@ def test_les_rhs_no_closure_is_dns_rhs():
     u = solenoidal(g, 2)
     zero = lambda u, GG: np.zeros((6,) + g.phys_shape)
     assert np.allclose(OC.les_rhs(u, g, 1e-2, zero), S.convectiondiffusion(u, g, 1e-2), atol=1e-15)
+
+
+def test_verstappen_closed_form():
+    """Q-R model on constant gradients: a trace-free diagonal strain gives
+    q = (a² + b² + (a+b)²)/2, r = -ab(a+b); pure shear has r = 0 (tr S³ = 0)."""
+    a, b, c, d = 0.7, -1.9, 1.0, 0.3
+    tau = OC.verstappen(_const_grad(3, 4, {0: a, 4: b, 8: -a - b}), c, d)
+    q = (a * a + b * b + (a + b) ** 2) / 2
+    nu = c**2 * d**2 * abs(a * b * (a + b)) / q
+    assert np.allclose(tau[0], -2 * nu * a, rtol=1e-14)
+    assert np.allclose(tau[1], -2 * nu * b, rtol=1e-14)
+    assert np.allclose(tau[2], 2 * nu * (a + b), rtol=1e-14)
+    assert np.allclose(tau[3:], 0)
+    assert np.allclose(OC.verstappen(_const_grad(3, 4, {3: 1.3}), c, d), 0, atol=1e-16)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_bardina_single_mode(D):
+    """u_x = cos(3 x_last): before the trace removal tau_xx =
+    -g3²/2 + (g6 - g3²) cos(6 x_last)/2 (g_k the filter's transfer at |k|;
+    the filtered u u loses its mean to twothirds!, which also zeroes k = 0,
+    while (G~ u)(G~ u) keeps g3²/2), all other components zero"""
+    n, s, d = 32, 2.0, 0.3
+    g = S.Grid(D, n)
+    x = np.arange(n) * (2 * np.pi / n)
+    z = x.reshape((n,) + (1,) * (D - 1))
+    ux = np.broadcast_to(np.cos(3 * z), g.phys_shape).copy()
+    u = np.stack([S.to_spec(ux, g)] + [S.to_spec(np.zeros(g.phys_shape), g) for _ in range(D - 1)])
+    tau = OC.bardina(u, s, d, g)
+    g3, g6 = np.exp(-(s * d) ** 2 * 9 / 24), np.exp(-(s * d) ** 2 * 36 / 24)
+    T = -0.5 * g3**2 + 0.5 * (g6 - g3**2) * np.cos(6 * z)
+    T = np.broadcast_to(T, g.phys_shape)
+    assert np.allclose(tau[0], (D - 1) / D * T, atol=1e-14)
+    assert np.allclose(tau[1], -T / D, atol=1e-14)
+    if D == 3:
+        assert np.allclose(tau[2], -T / 3, atol=1e-14)
+        assert np.allclose(tau[3:], 0, atol=1e-14)
+    else:
+        assert np.allclose(tau[2], 0, atol=1e-14)
+
+
+def test_eddy_viscosity_dissipation_and_transfer():
+    """tau = -nu_t (G + G^T): eps = 2 nu_t S:S pointwise, and for solenoidal u
+    the transfer clo = -nu_t k^2 u gives -nu_t sum_shell w k^2 |u|^2"""
+    from oracle import analysis as OA
+
+    g = S.Grid(3, 16)
+    u = solenoidal(g, 31)
+    u = np.stack([S.twothirds(c, g) for c in u])
+    nut = 0.037
+    GG = OC.gradient_phys(u, g)
+    G = OC.grad_matrix(GG, 3)
+    Sm = (G + np.swapaxes(G, 0, 1)) / 2
+    tau = OC.pack_sym(-2 * nut * Sm, 3)
+    eps = OA.sfs_dissipation(tau, GG, 3)
+    assert np.allclose(eps, 2 * nut * np.sum(Sm**2, axis=(0, 1)), rtol=1e-12, atol=1e-14)
+    T = OA.shell_transfer(u, tau, g)
+    e = g.rfft_weight() * g.k2full() * sum(np.abs(u[i]) ** 2 for i in range(3))
+    sh = S.shell_of_modes(g)
+    ref = np.array([-nut * e[sh == i].sum() for i in range(1, len(T) + 1)])
+    assert np.allclose(T, ref, rtol=1e-11, atol=1e-14 * np.abs(ref).max())
+
+
+def test_moments_closed_form():
+    from oracle import analysis as OA
+
+    m = OA.moments([1.0, 2.0, 3.0, 4.0])
+    assert m["mean"] == 2.5 and m["median"] == 2.5 and m["backscatter"] == 0.0
+    assert abs(m["std"] - np.sqrt(5 / 3)) < 1e-15 and abs(m["skewness"]) < 1e-15
+    m = OA.moments([-1.0, 0.0, 0.0, 0.0, 5.0])
+    s = np.sqrt(22.8 / 4)
+    assert abs(m["mean"] - 0.8) < 1e-15 and m["median"] == 0.0 and m["backscatter"] == 0.2
+    assert abs(m["std"] - s) < 1e-14
+    assert abs(m["skewness"] - (-1.8**3 - 3 * 0.8**3 + 4.2**3) / 5 / s**3) < 1e-13
+    assert OA.moments([2.0, 2.0, 2.0])["skewness"] == 0.0
