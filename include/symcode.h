@@ -172,7 +172,8 @@ int sc_apriori_metrics(sc_ctx* ctx, const void* model, const void* ref, long n, 
  * closure: 0 none, 1 Clark(p1 = Delta), 2 Smagorinsky(p1 = C_S, p2 = Delta),
  * 3 dynamic Smagorinsky(p1 = Delta), 4 the learned closure of
  * sc_set_cnn_closure, 5 Verstappen(p1 = C, p2 = Delta), 6 Bardina(p1 = s,
- * p2 = Delta); the band forcing is added when enabled */
+ * p2 = Delta), 7 the pointwise MLP of sc_set_pmlp_closure; the band forcing
+ * is added when enabled */
 int sc_les_rhs(sc_ctx* ctx, void* du, const void* u, double visc, int closure, double p1, double p2);
 int sc_les_wray3(sc_ctx* ctx, void* u, double dt, double visc, int closure, double p1, double p2);
 /* solve_les! with RK4 in place of wray3 (src/les.jl:188-247; RK4 has no
@@ -263,6 +264,51 @@ int sc_cnn_profile_report(sc_cnn* net, char* buf, int buflen);
 int sc_set_cnn_closure(sc_ctx* ctx, sc_cnn* net, double s_in, double s_out, double extra);
 /* the attached learned closure's tau (packed, physical) for the spectral u */
 int sc_closure_cnn(sc_ctx* ctx, void* tau, const void* u);
+
+/* Pointwise MLP closure: the reference's `mlp` net (src/nets.jl:247-298)
+ * behind `fullchain` (src/nets.jl:311-334).  D in {2, 3}; input channels
+ * D^2 (+1 with use_redelta, the standardised log Re_Delta), hidden widths
+ * c1, c2, c3 with GELU, output D == 3 ? 6 : 3 with the `symm` trace removal.
+ * FP32 parameters in Lux's order [W1 (c_in, c_out) column-major, b1, W2, b2,
+ * W3, b3, W4]; voxel matrices channels-last [nvox][c] FP32; nvox_max bounds
+ * every call (activations for that many voxels are kept for backward). */
+typedef struct sc_pmlp sc_pmlp;
+int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out);
+int sc_pmlp_destroy(sc_pmlp* net);
+const char* sc_pmlp_last_error(const sc_pmlp* net);
+/* info[0..7] = D, nin, c1, c2, c3, nout, nparams, nvox_max */
+int sc_pmlp_info(const sc_pmlp* net, int64_t* info);
+int sc_pmlp_set_stream(sc_pmlp* net, void* hip_stream);
+int sc_pmlp_get_params(sc_pmlp* net, float* host);
+int sc_pmlp_set_params(sc_pmlp* net, const float* host);
+int sc_pmlp_get_grads(sc_pmlp* net, float* host);
+int sc_pmlp_zero_grad(sc_pmlp* net);
+/* create_dataloader's pair (src/data.jl:705-741) from the physical velocity
+ * gradient GG (D^2 comps, nonsym order, component stride np) and, when tau
+ * (packed, physical) is given, the target: x = G / (|G| + eps) (+ z),
+ * y = tau / (Delta^2 |G|^2 + eps), z the standardised log Re_Delta of the
+ * snapshot; fp64 != 0 for double fields, eps = eps of the field type;
+ * x [np][nin], y [np][nout] FP32 (the training precision) */
+int sc_pmlp_features(sc_pmlp* net, const void* GG, const void* tau, int fp64, long np, double delta, double z,
+                     void* x, void* y);
+/* y = symm(mlp(x)); keeps the activations for sc_pmlp_backward */
+int sc_pmlp_forward(sc_pmlp* net, const void* x, void* y, long nvox);
+/* accumulate parameter gradients of the last forward for dy = dL/dy */
+int sc_pmlp_backward(sc_pmlp* net, const void* x, const void* dy, long nvox);
+/* kind 0 MSE, kind 1 the relative loss of create_loss (src/training.jl:3-9) */
+int sc_pmlp_loss(sc_pmlp* net, const void* y, const void* target, void* dy, long nvox, int kind, double* loss);
+/* per-array ClipNorm(clip) (clip <= 0: none) then AdamW (src/training.jl:50) */
+int sc_pmlp_adamw(sc_pmlp* net, double lr, double beta1, double beta2, double eps, double weight_decay, double clip);
+/* attach net as closure kind 7 of sc_les_*: fullchain's model
+ * (src/nets.jl:311-334), tau = Delta^2 |G|^2 net(G / (|G| + eps(T)) [, z])
+ * with G = C2R(2/3 grad u); with a Re_Delta input, z = (log Re_Delta -
+ * redelta_mu) / redelta_sigma of the live field, Re_Delta = Delta^2
+ * sqrt(mean |G|^2) / visc.  Inference runs in FP64 (the reference widens the
+ * trained parameters with f64) on FP64 or FP32 fields; net = NULL detaches. */
+int sc_set_pmlp_closure(sc_ctx* ctx, sc_pmlp* net, double delta, double visc, double redelta_mu,
+                        double redelta_sigma);
+/* the attached pointwise closure's tau (packed, physical) for the spectral u */
+int sc_closure_pmlp(sc_ctx* ctx, void* tau, const void* u);
 
 #ifdef __cplusplus
 }

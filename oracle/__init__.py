@@ -28,6 +28,9 @@ Pinning (what checks each function; see ``tests/test_oracle.py``):
 * ``gaussianfilter``: closed form with the |k|^2 < 9 carve-out.
 * ``band forcing``: injection equals the requested rate exactly.
 * ``randomfield``: shell energies equal the prescribed profile.
+* ``pmlp`` (pointwise MLP closure): forward and gradients against torch
+  autograd, fullchain's degree-two homogeneity, the Re_Delta shift
+  (``tests/test_pmlp_oracle.py``).
 
 Functions with no independent check are marked "parity unpinned" in their
 docstrings.

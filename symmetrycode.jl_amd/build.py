@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "symcode", "libsymcode_hip.so")
-SOURCES = ["api.hip", "lib_f64.hip", "lib_f32.hip", "cnn.hip"]
+SOURCES = ["api.hip", "lib_f64.hip", "lib_f32.hip", "cnn.hip", "pmlp.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are disabled for all device code
@@ -47,7 +47,7 @@ def build(jobs=8, verbose=False, defines=(), out=None, bdir=None):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(one, SOURCES))
     tmp = out + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lrocblas", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")

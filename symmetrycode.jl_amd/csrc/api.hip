@@ -4,6 +4,7 @@
 #include <string>
 #include "../../include/symcode.h"
 #include "ops.hpp"
+#include "pmlp_link.hpp"
 #include "cnn_link.hpp"
 
 struct sc_ctx : sc::Ctx {};
@@ -292,7 +293,7 @@ int sc_les_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, in
   return guard(c, [&] {
     need_fft(c);
     need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
-    need(closure >= 0 && closure <= 6, "closure kind");
+    need(closure >= 0 && closure <= 7, "closure kind");
     double d = 0;
     TYPED(c, d = sc::Ops<T>::wray3_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
     if (dt) *dt = d;
@@ -301,7 +302,7 @@ int sc_les_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, in
 int sc_les_rk4(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 6, "closure kind");
+    need(closure >= 0 && closure <= 7, "closure kind");
     TYPED(c, sc::Ops<T>::rk4_dev(*c, (CT*)u, dt, nu, 0.0, closure, p1, p2));
   });
 }
@@ -310,7 +311,7 @@ int sc_les_rk4_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, int 
   return guard(c, [&] {
     need_fft(c);
     need(cfl > 0 && dtmax > 0, "cfl and dtmax must be positive");
-    need(closure >= 0 && closure <= 6, "closure kind");
+    need(closure >= 0 && closure <= 7, "closure kind");
     double d = 0;
     TYPED(c, d = sc::Ops<T>::rk4_dev(*c, (CT*)u, dtmax, nu, cfl, closure, p1, p2));
     if (dt) *dt = d;
@@ -456,14 +457,14 @@ int sc_closure_dynsmag(sc_ctx* c, void* tau, const void* u, double delta, double
 int sc_les_rhs(sc_ctx* c, void* du, const void* u, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 6, "closure kind");
+    need(closure >= 0 && closure <= 7, "closure kind");
     TYPED(c, sc::Ops<T>::les_rhs(*c, (CT*)du, (const CT*)u, nu, closure, p1, p2));
   });
 }
 int sc_les_wray3(sc_ctx* c, void* u, double dt, double nu, int closure, double p1, double p2) {
   return guard(c, [&] {
     need_fft(c);
-    need(closure >= 0 && closure <= 6, "closure kind");
+    need(closure >= 0 && closure <= 7, "closure kind");
     TYPED(c, sc::Ops<T>::les_wray3(*c, (CT*)u, dt, nu, closure, p1, p2));
   });
 }
@@ -502,6 +503,35 @@ int sc_set_cnn_closure(sc_ctx* c, sc_cnn* net, double s_in, double s_out, double
     c->cnn_sin = s_in;
     c->cnn_sout = s_out;
     c->cnn_extra = extra;
+  });
+}
+int sc_set_pmlp_closure(sc_ctx* c, sc_pmlp* net, double delta, double visc, double redelta_mu,
+                        double redelta_sigma) {
+  return guard(c, [&] {
+    if (net) {
+      need(pmlplink::dim(net) == c->D, "the network's dimension differs from the grid's");
+      need(delta > 0, "delta > 0");
+      if (pmlplink::uses_redelta(net))
+        need(visc > 0 && redelta_sigma > 0, "a Re_Delta input needs visc > 0 and redelta_sigma > 0");
+    }
+    c->pmlp = net;
+    c->pmlp_delta = delta;
+    c->pmlp_visc = visc;
+    c->pmlp_mu = redelta_mu;
+    c->pmlp_sigma = redelta_sigma;
+  });
+}
+int sc_closure_pmlp(sc_ctx* c, void* tau, const void* u) {
+  return guard(c, [&] {
+    need(c->pmlp != nullptr, "no network attached (sc_set_pmlp_closure)");
+    need_fft(c);
+    const int ng = c->D * c->D;
+    TYPED(c, {
+      T* GG = reinterpret_cast<T*>(c->work(1, size_t(ng) * c->phys_elems * sizeof(T)));
+      sc::Ops<T>::gradient_phys(*c, GG, (const CT*)u);
+      pmlplink::closure_tau<T>(c->pmlp, GG, c->phys_elems, c->pmlp_delta, c->pmlp_visc, c->pmlp_mu,
+                               c->pmlp_sigma, (T*)tau, c->stream);
+    });
   });
 }
 int sc_closure_cnn(sc_ctx* c, void* tau, const void* u) {

@@ -6,7 +6,8 @@
 #include <stdexcept>
 #include "solver_kernels.hpp"
 
-struct sc_cnn;  // learned closure (cnn.hip)
+struct sc_cnn;   // learned closure (cnn.hip)
+struct sc_pmlp;  // pointwise MLP closure (pmlp.hip)
 
 namespace sc {
 
@@ -55,6 +56,9 @@ struct Ctx {
   // learned closure of les_rhs (closure kind 4): tau = cnn_sout * net(v / cnn_sin, cnn_extra)
   ::sc_cnn* cnn = nullptr;
   double cnn_sin = 1.0, cnn_sout = 1.0, cnn_extra = 0.0;
+  // pointwise MLP closure (closure kind 7): tau = Delta^2 |G|^2 mlp(G / |G|, z)
+  ::sc_pmlp* pmlp = nullptr;
+  double pmlp_delta = 0.0, pmlp_visc = 1.0, pmlp_mu = 0.0, pmlp_sigma = 1.0;
   // forcing applied inside the RHS: 0 none, 1 linear band
   int force_kind = 0;
   double injection = 0.1;

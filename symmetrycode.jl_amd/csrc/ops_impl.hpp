@@ -9,6 +9,7 @@
 #include "fft_passes.hpp"
 #include "rhs_passes.hpp"
 #include "cnn_link.hpp"
+#include "pmlp_link.hpp"
 
 namespace sc {
 
@@ -834,13 +835,14 @@ template <typename T> void Ops<T>::gradient_phys(Ctx& c, T* GG, const CT* u) {
 // les! (src/les.jl:4-33) for the built-in closures:
 // closure 1 = Clark(Delta=p1), 2 = Smagorinsky(C_S=p1, Delta=p2),
 // 3 = dynamic Smagorinsky(Delta=p1), 4 = the attached learned closure,
-// 5 = Verstappen Q-R(C=p1, Delta=p2), 6 = Bardina(s=p1, Delta=p2), 0 = none.
+// 5 = Verstappen Q-R(C=p1, Delta=p2), 6 = Bardina(s=p1, Delta=p2), 7 = the
+// attached pointwise MLP, 0 = none.
 // The band forcing is added when it is enabled on the context.
 template <typename T>
 void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double p1, double p2) {
   const int nt = c.tdim();
   const int ng = c.D * c.D;
-  if (closure < 0 || closure > 6) throw Error(1, "unknown closure kind");
+  if (closure < 0 || closure > 7) throw Error(1, "unknown closure kind");
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, nt));
   nonlinearity(c, B, u, nullptr);
   if (closure != 0) {
@@ -851,6 +853,12 @@ void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double
       cnn_tau(c, tau, u);
     } else if (closure == 6) {
       bardina(c, tau, u, p1, p2);
+    } else if (closure == 7) {
+      if (!c.pmlp) throw Error(1, "closure 7 needs a network attached with sc_set_pmlp_closure");
+      T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));
+      gradient_phys(c, GG, u);
+      pmlplink::closure_tau<T>(c.pmlp, GG, c.phys_elems, c.pmlp_delta, c.pmlp_visc, c.pmlp_mu, c.pmlp_sigma, tau,
+                               c.stream);
     } else {
       T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));
       gradient_phys(c, GG, u);

@@ -1,0 +1,623 @@
+// Pointwise MLP closure: the reference's `mlp` learned closure and its
+// `fullchain` wrapper (src/nets.jl:37-81,247-334; src/data.jl:689-741).
+//
+//   x = [∇ū / (|∇ū| + eps) , z]          (9 or 4 gradient channels, z = the
+//                                          standardised log Re_Δ when used)
+//   h1 = gelu(W1ᵀ x + b1), h2 = gelu(W2ᵀ h1 + b2), h3 = gelu(W3ᵀ h2 + b3)
+//   y = symm(W4ᵀ h3)                      (trace removal of xx, yy[, zz])
+//   τ = Δ² |∇ū|² y                        (packed xx, yy, zz, xy, yz, zx)
+//
+// with |∇ū|² = Σ_ij G_ij² pointwise and eps = eps(Float64).  Training uses the
+// dataloader's pairs (x, y = τ / (Δ² |∇ū|² + eps)) in FP32.  Layout: voxel
+// matrices are channels-fastest, [nvox][c] (= column-major c x nvox, the
+// layout rocBLAS sees); weights are Lux's (c_in, c_out) column-major
+// (c_in fastest), so the flat parameter vector [W1, b1, W2, b2, W3, b3, W4]
+// is Lux's own parameter order.  Every layer is a plain GEMM over the voxels
+// (rocBLAS); bias + GELU, GELU', the output trace removal, the feature
+// normalisation, the loss and the optimiser are this file's kernels.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+#include "../../include/symcode.h"
+#include "pmlp_link.hpp"
+#include "optim.hpp"
+
+namespace pmlp {
+
+#define PM_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define PM_BLAS(call)                                                                   \
+  do {                                                                                  \
+    rocblas_status s_ = (call);                                                         \
+    if (s_ != rocblas_status_success)                                                   \
+      throw std::runtime_error(std::string(#call) + ": " + rocblas_status_to_string(s_)); \
+  } while (0)
+
+namespace {
+
+// tanh-form GELU (NNlib's gelu) and its derivative
+template <typename E>
+__device__ __forceinline__ E gelu(E z) {
+  const E u = E(0.7978845608028654) * (z + E(0.044715) * z * z * z);
+  return E(0.5) * z * (E(1) + tanh(u));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+  const float t = tanhf(u);
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+__device__ __forceinline__ long gid() { return long(blockIdx.x) * blockDim.x + threadIdx.x; }
+
+// create_dataloader's normalised pair of one voxel set (src/data.jl:729-741):
+// x[v][c] = G_c / (sqrt(A2) + eps), x[v][ng] = z (with has_re), and (tau
+// non-null) y[v][c] = tau_c / (Δ² A2 + eps); A2 = Σ_c G_c², eps = eps(T), all
+// in FP64, stored in the FP32 training precision
+template <typename T>
+__global__ void k_features(const T* GG, long np, int ng, double d2, int has_re, float z, const T* tau, int nt,
+                           float* x, int nin, float* y) {
+  const long v = gid();
+  if (v >= np) return;
+  const double eps = double(std::numeric_limits<T>::epsilon());
+  double a2 = 0;
+  for (int c = 0; c < ng; ++c) {
+    const double g = double(GG[c * np + v]);
+    a2 += g * g;
+  }
+  const double den = sqrt(a2) + eps;
+  for (int c = 0; c < ng; ++c) x[v * nin + c] = float(double(GG[c * np + v]) / den);
+  if (has_re) x[v * nin + ng] = z;
+  if (tau) {
+    const double r = d2 * a2 + eps;
+    for (int c = 0; c < nt; ++c) y[v * nt + c] = float(double(tau[c * np + v]) / r);
+  }
+}
+
+// fullchain's input (src/nets.jl:317-325) for voxels [v0, v0 + nv) of the
+// field, FP64: x[i][c] = G_c / (sqrt(A2) + eps(T)), x[i][ng] = *z when z
+template <typename T>
+__global__ void k_features_inf(const T* GG, long np, long v0, long nv, int ng, const double* z, double* x, int nin) {
+  const long i = gid();
+  if (i >= nv) return;
+  const long v = v0 + i;
+  double a2 = 0;
+  for (int c = 0; c < ng; ++c) {
+    const double g = double(GG[c * np + v]);
+    a2 += g * g;
+  }
+  const double den = sqrt(a2) + double(std::numeric_limits<T>::epsilon());
+  for (int c = 0; c < ng; ++c) x[i * nin + c] = double(GG[c * np + v]) / den;
+  if (z) x[i * nin + ng] = *z;
+}
+
+// block partial sums of A2 = Σ_c G_c² over the field (FP64)
+template <typename T>
+__global__ void k_a2_partials(const T* GG, long np, int ng, double* part) {
+  __shared__ double sm[256];
+  double acc = 0;
+  for (long v = gid(); v < np; v += long(gridDim.x) * blockDim.x)
+    for (int c = 0; c < ng; ++c) {
+      const double g = double(GG[c * np + v]);
+      acc += g * g;
+    }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// the live standardised log Re_Δ of fullchain: Re_Δ = Δ² sqrt(mean A2) / ν,
+// z = (log Re_Δ - μ) / σ, rounded to the field precision (append_redelta,
+// src/data.jl:689-694)
+__global__ void k_redelta_z(const double* part, int nblk, long np, double d2, double visc, double mu, double sigma,
+                            int fp32, double* z) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0;
+  for (int i = 0; i < nblk; ++i) s += part[i];
+  const double re = d2 * sqrt(s / double(np)) / visc;
+  const double zz = (log(re) - mu) / sigma;
+  *z = fp32 ? double(float(zz)) : zz;
+}
+
+__global__ void k_widen(const float* p, double* q, long n) {
+  const long i = gid();
+  if (i < n) q[i] = double(p[i]);
+}
+
+// z += b (per channel); a = gelu(z)
+__global__ void k_bias_gelu(float* z, const float* b, float* a, long nv, int co) {
+  const long i = gid();
+  if (i >= nv * co) return;
+  const float t = z[i] + b[i % co];
+  z[i] = t;
+  a[i] = gelu(t);
+}
+
+// inference: z = gelu(z + b) in place
+template <typename E>
+__global__ void k_bias_gelu_inf(E* z, const E* b, long n, int co) {
+  const long i = gid();
+  if (i < n) z[i] = gelu(z[i] + b[i % co]);
+}
+
+// d *= gelu'(z)
+__global__ void k_gelu_back(float* d, const float* z, long n) {
+  const long i = gid();
+  if (i >= n) return;
+  d[i] *= gelu_grad(z[i]);
+}
+
+// trace removal of packed (xx, yy[, zz], ...) rows, in place: the `symm`
+// layer of `mlp` (src/nets.jl:275-293); the map is its own transpose
+template <typename E>
+__global__ void k_tracefree(E* y, long nv, int nt, int D) {
+  const long v = gid();
+  if (v >= nv) return;
+  E* r = y + v * nt;
+  if (D == 3) {
+    const E t = (r[0] + r[1] + r[2]) / E(3);
+    r[0] -= t;
+    r[1] -= t;
+    r[2] -= t;
+  } else {
+    const E t = (r[0] + r[1]) / E(2);
+    r[0] -= t;
+    r[1] -= t;
+  }
+}
+
+// sum of squares of x[0, n) in block partials (FP64)
+__global__ void k_sumsq(const float* x, long n, double* part) {
+  __shared__ double sm[256];
+  double acc = 0;
+  for (long i = gid(); i < n; i += long(gridDim.x) * blockDim.x) acc += double(x[i]) * double(x[i]);
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// squared error partials and dy = scale * (y - t), both [nv][nt]
+__global__ void k_loss(const float* y, const float* t, float* dy, long n, double scale, double* part) {
+  __shared__ double sm[256];
+  double acc = 0;
+  for (long i = gid(); i < n; i += long(gridDim.x) * blockDim.x) {
+    const float e = y[i] - t[i];
+    acc += double(e) * double(e);
+    dy[i] = float(scale * double(e));
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// fullchain's output scaling for voxels [v0, v0 + nv): tau_c = y_c (Δ² A2)
+// in the solver's precision, A2 recomputed from GG
+template <typename T>
+__global__ void k_tau_out(const double* y, const T* GG, long np, long v0, long nv, int ng, int nt, double d2, T* tau) {
+  const long i = gid();
+  if (i >= nv) return;
+  const long v = v0 + i;
+  double a2 = 0;
+  for (int c = 0; c < ng; ++c) {
+    const double g = double(GG[c * np + v]);
+    a2 += g * g;
+  }
+  const double s = d2 * a2;
+  for (int c = 0; c < nt; ++c) tau[c * np + v] = T(y[i * nt + c] * s);
+}
+
+unsigned nb(long n) { return unsigned((n + 255) / 256); }
+
+}  // namespace
+}  // namespace pmlp
+
+struct sc_pmlp {
+  int D = 3, nin = 9, nt = 6, ng = 9, w[3] = {0, 0, 0};
+  bool redelta = false;
+  long nvmax = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  rocblas_handle blas = nullptr;
+  std::string err;
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  long plen = 0;
+  long off[8] = {};  // W1 b1 W2 b2 W3 b3 W4 end
+  // activations of the last forward (z pre-activation, a = gelu(z)) per
+  // hidden layer, two gradient scratch buffers, a vector of ones (bias sums)
+  float *z[3] = {}, *a[3] = {}, *d0 = nullptr, *d1 = nullptr, *ones = nullptr;
+  // FP64 inference (the closure): widened parameters, the live Re_Δ input,
+  // input / ping-pong / output buffers for chunks of up to icap voxels
+  double *pd = nullptr, *zdev = nullptr, *xd = nullptr, *hA = nullptr, *hB = nullptr, *yd = nullptr;
+  long icap = 0;
+  double *part = nullptr, *gnorm = nullptr;
+  int adam_t = 0;
+  long last_nv = 0;
+  int maxw() const { return std::max({w[0], w[1], w[2], nin, nt}); }
+  int cin(int l) const { return l == 0 ? nin : w[l - 1]; }
+  int cout(int l) const { return l == 3 ? nt : w[l]; }
+};
+
+namespace {
+using namespace pmlp;
+using namespace optim;
+thread_local std::string g_pmlp_err;
+
+template <class F>
+int pguard(sc_pmlp* c, F&& f) {
+  if (!c) return 1;
+  (void)hipGetLastError();
+  try {
+    f();
+    PM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  } catch (std::exception& e) {
+    c->err = e.what();
+    return 3;
+  }
+}
+
+// Z (co x nv) = Wᵀ (co x ci) X (ci x nv)
+void gemm_fwd(sc_pmlp* c, int l, const float* X, float* Z, long nv) {
+  const float one = 1.f, zero = 0.f;
+  const int ci = c->cin(l), co = c->cout(l);
+  PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
+                        c->p + c->off[2 * l], ci, X, ci, &zero, Z, co));
+}
+
+void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
+  if (nv < 1 || nv > c->nvmax) throw std::runtime_error("voxel count out of range");
+  const float* in = x;
+  for (int l = 0; l < 3; ++l) {
+    gemm_fwd(c, l, in, c->z[l], nv);
+    hipLaunchKernelGGL(k_bias_gelu, dim3(nb(nv * c->w[l])), dim3(256), 0, c->stream, c->z[l], c->p + c->off[2 * l + 1],
+                       c->a[l], nv, c->w[l]);
+    PM_HIP(hipGetLastError());
+    in = c->a[l];
+  }
+  gemm_fwd(c, 3, in, y, nv);
+  hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
+  PM_HIP(hipGetLastError());
+  c->last_nv = nv;
+}
+
+// accumulate parameter gradients of the last forward for dy = dL/dy
+void backward_impl(sc_pmlp* c, const float* x, const float* dy, long nv) {
+  if (nv != c->last_nv) throw std::runtime_error("backward needs the voxel count of the last forward");
+  const float one = 1.f, zero = 0.f;
+  // output: dL/d(raw) = P(dy) (the trace removal is self-adjoint)
+  float* d = c->d0;
+  PM_HIP(hipMemcpyAsync(d, dy, size_t(nv) * c->nt * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, d, nv, c->nt, c->D);
+  PM_HIP(hipGetLastError());
+  for (int l = 3; l >= 0; --l) {
+    const int ci = c->cin(l), co = c->cout(l);
+    const float* A = l == 0 ? x : c->a[l - 1];
+    // dW (ci x co) += A (ci x nv) dZᵀ (nv x co)
+    PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, ci, co, int(nv), &one, A, ci, d,
+                          co, &one, c->g + c->off[2 * l], ci));
+    if (l < 3) {
+      // db (co) += dZ (co x nv) 1
+      PM_BLAS(rocblas_sgemv(c->blas, rocblas_operation_none, co, int(nv), &one, d, co, c->ones, 1, &one,
+                            c->g + c->off[2 * l + 1], 1));
+    }
+    if (l > 0) {
+      // dA (ci x nv) = W (ci x co) dZ (co x nv), then dZ_{l-1} = dA * gelu'(z_{l-1})
+      float* dn = d == c->d0 ? c->d1 : c->d0;
+      PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, ci, int(nv), co, &one,
+                            c->p + c->off[2 * l], ci, d, co, &zero, dn, ci));
+      hipLaunchKernelGGL(k_gelu_back, dim3(nb(nv * ci)), dim3(256), 0, c->stream, dn, c->z[l - 1], nv * ci);
+      PM_HIP(hipGetLastError());
+      d = dn;
+    }
+  }
+}
+
+double sum_partials(sc_pmlp* c, unsigned nblk) {
+  std::vector<double> h(nblk);
+  PM_HIP(hipMemcpyAsync(h.data(), c->part, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PM_HIP(hipStreamSynchronize(c->stream));
+  double s = 0;
+  for (double v : h) s += v;
+  return s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------ solver-side interface
+namespace pmlplink {
+int grad_channels(const sc_pmlp* c) { return c->ng; }
+int dim(const sc_pmlp* c) { return c->D; }
+bool uses_redelta(const sc_pmlp* c) { return c->redelta; }
+
+// one chunk of FP64 inference: y (nt x nv) = symm(mlp(x (nin x nv)))
+void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
+  const double one = 1.0, zero = 0.0;
+  const double* in = x;
+  double* out = c->hA;
+  for (int l = 0; l < 4; ++l) {
+    const int ci = c->cin(l), co = c->cout(l);
+    double* dst = l == 3 ? y : out;
+    PM_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
+                          c->pd + c->off[2 * l], ci, in, ci, &zero, dst, co));
+    if (l < 3) {
+      hipLaunchKernelGGL(k_bias_gelu_inf<double>, dim3(nb(nv * co)), dim3(256), 0, c->stream, dst,
+                         (const double*)c->pd + c->off[2 * l + 1], nv * co, co);
+      PM_HIP(hipGetLastError());
+      in = dst;
+      out = dst == c->hA ? c->hB : c->hA;
+    }
+  }
+  hipLaunchKernelGGL(k_tracefree<double>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
+  PM_HIP(hipGetLastError());
+}
+
+// fullchain's model (src/nets.jl:316-331) on the solver's stream s: the
+// network runs in FP64 (the reference widens the trained parameters with
+// `f64` before inference), over chunks of at most 2^20 voxels
+template <typename T>
+void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, double mu, double sigma, T* tau,
+                 hipStream_t s) {
+  constexpr long CHUNK = 1L << 20;
+  hipStream_t old = c->stream;
+  c->stream = s;
+  PM_BLAS(rocblas_set_stream(c->blas, s));
+  try {
+    const long cap = std::min(np, CHUNK);
+    if (cap > c->icap) {
+      for (double** q : {&c->xd, &c->hA, &c->hB, &c->yd})
+        if (*q) {
+          PM_HIP(hipFree(*q));
+          *q = nullptr;
+        }
+      c->icap = 0;
+      PM_HIP(hipMalloc(&c->xd, size_t(cap) * c->nin * sizeof(double)));
+      PM_HIP(hipMalloc(&c->hA, size_t(cap) * c->maxw() * sizeof(double)));
+      PM_HIP(hipMalloc(&c->hB, size_t(cap) * c->maxw() * sizeof(double)));
+      PM_HIP(hipMalloc(&c->yd, size_t(cap) * c->nt * sizeof(double)));
+      c->icap = cap;
+    }
+    if (!c->pd) {
+      PM_HIP(hipMalloc(&c->pd, c->plen * sizeof(double)));
+      PM_HIP(hipMalloc(&c->zdev, sizeof(double)));
+    }
+    hipLaunchKernelGGL(k_widen, dim3(nb(c->plen)), dim3(256), 0, s, c->p, c->pd, c->plen);
+    PM_HIP(hipGetLastError());
+    const double d2 = delta * delta;
+    if (c->redelta) {
+      const unsigned nblk = std::min(nb(np), 1024u);
+      hipLaunchKernelGGL(k_a2_partials<T>, dim3(nblk), dim3(256), 0, s, GG, np, c->ng, c->part);
+      PM_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_redelta_z, dim3(1), dim3(64), 0, s, (const double*)c->part, int(nblk), np, d2, visc, mu,
+                         sigma, sizeof(T) == 4 ? 1 : 0, c->zdev);
+      PM_HIP(hipGetLastError());
+    }
+    for (long v0 = 0; v0 < np; v0 += c->icap) {
+      const long nv = std::min(c->icap, np - v0);
+      hipLaunchKernelGGL(k_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->ng,
+                         c->redelta ? (const double*)c->zdev : nullptr, c->xd, c->nin);
+      PM_HIP(hipGetLastError());
+      infer_chunk(c, c->xd, c->yd, nv);
+      hipLaunchKernelGGL(k_tau_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->ng,
+                         c->nt, d2, tau);
+      PM_HIP(hipGetLastError());
+    }
+  } catch (...) {
+    c->stream = old;
+    rocblas_set_stream(c->blas, old);
+    throw;
+  }
+  c->stream = old;
+  PM_BLAS(rocblas_set_stream(c->blas, old));
+}
+template void closure_tau<float>(sc_pmlp*, const float*, long, double, double, double, double, float*, hipStream_t);
+template void closure_tau<double>(sc_pmlp*, const double*, long, double, double, double, double, double*,
+                                  hipStream_t);
+}  // namespace pmlplink
+
+extern "C" {
+
+int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  // every activation matrix (nvox_max x width) stays below 2^31 entries
+  if ((D != 2 && D != 3) || c1 < 1 || c2 < 1 || c3 < 1 || c1 > 1024 || c2 > 1024 || c3 > 1024 || nvox_max < 1 ||
+      nvox_max * std::max({c1, c2, c3, 10}) >= (1L << 31))
+    return 1;
+  g_pmlp_err.clear();
+  (void)hipGetLastError();
+  sc_pmlp* c = new sc_pmlp();
+  c->D = D;
+  c->ng = D * D;
+  c->nt = D == 3 ? 6 : 3;
+  c->redelta = use_redelta != 0;
+  c->nin = c->ng + (c->redelta ? 1 : 0);
+  c->w[0] = c1; c->w[1] = c2; c->w[2] = c3;
+  c->nvmax = nvox_max;
+  try {
+    PM_HIP(hipStreamCreate(&c->stream));
+    c->own_stream = true;
+    PM_BLAS(rocblas_create_handle(&c->blas));
+    PM_BLAS(rocblas_set_stream(c->blas, c->stream));
+    long o = 0;
+    for (int l = 0; l < 4; ++l) {
+      c->off[2 * l] = o;
+      o += long(c->cin(l)) * c->cout(l);
+      if (l < 3) {
+        c->off[2 * l + 1] = o;
+        o += c->cout(l);
+      }
+    }
+    c->off[7] = o;
+    c->plen = o;
+    // glorot_uniform per weight (fan_in + fan_out = c_in + c_out for 1x1
+    // kernels), zero biases, from a splitmix64 stream
+    std::vector<float> h(o, 0.f);
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 777;
+    auto rnd = [&]() {
+      s += 0x9E3779B97F4A7C15ull;
+      uint64_t zz = s;
+      zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+      zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+      zz ^= zz >> 31;
+      return double(zz >> 11) * 0x1.0p-53;
+    };
+    for (int l = 0; l < 4; ++l) {
+      const double lim = std::sqrt(6.0 / (c->cin(l) + c->cout(l)));
+      for (long i = 0; i < long(c->cin(l)) * c->cout(l); ++i) h[c->off[2 * l] + i] = float((2 * rnd() - 1) * lim);
+    }
+    for (float** q : {&c->p, &c->g, &c->m, &c->v}) PM_HIP(hipMalloc(q, o * sizeof(float)));
+    PM_HIP(hipMemcpy(c->p, h.data(), o * sizeof(float), hipMemcpyHostToDevice));
+    for (float* q : {c->g, c->m, c->v}) PM_HIP(hipMemset(q, 0, o * sizeof(float)));
+    const size_t nv = size_t(nvox_max);
+    for (int l = 0; l < 3; ++l) {
+      PM_HIP(hipMalloc(&c->z[l], nv * c->w[l] * sizeof(float)));
+      PM_HIP(hipMalloc(&c->a[l], nv * c->w[l] * sizeof(float)));
+    }
+    PM_HIP(hipMalloc(&c->d0, nv * c->maxw() * sizeof(float)));
+    PM_HIP(hipMalloc(&c->d1, nv * c->maxw() * sizeof(float)));
+    PM_HIP(hipMalloc(&c->ones, nv * sizeof(float)));
+    std::vector<float> one(nv, 1.f);
+    PM_HIP(hipMemcpy(c->ones, one.data(), nv * sizeof(float), hipMemcpyHostToDevice));
+    PM_HIP(hipMalloc(&c->part, 1024 * sizeof(double)));
+    PM_HIP(hipMalloc(&c->gnorm, (7 * NORM_BLOCKS + 8) * sizeof(double)));
+  } catch (std::exception& e) {
+    g_pmlp_err = e.what();
+    sc_pmlp_destroy(c);
+    return 3;
+  }
+  *out = c;
+  return 0;
+}
+
+int sc_pmlp_destroy(sc_pmlp* c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->p, c->g, c->m, c->v, c->d0, c->d1, c->ones, c->pd, c->zdev, c->xd, c->hA, c->hB, c->yd, c->part, c->gnorm};
+  for (void* q : ptrs) if (q) (void)hipFree(q);
+  for (int l = 0; l < 3; ++l) {
+    if (c->z[l]) (void)hipFree(c->z[l]);
+    if (c->a[l]) (void)hipFree(c->a[l]);
+  }
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char* sc_pmlp_last_error(const sc_pmlp* c) { return c ? c->err.c_str() : g_pmlp_err.c_str(); }
+
+int sc_pmlp_info(const sc_pmlp* c, int64_t* info) {
+  if (!c || !info) return 1;
+  info[0] = c->D; info[1] = c->nin; info[2] = c->w[0]; info[3] = c->w[1]; info[4] = c->w[2];
+  info[5] = c->nt; info[6] = c->plen; info[7] = c->nvmax;
+  return 0;
+}
+
+int sc_pmlp_set_stream(sc_pmlp* c, void* s) {
+  return pguard(c, [&] {
+    PM_HIP(hipStreamSynchronize(c->stream));
+    if (c->own_stream) PM_HIP(hipStreamDestroy(c->stream));
+    c->stream = (hipStream_t)s;
+    c->own_stream = false;
+    PM_BLAS(rocblas_set_stream(c->blas, c->stream));
+  });
+}
+
+int sc_pmlp_get_params(sc_pmlp* c, float* host) {
+  return pguard(c, [&] { PM_HIP(hipMemcpyAsync(host, c->p, c->plen * 4, hipMemcpyDeviceToHost, c->stream)); });
+}
+int sc_pmlp_set_params(sc_pmlp* c, const float* host) {
+  return pguard(c, [&] { PM_HIP(hipMemcpyAsync(c->p, host, c->plen * 4, hipMemcpyHostToDevice, c->stream)); });
+}
+int sc_pmlp_get_grads(sc_pmlp* c, float* host) {
+  return pguard(c, [&] { PM_HIP(hipMemcpyAsync(host, c->g, c->plen * 4, hipMemcpyDeviceToHost, c->stream)); });
+}
+int sc_pmlp_zero_grad(sc_pmlp* c) {
+  return pguard(c, [&] { PM_HIP(hipMemsetAsync(c->g, 0, c->plen * 4, c->stream)); });
+}
+
+int sc_pmlp_features(sc_pmlp* c, const void* GG, const void* tau, int fp64, long np, double delta, double z, void* x,
+                     void* y) {
+  return pguard(c, [&] {
+    if (np < 1 || !x || (tau && !y)) throw std::runtime_error("arguments");
+    if (fp64)
+      hipLaunchKernelGGL(k_features<double>, dim3(nb(np)), dim3(256), 0, c->stream, (const double*)GG, np, c->ng,
+                         delta * delta, c->redelta ? 1 : 0, float(z), (const double*)tau, c->nt, (float*)x, c->nin,
+                         (float*)y);
+    else
+      hipLaunchKernelGGL(k_features<float>, dim3(nb(np)), dim3(256), 0, c->stream, (const float*)GG, np, c->ng,
+                         delta * delta, c->redelta ? 1 : 0, float(z), (const float*)tau, c->nt, (float*)x, c->nin,
+                         (float*)y);
+    PM_HIP(hipGetLastError());
+  });
+}
+
+int sc_pmlp_forward(sc_pmlp* c, const void* x, void* y, long nvox) {
+  return pguard(c, [&] { forward_impl(c, (const float*)x, (float*)y, nvox); });
+}
+
+int sc_pmlp_backward(sc_pmlp* c, const void* x, const void* dy, long nvox) {
+  return pguard(c, [&] { backward_impl(c, (const float*)x, (const float*)dy, nvox); });
+}
+
+int sc_pmlp_loss(sc_pmlp* c, const void* y, const void* target, void* dy, long nvox, int kind, double* loss) {
+  return pguard(c, [&] {
+    if (kind != 0 && kind != 1) throw std::runtime_error("loss kind must be 0 (mse) or 1 (relative)");
+    const long n = nvox * c->nt;
+    const unsigned nblk = std::min(nb(n), 1024u);
+    double inv = 1.0 / double(n);
+    if (kind == 1) {
+      hipLaunchKernelGGL(k_sumsq, dim3(nblk), dim3(256), 0, c->stream, (const float*)target, n, c->part);
+      PM_HIP(hipGetLastError());
+      inv = 1.0 / (sum_partials(c, nblk) + double(FLT_EPSILON));
+    }
+    hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, c->stream, (const float*)y, (const float*)target, (float*)dy, n,
+                       2.0 * inv, c->part);
+    PM_HIP(hipGetLastError());
+    const double s = sum_partials(c, nblk);
+    if (loss) *loss = s * inv;
+  });
+}
+
+int sc_pmlp_adamw(sc_pmlp* c, double lr, double b1, double b2, double eps, double weight_decay, double clip) {
+  return pguard(c, [&] {
+    c->adam_t += 1;
+    const float c1 = float(1.0 - std::pow(b1, c->adam_t)), c2 = float(1.0 - std::pow(b2, c->adam_t));
+    Arrays A;
+    for (int i = 0; i < 8; ++i) A.b[i] = c->off[i];
+    double* nrm2 = c->gnorm + 7 * NORM_BLOCKS;
+    if (clip > 0) {
+      hipLaunchKernelGGL(k_array_sumsq, dim3(NORM_BLOCKS, 7), dim3(256), 0, c->stream, c->g, A, c->gnorm);
+      PM_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c->stream, c->gnorm, nrm2);
+      PM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_adamw, dim3(unsigned((c->plen + 255) / 256)), dim3(256), 0, c->stream, c->p, c->g, c->m, c->v,
+                       c->plen, float(lr), float(b1), float(b2), float(eps), float(weight_decay), c1, c2, nrm2, A,
+                       clip);
+    PM_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

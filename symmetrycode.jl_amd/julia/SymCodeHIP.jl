@@ -265,7 +265,6 @@ struct CNNClosure
     s_out::Cdouble
     extra::Cdouble
 end
-const LESModel = Union{BuiltinClosure,CNNClosure}
 kind(m::BuiltinClosure) = m.kind
 kind(::CNNClosure) = Cint(4)
 kind(::Nothing) = Cint(0)
@@ -279,6 +278,67 @@ function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::CNNClosu
     check(ccall((:sc_les_rhs, libsymcode), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
                 u.ctx.ptr, du.ptr, u.ptr, visc, 4, 0, 0), u.ctx.ptr)
+end
+
+# ------------------------------------------------------------ pointwise MLP closure
+"The reference's pointwise `mlp` (src/nets.jl:247-298) on the device (sc_pmlp_*)."
+mutable struct PointwiseMLP
+    ptr::Ptr{Cvoid}
+    D::Int
+end
+function PointwiseMLP(D, nchan; use_redelta = false, same_as_equi = false, seed = 0, nvox_max = 1 << 21)
+    w = same_as_equi ? nchan .* (D == 3 ? 48 : 8) : nchan
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:sc_pmlp_create, libsymcode), Cint,
+               (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
+               D, use_redelta, w[1], w[2], w[3], seed, nvox_max, out)
+    rc == 0 || error("sc_pmlp_create failed ($rc)")
+    net = PointwiseMLP(out[], D)
+    finalizer(x -> ccall((:sc_pmlp_destroy, libsymcode), Cint, (Ptr{Cvoid},), x.ptr), net)
+    net
+end
+
+# Lux parameters of `mlp` (layers lift, mid_1, mid_2, sink; each weight
+# (1,…,1, c_in, c_out)) in the library's flat order [W1, b1, W2, b2, W3, b3, W4]
+function setparams!(net::PointwiseMLP, ps)
+    h = Float32[]
+    for (l, p) in enumerate((ps.lift, ps.mid_1, ps.mid_2, ps.sink))
+        append!(h, vec(p.weight))
+        l < 4 && append!(h, p.bias)
+    end
+    ccall((:sc_pmlp_set_params, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Float32}), net.ptr, h) == 0 ||
+        error("sc_pmlp_set_params failed (parameter count?)")
+    net
+end
+
+adamw!(net::PointwiseMLP, lr; clip = 1.0, β = (0.9, 0.999), ϵ = 1e-8, λ = 0.0) =
+    ccall((:sc_pmlp_adamw, libsymcode), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble),
+          net.ptr, lr, β[1], β[2], ϵ, λ, clip) == 0 || error("sc_pmlp_adamw failed")
+
+"""
+`fullchain(setup, net, ps, st, Δ; use_redelta, redelta_norm)` (src/nets.jl:311-334)
+for a `PointwiseMLP`: les! runs it as closure kind 7 (FP64 inference, live Re_Δ).
+"""
+struct MLPClosure
+    net::PointwiseMLP
+    Δ::Cdouble
+    visc::Cdouble
+    μ::Cdouble
+    σ::Cdouble
+end
+MLPClosure(net::PointwiseMLP, Δ; visc = 0.0, redelta_norm = (; μ = 0.0, σ = 1.0)) =
+    MLPClosure(net, Δ, visc, redelta_norm.μ, redelta_norm.σ)
+kind(::MLPClosure) = Cint(7)
+const LESModel = Union{BuiltinClosure,CNNClosure,MLPClosure}
+attach(m::MLPClosure, ctx) = check(ccall((:sc_set_pmlp_closure, libsymcode), Cint,
+                                         (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble),
+                                         ctx.ptr, m.net.ptr, m.Δ, m.visc, m.μ, m.σ), ctx.ptr)
+
+function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::MLPClosure, visc)
+    attach(model, u.ctx)
+    check(ccall((:sc_les_rhs, libsymcode), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
+                u.ctx.ptr, du.ptr, u.ptr, visc, 7, 0, 0), u.ctx.ptr)
 end
 
 # ------------------------------------------------------------ adjoint

@@ -110,6 +110,21 @@ SIGNATURES = {
     "sc_les_rk4_vjp": [P, P, P, P, c_double, c_double, c_int],
     "sc_les_wray3_vjp": [P, P, P, P, c_double, c_double, c_int],
     "sc_lincomb": [P, P, P, c_double, P, c_double, c_int],
+    "sc_pmlp_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.c_long, ctypes.POINTER(P)],
+    "sc_pmlp_destroy": [P],
+    "sc_pmlp_info": [P, ctypes.POINTER(c_int64)],
+    "sc_pmlp_set_stream": [P, P],
+    "sc_pmlp_get_params": [P, P],
+    "sc_pmlp_set_params": [P, P],
+    "sc_pmlp_get_grads": [P, P],
+    "sc_pmlp_zero_grad": [P],
+    "sc_pmlp_features": [P, P, P, c_int, ctypes.c_long, c_double, c_double, P, P],
+    "sc_pmlp_forward": [P, P, P, ctypes.c_long],
+    "sc_pmlp_backward": [P, P, P, ctypes.c_long],
+    "sc_pmlp_loss": [P, P, P, P, ctypes.c_long, c_int, DP],
+    "sc_pmlp_adamw": [P, c_double, c_double, c_double, c_double, c_double, c_double],
+    "sc_set_pmlp_closure": [P, P, c_double, c_double, c_double, c_double],
+    "sc_closure_pmlp": [P, P, P],
 }
 
 
@@ -134,6 +149,8 @@ def lib():
         L.sc_last_error.restype = ctypes.c_char_p
         L.sc_cnn_last_error.argtypes = [P]
         L.sc_cnn_last_error.restype = ctypes.c_char_p
+        L.sc_pmlp_last_error.argtypes = [P]
+        L.sc_pmlp_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -142,6 +159,12 @@ def check_cnn(rc, net):
     if rc != 0:
         msg = lib().sc_cnn_last_error(net)
         raise SymcodeError(f"symcode cnn error {rc}: {msg.decode() if msg else ''}")
+
+
+def check_pmlp(rc, net):
+    if rc != 0:
+        msg = lib().sc_pmlp_last_error(net)
+        raise SymcodeError(f"symcode pmlp error {rc}: {msg.decode() if msg else ''}")
 
 
 def check(rc, ctx=None):

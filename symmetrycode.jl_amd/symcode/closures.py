@@ -172,7 +172,7 @@ def les_wray3(u, dt, g: Grid, model, visc):
 def _kind(model):
     if model is None:
         return 0, 0.0, 0.0
-    if isinstance(model, LearnedClosure):
+    if hasattr(model, "attach"):
         model.attach()
     return model.kind, model.p1, model.p2
 

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared() if not hasattr(L, n)]
     assert not missing, missing
     # and the ctypes table covers them all
-    assert set(declared()) - {"sc_last_error", "sc_cnn_last_error"} <= set(_lib.SIGNATURES)
+    assert set(declared()) - {"sc_last_error", "sc_cnn_last_error", "sc_pmlp_last_error"} <= set(_lib.SIGNATURES)
 
 
 def test_create_validates_arguments_without_gpu():
