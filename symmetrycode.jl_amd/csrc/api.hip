@@ -68,6 +68,7 @@ int sc_create(int D, int n, double l, int precision, int device, sc_ctx** out) {
   c->phys_elems = long(n) * n * c->nz;
   try {
     SC_HIP(hipSetDevice(device));
+    SC_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     SC_HIP(hipStreamCreate(&c->stream));
     c->own_stream = true;
     bool ok = precision == 64 ? sc::Ops<double>::supported(n) : sc::Ops<float>::supported(n);

@@ -25,6 +25,7 @@ struct Error : std::runtime_error {
 
 struct Ctx {
   int D = 3, n = 0, nh = 0, nhp = 0, nz = 0, prec = 64, device = 0;
+  int ncu = 256;  // compute units of the device (persistent grids)
   double l = 0;
   long spec_elems = 0, phys_elems = 0;  // per component
   hipStream_t stream = nullptr;

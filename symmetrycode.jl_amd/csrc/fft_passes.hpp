@@ -420,4 +420,134 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   }
 }
 
+// Persistent form of k_x_products_w for FP64 rows whose inputs are band-limited
+// to kx <= kin < 64 XP_CHUNKS: wavefront gw walks rows gw, gw + W, ... (W waves
+// in the grid).  Each velocity row is staged in LDS with 16-byte direct
+// global->LDS loads issued one transform ahead of its use (the next component
+// while the current one is transformed, the next row's first component while
+// the current row's products are transformed), so no register holds an
+// in-flight load and the loads' latency overlaps the FFTs.  The staging row is
+// zero beyond kin (entries above kin are loaded from a zero source, those past
+// 64 XP_CHUNKS are cleared once), so the C2R pre-processing reads it without
+// bounds tests.
+#ifndef SC_XP_WAVES
+#define SC_XP_WAVES 4
+#endif
+constexpr int XP_CHUNKS = 5;  // direct loads of 64 entries per staged row
+
+// stg[k] = k <= kin ? src[k] : 0 for k < 64 XP_CHUNKS; lane l of chunk j fills
+// stg[64 j + l] (the destination of a direct load is the wave's base + 16 l)
+DEV void xp_stage(const C<double>* src, const C<double>* zero, C<double>* stg, int kin, int lane) {
+  // the wave's earlier LDS reads of stg have returned (lgkmcnt(0))
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+#pragma unroll
+  for (int j = 0; j < XP_CHUNKS; ++j) {
+    const int k = j * 64 + lane;
+    const C<double>* a = k <= kin ? src + k : zero;
+    __builtin_amdgcn_global_load_lds((const void*)a, (__attribute__((address_space(3))) void*)(stg + j * 64), 16, 0,
+                                     0);
+  }
+}
+
+template <int M, int D>
+__global__ void __launch_bounds__(64 * SC_XP_WAVES, 8 / SC_XP_WAVES)
+k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>* __restrict__ twm,
+               const C<double>* __restrict__ twn, RowGeom g, unsigned long long* vmax_bits) {
+  using T = double;
+  constexpr int TPP = Plan<M>::tpp;
+  static_assert(TPP <= 64, "one wavefront per row");
+  static_assert(64 * XP_CHUNKS <= M + 1, "staged chunks fit the row");
+  constexpr int NW = SC_XP_WAVES;
+  constexpr int NP = D == 3 ? 6 : 3;
+  constexpr int XS = (M + 1 + 7) / 8 * 8;  // staging row: entries 0..M
+  using PI = Pencil<T, M, +1, TPP, true>;
+  using PF = Pencil<T, M, -1, TPP, true>;
+  using PFR = Pencil<T, M, -1, TPP, true, PlanRev<M>>;
+  constexpr bool DIRECT = M / PlanInfo<M>::R(PlanInfo<M>::NS - 1) == TPP;
+  constexpr int E = PlanInfo<M>::E_LAST;
+  __shared__ C<T> lds[NW * M];
+  __shared__ C<T> stg[NW * XS];
+  __shared__ C<T> twm_l[M];
+  __shared__ C<T> twn_l[M + 1];
+  for (int i = threadIdx.x; i < M; i += blockDim.x) twm_l[i] = twm[i];
+  for (int i = threadIdx.x; i <= M; i += blockDim.x) twn_l[i] = twn[i];
+  for (int i = threadIdx.x; i < NW * XS; i += blockDim.x) stg[i] = mk<T>(T(0), T(0));
+  __syncthreads();
+  const C<T>* zero = twn + 2 * M;
+  const int w = threadIdx.x >> 6, lane0 = threadIdx.x & 63;
+  const int p0 = lane0 < TPP ? lane0 : 0;
+  C<T>* ll = lds + w * M;
+  C<T>* sw = stg + w * XS;
+  const long W = long(gridDim.x) * NW;
+  const int kin = g.kin;
+  long row = long(blockIdx.x) * NW + w;
+  if (row < g.nrows) xp_stage(in + row * g.nhp, zero, sw, kin, lane0);
+  double vm = 0.0;
+  constexpr int PA[6] = {0, 1, 2, 0, 1, 2};
+  constexpr int PB[6] = {0, 1, 2, 1, 2, 0};
+  constexpr int QA[3] = {0, 1, 0};
+  constexpr int QB[3] = {0, 1, 1};
+  const T s = T(g.scale);
+  for (; row < g.nrows; row += W) {
+    // opaque copies of the lane indices: every per-lane address is formed
+    // inside the loop instead of being kept live across it
+    int p = p0, lane = lane0;
+    asm volatile("" : "+v"(p), "+v"(lane));
+    C<T> u[D][E];
+    C<T> v[PlanInfo<M>::MAXE > PFR::MAXE ? PlanInfo<M>::MAXE : PFR::MAXE];
+    const long nxt = row + W;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      PI::load(v, p, [&](int k) { return c2r_pre<T, M>(k, twn_l, [&](int kk) { return sw[kk]; }); });
+      // the staging row is free: the next component, or the next row's first
+      // (after the last row, this row's again: unused, but every wave issues
+      // the same loads, so the waits can count them)
+      if (c + 1 < D) xp_stage(in + long(c + 1) * g.s_comp_spec + row * g.nhp, zero, sw, kin, lane);
+      else xp_stage(in + (nxt < g.nrows ? nxt : row) * g.nhp, zero, sw, kin, lane);
+      PI::run(v, p, ll, 1, twm_l);
+#pragma unroll
+      for (int e = 0; e < E; ++e) u[c][e] = v[e];
+    }
+    if (vmax_bits != nullptr) {
+      PI::for_out(p, [&](int e, int) {
+        double a = 0, b = 0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          a += double(u[c][e].x) * double(u[c][e].x);
+          b += double(u[c][e].y) * double(u[c][e].y);
+        }
+        vm = fmax(vm, fmax(a, b));
+      });
+    }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const int i = D == 3 ? PA[q] : QA[q];
+      const int j = D == 3 ? PB[q] : QB[q];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
+      if constexpr (DIRECT) {
+        PFR::run(v, p, ll, 1, twm_l);
+        PFR::out_to_lds(v, p, ll, 1);
+      } else {
+        PI::out_to_in(v, p, ll, 1);
+        PF::run(v, p, ll, 1, twm_l);
+        PF::out_to_lds(v, p, ll, 1);
+      }
+      xsync<true>();
+      C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
+      // lanes past k = M store X[M] again (same address, same value), so
+      // every store is issued and the waits can count them
+#pragma unroll
+      for (int t = 0; t < (M + 64) / 64; ++t) {
+        const int k = min(lane + 64 * t, M);
+        dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn_l), s);
+      }
+    }
+  }
+  if (vmax_bits != nullptr) {
+    for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o));
+    if (lane0 == 0 && vm > 0.0) atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(sqrt(vm)));
+  }
+}
+
 }  // namespace sc

@@ -181,6 +181,9 @@ static void x_r2c(Ctx& c, C<T>* out, const T* in, int ncomp, double scale) {
   SC_HIP(hipGetLastError());
 }
 
+#ifndef SC_XPROD_PREFETCH
+#define SC_XPROD_PREFETCH 1
+#endif
 #ifndef SC_XPROD_WAVE
 #define SC_XPROD_WAVE 1
 #endif
@@ -191,6 +194,22 @@ static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale,
   auto twm = reinterpret_cast<const C<T>*>(c.tw_m);
   auto twn = reinterpret_cast<const C<T>*>(c.tw_n);
   c.prof_begin("x_products");
+  if constexpr (std::is_same<T, double>::value) {
+    if (SC_XPROD_PREFETCH && M == 405 && g.kin < XP_CHUNKS * 64) {
+      // persistent: two blocks of SC_XP_WAVES wavefronts per compute unit
+      constexpr int NW = SC_XP_WAVES;
+      const long blocks = std::min<long>((g.nrows + NW - 1) / NW, 2L * c.ncu);
+      if (c.D == 3)
+        hipLaunchKernelGGL((k_x_products_p<405, 3>), dim3(unsigned(blocks)), dim3(64 * NW), 0, c.stream, in, out,
+                           twm, twn, g, vmax);
+      else
+        hipLaunchKernelGGL((k_x_products_p<405, 2>), dim3(unsigned(blocks)), dim3(64 * NW), 0, c.stream, in, out,
+                           twm, twn, g, vmax);
+      c.prof_end();
+      SC_HIP(hipGetLastError());
+      return;
+    }
+  }
   if (SC_XPROD_WAVE && tpp_rt(M) <= 64) {
     constexpr int NW = SC_XW_WAVES;
     dim3 grid(unsigned((g.nrows + NW - 1) / NW), 1);
@@ -326,9 +345,12 @@ void Ops<T>::init_tables(Ctx& c) {
     long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)M;
     hm[k] = mk<T>(T(std::cos(a)), T(std::sin(a)));
   }
-  SC_HIP(hipMalloc(&c.tw_n, n * sizeof(C<T>)));
+  // tw_n carries four zero entries after its n twiddles (a zero source for
+  // direct-to-LDS loads)
+  hn.resize(n + 4, mk<T>(T(0), T(0)));
+  SC_HIP(hipMalloc(&c.tw_n, (n + 4) * sizeof(C<T>)));
   SC_HIP(hipMalloc(&c.tw_m, (M > 0 ? M : 1) * sizeof(C<T>)));
-  SC_HIP(hipMemcpy(c.tw_n, hn.data(), n * sizeof(C<T>), hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(c.tw_n, hn.data(), (n + 4) * sizeof(C<T>), hipMemcpyHostToDevice));
   SC_HIP(hipMemcpy(c.tw_m, hm.data(), M * sizeof(C<T>), hipMemcpyHostToDevice));
 }
 
