@@ -274,9 +274,17 @@ int sc_closure_cnn(sc_ctx* ctx, void* tau, const void* u);
  * every call (activations for that many voxels are kept for backward). */
 typedef struct sc_pmlp sc_pmlp;
 int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out);
+/* The tensor-basis network: tbnn_net (src/nets.jl:535-556) behind tbnn
+ * (:558-583).  Inputs are the tb_kernel! invariants of the normalised
+ * gradient (5 in 3D, 2 in 2D; +1 with use_redelta), outputs the coefficients
+ * of its deviatoric basis (7 in 3D, 2 in 2D), contracted with the basis to the
+ * packed stress.  Input rows carry the basis tensors after the network
+ * inputs (row width info[9]); every other sc_pmlp_* call takes either kind. */
+int sc_tbnn_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out);
 int sc_pmlp_destroy(sc_pmlp* net);
 const char* sc_pmlp_last_error(const sc_pmlp* net);
-/* info[0..7] = D, nin, c1, c2, c3, nout, nparams, nvox_max */
+/* info[0..9] = D, network inputs, c1, c2, c3, stress components, nparams,
+ * nvox_max, kind (0 mlp, 1 TBNN), input row width */
 int sc_pmlp_info(const sc_pmlp* net, int64_t* info);
 int sc_pmlp_set_stream(sc_pmlp* net, void* hip_stream);
 int sc_pmlp_get_params(sc_pmlp* net, float* host);
@@ -288,10 +296,13 @@ int sc_pmlp_zero_grad(sc_pmlp* net);
  * (packed, physical) is given, the target: x = G / (|G| + eps) (+ z),
  * y = tau / (Delta^2 |G|^2 + eps), z the standardised log Re_Delta of the
  * snapshot; fp64 != 0 for double fields, eps = eps of the field type;
- * x [np][nin], y [np][nout] FP32 (the training precision) */
+ * x [np][row width], y [np][stress components] FP32 (the training precision).
+ * For the TBNN, x rows are [invariants, z, basis (component fastest)]
+ * (create_dataloader_tbnn, src/data.jl:759-805). */
 int sc_pmlp_features(sc_pmlp* net, const void* GG, const void* tau, int fp64, long np, double delta, double z,
                      void* x, void* y);
-/* y = symm(mlp(x)); keeps the activations for sc_pmlp_backward */
+/* y = symm(mlp(x)) (TBNN: the basis contraction of its coefficients);
+ * keeps the activations for sc_pmlp_backward */
 int sc_pmlp_forward(sc_pmlp* net, const void* x, void* y, long nvox);
 /* accumulate parameter gradients of the last forward for dy = dL/dy */
 int sc_pmlp_backward(sc_pmlp* net, const void* x, const void* dy, long nvox);
@@ -303,10 +314,15 @@ int sc_pmlp_adamw(sc_pmlp* net, double lr, double beta1, double beta2, double ep
  * (src/nets.jl:311-334), tau = Delta^2 |G|^2 net(G / (|G| + eps(T)) [, z])
  * with G = C2R(2/3 grad u); with a Re_Delta input, z = (log Re_Delta -
  * redelta_mu) / redelta_sigma of the live field, Re_Delta = Delta^2
- * sqrt(mean |G|^2) / visc.  Inference runs in FP64 (the reference widens the
- * trained parameters with f64) on FP64 or FP32 fields; net = NULL detaches. */
+ * sqrt(mean |G|^2) / visc.  A TBNN handle gives tbnn's model instead.
+ * Inference runs in FP64 (the reference widens the trained parameters with
+ * f64) on FP64 or FP32 fields; net = NULL detaches. */
 int sc_set_pmlp_closure(sc_ctx* ctx, sc_pmlp* net, double delta, double visc, double redelta_mu,
                         double redelta_sigma);
+/* symmetrize_pointwise (src/nets.jl:350-389) for the attached mlp: closure 7
+ * averages R_e^T m(R_e G R_e^T) R_e over the cube's 48 (2D: 8) symmetries.
+ * Attaching a network switches it off. */
+int sc_set_pmlp_symmetrize(sc_ctx* ctx, int on);
 /* the attached pointwise closure's tau (packed, physical) for the spectral u */
 int sc_closure_pmlp(sc_ctx* ctx, void* tau, const void* u);
 

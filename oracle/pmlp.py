@@ -13,6 +13,11 @@ oracle/__init__.py.
   standardised log Re_Delta input (append_redelta, src/data.jl:689-694).
 * ``flat`` / ``unflat``: Lux's parameter order [W1, b1, W2, b2, W3, b3, W4],
   each W column-major (c_in fastest).
+* ``tb_features`` / ``tbnn_forward`` / ``tbnn_backward`` / ``tbnn_model``:
+  tb_kernel!'s invariants and deviatoric basis (src/nets.jl:416-500),
+  tbnn_net and the tbnn closure (:535-583).
+* ``symmetrize``: symmetrize_pointwise (src/nets.jl:350-389) over
+  group_stuff's elements (src/symmetry.jl:42-67).
 
 Pinned (tests/test_pmlp_oracle.py): forward and every gradient against
 torch.autograd of the same network built from torch.nn.functional's
@@ -20,6 +25,13 @@ gelu(approximate="tanh"); the trace removal is idempotent and leaves a
 trace-free output; fullchain is homogeneous of degree two in G, tau(aG) =
 a^2 tau(G), without the Re_Delta input, and with it the input z moves by
 log(a) / sigma; the features have unit row norm up to eps and reproduce tau.
+The TBNN's invariants are unchanged and its basis tensors rotate with any
+orthogonal Q (A -> Q A Q^T), are symmetric and trace-free, and vanish as the
+closed forms say for pure strain and pure rotation; its forward and gradients
+equal torch.autograd's.  The group tables form a group of 48 (8) distinct
+signed permutation matrices closed under products; the symmetrized model is
+exactly equivariant under every element and leaves an equivariant model
+unchanged.
 """
 import math
 
@@ -46,10 +58,13 @@ def tracefree(y, D):
     return y
 
 
-def layer_shapes(D, widths, use_redelta):
-    nin = D * D + (1 if use_redelta else 0)
+def layer_shapes(D, widths, use_redelta, tbnn=False):
     nt = 6 if D == 3 else 3
-    ch = [nin, *widths, nt]
+    if tbnn:
+        nin, nout = (5 if D == 3 else 2) + (1 if use_redelta else 0), (7 if D == 3 else 2)
+    else:
+        nin, nout = D * D + (1 if use_redelta else 0), nt
+    ch = [nin, *widths, nout]
     return [(ch[i], ch[i + 1]) for i in range(4)]
 
 
@@ -86,8 +101,8 @@ def array_bounds(shapes):
     return b
 
 
-def forward(p, x, D):
-    """x [nv][nin] -> (y [nv][nt], cache)"""
+def mlp_core(p, x):
+    """the four PointwiseConv layers without the output head"""
     a = np.asarray(x, np.float64)
     zs, acts = [], [a]
     for l in range(1, 4):
@@ -95,14 +110,12 @@ def forward(p, x, D):
         a = gelu(z)
         zs.append(z)
         acts.append(a)
-    y = tracefree(a @ p["W4"], D)
-    return y, (zs, acts)
+    return a @ p["W4"], (zs, acts)
 
 
-def backward(p, cache, dy, D):
-    """gradients {'W1', 'b1', ..., 'W4'} of sum(dy * y) for the cached forward"""
+def core_backward(p, cache, d):
+    """parameter gradients for the cotangent d of mlp_core's output"""
     zs, acts = cache
-    d = tracefree(dy, D)  # the trace removal is symmetric
     g = {"W4": acts[3].T @ d}
     d = d @ p["W4"].T
     for l in range(3, 0, -1):
@@ -111,6 +124,17 @@ def backward(p, cache, dy, D):
         g[f"b{l}"] = d.sum(axis=0)
         d = d @ p[f"W{l}"].T
     return g
+
+
+def forward(p, x, D):
+    """x [nv][nin] -> (y [nv][nt], cache)"""
+    raw, cache = mlp_core(p, x)
+    return tracefree(raw, D), cache
+
+
+def backward(p, cache, dy, D):
+    """gradients {'W1', 'b1', ..., 'W4'} of sum(dy * y) for the cached forward"""
+    return core_backward(p, cache, tracefree(dy, D))  # the trace removal is symmetric
 
 
 def relative_loss(yhat, y):
@@ -163,3 +187,119 @@ def fullchain(p, GG, delta, D, eps=np.finfo(np.float64).eps, redelta=None):
     s = delta**2 * a2_of(GG).ravel()
     tau = y * s[:, None]
     return np.stack([tau[:, c].reshape(GG.shape[1:]) for c in range(tau.shape[1])])
+
+
+# ------------------------------------------------------------------ TBNN
+def grad_matrices(GG, D):
+    """(np, D, D) matrices A[i][j] = G_ij of the component-major gradient
+    (nonsym order: G_ij at component i + D j)"""
+    G = np.stack([np.asarray(g, np.float64).ravel() for g in GG], axis=1)
+    A = np.empty((G.shape[0], D, D))
+    for i in range(D):
+        for j in range(D):
+            A[:, i, j] = G[:, i + D * j]
+    return A, (G * G).sum(axis=1)
+
+
+def _dev(B):
+    D = B.shape[-1]
+    return B - np.trace(B, axis1=1, axis2=2)[:, None, None] / D * np.eye(D)
+
+
+def _pack(B, D):
+    if D == 3:
+        return np.stack([B[:, 0, 0], B[:, 1, 1], B[:, 2, 2], B[:, 0, 1], B[:, 1, 2], B[:, 2, 0]], axis=1)
+    return np.stack([B[:, 0, 0], B[:, 1, 1], B[:, 0, 1]], axis=1)
+
+
+def tb_features(GG, D, eps=np.finfo(np.float64).eps):
+    """tb_kernel! (src/nets.jl:455-500): (invariants [np][ni], basis
+    [np][nb][nt] packed, |G|^2 [np]) of A = G / (|G| + eps)"""
+    A, a2 = grad_matrices(GG, D)
+    A = A / (np.sqrt(a2) + eps)[:, None, None]
+    At = np.swapaxes(A, 1, 2)
+    S, R = (A + At) / 2, (A - At) / 2
+    tr = lambda M: np.trace(M, axis1=1, axis2=2)
+    SS, RR, SR, RS = S @ S, R @ R, S @ R, R @ S
+    if D == 3:
+        inv = [tr(SS), tr(RR), tr(SS @ S), tr(SR @ R), tr(SS @ R @ R)]
+        bas = [S, SS, RR, SR - RS, RS @ R, RS @ S - SS @ R, RS @ R @ R - RR @ S @ R]
+    else:
+        inv = [tr(SS), tr(RR)]
+        bas = [S, SR - RS]
+    return np.stack(inv, axis=1), np.stack([_pack(_dev(B), D) for B in bas], axis=1), a2
+
+
+def tb_rows(GG, D, z=None, eps=np.finfo(np.float64).eps):
+    """create_dataloader_tbnn's input rows [invariants, z, basis (component
+    fastest)]"""
+    inv, bas, _ = tb_features(GG, D, eps)
+    cols = [inv] + ([np.full((inv.shape[0], 1), z)] if z is not None else []) + [bas.reshape(bas.shape[0], -1)]
+    return np.concatenate(cols, axis=1)
+
+
+def tbnn_forward(p, x, D, nin):
+    """rows x -> (m = sum_b w_b B_b [nv][nt], cache); w = mlp_core(x[:, :nin])"""
+    nt, nb = (6, 7) if D == 3 else (3, 2)
+    w, cache = mlp_core(p, x[:, :nin])
+    B = np.asarray(x[:, nin:], np.float64).reshape(-1, nb, nt)
+    return np.einsum("vb,vbt->vt", w, B), (cache, B)
+
+
+def tbnn_backward(p, cache, dm):
+    core, B = cache
+    return core_backward(p, core, np.einsum("vt,vbt->vb", dm, B))
+
+
+def tbnn_model(p, GG, delta, D, eps=np.finfo(np.float64).eps, redelta=None):
+    """tbnn(net, ps, st, Δ, g; use_redelta, redelta_norm, visc) (src/nets.jl:558-583)"""
+    z = None if redelta is None else redelta_z(GG, delta, *redelta)
+    x = tb_rows(GG, D, z, eps)
+    nin = (5 if D == 3 else 2) + (0 if z is None else 1)
+    m, _ = tbnn_forward(p, x, D, nin)
+    m = m * (delta**2 * a2_of(GG).ravel())[:, None]
+    return np.stack([m[:, c].reshape(GG.shape[1:]) for c in range(m.shape[1])])
+
+
+# ------------------------------------------------- the cube's symmetry group
+def group_elements(D):
+    """group_stuff (src/symmetry.jl:42-67): [(permutation, signs)], the
+    permutation index fastest, 0-based axes"""
+    if D == 3:
+        perms = [(0, 1, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0), (1, 0, 2), (0, 2, 1)]
+        signs = [(1, 1, 1), (-1, 1, 1), (1, -1, 1), (1, 1, -1), (-1, -1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, -1)]
+    else:
+        perms = [(0, 1), (1, 0)]
+        signs = [(1, 1), (-1, 1), (1, -1), (-1, -1)]
+    return [(p, s) for s in signs for p in perms]
+
+
+def element_matrix(p, s):
+    """R with (R A R^T)_ab = s_a s_b A_{p_a p_b}: R_{a, p_a} = s_a"""
+    D = len(p)
+    R = np.zeros((D, D))
+    for a in range(D):
+        R[a, p[a]] = s[a]
+    return R
+
+
+def symmetrize(model, GG, D):
+    """symmetrize_pointwise (src/nets.jl:350-389): 1/|G| sum_g R_g^T
+    model(R_g A R_g^T) R_g for a pointwise model(GG) -> packed tau"""
+    names = [(i, j) for j in range(D) for i in range(D)]  # nonsym order: i + D j
+    comps = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)] if D == 3 else [(0, 0), (1, 1), (0, 1)]
+    pidx = {}
+    for t, (a, b) in enumerate(comps):
+        pidx[(a, b)] = pidx[(b, a)] = t
+    acc = None
+    els = group_elements(D)
+    for p, s in els:
+        Arot = np.stack([s[a] * s[b] * GG[p[a] + D * p[b]] for (a, b) in names])
+        y = model(Arot)
+        q = [0] * D
+        for a in range(D):
+            q[p[a]] = a
+        qs = [s[q[a]] for a in range(D)]
+        term = np.stack([qs[a] * qs[b] * y[pidx[(q[a], q[b])]] for (a, b) in comps])
+        acc = term if acc is None else acc + term
+    return acc / len(els)

@@ -516,10 +516,17 @@ int sc_set_pmlp_closure(sc_ctx* c, sc_pmlp* net, double delta, double visc, doub
         need(visc > 0 && redelta_sigma > 0, "a Re_Delta input needs visc > 0 and redelta_sigma > 0");
     }
     c->pmlp = net;
+    c->pmlp_sym = 0;
     c->pmlp_delta = delta;
     c->pmlp_visc = visc;
     c->pmlp_mu = redelta_mu;
     c->pmlp_sigma = redelta_sigma;
+  });
+}
+int sc_set_pmlp_symmetrize(sc_ctx* c, int on) {
+  return guard(c, [&] {
+    need(!on || (c->pmlp && pmlplink::arch(c->pmlp) == 0), "the group average needs an attached mlp network");
+    c->pmlp_sym = on ? 1 : 0;
   });
 }
 int sc_closure_pmlp(sc_ctx* c, void* tau, const void* u) {
@@ -531,7 +538,7 @@ int sc_closure_pmlp(sc_ctx* c, void* tau, const void* u) {
       T* GG = reinterpret_cast<T*>(c->work(1, size_t(ng) * c->phys_elems * sizeof(T)));
       sc::Ops<T>::gradient_phys(*c, GG, (const CT*)u);
       pmlplink::closure_tau<T>(c->pmlp, GG, c->phys_elems, c->pmlp_delta, c->pmlp_visc, c->pmlp_mu,
-                               c->pmlp_sigma, (T*)tau, c->stream);
+                               c->pmlp_sigma, c->pmlp_sym, (T*)tau, c->stream);
     });
   });
 }

@@ -60,6 +60,7 @@ struct Ctx {
   // pointwise MLP closure (closure kind 7): tau = Delta^2 |G|^2 mlp(G / |G|, z)
   ::sc_pmlp* pmlp = nullptr;
   double pmlp_delta = 0.0, pmlp_visc = 1.0, pmlp_mu = 0.0, pmlp_sigma = 1.0;
+  int pmlp_sym = 0;  // group-average the attached mlp (symmetrize_pointwise)
   // forcing applied inside the RHS: 0 none, 1 linear band
   int force_kind = 0;
   double injection = 0.1;

@@ -879,7 +879,7 @@ void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double
       if (!c.pmlp) throw Error(1, "closure 7 needs a network attached with sc_set_pmlp_closure");
       T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));
       gradient_phys(c, GG, u);
-      pmlplink::closure_tau<T>(c.pmlp, GG, c.phys_elems, c.pmlp_delta, c.pmlp_visc, c.pmlp_mu, c.pmlp_sigma, tau,
+      pmlplink::closure_tau<T>(c.pmlp, GG, c.phys_elems, c.pmlp_delta, c.pmlp_visc, c.pmlp_mu, c.pmlp_sigma, c.pmlp_sym, tau,
                                c.stream);
     } else {
       T* GG = reinterpret_cast<T*>(c.work(1, size_t(ng) * c.phys_elems * sizeof(T)));

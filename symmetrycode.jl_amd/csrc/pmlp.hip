@@ -227,6 +227,240 @@ __global__ void k_tau_out(const double* y, const T* GG, long np, long v0, long n
   for (int c = 0; c < nt; ++c) tau[c * np + v] = T(y[i * nt + c] * s);
 }
 
+// ------------------------------------------------ the cube's symmetry group
+// group_stuff (src/symmetry.jl:42-67): element e = (permutation e % NPERM,
+// signs e / NPERM), in the reference's order; (R A Rᵀ)_ab = s_a s_b A_{p_a p_b}
+__constant__ int c_perm3[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}, {1, 0, 2}, {0, 2, 1}};
+__constant__ int c_sign3[8][3] = {{1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {1, 1, -1},
+                                  {-1, -1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
+__constant__ int c_perm2[2][2] = {{0, 1}, {1, 0}};
+__constant__ int c_sign2[4][2] = {{1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+
+__device__ __forceinline__ void group_elem(int D, int e, int* p, int* sg) {
+  if (D == 3) {
+    const int pi = e % 6, si = e / 6;
+    for (int a = 0; a < 3; ++a) {
+      p[a] = c_perm3[pi][a];
+      sg[a] = c_sign3[si][a];
+    }
+  } else {
+    const int pi = e % 2, si = e / 2;
+    for (int a = 0; a < 2; ++a) {
+      p[a] = c_perm2[pi][a];
+      sg[a] = c_sign2[si][a];
+    }
+  }
+}
+// packed index of the symmetric component (a, b): xx, yy, zz, xy, yz, zx (3D)
+// or xx, yy, xy (2D)
+__device__ __forceinline__ int packed_index(int D, int a, int b) {
+  if (a == b) return a;
+  if (D == 2) return 2;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo == 0 ? (hi == 1 ? 3 : 5) : 4;
+}
+
+// symmetrize_pointwise's inputs (src/nets.jl:350-389) for voxels [v0, v0 + nv)
+// and every group element: row e nv + i holds fullchain's input for R_e A R_eᵀ
+// of voxel v0 + i (its norm taken over the rotated components, in order)
+template <typename T>
+__global__ void k_features_sym(const T* GG, long np, long v0, long nv, int D, int G, const double* z, double* x,
+                               int nin) {
+  const long id = gid();
+  if (id >= nv * G) return;
+  const int e = int(id / nv);
+  const long v = v0 + (id - long(e) * nv);
+  int p[3], sg[3];
+  group_elem(D, e, p, sg);
+  double r[9];
+  for (int b = 0; b < D; ++b)
+    for (int a = 0; a < D; ++a) r[a + D * b] = double(T(sg[a] * sg[b]) * GG[long(p[a] + D * p[b]) * np + v]);
+  const int ng = D * D;
+  double a2 = 0;
+  for (int k = 0; k < ng; ++k) a2 += r[k] * r[k];
+  const double den = sqrt(a2) + double(std::numeric_limits<T>::epsilon());
+  for (int k = 0; k < ng; ++k) x[id * nin + k] = r[k] / den;
+  if (z) x[id * nin + ng] = *z;
+}
+
+// the group average of the conjugated-back outputs, scaled to tau:
+// tau_ab = Δ² A2 / |G| Σ_e qs_a qs_b y_e[q_a q_b], (q, qs) the inverse element
+template <typename T>
+__global__ void k_sym_out(const double* y, const T* GG, long np, long v0, long nv, int D, int nt, int G, double d2,
+                          T* tau) {
+  const long i = gid();
+  if (i >= nv) return;
+  const long v = v0 + i;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int ca[6] = {0, 1, 2, 0, 1, 2}, cb[6] = {0, 1, 2, 1, 2, 0};
+  const int da[3] = {0, 1, 0}, db[3] = {0, 1, 1};
+  for (int e = 0; e < G; ++e) {
+    int p[3], sg[3], q[3], qs[3];
+    group_elem(D, e, p, sg);
+    for (int a = 0; a < D; ++a) q[p[a]] = a;
+    for (int a = 0; a < D; ++a) qs[a] = sg[q[a]];
+    const double* row = y + (long(e) * nv + i) * nt;
+    for (int t = 0; t < nt; ++t) {
+      const int a = D == 3 ? ca[t] : da[t], b = D == 3 ? cb[t] : db[t];
+      acc[t] += double(qs[a] * qs[b]) * row[packed_index(D, q[a], q[b])];
+    }
+  }
+  double a2 = 0;
+  for (int c = 0; c < D * D; ++c) {
+    const double g = double(GG[c * np + v]);
+    a2 += g * g;
+  }
+  const double sc = d2 * a2;
+  for (int t = 0; t < nt; ++t) tau[t * np + v] = T(acc[t] / double(G) * sc);
+}
+
+// ------------------------------------------------------------------ TBNN
+// tb_kernel! (src/nets.jl:455-500) on one voxel's gradient G (nonsym order,
+// G_ij at i + D j): A = G / (|G| + eps), S, R its symmetric and antisymmetric
+// parts; inv[ni] the invariants, bas[t + nt b] the deviatoric basis tensors
+// (packed); returns |G|² (FP64)
+struct Mat3 {
+  double m[3][3];
+};
+__device__ __forceinline__ Mat3 mmul(const Mat3& A, const Mat3& B) {
+  Mat3 C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
+  return C;
+}
+__device__ __forceinline__ Mat3 msub(const Mat3& A, const Mat3& B) {
+  Mat3 C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.m[i][j] = A.m[i][j] - B.m[i][j];
+  return C;
+}
+__device__ __forceinline__ double mtr(const Mat3& A) { return A.m[0][0] + A.m[1][1] + A.m[2][2]; }
+// deviator, packed xx yy zz xy yz zx at out[t * nt]
+__device__ __forceinline__ void put_dev3(const Mat3& B, double* out, int stride) {
+  const double t = mtr(B) / 3;
+  out[0 * stride] = B.m[0][0] - t;
+  out[1 * stride] = B.m[1][1] - t;
+  out[2 * stride] = B.m[2][2] - t;
+  out[3 * stride] = B.m[0][1];
+  out[4 * stride] = B.m[1][2];
+  out[5 * stride] = B.m[2][0];
+}
+
+template <typename T>
+__device__ double tb_voxel(int D, const T* GG, long np, long v, double* inv, double* bas) {
+  const int ng = D * D;
+  double g[9];
+  double a2 = 0;
+  for (int c = 0; c < ng; ++c) {
+    g[c] = double(GG[c * np + v]);
+    a2 += g[c] * g[c];
+  }
+  const double den = sqrt(a2) + double(std::numeric_limits<T>::epsilon());
+  if (D == 3) {
+    Mat3 S, R;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const double aij = g[i + 3 * j] / den, aji = g[j + 3 * i] / den;
+        S.m[i][j] = (aij + aji) / 2;
+        R.m[i][j] = (aij - aji) / 2;
+      }
+    const Mat3 SS = mmul(S, S), RR = mmul(R, R), SR = mmul(S, R), RS = mmul(R, S);
+    inv[0] = mtr(SS);
+    inv[1] = mtr(RR);
+    inv[2] = mtr(mmul(SS, S));
+    inv[3] = mtr(mmul(SR, R));
+    inv[4] = mtr(mmul(mmul(SS, R), R));
+    put_dev3(S, bas + 0 * 6, 1);
+    put_dev3(SS, bas + 1 * 6, 1);
+    put_dev3(RR, bas + 2 * 6, 1);
+    put_dev3(msub(SR, RS), bas + 3 * 6, 1);
+    const Mat3 RSR = mmul(RS, R);
+    put_dev3(RSR, bas + 4 * 6, 1);
+    put_dev3(msub(mmul(RS, S), mmul(SS, R)), bas + 5 * 6, 1);
+    put_dev3(msub(mmul(RSR, R), mmul(mmul(RR, S), R)), bas + 6 * 6, 1);
+  } else {
+    // 2D: A = [xx xy; yx yy] with G nonsym order xx, yx, xy, yy
+    const double axx = g[0] / den, ayx = g[1] / den, axy = g[2] / den, ayy = g[3] / den;
+    const double s00 = axx, s11 = ayy, s01 = (axy + ayx) / 2, r01 = (axy - ayx) / 2;
+    // S = [s00 s01; s01 s11], R = [0 r01; -r01 0]
+    inv[0] = s00 * s00 + s01 * s01 + s01 * s01 + s11 * s11;  // tr(S S)
+    inv[1] = -r01 * r01 - r01 * r01;                          // tr(R R)
+    const double t = (s00 + s11) / 2;
+    bas[0] = s00 - t;
+    bas[1] = s11 - t;
+    bas[2] = s01;
+    // S R - R S
+    const double c00 = s01 * (-r01) - r01 * s01;
+    const double c01 = s00 * r01 - r01 * s11;
+    const double c11 = s01 * r01 - (-r01) * s01;
+    const double t2 = (c00 + c11) / 2;
+    bas[3] = c00 - t2;
+    bas[4] = c11 - t2;
+    bas[5] = c01;
+  }
+  return a2;
+}
+
+// create_dataloader_tbnn's pair (src/data.jl:759-805): x[v] = [invariants,
+// z, basis (t + nt b)], y = tau / (Δ² A2 + eps), FP32
+template <typename T>
+__global__ void k_tb_features(const T* GG, long np, int D, double d2, int has_re, float z, const T* tau, int nt,
+                              float* x, int ctot, int ni, int nbas, float* y) {
+  const long v = gid();
+  if (v >= np) return;
+  double inv[5], bas[42];
+  const double a2 = tb_voxel<T>(D, GG, np, v, inv, bas);
+  float* r = x + v * ctot;
+  for (int k = 0; k < ni; ++k) r[k] = float(inv[k]);
+  const int o = ni + (has_re ? 1 : 0);
+  if (has_re) r[ni] = z;
+  for (int k = 0; k < nt * nbas; ++k) r[o + k] = float(bas[k]);
+  if (tau) {
+    const double den = d2 * a2 + double(std::numeric_limits<T>::epsilon());
+    for (int c = 0; c < nt; ++c) y[v * nt + c] = float(double(tau[c * np + v]) / den);
+  }
+}
+
+// the TBNN model's input (tbnn, src/nets.jl:558-583) for voxels [v0, v0 + nv), FP64
+template <typename T>
+__global__ void k_tb_features_inf(const T* GG, long np, long v0, long nv, int D, const double* z, double* x, int ctot,
+                                  int ni, int nt, int nbas) {
+  const long i = gid();
+  if (i >= nv) return;
+  double inv[5], bas[42];
+  tb_voxel<T>(D, GG, np, v0 + i, inv, bas);
+  double* r = x + i * ctot;
+  for (int k = 0; k < ni; ++k) r[k] = inv[k];
+  const int o = ni + (z ? 1 : 0);
+  if (z) r[ni] = *z;
+  for (int k = 0; k < nt * nbas; ++k) r[o + k] = bas[k];
+}
+
+// basis contraction y[v][t] = Σ_b w[v][b] x[v][boff + t + nt b]
+template <typename E>
+__global__ void k_tb_contract(const E* w, const E* x, long nv, int ctot, int boff, int nt, int nbas, E* y) {
+  const long v = gid();
+  if (v >= nv) return;
+  const E* bx = x + v * ctot + boff;
+  for (int t = 0; t < nt; ++t) {
+    E m = E(0);
+    for (int b = 0; b < nbas; ++b) m += bx[t + nt * b] * w[v * nbas + b];
+    y[v * nt + t] = m;
+  }
+}
+
+// its transpose: dw[v][b] = Σ_t dy[v][t] x[v][boff + t + nt b]
+__global__ void k_tb_dw(const float* dy, const float* x, long nv, int ctot, int boff, int nt, int nbas, float* dw) {
+  const long v = gid();
+  if (v >= nv) return;
+  const float* bx = x + v * ctot + boff;
+  for (int b = 0; b < nbas; ++b) {
+    float a = 0.f;
+    for (int t = 0; t < nt; ++t) a += dy[v * nt + t] * bx[t + nt * b];
+    dw[v * nbas + b] = a;
+  }
+}
+
 unsigned nb(long n) { return unsigned((n + 255) / 256); }
 
 }  // namespace
@@ -234,6 +468,10 @@ unsigned nb(long n) { return unsigned((n + 255) / 256); }
 
 struct sc_pmlp {
   int D = 3, nin = 9, nt = 6, ng = 9, w[3] = {0, 0, 0};
+  // arch 0: mlp (output nt, trace removal); 1: TBNN (inputs ni invariants
+  // [+ z], output the nbas basis coefficients, rows of the input matrix carry
+  // the basis tensors after the network inputs: ctot = nin + nt nbas)
+  int arch = 0, ni = 0, nbas = 0, ctot = 9;
   bool redelta = false;
   long nvmax = 0;
   hipStream_t stream = nullptr;
@@ -253,9 +491,11 @@ struct sc_pmlp {
   double *part = nullptr, *gnorm = nullptr;
   int adam_t = 0;
   long last_nv = 0;
-  int maxw() const { return std::max({w[0], w[1], w[2], nin, nt}); }
+  int nout() const { return arch == 1 ? nbas : nt; }
+  int maxw() const { return std::max({w[0], w[1], w[2], nin, nt, nout()}); }
   int cin(int l) const { return l == 0 ? nin : w[l - 1]; }
-  int cout(int l) const { return l == 3 ? nt : w[l]; }
+  int cout(int l) const { return l == 3 ? nout() : w[l]; }
+  int ldin(int l) const { return l == 0 ? ctot : cin(l); }
 };
 
 namespace {
@@ -277,12 +517,12 @@ int pguard(sc_pmlp* c, F&& f) {
   }
 }
 
-// Z (co x nv) = Wᵀ (co x ci) X (ci x nv)
+// Z (co x nv) = Wᵀ (co x ci) X (ci x nv, leading dimension ldin(l))
 void gemm_fwd(sc_pmlp* c, int l, const float* X, float* Z, long nv) {
   const float one = 1.f, zero = 0.f;
   const int ci = c->cin(l), co = c->cout(l);
   PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                        c->p + c->off[2 * l], ci, X, ci, &zero, Z, co));
+                        c->p + c->off[2 * l], ci, X, c->ldin(l), &zero, Z, co));
 }
 
 void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
@@ -295,8 +535,15 @@ void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
     PM_HIP(hipGetLastError());
     in = c->a[l];
   }
-  gemm_fwd(c, 3, in, y, nv);
-  hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
+  if (c->arch == 1) {
+    // basis coefficients, then the contraction with the rows' basis tensors
+    gemm_fwd(c, 3, in, c->d1, nv);
+    hipLaunchKernelGGL(k_tb_contract<float>, dim3(nb(nv)), dim3(256), 0, c->stream, (const float*)c->d1, x, nv, c->ctot,
+                       c->nin, c->nt, c->nbas, y);
+  } else {
+    gemm_fwd(c, 3, in, y, nv);
+    hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
+  }
   PM_HIP(hipGetLastError());
   c->last_nv = nv;
 }
@@ -305,17 +552,22 @@ void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
 void backward_impl(sc_pmlp* c, const float* x, const float* dy, long nv) {
   if (nv != c->last_nv) throw std::runtime_error("backward needs the voxel count of the last forward");
   const float one = 1.f, zero = 0.f;
-  // output: dL/d(raw) = P(dy) (the trace removal is self-adjoint)
   float* d = c->d0;
-  PM_HIP(hipMemcpyAsync(d, dy, size_t(nv) * c->nt * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-  hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, d, nv, c->nt, c->D);
+  if (c->arch == 1) {
+    // dL/dw_b = Σ_t dy_t basis_tb
+    hipLaunchKernelGGL(k_tb_dw, dim3(nb(nv)), dim3(256), 0, c->stream, dy, x, nv, c->ctot, c->nin, c->nt, c->nbas, d);
+  } else {
+    // dL/d(raw) = P(dy) (the trace removal is self-adjoint)
+    PM_HIP(hipMemcpyAsync(d, dy, size_t(nv) * c->nt * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, d, nv, c->nt, c->D);
+  }
   PM_HIP(hipGetLastError());
   for (int l = 3; l >= 0; --l) {
     const int ci = c->cin(l), co = c->cout(l);
     const float* A = l == 0 ? x : c->a[l - 1];
     // dW (ci x co) += A (ci x nv) dZᵀ (nv x co)
-    PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, ci, co, int(nv), &one, A, ci, d,
-                          co, &one, c->g + c->off[2 * l], ci));
+    PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, ci, co, int(nv), &one, A,
+                          c->ldin(l), d, co, &one, c->g + c->off[2 * l], ci));
     if (l < 3) {
       // db (co) += dZ (co x nv) 1
       PM_BLAS(rocblas_sgemv(c->blas, rocblas_operation_none, co, int(nv), &one, d, co, c->ones, 1, &one,
@@ -349,17 +601,20 @@ namespace pmlplink {
 int grad_channels(const sc_pmlp* c) { return c->ng; }
 int dim(const sc_pmlp* c) { return c->D; }
 bool uses_redelta(const sc_pmlp* c) { return c->redelta; }
+int arch(const sc_pmlp* c) { return c->arch; }
 
-// one chunk of FP64 inference: y (nt x nv) = symm(mlp(x (nin x nv)))
+// one chunk of FP64 inference on nv input rows (leading dimension ctot):
+// y (nt x nv) = symm(mlp(x)) or, for the TBNN, the basis contraction of the
+// coefficients
 void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
   const double one = 1.0, zero = 0.0;
   const double* in = x;
   double* out = c->hA;
   for (int l = 0; l < 4; ++l) {
     const int ci = c->cin(l), co = c->cout(l);
-    double* dst = l == 3 ? y : out;
+    double* dst = l == 3 && c->arch == 0 ? y : out;
     PM_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                          c->pd + c->off[2 * l], ci, in, ci, &zero, dst, co));
+                          c->pd + c->off[2 * l], ci, in, l == 0 ? c->ctot : ci, &zero, dst, co));
     if (l < 3) {
       hipLaunchKernelGGL(k_bias_gelu_inf<double>, dim3(nb(nv * co)), dim3(256), 0, c->stream, dst,
                          (const double*)c->pd + c->off[2 * l + 1], nv * co, co);
@@ -368,22 +623,31 @@ void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
       out = dst == c->hA ? c->hB : c->hA;
     }
   }
-  hipLaunchKernelGGL(k_tracefree<double>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
+  if (c->arch == 1)
+    hipLaunchKernelGGL(k_tb_contract<double>, dim3(nb(nv)), dim3(256), 0, c->stream, (const double*)out, x, nv,
+                       c->ctot, c->nin, c->nt, c->nbas, y);
+  else
+    hipLaunchKernelGGL(k_tracefree<double>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
   PM_HIP(hipGetLastError());
 }
 
-// fullchain's model (src/nets.jl:316-331) on the solver's stream s: the
-// network runs in FP64 (the reference widens the trained parameters with
-// `f64` before inference), over chunks of at most 2^20 voxels
+// fullchain's model (src/nets.jl:316-331), or tbnn's (:558-583), on the
+// solver's stream s: the network runs in FP64 (the reference widens the
+// trained parameters with `f64` before inference), over chunks of at most
+// 2^20 input rows; sym != 0 averages the mlp over the cube's symmetry group
+// (symmetrize_pointwise, src/nets.jl:350-389), |G| rows per voxel
 template <typename T>
-void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, double mu, double sigma, T* tau,
-                 hipStream_t s) {
+void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, double mu, double sigma, int sym,
+                 T* tau, hipStream_t s) {
   constexpr long CHUNK = 1L << 20;
+  if (sym && c->arch != 0) throw std::runtime_error("the group average applies to the mlp closure");
+  const int G = sym ? (c->D == 3 ? 48 : 8) : 1;
   hipStream_t old = c->stream;
   c->stream = s;
   PM_BLAS(rocblas_set_stream(c->blas, s));
   try {
-    const long cap = std::min(np, CHUNK);
+    const long vc = std::min(np, CHUNK / G);  // voxels per chunk
+    const long cap = vc * G;                  // input rows per chunk
     if (cap > c->icap) {
       for (double** q : {&c->xd, &c->hA, &c->hB, &c->yd})
         if (*q) {
@@ -391,7 +655,7 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
           *q = nullptr;
         }
       c->icap = 0;
-      PM_HIP(hipMalloc(&c->xd, size_t(cap) * c->nin * sizeof(double)));
+      PM_HIP(hipMalloc(&c->xd, size_t(cap) * c->ctot * sizeof(double)));
       PM_HIP(hipMalloc(&c->hA, size_t(cap) * c->maxw() * sizeof(double)));
       PM_HIP(hipMalloc(&c->hB, size_t(cap) * c->maxw() * sizeof(double)));
       PM_HIP(hipMalloc(&c->yd, size_t(cap) * c->nt * sizeof(double)));
@@ -412,14 +676,26 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
                          sigma, sizeof(T) == 4 ? 1 : 0, c->zdev);
       PM_HIP(hipGetLastError());
     }
-    for (long v0 = 0; v0 < np; v0 += c->icap) {
-      const long nv = std::min(c->icap, np - v0);
-      hipLaunchKernelGGL(k_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->ng,
-                         c->redelta ? (const double*)c->zdev : nullptr, c->xd, c->nin);
+    const double* zp = c->redelta ? (const double*)c->zdev : nullptr;
+    for (long v0 = 0; v0 < np; v0 += vc) {
+      const long nv = std::min(vc, np - v0);
+      if (c->arch == 1)
+        hipLaunchKernelGGL(k_tb_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->D, zp, c->xd,
+                           c->ctot, c->ni, c->nt, c->nbas);
+      else if (sym)
+        hipLaunchKernelGGL(k_features_sym<T>, dim3(nb(nv * G)), dim3(256), 0, s, GG, np, v0, nv, c->D, G, zp, c->xd,
+                           c->nin);
+      else
+        hipLaunchKernelGGL(k_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->ng, zp, c->xd,
+                           c->nin);
       PM_HIP(hipGetLastError());
-      infer_chunk(c, c->xd, c->yd, nv);
-      hipLaunchKernelGGL(k_tau_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->ng,
-                         c->nt, d2, tau);
+      infer_chunk(c, c->xd, c->yd, nv * G);
+      if (sym)
+        hipLaunchKernelGGL(k_sym_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->D,
+                           c->nt, G, d2, tau);
+      else
+        hipLaunchKernelGGL(k_tau_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->ng,
+                           c->nt, d2, tau);
       PM_HIP(hipGetLastError());
     }
   } catch (...) {
@@ -430,28 +706,41 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
   c->stream = old;
   PM_BLAS(rocblas_set_stream(c->blas, old));
 }
-template void closure_tau<float>(sc_pmlp*, const float*, long, double, double, double, double, float*, hipStream_t);
-template void closure_tau<double>(sc_pmlp*, const double*, long, double, double, double, double, double*,
+template void closure_tau<float>(sc_pmlp*, const float*, long, double, double, double, double, int, float*,
+                                 hipStream_t);
+template void closure_tau<double>(sc_pmlp*, const double*, long, double, double, double, double, int, double*,
                                   hipStream_t);
 }  // namespace pmlplink
 
 extern "C" {
 
-int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out) {
+namespace {
+int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max,
+                sc_pmlp** out) {
   if (!out) return 1;
   *out = nullptr;
-  // every activation matrix (nvox_max x width) stays below 2^31 entries
+  // every activation and input matrix (nvox_max x width) stays below 2^31 entries
   if ((D != 2 && D != 3) || c1 < 1 || c2 < 1 || c3 < 1 || c1 > 1024 || c2 > 1024 || c3 > 1024 || nvox_max < 1 ||
-      nvox_max * std::max({c1, c2, c3, 10}) >= (1L << 31))
+      nvox_max * std::max({c1, c2, c3, 48}) >= (1L << 31))
     return 1;
   g_pmlp_err.clear();
   (void)hipGetLastError();
   sc_pmlp* c = new sc_pmlp();
+  c->arch = arch;
   c->D = D;
   c->ng = D * D;
   c->nt = D == 3 ? 6 : 3;
   c->redelta = use_redelta != 0;
-  c->nin = c->ng + (c->redelta ? 1 : 0);
+  if (arch == 1) {
+    // invariants and basis of tb_kernel! (src/nets.jl:416-453)
+    c->ni = D == 3 ? 5 : 2;
+    c->nbas = D == 3 ? 7 : 2;
+    c->nin = c->ni + (c->redelta ? 1 : 0);
+    c->ctot = c->nin + c->nt * c->nbas;
+  } else {
+    c->nin = c->ng + (c->redelta ? 1 : 0);
+    c->ctot = c->nin;
+  }
   c->w[0] = c1; c->w[1] = c2; c->w[2] = c3;
   c->nvmax = nvox_max;
   try {
@@ -509,6 +798,15 @@ int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed
   *out = c;
   return 0;
 }
+}  // namespace
+
+int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out) {
+  return create_impl(0, D, use_redelta, c1, c2, c3, seed, nvox_max, out);
+}
+
+int sc_tbnn_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out) {
+  return create_impl(1, D, use_redelta, c1, c2, c3, seed, nvox_max, out);
+}
 
 int sc_pmlp_destroy(sc_pmlp* c) {
   if (!c) return 0;
@@ -530,7 +828,7 @@ const char* sc_pmlp_last_error(const sc_pmlp* c) { return c ? c->err.c_str() : g
 int sc_pmlp_info(const sc_pmlp* c, int64_t* info) {
   if (!c || !info) return 1;
   info[0] = c->D; info[1] = c->nin; info[2] = c->w[0]; info[3] = c->w[1]; info[4] = c->w[2];
-  info[5] = c->nt; info[6] = c->plen; info[7] = c->nvmax;
+  info[5] = c->nt; info[6] = c->plen; info[7] = c->nvmax; info[8] = c->arch; info[9] = c->ctot;
   return 0;
 }
 
@@ -561,14 +859,24 @@ int sc_pmlp_features(sc_pmlp* c, const void* GG, const void* tau, int fp64, long
                      void* y) {
   return pguard(c, [&] {
     if (np < 1 || !x || (tau && !y)) throw std::runtime_error("arguments");
-    if (fp64)
-      hipLaunchKernelGGL(k_features<double>, dim3(nb(np)), dim3(256), 0, c->stream, (const double*)GG, np, c->ng,
-                         delta * delta, c->redelta ? 1 : 0, float(z), (const double*)tau, c->nt, (float*)x, c->nin,
-                         (float*)y);
-    else
-      hipLaunchKernelGGL(k_features<float>, dim3(nb(np)), dim3(256), 0, c->stream, (const float*)GG, np, c->ng,
-                         delta * delta, c->redelta ? 1 : 0, float(z), (const float*)tau, c->nt, (float*)x, c->nin,
-                         (float*)y);
+    const double d2 = delta * delta;
+    const int re = c->redelta ? 1 : 0;
+    if (c->arch == 1) {
+      if (fp64)
+        hipLaunchKernelGGL(k_tb_features<double>, dim3(nb(np)), dim3(256), 0, c->stream, (const double*)GG, np, c->D,
+                           d2, re, float(z), (const double*)tau, c->nt, (float*)x, c->ctot, c->ni, c->nbas,
+                           (float*)y);
+      else
+        hipLaunchKernelGGL(k_tb_features<float>, dim3(nb(np)), dim3(256), 0, c->stream, (const float*)GG, np, c->D,
+                           d2, re, float(z), (const float*)tau, c->nt, (float*)x, c->ctot, c->ni, c->nbas,
+                           (float*)y);
+    } else if (fp64) {
+      hipLaunchKernelGGL(k_features<double>, dim3(nb(np)), dim3(256), 0, c->stream, (const double*)GG, np, c->ng, d2,
+                         re, float(z), (const double*)tau, c->nt, (float*)x, c->nin, (float*)y);
+    } else {
+      hipLaunchKernelGGL(k_features<float>, dim3(nb(np)), dim3(256), 0, c->stream, (const float*)GG, np, c->ng, d2,
+                         re, float(z), (const float*)tau, c->nt, (float*)x, c->nin, (float*)y);
+    }
     PM_HIP(hipGetLastError());
   });
 }

@@ -286,13 +286,19 @@ mutable struct PointwiseMLP
     ptr::Ptr{Cvoid}
     D::Int
 end
-function PointwiseMLP(D, nchan; use_redelta = false, same_as_equi = false, seed = 0, nvox_max = 1 << 21)
+# tbnn = true: the tensor-basis network (tbnn_net, src/nets.jl:535-556) instead
+function PointwiseMLP(D, nchan; use_redelta = false, same_as_equi = false, seed = 0, nvox_max = 1 << 21,
+                      tbnn = false)
     w = same_as_equi ? nchan .* (D == 3 ? 48 : 8) : nchan
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    rc = ccall((:sc_pmlp_create, libsymcode), Cint,
-               (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
-               D, use_redelta, w[1], w[2], w[3], seed, nvox_max, out)
-    rc == 0 || error("sc_pmlp_create failed ($rc)")
+    rc = if tbnn
+        ccall((:sc_tbnn_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
+              D, use_redelta, w[1], w[2], w[3], seed, nvox_max, out)
+    else
+        ccall((:sc_pmlp_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
+              D, use_redelta, w[1], w[2], w[3], seed, nvox_max, out)
+    end
+    rc == 0 || error("network creation failed ($rc)")
     net = PointwiseMLP(out[], D)
     finalizer(x -> ccall((:sc_pmlp_destroy, libsymcode), Cint, (Ptr{Cvoid},), x.ptr), net)
     net
@@ -302,7 +308,8 @@ end
 # (1,…,1, c_in, c_out)) in the library's flat order [W1, b1, W2, b2, W3, b3, W4]
 function setparams!(net::PointwiseMLP, ps)
     h = Float32[]
-    for (l, p) in enumerate((ps.lift, ps.mid_1, ps.mid_2, ps.sink))
+    layers = haskey(ps, :lift) ? (ps.lift, ps.mid_1, ps.mid_2, ps.sink) : values(ps)  # tbnn_net: layer_1..4
+    for (l, p) in enumerate(layers)
         append!(h, vec(p.weight))
         l < 4 && append!(h, p.bias)
     end
@@ -325,14 +332,17 @@ struct MLPClosure
     visc::Cdouble
     μ::Cdouble
     σ::Cdouble
+    symmetrize::Bool  # symmetrize_pointwise (src/nets.jl:350-389) of an mlp
 end
-MLPClosure(net::PointwiseMLP, Δ; visc = 0.0, redelta_norm = (; μ = 0.0, σ = 1.0)) =
-    MLPClosure(net, Δ, visc, redelta_norm.μ, redelta_norm.σ)
+MLPClosure(net::PointwiseMLP, Δ; visc = 0.0, redelta_norm = (; μ = 0.0, σ = 1.0), symmetrize = false) =
+    MLPClosure(net, Δ, visc, redelta_norm.μ, redelta_norm.σ, symmetrize)
 kind(::MLPClosure) = Cint(7)
 const LESModel = Union{BuiltinClosure,CNNClosure,MLPClosure}
-attach(m::MLPClosure, ctx) = check(ccall((:sc_set_pmlp_closure, libsymcode), Cint,
-                                         (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble),
-                                         ctx.ptr, m.net.ptr, m.Δ, m.visc, m.μ, m.σ), ctx.ptr)
+function attach(m::MLPClosure, ctx)
+    check(ccall((:sc_set_pmlp_closure, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cdouble),
+                ctx.ptr, m.net.ptr, m.Δ, m.visc, m.μ, m.σ), ctx.ptr)
+    m.symmetrize && check(ccall((:sc_set_pmlp_symmetrize, libsymcode), Cint, (Ptr{Cvoid}, Cint), ctx.ptr, 1), ctx.ptr)
+end
 
 function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::MLPClosure, visc)
     attach(model, u.ctx)

@@ -111,6 +111,7 @@ SIGNATURES = {
     "sc_les_wray3_vjp": [P, P, P, P, c_double, c_double, c_int],
     "sc_lincomb": [P, P, P, c_double, P, c_double, c_int],
     "sc_pmlp_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.c_long, ctypes.POINTER(P)],
+    "sc_tbnn_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.c_long, ctypes.POINTER(P)],
     "sc_pmlp_destroy": [P],
     "sc_pmlp_info": [P, ctypes.POINTER(c_int64)],
     "sc_pmlp_set_stream": [P, P],
@@ -125,6 +126,7 @@ SIGNATURES = {
     "sc_pmlp_adamw": [P, c_double, c_double, c_double, c_double, c_double, c_double],
     "sc_set_pmlp_closure": [P, P, c_double, c_double, c_double, c_double],
     "sc_closure_pmlp": [P, P, P],
+    "sc_set_pmlp_symmetrize": [P, c_int],
 }
 
 

@@ -1,8 +1,9 @@
-"""Host-side wrapper of the pointwise MLP closure (sc_pmlp_* in
-include/symcode.h): the reference's `mlp` learned closure ("conv" arch,
-src/nets.jl:247-298), its training pairs (create_dataloader,
-src/data.jl:705-741) and its solver-facing wrapper (fullchain,
-src/nets.jl:311-334).
+"""Host-side wrapper of the pointwise learned closures (sc_pmlp_* in
+include/symcode.h): the reference's `mlp` ("conv" arch, src/nets.jl:247-298)
+with its training pairs (create_dataloader, src/data.jl:705-741), its
+solver-facing wrapper (fullchain, src/nets.jl:311-334) and its group average
+(symmetrize_pointwise, :350-389, the "convsym" model); and the tensor-basis
+network (tbnn_net / tbnn, :416-583, with create_dataloader_tbnn).
 
     x = [∇ū / (|∇ū| + eps), z]         z = standardised log Re_Δ (optional)
     y = symm(W4ᵀ gelu(W3ᵀ gelu(W2ᵀ gelu(W1ᵀ x + b1) + b2) + b3))
@@ -39,29 +40,26 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-class PointwiseMLP:
-    """mlp(setup, nchan; same_as_equi, use_redelta): three GELU hidden layers
-    of widths ``widths`` (times |G| with ``same_as_equi``), a bias-free output
-    layer of D == 3 ? 6 : 3 channels and the trace removal.  Parameters are
-    glorot-uniform (zero biases) from ``seed``; the flat FP32 vector follows
-    Lux's order [W1, b1, W2, b2, W3, b3, W4] with each W (c_in, c_out)
-    column-major.  ``nvox_max`` bounds the voxels of one training call."""
+class _PointwiseNet:
+    """common device-side state of the pointwise networks (sc_pmlp_*)"""
+    _create = "sc_pmlp_create"
 
-    def __init__(self, D, widths=TIERS["p3000"], use_redelta=False, same_as_equi=False, seed=0,
-                 nvox_max=1 << 21, device=0):
+    def __init__(self, D, widths, use_redelta, seed, nvox_max, device):
         if len(widths) != 3:
             raise ValueError("the network has three hidden layers")
-        w = [int(c) * (GROUP_ORDER.get(D, 1) if same_as_equi else 1) for c in widths]
+        w = [int(c) for c in widths]
         net = ctypes.c_void_p()
-        rc = lib().sc_pmlp_create(D, 1 if use_redelta else 0, w[0], w[1], w[2], seed, nvox_max, ctypes.byref(net))
+        rc = getattr(lib(), self._create)(D, 1 if use_redelta else 0, w[0], w[1], w[2], seed, nvox_max,
+                                          ctypes.byref(net))
         if rc != 0:
             if rc == 1:
-                raise ValueError("sc_pmlp_create: D in {2, 3}, widths in [1, 1024], nvox_max * width < 2^31")
-            raise RuntimeError("sc_pmlp_create failed: " + lib().sc_pmlp_last_error(None).decode())
+                raise ValueError(f"{self._create}: D in {{2, 3}}, widths in [1, 1024], nvox_max * width < 2^31")
+            raise RuntimeError(f"{self._create} failed: " + lib().sc_pmlp_last_error(None).decode())
         self.net = net
-        info = (ctypes.c_int64 * 8)()
+        info = (ctypes.c_int64 * 10)()
         check_pmlp(lib().sc_pmlp_info(net, info), net)
-        self.D, self.nin, c1, c2, c3, self.nout, self.nparams, self.nvox_max = list(info)
+        (self.D, self.nin, c1, c2, c3, self.nout, self.nparams, self.nvox_max, self.kind,
+         self.row) = list(info)
         self.widths = (c1, c2, c3)
         self.use_redelta = bool(use_redelta)
         self.device = torch.device("cuda", device)
@@ -77,9 +75,12 @@ class PointwiseMLP:
             pass
 
     # --------------------------------------------------------- parameters
+    def net_outputs(self):
+        return self.nout
+
     def layer_shapes(self):
         """[(c_in, c_out)] of the four layers"""
-        ch = [self.nin, *self.widths, self.nout]
+        ch = [self.nin, *self.widths, self.net_outputs()]
         return [(ch[i], ch[i + 1]) for i in range(4)]
 
     def get_params(self):
@@ -116,14 +117,14 @@ class PointwiseMLP:
     def features(self, GG, delta, tau=None, z=0.0):
         """create_dataloader's normalised pair from the physical gradient GG
         (D² components) and, when given, the physical stress tau (packed):
-        returns x (np, nin) [and y (np, nout)], FP32"""
+        returns x (np, row) [and y (np, nout)], FP32"""
         np_ = GG[0].numel()
         fp64 = GG.dtype == torch.float64
         if tau is not None and (tau.dtype != GG.dtype or tau.shape[0] != self.nout):
             raise ValueError("tau must be the packed stress in GG's dtype")
         if GG.shape[0] != self.D * self.D:
             raise ValueError("GG must hold D^2 components")
-        x = torch.empty((np_, self.nin), dtype=torch.float32, device=self.device)
+        x = torch.empty((np_, self.row), dtype=torch.float32, device=self.device)
         y = torch.empty((np_, self.nout), dtype=torch.float32, device=self.device) if tau is not None else None
         check_pmlp(lib().sc_pmlp_features(self.net, _vp(GG), _vp(tau) if tau is not None else None, int(fp64), np_,
                                           float(delta), float(z), _vp(x), _vp(y) if y is not None else None),
@@ -137,7 +138,7 @@ class PointwiseMLP:
         return t
 
     def forward(self, x, y=None):
-        x = self._flat(x, self.nin)
+        x = self._flat(x, self.row)
         nv = x.shape[0]
         y = torch.empty((nv, self.nout), dtype=torch.float32, device=self.device) if y is None else y
         check_pmlp(lib().sc_pmlp_forward(self.net, _vp(x), _vp(y), nv), self.net)
@@ -145,7 +146,7 @@ class PointwiseMLP:
 
     def backward(self, x, dy):
         """accumulate the parameter gradients of the last forward (on x)"""
-        x = self._flat(x, self.nin)
+        x = self._flat(x, self.row)
         dy = self._flat(dy, self.nout)
         check_pmlp(lib().sc_pmlp_backward(self.net, _vp(x), _vp(dy), x.shape[0]), self.net)
 
@@ -175,6 +176,36 @@ class PointwiseMLP:
         return l
 
 
+class PointwiseMLP(_PointwiseNet):
+    """mlp(setup, nchan; same_as_equi, use_redelta): three GELU hidden layers
+    of widths ``widths`` (times |G| with ``same_as_equi``), a bias-free output
+    layer of D == 3 ? 6 : 3 channels and the trace removal.  Parameters are
+    glorot-uniform (zero biases) from ``seed``; the flat FP32 vector follows
+    Lux's order [W1, b1, W2, b2, W3, b3, W4] with each W (c_in, c_out)
+    column-major.  ``nvox_max`` bounds the voxels of one training call."""
+
+    def __init__(self, D, widths=TIERS["p3000"], use_redelta=False, same_as_equi=False, seed=0,
+                 nvox_max=1 << 21, device=0):
+        w = [int(c) * (GROUP_ORDER.get(D, 1) if same_as_equi else 1) for c in widths]
+        super().__init__(D, w, use_redelta, seed, nvox_max, device)
+
+
+class TBNN(_PointwiseNet):
+    """tbnn_net(setup, nchan; use_redelta) (src/nets.jl:535-556): three GELU
+    hidden layers on the tb_kernel! invariants [+ z], a bias-free output layer
+    of the basis coefficients (7 in 3D, 2 in 2D), contracted with the rows'
+    deviatoric basis tensors.  Input rows: [invariants, z, basis (component
+    fastest)], ``row`` wide."""
+    _create = "sc_tbnn_create"
+
+    def __init__(self, D, widths=TIERS["p3000"], use_redelta=False, seed=0, nvox_max=1 << 21, device=0):
+        super().__init__(D, widths, use_redelta, seed, nvox_max, device)
+        self.nbasis = 7 if D == 3 else 2
+
+    def net_outputs(self):
+        return self.nbasis
+
+
 def redelta_norm(pairs):
     """compute_redelta_norm (src/training.jl:180-186): (μ, σ) of log Re_Δ over
     the pairs, σ the corrected standard deviation"""
@@ -184,8 +215,9 @@ def redelta_norm(pairs):
     return float(np.mean(logre)), float(np.std(logre, ddof=1))
 
 
-def pointwise_pairs(net: PointwiseMLP, pairs, g: Grid, norm=None):
-    """create_dataloader (src/data.jl:705-741) over Pairs (ubar, tau spectral
+def pointwise_pairs(net, pairs, g: Grid, norm=None):
+    """create_dataloader (src/data.jl:705-741; for a TBNN
+    create_dataloader_tbnn, :759-805) over Pairs (ubar, tau spectral
     on the LES grid, their Δ and Re_Δ): per snapshot x = ∇ū / (|∇ū| + eps)
     [+ z(Re_Δ)], y = τ / (Δ² |∇ū|² + eps) with ∇ū and τ 2/3-truncated in
     physical space; returns (X, Y) stacked over the snapshots, FP32.  ``norm``
@@ -193,7 +225,7 @@ def pointwise_pairs(net: PointwiseMLP, pairs, g: Grid, norm=None):
     if net.use_redelta and norm is None:
         raise ValueError("a network with the Re_Delta input needs norm = redelta_norm(pairs)")
     np_ = g.phys_elems
-    X = torch.empty((len(pairs) * np_, net.nin), dtype=torch.float32, device=net.device)
+    X = torch.empty((len(pairs) * np_, net.row), dtype=torch.float32, device=net.device)
     Y = torch.empty((len(pairs) * np_, net.nout), dtype=torch.float32, device=net.device)
     t = g.spacetensorfield()
     for i, p in enumerate(pairs):
@@ -211,26 +243,40 @@ def pointwise_pairs(net: PointwiseMLP, pairs, g: Grid, norm=None):
 class PointwiseClosure(Closure):
     """fullchain(setup, net, ...; use_redelta, redelta_norm) as closure kind 7
     of the LES solver: τ = Δ² |∇ū|² net(∇ū / (|∇ū| + eps) [, z]) with the live
-    field's z = (log(Δ² √⟨|∇ū|²⟩ / ν) - μ) / σ.  It stays attached to the grid
-    until another model is attached."""
+    field's z = (log(Δ² √⟨|∇ū|²⟩ / ν) - μ) / σ; for a TBNN, tbnn(...)'s
+    τ = Δ² |∇ū|² Σ_b w_b B_b.  ``symmetrize`` wraps an mlp in
+    symmetrize_pointwise (the "convsym" model).  It stays attached to the
+    grid until another model is attached."""
     kind = 7
 
-    def __init__(self, net: PointwiseMLP, g: Grid, delta, visc=0.0, norm=None):
+    def __init__(self, net, g: Grid, delta, visc=0.0, norm=None, symmetrize=False):
         super().__init__(g)
         if net.use_redelta and (norm is None or visc <= 0):
             raise ValueError("a network with the Re_Delta input needs visc > 0 and norm = (mu, sigma)")
+        if symmetrize and net.kind != 0:
+            raise ValueError("the group average applies to the mlp")
         self.net, self.delta, self.visc = net, float(delta), float(visc)
         self.mu, self.sigma = (float(norm[0]), float(norm[1])) if norm is not None else (0.0, 1.0)
+        self.symmetrize = bool(symmetrize)
         self.attach()
 
     def attach(self):
         S._c(lib().sc_set_pmlp_closure(self.g.ctx, self.net.net, self.delta, self.visc, self.mu, self.sigma), self.g)
+        if self.symmetrize:
+            S._c(lib().sc_set_pmlp_symmetrize(self.g.ctx, 1), self.g)
 
     def __call__(self, u, GG=None):
+        self.attach()
         tau = self.g.spacetensorfield()
         S._c(lib().sc_closure_pmlp(self.g.ctx, _p(tau), _p(u)), self.g)
         return tau
 
 
-def create_pointwise_closure(net, g, delta, visc=0.0, norm=None):
-    return PointwiseClosure(net, g, delta, visc, norm)
+def create_pointwise_closure(net, g, delta, visc=0.0, norm=None, symmetrize=False):
+    return PointwiseClosure(net, g, delta, visc, norm, symmetrize)
+
+
+def symmetrize_pointwise(model: PointwiseClosure):
+    """symmetrize_pointwise (src/nets.jl:350-389) of an mlp closure: the same
+    parameters, exactly equivariant under the cube's symmetries"""
+    return PointwiseClosure(model.net, model.g, model.delta, model.visc, (model.mu, model.sigma), symmetrize=True)

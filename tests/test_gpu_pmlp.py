@@ -261,3 +261,129 @@ def test_training_on_filtered_pairs():
     u0 = pairs[0].ubar.clone()
     nred = sc.solve_les(u0, [0.0, 0.02], gl, visc, model, 0.3, forced=False)
     assert nred == 2 and torch.isfinite(u0).all()
+
+
+# ------------------------------------------------------------------ TBNN
+def tbnn_of(D, widths=(7, 9, 5), use_redelta=False, seed=1, nvox_max=4096):
+    from symcode.pmlp import TBNN
+
+    return TBNN(D, widths, use_redelta=use_redelta, seed=seed, nvox_max=nvox_max)
+
+
+@pytest.mark.parametrize("D,n,dtype", [(3, 16, torch.float64), (3, 16, torch.float32), (2, 32, torch.float64)])
+def test_tbnn_features_parity(D, n, dtype):
+    import symcode as sc
+
+    print_case("tbnn features", D, n, dtype)
+    go = S.Grid(D, n)
+    g = sc.Grid(D, n, dtype=dtype)
+    net = tbnn_of(D, use_redelta=True, nvox_max=n**D)
+    assert net.row == (5 if D == 3 else 2) + 1 + (42 if D == 3 else 6)
+    u = oracle_solenoidal(go, 7, trunc=True)
+    GG = sc.gradient_phys(g.upload_spectral(u), g)
+    tau = g.upload_physical(np.stack([rand_phys(go.phys_shape, 30 + i) for i in range(go.tensordim)]))
+    delta, z = 0.35, 0.4
+    x, y = net.features(GG, delta, tau, z)
+    eps = np.finfo(np.float64 if dtype == torch.float64 else np.float32).eps
+    GGo = g.download_physical(GG)
+    xo = OP.tb_rows(GGo, D, z, eps)
+    _, yo = OP.features(GGo, g.download_physical(tau), delta, None, eps)
+    assert rel(x.cpu().numpy(), xo) < 1e-6
+    assert rel(y.cpu().numpy(), yo) < 1e-6
+
+
+@pytest.mark.parametrize("D,use_redelta", [(3, True), (2, False)])
+def test_tbnn_forward_loss_backward_parity(D, use_redelta):
+    print_case("tbnn forward backward", D, use_redelta)
+    net = tbnn_of(D, use_redelta=use_redelta)
+    rng = np.random.default_rng(D + 10)
+    GG = rng.standard_normal((D * D, 3000))
+    x = OP.tb_rows(GG, D, 0.2 if use_redelta else None).astype(np.float32)
+    t = rng.standard_normal((3000, net.nout)).astype(np.float32)
+    xd, td = torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda")
+    p = oracle_params(net)
+    net.zero_grad()
+    y = net.forward(xd)
+    mo, cache = OP.tbnn_forward(p, x.astype(np.float64), D, net.nin)
+    assert rel(y.cpu().numpy(), mo) < 2e-6
+    loss, dy = net.loss(y, td, "relative")
+    lo, dmo = OP.relative_loss(mo, t.astype(np.float64))
+    assert loss == pytest.approx(lo, rel=1e-5)
+    net.backward(xd, dy)
+    go_ = OP.tbnn_backward(p, cache, dmo)
+    gg = OP.unflat(net.get_grads(), net.layer_shapes())
+    for name in go_:
+        assert rel(gg[name], go_[name]) < 2e-5, name
+
+
+@pytest.mark.parametrize("D,n,use_redelta", [(3, 16, True), (2, 32, False)])
+def test_tbnn_closure_parity_fp64(D, n, use_redelta):
+    import symcode as sc
+    from symcode.pmlp import PointwiseClosure
+
+    print_case("tbnn closure fp64", D, n, use_redelta)
+    go = S.Grid(D, n)
+    g = sc.Grid(D, n, dtype=torch.float64)
+    net = tbnn_of(D, (11, 13, 7), use_redelta=use_redelta, seed=4)
+    delta, visc, norm = 2 * np.pi / n * 2, 2e-3, (1.1, 0.8)
+    model = PointwiseClosure(net, g, delta, visc, norm if use_redelta else None)
+    u = oracle_solenoidal(go, 37, trunc=True)
+    ud = g.upload_spectral(u)
+    p = oracle_params(net)
+    re = (visc, *norm) if use_redelta else None
+    to = OP.tbnn_model(p, OC.gradient_phys(u, go), delta, D, redelta=re)
+    assert rel(g.download_physical(model(ud)), to) < 1e-12
+    du = sc.les_rhs(g.vectorfield(), ud, g, model, visc)
+    duo = OC.les_rhs(u, go, visc, lambda u_, GG_: OP.tbnn_model(p, GG_, delta, D, redelta=re))
+    assert rel(g.download_spectral(du), duo) < 1e-12
+    with pytest.raises(ValueError):
+        PointwiseClosure(net, g, delta, visc, norm, symmetrize=True)
+
+
+def test_tbnn_training_teacher():
+    """a teacher TBNN's outputs on the rows of a gradient field are learnable:
+    the reference semantics lower the relative loss tenfold"""
+    import symcode as sc
+    from symcode import training as T
+
+    print_case("tbnn teacher training")
+    go = S.Grid(3, 16)
+    g = sc.Grid(3, 16, dtype=torch.float64)
+    GG = sc.gradient_phys(g.upload_spectral(oracle_solenoidal(go, 8, trunc=True)), g)
+    teacher = tbnn_of(3, (10, 18, 32), seed=9, nvox_max=16**3)
+    X = teacher.features(GG, 0.3)
+    Yt = teacher.forward(X).clone()
+    student = tbnn_of(3, (10, 18, 32), seed=2, nvox_max=16**3)
+    lt = T.reference_train(student, X, Yt, 150, lr=3e-3)
+    print("teacher-student loss", lt[0], lt[-1], flush=True)
+    assert lt[-1] < 0.1 * lt[0]
+
+
+# ------------------------------------------- symmetrize_pointwise (convsym)
+@pytest.mark.parametrize("D,n,use_redelta", [(3, 16, True), (2, 32, False)])
+def test_symmetrized_closure_parity_fp64(D, n, use_redelta):
+    import symcode as sc
+    from symcode.pmlp import PointwiseClosure, symmetrize_pointwise
+
+    print_case("symmetrized mlp closure fp64", D, n, use_redelta)
+    go = S.Grid(D, n)
+    g = sc.Grid(D, n, dtype=torch.float64)
+    net = net_of(D, (11, 13, 7), use_redelta=use_redelta, seed=4)
+    delta, visc, norm = 2 * np.pi / n * 2, 2e-3, (1.1, 0.8)
+    plain = PointwiseClosure(net, g, delta, visc, norm if use_redelta else None)
+    model = symmetrize_pointwise(plain)
+    u = oracle_solenoidal(go, 39, trunc=True)
+    ud = g.upload_spectral(u)
+    p = oracle_params(net)
+    re = (visc, *norm) if use_redelta else None
+    GGo = OC.gradient_phys(u, go)
+    fc = lambda G: OP.fullchain(p, G, delta, D, redelta=re)
+    to = OP.symmetrize(fc, GGo, D)
+    tau = g.download_physical(model(ud))
+    assert rel(tau, to) < 1e-12
+    assert rel(tau, fc(GGo)) > 1e-3  # the average differs from the plain network
+    du = sc.les_rhs(g.vectorfield(), ud, g, model, visc)
+    duo = OC.les_rhs(u, go, visc, lambda u_, GG_: OP.symmetrize(fc, GG_, D))
+    assert rel(g.download_spectral(du), duo) < 1e-12
+    # attaching the plain closure switches the average off again
+    assert rel(g.download_physical(plain(ud)), fc(GGo)) < 1e-12

@@ -14,9 +14,9 @@ import torch
 from oracle import pmlp as OP
 
 
-def rand_params(D, widths, use_redelta, seed):
+def rand_params(D, widths, use_redelta, seed, tbnn=False):
     rng = np.random.default_rng(seed)
-    shapes = OP.layer_shapes(D, widths, use_redelta)
+    shapes = OP.layer_shapes(D, widths, use_redelta, tbnn)
     p = {}
     for l, (ci, co) in enumerate(shapes):
         p[f"W{l + 1}"] = rng.uniform(-1, 1, (ci, co)) * math.sqrt(6 / (ci + co))
@@ -141,3 +141,137 @@ def test_fullchain_redelta_shift():
     t1 = OP.fullchain(p, GG, delta, 3, redelta=(visc, mu, sigma))
     t2 = OP.fullchain(p, a * GG, delta, 3, redelta=(visc, mu + math.log(a), sigma))
     assert np.allclose(t2, a * a * t1, rtol=1e-12, atol=1e-14)
+
+
+# ------------------------------------------------------------------ TBNN
+def rand_orthogonal(D, seed):
+    q, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((D, D)))
+    return q
+
+
+def rotate_grad(GG, Q, D):
+    """component-major gradient of Q A Q^T (A[i][j] at component i + D j)"""
+    A, _ = OP.grad_matrices(GG, D)
+    B = Q @ A @ Q.T
+    shape = GG.shape[1:]
+    return np.stack([B[:, i, j].reshape(shape) for j in range(D) for i in range(D)])
+
+
+def unpack(bas, D):
+    """packed [.., nt] -> [.., D, D]"""
+    out = np.zeros(bas.shape[:-1] + (D, D))
+    comps = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)] if D == 3 else [(0, 0), (1, 1), (0, 1)]
+    for t, (a, b) in enumerate(comps):
+        out[..., a, b] = bas[..., t]
+        out[..., b, a] = bas[..., t]
+    return out
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_tb_invariance_and_equivariance(D):
+    GG = np.random.default_rng(3).standard_normal((D * D, 5, 5))
+    Q = rand_orthogonal(D, 4)
+    inv, bas, a2 = OP.tb_features(GG, D)
+    inv2, bas2, a22 = OP.tb_features(rotate_grad(GG, Q, D), D)
+    assert np.allclose(a22, a2, rtol=1e-13)
+    assert np.allclose(inv2, inv, rtol=1e-11, atol=1e-13)
+    B = unpack(bas, D)
+    B2 = unpack(bas2, D)
+    assert np.allclose(B2, Q @ B @ Q.T, atol=1e-12)
+    # symmetric (by packing) and trace-free
+    assert np.abs(np.trace(B, axis1=-2, axis2=-1)).max() < 1e-14
+
+
+def test_tb_closed_forms():
+    """pure strain A = diag(1, 2, -3) and pure rotation A = [[0, w], [-w, 0]]"""
+    GG = np.zeros((9, 1))
+    for i, v in enumerate((1.0, 2.0, -3.0)):
+        GG[i + 3 * i, 0] = v
+    inv, bas, a2 = OP.tb_features(GG, 3)
+    n = math.sqrt(14.0) + np.finfo(float).eps
+    s = np.array([1.0, 2.0, -3.0]) / n
+    assert a2[0] == 14.0
+    assert inv[0] == pytest.approx([np.sum(s**2), 0, np.sum(s**3), 0, 0], abs=1e-15)
+    assert np.allclose(bas[0, 0, :3], s - s.mean(), atol=1e-15) and np.all(bas[0, 0, 3:] == 0)
+    assert np.allclose(bas[0, 1, :3], s**2 - np.mean(s**2), atol=1e-15)
+    assert np.abs(bas[0, 2:]).max() < 1e-15  # every R-term vanishes
+    # 2D pure rotation: S = 0, tr(R R) = -2 r^2 with r = 1/sqrt(2)
+    G2 = np.array([[0.0], [-1.0], [1.0], [0.0]])  # xx, yx, xy, yy
+    inv, bas, _ = OP.tb_features(G2, 2)
+    assert inv[0] == pytest.approx([0.0, -1.0], abs=1e-15)
+    assert np.abs(bas).max() == 0
+
+
+@pytest.mark.parametrize("D,use_redelta", [(3, False), (3, True), (2, True)])
+def test_tbnn_forward_backward_match_autograd(D, use_redelta):
+    p, shapes = rand_params(D, (6, 5, 4), use_redelta, 8, tbnn=True)
+    nin = shapes[0][0]
+    GG = np.random.default_rng(9).standard_normal((D * D, 30))
+    x = OP.tb_rows(GG, D, z=0.3 if use_redelta else None)
+    m, cache = OP.tbnn_forward(p, x, D, nin)
+    tp = {k: torch.tensor(v, requires_grad=True) for k, v in p.items()}
+    a = torch.tensor(x[:, :nin])
+    for l in range(1, 4):
+        a = torch.nn.functional.gelu(a @ tp[f"W{l}"] + tp[f"b{l}"], approximate="tanh")
+    w = a @ tp["W4"]
+    nt, nb = (6, 7) if D == 3 else (3, 2)
+    B = torch.tensor(x[:, nin:]).reshape(-1, nb, nt)
+    tm = (w[:, :, None] * B).sum(dim=1)
+    assert np.allclose(m, tm.detach().numpy(), rtol=1e-13, atol=1e-14)
+    target = np.random.default_rng(10).standard_normal(m.shape)
+    loss, dm = OP.relative_loss(m, target)
+    tl = ((tm - torch.tensor(target)) ** 2).sum() / ((torch.tensor(target) ** 2).sum() + np.finfo(np.float32).eps)
+    tl.backward()
+    g = OP.tbnn_backward(p, cache, dm)
+    for k in p:
+        assert np.allclose(g[k], tp[k].grad.numpy(), rtol=1e-11, atol=1e-14), k
+    # the TBNN closure is equivariant under any rotation (z is invariant)
+    re = (1e-3, 0.5, 2.0) if use_redelta else None
+    tau = OP.tbnn_model(p, GG.reshape(D * D, 5, 6), 0.3, D, redelta=re)
+    Q = rand_orthogonal(D, 11)
+    tau2 = OP.tbnn_model(p, rotate_grad(GG.reshape(D * D, 5, 6), Q, D), 0.3, D, redelta=re)
+    T1 = unpack(np.moveaxis(tau, 0, -1), D)
+    T2 = unpack(np.moveaxis(tau2, 0, -1), D)
+    assert np.allclose(T2, Q @ T1 @ Q.T, atol=1e-10 * np.abs(T1).max())
+
+
+# ------------------------------------------------- the cube's symmetry group
+@pytest.mark.parametrize("D", [2, 3])
+def test_group_tables(D):
+    els = OP.group_elements(D)
+    mats = [OP.element_matrix(p, s) for p, s in els]
+    assert len(mats) == (48 if D == 3 else 8)
+    keys = {m.tobytes() for m in mats}
+    assert len(keys) == len(mats)
+    assert np.array_equal(mats[0], np.eye(D))
+    for a in mats:
+        assert np.allclose(a @ a.T, np.eye(D))
+        for b in mats[:8]:
+            assert (a @ b).tobytes() in keys
+    # (R A R^T)_ab = s_a s_b A_{p_a p_b}
+    A = np.random.default_rng(0).standard_normal((D, D))
+    p, s = els[-1]
+    R = OP.element_matrix(p, s)
+    RA = R @ A @ R.T
+    for a in range(D):
+        for b in range(D):
+            assert RA[a, b] == pytest.approx(s[a] * s[b] * A[p[a], p[b]], abs=1e-15)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_symmetrize_equivariant_and_idempotent(D):
+    p, _ = rand_params(D, (5, 6, 7), False, 13)
+    GG = np.random.default_rng(14).standard_normal((D * D, 4, 5))
+    model = lambda G: OP.fullchain(p, G, 0.3, D)
+    msym = OP.symmetrize(model, GG, D)
+    for pe, se in OP.group_elements(D)[1:]:
+        R = OP.element_matrix(pe, se)
+        a = unpack(np.moveaxis(OP.symmetrize(model, rotate_grad(GG, R, D), D), 0, -1), D)
+        b = unpack(np.moveaxis(msym, 0, -1), D)
+        assert np.allclose(a, R @ b @ R.T, atol=1e-13 * np.abs(b).max())
+    # the plain mlp is not equivariant, so the average changes it ...
+    assert not np.allclose(msym, model(GG), atol=1e-6)
+    # ... while an equivariant model is its own average
+    pt, _ = rand_params(D, (5, 6, 7), False, 15, tbnn=True)
+    tb = lambda G: OP.tbnn_model(pt, G, 0.3, D)
+    assert np.allclose(OP.symmetrize(tb, GG, D), tb(GG), atol=1e-13 * np.abs(tb(GG)).max())
