@@ -281,10 +281,21 @@ int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed
  * packed stress.  Input rows carry the basis tensors after the network
  * inputs (row width info[9]); every other sc_pmlp_* call takes either kind. */
 int sc_tbnn_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out);
+/* The group-equivariant network: equivariant_net (src/nets.jl:102-238), |G|
+ * = 48 (2D: 8) copies of each hidden channel whose weights are synthesised
+ * from compact trainables by get_weight_synthesis (src/symmetry.jl:156-197),
+ * the sink's D^2 outputs made symmetric and trace-free.  The trainable vector
+ * (get/set_params, gradients, AdamW) is the compact one in Lux's order: lift
+ * (weight (D^2, c1), bias[, weight_re (1, c1)]), mid_1 and mid_2 (weight
+ * (|G|, c_out, c_in), bias), sink (weight (D^2, c3)); fullchain inputs and
+ * closure 7 as for the mlp. */
+int sc_equi_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out);
 int sc_pmlp_destroy(sc_pmlp* net);
 const char* sc_pmlp_last_error(const sc_pmlp* net);
-/* info[0..9] = D, network inputs, c1, c2, c3, stress components, nparams,
- * nvox_max, kind (0 mlp, 1 TBNN), input row width */
+/* info[0..11] = D, network inputs, c1, c2, c3, stress components, nparams
+ * (trainable), nvox_max, kind (0 mlp, 1 TBNN, 2 equivariant), input row
+ * width, |G| copies per hidden channel (1 unless equivariant), network
+ * parameters */
 int sc_pmlp_info(const sc_pmlp* net, int64_t* info);
 int sc_pmlp_set_stream(sc_pmlp* net, void* hip_stream);
 int sc_pmlp_get_params(sc_pmlp* net, float* host);

@@ -18,6 +18,10 @@ oracle/__init__.py.
   tbnn_net and the tbnn closure (:535-583).
 * ``symmetrize``: symmetrize_pointwise (src/nets.jl:350-389) over
   group_stuff's elements (src/symmetry.jl:42-67).
+* ``weight_synthesis`` / ``equi_project`` / ``equi_net`` /
+  ``equi_fullchain``: equivariant_net (src/nets.jl:102-238) with
+  get_weight_synthesis (src/symmetry.jl:156-197), in torch float64 so that
+  the trainables' gradients come from autograd.
 
 Pinned (tests/test_pmlp_oracle.py): forward and every gradient against
 torch.autograd of the same network built from torch.nn.functional's
@@ -31,7 +35,10 @@ closed forms say for pure strain and pure rotation; its forward and gradients
 equal torch.autograd's.  The group tables form a group of 48 (8) distinct
 signed permutation matrices closed under products; the symmetrized model is
 exactly equivariant under every element and leaves an equivariant model
-unchanged.
+unchanged.  The synthesised equivariant network is exactly equivariant under
+every element for random trainables; in 2D the synthesis columns are
+orthonormal fixed points of the brute-force Reynolds projectors of
+get_weight_projectors (src/symmetry.jl:92-127).
 """
 import math
 
@@ -303,3 +310,143 @@ def symmetrize(model, GG, D):
         term = np.stack([qs[a] * qs[b] * y[pidx[(q[a], q[b])]] for (a, b) in comps])
         acc = term if acc is None else acc + term
     return acc / len(els)
+
+
+# ------------------------------------------------------ equivariant_net
+# Written with torch (CPU, float64) so that the compact gradients come from
+# autograd; Julia's column-major reshape is _reshape_f.
+def _reshape_f(t, shape):
+    """column-major reshape (Julia's reshape) of a torch tensor"""
+    nd = t.dim()
+    t = t.permute(*reversed(range(nd))).reshape(tuple(reversed(shape)))
+    return t.permute(*reversed(range(len(shape))))
+
+
+def cayley_tables(D):
+    """group_stuff (src/symmetry.jl:42-67): Cayley table and inverse indices"""
+    mats = [element_matrix(p, s) for p, s in group_elements(D)]
+    n = len(mats)
+    key = {m.astype(int).tobytes(): e for e, m in enumerate(mats)}
+    cayley = np.array([[key[(mats[a] @ mats[b]).astype(int).tobytes()] for b in range(n)] for a in range(n)])
+    unit = key[np.eye(D).astype(int).tobytes()]
+    inverse = np.array([int(np.nonzero(cayley[a] == unit)[0][0]) for a in range(n)])
+    return cayley, inverse
+
+
+def weight_synthesis(D):
+    """get_weight_synthesis (src/symmetry.jl:156-197) with the entries of the
+    synthesis matrices rounded to FP32, the precision the reference stores
+    them in; returns (s_lift, s_mid, s_sink) as float64"""
+    els = group_elements(D)
+    cayley, inverse = cayley_tables(D)
+    nreg, nten = len(els), D * D
+    alpha = float(np.float32(1 / math.sqrt(nreg)))
+    Q = np.zeros((nreg, nten, nten))
+    for h, (p, s) in enumerate(els):
+        for j in range(D):
+            for i in range(D):
+                for y in range(D):
+                    for x in range(D):
+                        Q[h, x + y * D, i + j * D] = s[x] * s[y] * (p[x] == i) * (p[y] == j)
+    relidx = np.array([[cayley[inverse[n], m] for n in range(nreg)] for m in range(nreg)])
+    s_mid = np.zeros((nreg * nreg, nreg))
+    for n in range(nreg):
+        for m in range(nreg):
+            s_mid[m + n * nreg, relidx[m, n]] = alpha
+    s_lift = np.zeros((nreg * nten, nten))
+    s_sink = np.zeros((nten * nreg, nten))
+    for h in range(nreg):
+        for mu in range(nten):
+            for nu in range(nten):
+                s_lift[h + mu * nreg, nu] = alpha * Q[h, mu, nu]
+                s_sink[mu + h * nten, nu] = alpha * Q[h, mu, nu]
+    return s_lift, s_mid, s_sink
+
+
+def equi_shapes(D, widths, use_redelta):
+    """the compact trainable arrays in Lux's order: lift (weight, bias[,
+    weight_re]), mid_1, mid_2 (weight, bias), sink"""
+    nreg, nten = (48 if D == 3 else 8), D * D
+    c1, c2, c3 = widths
+    sh = [("lift.weight", (nten, c1)), ("lift.bias", (c1,))]
+    if use_redelta:
+        sh.append(("lift.weight_re", (1, c1)))
+    sh += [("mid_1.weight", (nreg, c2, c1)), ("mid_1.bias", (c2,)),
+           ("mid_2.weight", (nreg, c3, c2)), ("mid_2.bias", (c3,)), ("sink.weight", (nten, c3))]
+    return sh
+
+
+def equi_unflat(h, D, widths, use_redelta):
+    """flat trainables -> {name: torch float64 tensor (column-major shapes)}"""
+    out, o = {}, 0
+    for name, shp in equi_shapes(D, widths, use_redelta):
+        n = int(np.prod(shp))
+        import torch
+
+        out[name] = _reshape_f(torch.tensor(np.asarray(h[o:o + n], np.float64)), shp)
+        o += n
+    return out
+
+
+def equi_project(ps, D, syn):
+    """project (src/nets.jl:130-170): the network weights (c_in, c_out) and
+    biases of the four layers from the trainables"""
+    import torch
+
+    s_lift, s_mid, s_sink = (torch.tensor(a) for a in syn)
+    nreg, nten = (48 if D == 3 else 8), D * D
+    layers = []
+    w, b = ps["lift.weight"], ps["lift.bias"]
+    c_out = w.shape[1]
+    w = _reshape_f(s_lift @ w, (nreg, nten, c_out)).permute(1, 0, 2)
+    w = _reshape_f(w, (nten, nreg * c_out))
+    if "lift.weight_re" in ps:
+        wre = _reshape_f(ps["lift.weight_re"].repeat(nreg, 1), (1, nreg * c_out))
+        w = torch.cat([w, wre], dim=0)
+    layers.append((w, _reshape_f(b.reshape(1, -1).repeat(nreg, 1), (nreg * c_out,))))
+    for i in (1, 2):
+        w, b = ps[f"mid_{i}.weight"], ps[f"mid_{i}.bias"]
+        _, c_out, c_in = w.shape
+        w = s_mid @ _reshape_f(w, (nreg, c_out * c_in))
+        w = _reshape_f(w, (nreg, nreg, c_out, c_in)).permute(1, 3, 0, 2)
+        layers.append((_reshape_f(w, (nreg * c_in, nreg * c_out)),
+                       _reshape_f(b.reshape(1, -1).repeat(nreg, 1), (nreg * c_out,))))
+    w = ps["sink.weight"]
+    c_in = w.shape[1]
+    w = _reshape_f(s_sink @ w, (nten, nreg, c_in)).permute(1, 2, 0)
+    layers.append((_reshape_f(w, (nreg * c_in, nten)), None))
+    return layers
+
+
+def equi_net(layers, x, D):
+    """the projected network on rows x [nv][nin] with the symmetrising `symm`
+    layer (src/nets.jl:196-215): packed [nv][nt]"""
+    import torch
+
+    a = x
+    for w, b in layers[:3]:
+        a = torch.nn.functional.gelu(a @ w + b, approximate="tanh")
+    s = a @ layers[3][0]
+    if D == 3:
+        xx, yy, zz = s[:, 0], s[:, 4], s[:, 8]
+        xy, yz, zx = (s[:, 1] + s[:, 3]) / 2, (s[:, 5] + s[:, 7]) / 2, (s[:, 2] + s[:, 6]) / 2
+        t = (xx + yy + zz) / 3
+        return torch.stack([xx - t, yy - t, zz - t, xy, yz, zx], dim=1)
+    xx, yy, xy = s[:, 0], s[:, 3], (s[:, 1] + s[:, 2]) / 2
+    t = (xx + yy) / 2
+    return torch.stack([xx - t, yy - t, xy], dim=1)
+
+
+def equi_fullchain(h, D, widths, GG, delta, eps=np.finfo(np.float64).eps, redelta=None):
+    """fullchain of equivariant_net at inference (parameters widened to
+    FP64): packed tau, component-major"""
+    import torch
+
+    syn = weight_synthesis(D)
+    layers = equi_project(equi_unflat(h, D, widths, redelta is not None), D, syn)
+    z = None if redelta is None else redelta_z(GG, delta, *redelta)
+    x, _ = features(GG, None, delta, z, eps)
+    y = equi_net(layers, torch.tensor(x), D).numpy()
+    s = delta**2 * a2_of(GG).ravel()
+    tau = y * s[:, None]
+    return np.stack([tau[:, c].reshape(GG.shape[1:]) for c in range(tau.shape[1])])

@@ -694,7 +694,7 @@ struct sc_cnn {
   long part_len = 0;
   double* lossp = nullptr;
   double* hostd = nullptr;
-  double* gnorm = nullptr;  // per-array gradient norm partials [7][NORM_BLOCKS] then the 7 squared norms
+  double* gnorm = nullptr;  // per-array gradient norm partials, then the squared norms (NORM_DOUBLES)
   int adam_t = 0;
   bool tracefree = false;  // output channels 0-2 (xx, yy, zz) lose their mean (cout 6)
   int ncu = 256;  // compute units of the device (persistent wgrad grid)
@@ -979,7 +979,7 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
       CNN_HIP(hipMalloc(&c->dz[l], nv * width * 2));
     }
     CNN_HIP(hipMalloc(&c->lossp, ((nv + 255) / 256) * sizeof(double) + 16));
-    CNN_HIP(hipMalloc(&c->gnorm, (7 * NORM_BLOCKS + 8) * sizeof(double)));
+    CNN_HIP(hipMalloc(&c->gnorm, NORM_DOUBLES * sizeof(double)));
     CNN_HIP(hipHostMalloc(&c->hostd, 64));
     pack_all(c);
     CNN_HIP(hipStreamSynchronize(c->stream));
@@ -1100,11 +1100,12 @@ int sc_cnn_adamw(sc_cnn* c, double lr, double b1, double b2, double eps, double 
     for (int l = 0; l < 4; ++l) A.b[l] = c->woff[l];
     for (int l = 0; l < 3; ++l) A.b[4 + l] = c->boff[l];
     A.b[7] = c->wlen;
-    double* nrm2 = c->gnorm + 7 * NORM_BLOCKS;
+    A.n = 7;
+    double* nrm2 = c->gnorm + NORM_OFFSET;
     if (clip > 0) {
-      hipLaunchKernelGGL(k_array_sumsq, dim3(NORM_BLOCKS, 7), dim3(256), 0, c->stream, c->g, A, c->gnorm);
+      hipLaunchKernelGGL(k_array_sumsq, dim3(NORM_BLOCKS, A.n), dim3(256), 0, c->stream, c->g, A, c->gnorm);
       CNN_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c->stream, c->gnorm, nrm2);
+      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c->stream, c->gnorm, nrm2, A.n);
       CNN_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_adamw, dim3(unsigned((c->wlen + 255) / 256)), dim3(256), 0, c->stream, c->w, c->g, c->m, c->v,

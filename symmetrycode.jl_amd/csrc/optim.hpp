@@ -1,5 +1,5 @@
 // Optimiser kernels shared by the learned closures: per-array ClipNorm and
-// AdamW on a flat FP32 parameter vector of at most seven arrays (the rules of
+// AdamW on a flat FP32 parameter vector of at most eight arrays (the rules of
 // OptimiserChain(ClipNorm, AdamW), src/training.jl:50, Optimisers.jl 0.4).
 // Internal linkage: every source that includes this gets its own copy.
 #pragma once
@@ -8,13 +8,22 @@
 namespace optim {
 namespace {
 
-// Boundaries of the (up to seven) parameter arrays in the flat parameter
-// vector: array a is [b[a], b[a + 1]); unused trailing arrays are empty
-struct Arrays { long b[8]; };
+// Boundaries of the n (at most MAX_ARRAYS) parameter arrays in the flat
+// parameter vector: array a is [b[a], b[a + 1])
+constexpr int MAX_ARRAYS = 8;
+struct Arrays {
+  long b[MAX_ARRAYS + 1];
+  int n;
+};
+
+// norm workspace (doubles): partials [MAX_ARRAYS][NORM_BLOCKS], then the
+// squared norms at NORM_OFFSET
+constexpr int NORM_BLOCKS = 32;
+constexpr int NORM_OFFSET = MAX_ARRAYS * NORM_BLOCKS;
+constexpr int NORM_DOUBLES = NORM_OFFSET + MAX_ARRAYS;
 
 // squared L2 norm of each parameter array's gradient: block (j, a) sums a
 // strided share of array a into part[a][j]; k_finish_norms adds the shares
-constexpr int NORM_BLOCKS = 32;
 __global__ void k_array_sumsq(const float* g, Arrays A, double* part) {
   __shared__ double sm[256];
   const int a = blockIdx.y;
@@ -32,9 +41,9 @@ __global__ void k_array_sumsq(const float* g, Arrays A, double* part) {
   }
   if (threadIdx.x == 0) part[a * NORM_BLOCKS + blockIdx.x] = sm[0];
 }
-__global__ void k_finish_norms(const double* part, double* nrm2) {
+__global__ void k_finish_norms(const double* part, double* nrm2, int n) {
   const int a = threadIdx.x;
-  if (a >= 7) return;
+  if (a >= n) return;
   double s = 0;
   for (int j = 0; j < NORM_BLOCKS; ++j) s += part[a * NORM_BLOCKS + j];
   nrm2[a] = s;
@@ -50,7 +59,7 @@ __global__ void k_adamw(float* w, const float* g, float* m, float* v, long len, 
   float gi = g[i];
   if (clip > 0) {
     int a = 0;
-    while (a < 6 && i >= A.b[a + 1]) ++a;
+    while (a < A.n - 1 && i >= A.b[a + 1]) ++a;
     const double nrm = sqrt(nrm2[a]);
     if (nrm > clip) gi *= float(clip / nrm);
   }

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -461,6 +462,181 @@ __global__ void k_tb_dw(const float* dy, const float* x, long nv, int ctot, int 
   }
 }
 
+// ------------------------------------------------------ equivariant_net
+// The synthesis of equivariant_net's projected weights (src/nets.jl:102-173,
+// with get_weight_synthesis, src/symmetry.jl:156-197) from the compact
+// trainables, and its transpose for the gradient.  Tables gt: perm, sign,
+// pinv (inverse permutation) [nreg][3], relidx[m][n] = index of g_n^-1 g_m,
+// ninv[m][k] = the n with relidx[m][n] = k.  Network weights are Lux's
+// (c_in, c_out) column-major; compact arrays: lift (nten, c1), mid (nreg,
+// c_out, c_in), sink (nten, c3), all column-major.
+struct EqTab {
+  const int *perm, *sign, *pinv, *relidx, *ninv;
+};
+__device__ __forceinline__ EqTab eq_tab(const int* gt, int nreg) {
+  return {gt, gt + 3 * nreg, gt + 6 * nreg, gt + 9 * nreg, gt + 9 * nreg + nreg * nreg};
+}
+
+// lift: W[mu][h + nreg j] = alpha s_x s_y w[p_x + D p_y][j] (mu = x + D y),
+// W[nten][h + nreg j] = w_re[j]; b[h + nreg j] = bias[j]
+template <typename E>
+__global__ void k_eq_lift(const float* w, const float* wre, const float* bias, E* W, E* b, int D, int nreg, int nin,
+                          int c1, E alpha, const int* gt) {
+  const long id = gid();
+  const int nten = D * D;
+  const long ncol = long(nreg) * c1;
+  if (id < ncol) b[id] = E(bias[id / nreg]);
+  if (id >= ncol * nin) return;
+  const EqTab t = eq_tab(gt, nreg);
+  const int mu = int(id % nin);
+  const long k = id / nin;
+  const int h = int(k % nreg), j = int(k / nreg);
+  if (mu == nten) {
+    W[id] = E(wre[j]);
+    return;
+  }
+  const int x = mu % D, y = mu / D;
+  const int nu = t.perm[3 * h + x] + D * t.perm[3 * h + y];
+  W[id] = (alpha * E(t.sign[3 * h + x] * t.sign[3 * h + y])) * E(w[nu + nten * j]);
+}
+
+// mid: W[n + nreg ci][m + nreg co] = alpha w[relidx[m][n]][co][ci]; b[m + nreg co] = bias[co]
+template <typename E>
+__global__ void k_eq_mid(const float* w, const float* bias, E* W, E* b, int nreg, int cin, int cout, E alpha,
+                         const int* gt) {
+  const long id = gid();
+  const long nrow = long(nreg) * cin, ncol = long(nreg) * cout;
+  if (id < ncol) b[id] = E(bias[id / nreg]);
+  if (id >= nrow * ncol) return;
+  const EqTab t = eq_tab(gt, nreg);
+  const long r = id % nrow, k = id / nrow;
+  const int n = int(r % nreg), ci = int(r / nreg);
+  const int m = int(k % nreg), co = int(k / nreg);
+  W[id] = alpha * E(w[t.relidx[m * nreg + n] + long(nreg) * (co + long(cout) * ci)]);
+}
+
+// sink: W[h + nreg ci][mu] = alpha s_x s_y w[p_x + D p_y][ci]
+template <typename E>
+__global__ void k_eq_sink(const float* w, E* W, int D, int nreg, int c3, E alpha, const int* gt) {
+  const long id = gid();
+  const int nten = D * D;
+  const long nrow = long(nreg) * c3;
+  if (id >= nrow * nten) return;
+  const EqTab t = eq_tab(gt, nreg);
+  const long r = id % nrow;
+  const int mu = int(id / nrow);
+  const int h = int(r % nreg), ci = int(r / nreg);
+  const int x = mu % D, y = mu / D;
+  const int nu = t.perm[3 * h + x] + D * t.perm[3 * h + y];
+  W[id] = (alpha * E(t.sign[3 * h + x] * t.sign[3 * h + y])) * E(w[nu + nten * ci]);
+}
+
+// transposes (gradients of the compact arrays, accumulated)
+__global__ void k_eq_lift_T(const float* GW, const float* Gb, float* gw, float* gre, float* gb, int D, int nreg,
+                            int nin, int c1, float alpha, const int* gt) {
+  const long id = gid();
+  const int nten = D * D;
+  const EqTab t = eq_tab(gt, nreg);
+  if (id < long(nten) * c1) {
+    const int nu = int(id % nten), j = int(id / nten);
+    const int i = nu % D, jj = nu / D;
+    float acc = 0.f;
+    for (int h = 0; h < nreg; ++h) {
+      const int x = t.pinv[3 * h + i], y = t.pinv[3 * h + jj];
+      acc += float(t.sign[3 * h + x] * t.sign[3 * h + y]) * GW[(x + D * y) + long(nin) * (h + long(nreg) * j)];
+    }
+    gw[id] += alpha * acc;
+  } else if (id < long(nten) * c1 + c1) {
+    const int j = int(id - long(nten) * c1);
+    float ab = 0.f, ar = 0.f;
+    for (int h = 0; h < nreg; ++h) {
+      ab += Gb[h + long(nreg) * j];
+      if (gre) ar += GW[nten + long(nin) * (h + long(nreg) * j)];
+    }
+    gb[j] += ab;
+    if (gre) gre[j] += ar;
+  }
+}
+
+__global__ void k_eq_mid_T(const float* GW, const float* Gb, float* gw, float* gb, int nreg, int cin, int cout,
+                           float alpha, const int* gt) {
+  const long id = gid();
+  const EqTab t = eq_tab(gt, nreg);
+  const long nw = long(nreg) * cout * cin;
+  const long nrow = long(nreg) * cin;
+  if (id < nw) {
+    const int kk = int(id % nreg);
+    const long rest = id / nreg;
+    const int co = int(rest % cout), ci = int(rest / cout);
+    float acc = 0.f;
+    for (int m = 0; m < nreg; ++m) {
+      const int n = t.ninv[m * nreg + kk];
+      acc += GW[(n + long(nreg) * ci) + nrow * (m + long(nreg) * co)];
+    }
+    gw[id] += alpha * acc;
+  } else if (id < nw + cout) {
+    const int co = int(id - nw);
+    float acc = 0.f;
+    for (int m = 0; m < nreg; ++m) acc += Gb[m + long(nreg) * co];
+    gb[co] += acc;
+  }
+}
+
+__global__ void k_eq_sink_T(const float* GW, float* gw, int D, int nreg, int c3, float alpha, const int* gt) {
+  const long id = gid();
+  const int nten = D * D;
+  if (id >= long(nten) * c3) return;
+  const EqTab t = eq_tab(gt, nreg);
+  const int nu = int(id % nten), ci = int(id / nten);
+  const int i = nu % D, jj = nu / D;
+  const long nrow = long(nreg) * c3;
+  float acc = 0.f;
+  for (int h = 0; h < nreg; ++h) {
+    const int x = t.pinv[3 * h + i], y = t.pinv[3 * h + jj];
+    acc += float(t.sign[3 * h + x] * t.sign[3 * h + y]) * GW[(h + long(nreg) * ci) + nrow * (x + D * y)];
+  }
+  gw[id] += alpha * acc;
+}
+
+// equivariant_net's `symm` layer: the D^2 sink outputs (mu = x + D y) made
+// symmetric and trace-free, packed (src/nets.jl:196-215)
+template <typename E>
+__global__ void k_symm_sink(const E* s9, E* y, long nv, int D) {
+  const long v = gid();
+  if (v >= nv) return;
+  const E* q = s9 + v * D * D;
+  if (D == 3) {
+    const E xx = q[0], yy = q[4], zz = q[8];
+    const E xy = (q[1] + q[3]) / E(2), yz = (q[5] + q[7]) / E(2), zx = (q[2] + q[6]) / E(2);
+    const E t = (xx + yy + zz) / E(3);
+    E* o = y + v * 6;
+    o[0] = xx - t; o[1] = yy - t; o[2] = zz - t; o[3] = xy; o[4] = yz; o[5] = zx;
+  } else {
+    const E xx = q[0], yy = q[3], xy = (q[1] + q[2]) / E(2);
+    const E t = (xx + yy) / E(2);
+    E* o = y + v * 3;
+    o[0] = xx - t; o[1] = yy - t; o[2] = xy;
+  }
+}
+__global__ void k_symm_sink_T(const float* dy, float* d9, long nv, int D) {
+  const long v = gid();
+  if (v >= nv) return;
+  float* q = d9 + v * D * D;
+  if (D == 3) {
+    const float* g = dy + v * 6;
+    const float mm = (g[0] + g[1] + g[2]) / 3.f;
+    q[0] = g[0] - mm; q[4] = g[1] - mm; q[8] = g[2] - mm;
+    q[1] = q[3] = g[3] / 2.f;
+    q[5] = q[7] = g[4] / 2.f;
+    q[2] = q[6] = g[5] / 2.f;
+  } else {
+    const float* g = dy + v * 3;
+    const float mm = (g[0] + g[1]) / 2.f;
+    q[0] = g[0] - mm; q[3] = g[1] - mm;
+    q[1] = q[2] = g[2] / 2.f;
+  }
+}
+
 unsigned nb(long n) { return unsigned((n + 255) / 256); }
 
 }  // namespace
@@ -472,6 +648,11 @@ struct sc_pmlp {
   // [+ z], output the nbas basis coefficients, rows of the input matrix carry
   // the basis tensors after the network inputs: ctot = nin + nt nbas)
   int arch = 0, ni = 0, nbas = 0, ctot = 9;
+  // arch 2: equivariant_net, the mlp over |G| = nreg copies of each hidden
+  // channel whose weights are synthesised from compact trainable parameters
+  // (p, g, m, v hold the compact vector; pf, gf the synthesised network)
+  int nreg = 1, nten = 0;
+  int wf[3] = {0, 0, 0};  // network widths (w times nreg for arch 2)
   bool redelta = false;
   long nvmax = 0;
   hipStream_t stream = nullptr;
@@ -480,7 +661,14 @@ struct sc_pmlp {
   std::string err;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   long plen = 0;
-  long off[8] = {};  // W1 b1 W2 b2 W3 b3 W4 end
+  long cb[9] = {};  // bounds of the trainable arrays (ncarr of them)
+  int ncarr = 7;
+  // network parameters: W1 b1 W2 b2 W3 b3 W4 at fo[] (flen in all); the
+  // trainable vector itself except for arch 2
+  float *pf = nullptr, *gf = nullptr;
+  long fo[8] = {};
+  long flen = 0;
+  int* gt = nullptr;  // arch 2 group tables (device): perm, sign, pinv [nreg][3], relidx, ninv [nreg][nreg]
   // activations of the last forward (z pre-activation, a = gelu(z)) per
   // hidden layer, two gradient scratch buffers, a vector of ones (bias sums)
   float *z[3] = {}, *a[3] = {}, *d0 = nullptr, *d1 = nullptr, *ones = nullptr;
@@ -491,11 +679,14 @@ struct sc_pmlp {
   double *part = nullptr, *gnorm = nullptr;
   int adam_t = 0;
   long last_nv = 0;
-  int nout() const { return arch == 1 ? nbas : nt; }
-  int maxw() const { return std::max({w[0], w[1], w[2], nin, nt, nout()}); }
-  int cin(int l) const { return l == 0 ? nin : w[l - 1]; }
-  int cout(int l) const { return l == 3 ? nout() : w[l]; }
+  int nout() const { return arch == 1 ? nbas : (arch == 2 ? nten : nt); }
+  int maxw() const { return std::max({wf[0], wf[1], wf[2], nin, nt, nout()}); }
+  int cin(int l) const { return l == 0 ? nin : wf[l - 1]; }
+  int cout(int l) const { return l == 3 ? nout() : wf[l]; }
   int ldin(int l) const { return l == 0 ? ctot : cin(l); }
+  // the synthesis matrices' scale 1 / sqrt|G|, stored in FP32 as the
+  // reference's training precision keeps them
+  float alpha() const { return float(1.0 / std::sqrt(double(nreg))); }
 };
 
 namespace {
@@ -517,25 +708,76 @@ int pguard(sc_pmlp* c, F&& f) {
   }
 }
 
+// equivariant_net: network parameters dst (fo layout) from the trainables
+template <typename E>
+void synthesise(sc_pmlp* c, E* dst, E alpha) {
+  const int D = c->D, nreg = c->nreg;
+  const float* p = c->p;
+  const long* cb = c->cb;
+  const int re = c->redelta ? 1 : 0;
+  const long n0 = long(c->nin) * c->wf[0];
+  hipLaunchKernelGGL(k_eq_lift<E>, dim3(nb(n0)), dim3(256), 0, c->stream, p + cb[0], re ? p + cb[2] : nullptr,
+                     p + cb[1], dst + c->fo[0], dst + c->fo[1], D, nreg, c->nin, c->w[0], alpha, (const int*)c->gt);
+  for (int l = 1; l < 3; ++l) {
+    const long nw = long(c->wf[l - 1]) * c->wf[l];
+    const long wo = cb[2 + re + 2 * (l - 1)], bo = wo + long(nreg) * c->w[l] * c->w[l - 1];
+    hipLaunchKernelGGL(k_eq_mid<E>, dim3(nb(nw)), dim3(256), 0, c->stream, p + wo, p + bo, dst + c->fo[2 * l],
+                       dst + c->fo[2 * l + 1], nreg, c->w[l - 1], c->w[l], alpha, (const int*)c->gt);
+  }
+  const long n3 = long(c->wf[2]) * c->nten;
+  hipLaunchKernelGGL(k_eq_sink<E>, dim3(nb(n3)), dim3(256), 0, c->stream, p + cb[c->ncarr - 1], dst + c->fo[6], D,
+                     nreg, c->w[2], alpha, (const int*)c->gt);
+  PM_HIP(hipGetLastError());
+}
+
+// equivariant_net: trainable gradient g += Sᵀ gf
+void synthesise_T(sc_pmlp* c) {
+  const int D = c->D, nreg = c->nreg;
+  float* g = c->g;
+  const long* cb = c->cb;
+  const int re = c->redelta ? 1 : 0;
+  const float al = c->alpha();
+  const long n0 = long(c->nten) * c->w[0] + c->w[0];
+  hipLaunchKernelGGL(k_eq_lift_T, dim3(nb(n0)), dim3(256), 0, c->stream, (const float*)c->gf + c->fo[0],
+                     (const float*)c->gf + c->fo[1], g + cb[0], re ? g + cb[2] : nullptr, g + cb[1], D, nreg, c->nin,
+                     c->w[0], al, (const int*)c->gt);
+  for (int l = 1; l < 3; ++l) {
+    const long nw = long(nreg) * c->w[l] * c->w[l - 1];
+    const long wo = cb[2 + re + 2 * (l - 1)];
+    hipLaunchKernelGGL(k_eq_mid_T, dim3(nb(nw + c->w[l])), dim3(256), 0, c->stream,
+                       (const float*)c->gf + c->fo[2 * l], (const float*)c->gf + c->fo[2 * l + 1], g + wo,
+                       g + wo + nw, nreg, c->w[l - 1], c->w[l], al, (const int*)c->gt);
+  }
+  const long n3 = long(c->nten) * c->w[2];
+  hipLaunchKernelGGL(k_eq_sink_T, dim3(nb(n3)), dim3(256), 0, c->stream, (const float*)c->gf + c->fo[6],
+                     g + cb[c->ncarr - 1], D, nreg, c->w[2], al, (const int*)c->gt);
+  PM_HIP(hipGetLastError());
+}
+
 // Z (co x nv) = Wᵀ (co x ci) X (ci x nv, leading dimension ldin(l))
 void gemm_fwd(sc_pmlp* c, int l, const float* X, float* Z, long nv) {
   const float one = 1.f, zero = 0.f;
   const int ci = c->cin(l), co = c->cout(l);
   PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                        c->p + c->off[2 * l], ci, X, c->ldin(l), &zero, Z, co));
+                        c->pf + c->fo[2 * l], ci, X, c->ldin(l), &zero, Z, co));
 }
 
 void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
   if (nv < 1 || nv > c->nvmax) throw std::runtime_error("voxel count out of range");
+  if (c->arch == 2) synthesise<float>(c, c->pf, float(c->alpha()));
   const float* in = x;
   for (int l = 0; l < 3; ++l) {
     gemm_fwd(c, l, in, c->z[l], nv);
-    hipLaunchKernelGGL(k_bias_gelu, dim3(nb(nv * c->w[l])), dim3(256), 0, c->stream, c->z[l], c->p + c->off[2 * l + 1],
-                       c->a[l], nv, c->w[l]);
+    hipLaunchKernelGGL(k_bias_gelu, dim3(nb(nv * c->wf[l])), dim3(256), 0, c->stream, c->z[l],
+                       c->pf + c->fo[2 * l + 1], c->a[l], nv, c->wf[l]);
     PM_HIP(hipGetLastError());
     in = c->a[l];
   }
-  if (c->arch == 1) {
+  if (c->arch == 2) {
+    // the sink's D^2 outputs, then the symmetrising trace removal
+    gemm_fwd(c, 3, in, c->d1, nv);
+    hipLaunchKernelGGL(k_symm_sink<float>, dim3(nb(nv)), dim3(256), 0, c->stream, (const float*)c->d1, y, nv, c->D);
+  } else if (c->arch == 1) {
     // basis coefficients, then the contraction with the rows' basis tensors
     gemm_fwd(c, 3, in, c->d1, nv);
     hipLaunchKernelGGL(k_tb_contract<float>, dim3(nb(nv)), dim3(256), 0, c->stream, (const float*)c->d1, x, nv, c->ctot,
@@ -553,7 +795,10 @@ void backward_impl(sc_pmlp* c, const float* x, const float* dy, long nv) {
   if (nv != c->last_nv) throw std::runtime_error("backward needs the voxel count of the last forward");
   const float one = 1.f, zero = 0.f;
   float* d = c->d0;
-  if (c->arch == 1) {
+  if (c->arch == 2) {
+    hipLaunchKernelGGL(k_symm_sink_T, dim3(nb(nv)), dim3(256), 0, c->stream, dy, d, nv, c->D);
+    PM_HIP(hipMemsetAsync(c->gf, 0, c->flen * sizeof(float), c->stream));
+  } else if (c->arch == 1) {
     // dL/dw_b = Σ_t dy_t basis_tb
     hipLaunchKernelGGL(k_tb_dw, dim3(nb(nv)), dim3(256), 0, c->stream, dy, x, nv, c->ctot, c->nin, c->nt, c->nbas, d);
   } else {
@@ -567,22 +812,23 @@ void backward_impl(sc_pmlp* c, const float* x, const float* dy, long nv) {
     const float* A = l == 0 ? x : c->a[l - 1];
     // dW (ci x co) += A (ci x nv) dZᵀ (nv x co)
     PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, ci, co, int(nv), &one, A,
-                          c->ldin(l), d, co, &one, c->g + c->off[2 * l], ci));
+                          c->ldin(l), d, co, &one, c->gf + c->fo[2 * l], ci));
     if (l < 3) {
       // db (co) += dZ (co x nv) 1
       PM_BLAS(rocblas_sgemv(c->blas, rocblas_operation_none, co, int(nv), &one, d, co, c->ones, 1, &one,
-                            c->g + c->off[2 * l + 1], 1));
+                            c->gf + c->fo[2 * l + 1], 1));
     }
     if (l > 0) {
       // dA (ci x nv) = W (ci x co) dZ (co x nv), then dZ_{l-1} = dA * gelu'(z_{l-1})
       float* dn = d == c->d0 ? c->d1 : c->d0;
       PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, ci, int(nv), co, &one,
-                            c->p + c->off[2 * l], ci, d, co, &zero, dn, ci));
+                            c->pf + c->fo[2 * l], ci, d, co, &zero, dn, ci));
       hipLaunchKernelGGL(k_gelu_back, dim3(nb(nv * ci)), dim3(256), 0, c->stream, dn, c->z[l - 1], nv * ci);
       PM_HIP(hipGetLastError());
       d = dn;
     }
   }
+  if (c->arch == 2) synthesise_T(c);  // the trainable gradient += Sᵀ (network gradient)
 }
 
 double sum_partials(sc_pmlp* c, unsigned nblk) {
@@ -612,18 +858,20 @@ void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
   double* out = c->hA;
   for (int l = 0; l < 4; ++l) {
     const int ci = c->cin(l), co = c->cout(l);
-    double* dst = l == 3 && c->arch == 0 ? y : out;
+    double* dst = l == 3 && c->arch == 0 ? y : out;  // the others end in a head
     PM_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                          c->pd + c->off[2 * l], ci, in, l == 0 ? c->ctot : ci, &zero, dst, co));
+                          c->pd + c->fo[2 * l], ci, in, l == 0 ? c->ctot : ci, &zero, dst, co));
     if (l < 3) {
       hipLaunchKernelGGL(k_bias_gelu_inf<double>, dim3(nb(nv * co)), dim3(256), 0, c->stream, dst,
-                         (const double*)c->pd + c->off[2 * l + 1], nv * co, co);
+                         (const double*)c->pd + c->fo[2 * l + 1], nv * co, co);
       PM_HIP(hipGetLastError());
       in = dst;
       out = dst == c->hA ? c->hB : c->hA;
     }
   }
-  if (c->arch == 1)
+  if (c->arch == 2)
+    hipLaunchKernelGGL(k_symm_sink<double>, dim3(nb(nv)), dim3(256), 0, c->stream, (const double*)out, y, nv, c->D);
+  else if (c->arch == 1)
     hipLaunchKernelGGL(k_tb_contract<double>, dim3(nb(nv)), dim3(256), 0, c->stream, (const double*)out, x, nv,
                        c->ctot, c->nin, c->nt, c->nbas, y);
   else
@@ -639,14 +887,15 @@ void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
 template <typename T>
 void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, double mu, double sigma, int sym,
                  T* tau, hipStream_t s) {
-  constexpr long CHUNK = 1L << 20;
+  constexpr long CHUNK = 1L << 20, CHUNK_ELEMS = 1L << 27;
   if (sym && c->arch != 0) throw std::runtime_error("the group average applies to the mlp closure");
   const int G = sym ? (c->D == 3 ? 48 : 8) : 1;
   hipStream_t old = c->stream;
   c->stream = s;
   PM_BLAS(rocblas_set_stream(c->blas, s));
   try {
-    const long vc = std::min(np, CHUNK / G);  // voxels per chunk
+    // voxels per chunk: at most 2^20 input rows and 2^27 activations
+    const long vc = std::max(1L, std::min({np, CHUNK / G, CHUNK_ELEMS / (long(c->maxw()) * G)}));
     const long cap = vc * G;                  // input rows per chunk
     if (cap > c->icap) {
       for (double** q : {&c->xd, &c->hA, &c->hB, &c->yd})
@@ -662,11 +911,17 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
       c->icap = cap;
     }
     if (!c->pd) {
-      PM_HIP(hipMalloc(&c->pd, c->plen * sizeof(double)));
+      PM_HIP(hipMalloc(&c->pd, c->flen * sizeof(double)));
       PM_HIP(hipMalloc(&c->zdev, sizeof(double)));
     }
-    hipLaunchKernelGGL(k_widen, dim3(nb(c->plen)), dim3(256), 0, s, c->p, c->pd, c->plen);
-    PM_HIP(hipGetLastError());
+    if (c->arch == 2) {
+      // the synthesis in FP64 from the FP32 trainables: alpha is the FP32
+      // 1/sqrt|G| of the reference's synthesis matrices
+      synthesise<double>(c, c->pd, double(c->alpha()));
+    } else {
+      hipLaunchKernelGGL(k_widen, dim3(nb(c->plen)), dim3(256), 0, s, c->p, c->pd, c->plen);
+      PM_HIP(hipGetLastError());
+    }
     const double d2 = delta * delta;
     if (c->redelta) {
       const unsigned nblk = std::min(nb(np), 1024u);
@@ -715,13 +970,67 @@ template void closure_tau<double>(sc_pmlp*, const double*, long, double, double,
 extern "C" {
 
 namespace {
+// group_stuff's tables (src/symmetry.jl:42-67) for the synthesis kernels:
+// perm, sign, pinv [nreg][3], relidx[m][n] = index of g_n^-1 g_m (the Cayley
+// gather of get_weight_synthesis), ninv[m][k] = the n with relidx[m][n] = k
+std::vector<int> group_tables(int D) {
+  const int np = D == 3 ? 6 : 2, ns = D == 3 ? 8 : 4, nreg = np * ns;
+  const int P3[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}, {1, 0, 2}, {0, 2, 1}};
+  const int S3[8][3] = {{1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {1, 1, -1}, {-1, -1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
+  const int P2[2][3] = {{0, 1, 2}, {1, 0, 2}};
+  const int S2[4][3] = {{1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
+  std::vector<int> t(9 * nreg + 2 * nreg * nreg, 0);
+  int* perm = t.data();
+  int* sign = perm + 3 * nreg;
+  int* pinv = perm + 6 * nreg;
+  int* relidx = perm + 9 * nreg;
+  int* ninv = relidx + nreg * nreg;
+  std::vector<std::array<int, 9>> mats(nreg);
+  for (int e = 0; e < nreg; ++e) {
+    const int pi = e % np, si = e / np;
+    for (int a = 0; a < 3; ++a) {
+      perm[3 * e + a] = D == 3 ? P3[pi][a] : P2[pi][a];
+      sign[3 * e + a] = D == 3 ? S3[si][a] : S2[si][a];
+    }
+    for (int a = 0; a < D; ++a) pinv[3 * e + perm[3 * e + a]] = a;
+    std::array<int, 9> M{};
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) M[3 * i + j] = sign[3 * e + i] * (perm[3 * e + i] == j ? 1 : 0);
+    mats[e] = M;
+  }
+  auto find = [&](const std::array<int, 9>& M) {
+    for (int e = 0; e < nreg; ++e)
+      if (mats[e] == M) return e;
+    throw std::runtime_error("group tables: product outside the group");
+  };
+  auto mul = [&](int a, int b) {
+    std::array<int, 9> C{};
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j)
+        for (int k = 0; k < D; ++k) C[3 * i + j] += mats[a][3 * i + k] * mats[b][3 * k + j];
+    return find(C);
+  };
+  std::vector<int> inv(nreg);
+  for (int e = 0; e < nreg; ++e)
+    for (int f = 0; f < nreg; ++f)
+      if (mul(e, f) == 0) inv[e] = f;
+  for (int m = 0; m < nreg; ++m)
+    for (int n = 0; n < nreg; ++n) {
+      const int k = mul(inv[n], m);
+      relidx[m * nreg + n] = k;
+      ninv[m * nreg + k] = n;
+    }
+  return t;
+}
+
 int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max,
                 sc_pmlp** out) {
   if (!out) return 1;
   *out = nullptr;
   // every activation and input matrix (nvox_max x width) stays below 2^31 entries
+  const long nreg = arch == 2 ? (D == 3 ? 48 : 8) : 1;
   if ((D != 2 && D != 3) || c1 < 1 || c2 < 1 || c3 < 1 || c1 > 1024 || c2 > 1024 || c3 > 1024 || nvox_max < 1 ||
-      nvox_max * std::max({c1, c2, c3, 48}) >= (1L << 31))
+      nvox_max * std::max({nreg * c1, nreg * c2, nreg * c3, 48L}) >= (1L << 31))
     return 1;
   g_pmlp_err.clear();
   (void)hipGetLastError();
@@ -740,8 +1049,13 @@ int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64
   } else {
     c->nin = c->ng + (c->redelta ? 1 : 0);
     c->ctot = c->nin;
+    if (arch == 2) {
+      c->nreg = int(nreg);
+      c->nten = c->ng;
+    }
   }
   c->w[0] = c1; c->w[1] = c2; c->w[2] = c3;
+  for (int l = 0; l < 3; ++l) c->wf[l] = c->w[l] * c->nreg;
   c->nvmax = nvox_max;
   try {
     PM_HIP(hipStreamCreate(&c->stream));
@@ -750,17 +1064,48 @@ int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64
     PM_BLAS(rocblas_set_stream(c->blas, c->stream));
     long o = 0;
     for (int l = 0; l < 4; ++l) {
-      c->off[2 * l] = o;
+      c->fo[2 * l] = o;
       o += long(c->cin(l)) * c->cout(l);
       if (l < 3) {
-        c->off[2 * l + 1] = o;
+        c->fo[2 * l + 1] = o;
         o += c->cout(l);
       }
     }
-    c->off[7] = o;
-    c->plen = o;
-    // glorot_uniform per weight (fan_in + fan_out = c_in + c_out for 1x1
-    // kernels), zero biases, from a splitmix64 stream
+    c->fo[7] = o;
+    c->flen = o;
+    // trainable arrays with their glorot_uniform bounds (Lux's fans of each
+    // array; biases zero)
+    std::vector<double> lims;
+    if (arch == 2) {
+      // lift (weight, bias[, weight_re]), mid_1 and mid_2 (weight, bias), sink
+      const int re = c->redelta ? 1 : 0, nten = c->nten;
+      std::vector<std::pair<long, double>> arrs = {{long(nten) * c1, std::sqrt(6.0 / (nten + c1))}, {c1, 0.0}};
+      if (re) arrs.push_back({c1, std::sqrt(6.0 / (1 + c1))});
+      for (int l = 1; l < 3; ++l) {
+        arrs.push_back({nreg * c->w[l] * c->w[l - 1], std::sqrt(6.0 / (nreg * (c->w[l] + c->w[l - 1])))});
+        arrs.push_back({c->w[l], 0.0});
+      }
+      arrs.push_back({long(nten) * c3, std::sqrt(6.0 / (nten + c3))});
+      long q = 0;
+      for (size_t i = 0; i < arrs.size(); ++i) {
+        c->cb[i] = q;
+        q += arrs[i].first;
+        lims.push_back(arrs[i].second);
+      }
+      c->ncarr = int(arrs.size());
+      c->cb[c->ncarr] = q;
+      c->plen = q;
+    } else {
+      for (int i = 0; i < 8; ++i) c->cb[i] = c->fo[i];
+      for (int l = 0; l < 4; ++l) {
+        lims.push_back(std::sqrt(6.0 / (c->cin(l) + c->cout(l))));
+        if (l < 3) lims.push_back(0.0);
+      }
+      c->ncarr = 7;
+      c->plen = o;
+    }
+    o = c->plen;
+    // uniform(-lim, lim) from a splitmix64 stream
     std::vector<float> h(o, 0.f);
     uint64_t s = seed * 0x9E3779B97F4A7C15ull + 777;
     auto rnd = [&]() {
@@ -771,17 +1116,26 @@ int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64
       zz ^= zz >> 31;
       return double(zz >> 11) * 0x1.0p-53;
     };
-    for (int l = 0; l < 4; ++l) {
-      const double lim = std::sqrt(6.0 / (c->cin(l) + c->cout(l)));
-      for (long i = 0; i < long(c->cin(l)) * c->cout(l); ++i) h[c->off[2 * l] + i] = float((2 * rnd() - 1) * lim);
-    }
+    for (int a = 0; a < c->ncarr; ++a)
+      if (lims[a] > 0)
+        for (long i = c->cb[a]; i < c->cb[a + 1]; ++i) h[i] = float((2 * rnd() - 1) * lims[a]);
     for (float** q : {&c->p, &c->g, &c->m, &c->v}) PM_HIP(hipMalloc(q, o * sizeof(float)));
     PM_HIP(hipMemcpy(c->p, h.data(), o * sizeof(float), hipMemcpyHostToDevice));
     for (float* q : {c->g, c->m, c->v}) PM_HIP(hipMemset(q, 0, o * sizeof(float)));
+    if (arch == 2) {
+      PM_HIP(hipMalloc(&c->pf, c->flen * sizeof(float)));
+      PM_HIP(hipMalloc(&c->gf, c->flen * sizeof(float)));
+      const std::vector<int> t = group_tables(D);
+      PM_HIP(hipMalloc(&c->gt, t.size() * sizeof(int)));
+      PM_HIP(hipMemcpy(c->gt, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
+    } else {
+      c->pf = c->p;
+      c->gf = c->g;
+    }
     const size_t nv = size_t(nvox_max);
     for (int l = 0; l < 3; ++l) {
-      PM_HIP(hipMalloc(&c->z[l], nv * c->w[l] * sizeof(float)));
-      PM_HIP(hipMalloc(&c->a[l], nv * c->w[l] * sizeof(float)));
+      PM_HIP(hipMalloc(&c->z[l], nv * c->wf[l] * sizeof(float)));
+      PM_HIP(hipMalloc(&c->a[l], nv * c->wf[l] * sizeof(float)));
     }
     PM_HIP(hipMalloc(&c->d0, nv * c->maxw() * sizeof(float)));
     PM_HIP(hipMalloc(&c->d1, nv * c->maxw() * sizeof(float)));
@@ -789,7 +1143,7 @@ int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64
     std::vector<float> one(nv, 1.f);
     PM_HIP(hipMemcpy(c->ones, one.data(), nv * sizeof(float), hipMemcpyHostToDevice));
     PM_HIP(hipMalloc(&c->part, 1024 * sizeof(double)));
-    PM_HIP(hipMalloc(&c->gnorm, (7 * NORM_BLOCKS + 8) * sizeof(double)));
+    PM_HIP(hipMalloc(&c->gnorm, NORM_DOUBLES * sizeof(double)));
   } catch (std::exception& e) {
     g_pmlp_err = e.what();
     sc_pmlp_destroy(c);
@@ -808,11 +1162,20 @@ int sc_tbnn_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed
   return create_impl(1, D, use_redelta, c1, c2, c3, seed, nvox_max, out);
 }
 
+int sc_equi_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out) {
+  return create_impl(2, D, use_redelta, c1, c2, c3, seed, nvox_max, out);
+}
+
 int sc_pmlp_destroy(sc_pmlp* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->p, c->g, c->m, c->v, c->d0, c->d1, c->ones, c->pd, c->zdev, c->xd, c->hA, c->hB, c->yd, c->part, c->gnorm};
+  void* ptrs[] = {c->p, c->g, c->m, c->v, c->d0, c->d1, c->ones, c->pd, c->zdev, c->xd, c->hA, c->hB, c->yd, c->part, c->gnorm,
+                  c->gt};
   for (void* q : ptrs) if (q) (void)hipFree(q);
+  if (c->arch == 2) {
+    if (c->pf) (void)hipFree(c->pf);
+    if (c->gf) (void)hipFree(c->gf);
+  }
   for (int l = 0; l < 3; ++l) {
     if (c->z[l]) (void)hipFree(c->z[l]);
     if (c->a[l]) (void)hipFree(c->a[l]);
@@ -829,6 +1192,7 @@ int sc_pmlp_info(const sc_pmlp* c, int64_t* info) {
   if (!c || !info) return 1;
   info[0] = c->D; info[1] = c->nin; info[2] = c->w[0]; info[3] = c->w[1]; info[4] = c->w[2];
   info[5] = c->nt; info[6] = c->plen; info[7] = c->nvmax; info[8] = c->arch; info[9] = c->ctot;
+  info[10] = c->nreg; info[11] = c->flen;
   return 0;
 }
 
@@ -913,12 +1277,13 @@ int sc_pmlp_adamw(sc_pmlp* c, double lr, double b1, double b2, double eps, doubl
     c->adam_t += 1;
     const float c1 = float(1.0 - std::pow(b1, c->adam_t)), c2 = float(1.0 - std::pow(b2, c->adam_t));
     Arrays A;
-    for (int i = 0; i < 8; ++i) A.b[i] = c->off[i];
-    double* nrm2 = c->gnorm + 7 * NORM_BLOCKS;
+    A.n = c->ncarr;
+    for (int i = 0; i <= A.n; ++i) A.b[i] = c->cb[i];
+    double* nrm2 = c->gnorm + NORM_OFFSET;
     if (clip > 0) {
-      hipLaunchKernelGGL(k_array_sumsq, dim3(NORM_BLOCKS, 7), dim3(256), 0, c->stream, c->g, A, c->gnorm);
+      hipLaunchKernelGGL(k_array_sumsq, dim3(NORM_BLOCKS, A.n), dim3(256), 0, c->stream, c->g, A, c->gnorm);
       PM_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c->stream, c->gnorm, nrm2);
+      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(64), 0, c->stream, c->gnorm, nrm2, A.n);
       PM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_adamw, dim3(unsigned((c->plen + 255) / 256)), dim3(256), 0, c->stream, c->p, c->g, c->m, c->v,

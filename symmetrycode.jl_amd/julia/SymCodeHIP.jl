@@ -286,13 +286,17 @@ mutable struct PointwiseMLP
     ptr::Ptr{Cvoid}
     D::Int
 end
-# tbnn = true: the tensor-basis network (tbnn_net, src/nets.jl:535-556) instead
+# tbnn = true: the tensor-basis network (tbnn_net, src/nets.jl:535-556);
+# equi = true: equivariant_net (src/nets.jl:102-238, compact trainables)
 function PointwiseMLP(D, nchan; use_redelta = false, same_as_equi = false, seed = 0, nvox_max = 1 << 21,
-                      tbnn = false)
+                      tbnn = false, equi = false)
     w = same_as_equi ? nchan .* (D == 3 ? 48 : 8) : nchan
     out = Ref{Ptr{Cvoid}}(C_NULL)
     rc = if tbnn
         ccall((:sc_tbnn_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
+              D, use_redelta, w[1], w[2], w[3], seed, nvox_max, out)
+    elseif equi
+        ccall((:sc_equi_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
               D, use_redelta, w[1], w[2], w[3], seed, nvox_max, out)
     else
         ccall((:sc_pmlp_create, libsymcode), Cint, (Cint, Cint, Cint, Cint, Cint, UInt64, Clong, Ref{Ptr{Cvoid}}),
@@ -307,11 +311,12 @@ end
 # Lux parameters of `mlp` (layers lift, mid_1, mid_2, sink; each weight
 # (1,…,1, c_in, c_out)) in the library's flat order [W1, b1, W2, b2, W3, b3, W4]
 function setparams!(net::PointwiseMLP, ps)
+    # every leaf array in Lux's order, column-major: mlp / tbnn_net weights
+    # and biases; equivariant_net's compact lift (weight, bias[, weight_re]),
+    # mid_1, mid_2 and sink arrays
     h = Float32[]
-    layers = haskey(ps, :lift) ? (ps.lift, ps.mid_1, ps.mid_2, ps.sink) : values(ps)  # tbnn_net: layer_1..4
-    for (l, p) in enumerate(layers)
-        append!(h, vec(p.weight))
-        l < 4 && append!(h, p.bias)
+    for layer in values(ps), leaf in values(layer)
+        append!(h, vec(leaf))
     end
     ccall((:sc_pmlp_set_params, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Float32}), net.ptr, h) == 0 ||
         error("sc_pmlp_set_params failed (parameter count?)")

@@ -13,6 +13,6 @@ from .closures import (create_clark, create_smagorinsky, create_dynamic_smagorin
                        create_bardina,
                        create_learned_closure, les_rhs, les_wray3, les_wray3_cfl, les_rk4, les_rk4_cfl,
                        les_rhs_vjp, les_rk4_vjp, les_wray3_vjp, solve_les, gradient_phys)
-from .pmlp import PointwiseMLP, TBNN, PointwiseClosure, create_pointwise_closure, symmetrize_pointwise  # noqa: F401
+from .pmlp import PointwiseMLP, TBNN, EquivariantNet, PointwiseClosure, create_pointwise_closure, symmetrize_pointwise  # noqa: F401
 
 lib()  # fail loudly at import if the HIP library is missing

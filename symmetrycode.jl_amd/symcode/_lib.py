@@ -112,6 +112,7 @@ SIGNATURES = {
     "sc_lincomb": [P, P, P, c_double, P, c_double, c_int],
     "sc_pmlp_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.c_long, ctypes.POINTER(P)],
     "sc_tbnn_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.c_long, ctypes.POINTER(P)],
+    "sc_equi_create": [c_int, c_int, c_int, c_int, c_int, c_uint64, ctypes.c_long, ctypes.POINTER(P)],
     "sc_pmlp_destroy": [P],
     "sc_pmlp_info": [P, ctypes.POINTER(c_int64)],
     "sc_pmlp_set_stream": [P, P],

@@ -2,8 +2,9 @@
 include/symcode.h): the reference's `mlp` ("conv" arch, src/nets.jl:247-298)
 with its training pairs (create_dataloader, src/data.jl:705-741), its
 solver-facing wrapper (fullchain, src/nets.jl:311-334) and its group average
-(symmetrize_pointwise, :350-389, the "convsym" model); and the tensor-basis
-network (tbnn_net / tbnn, :416-583, with create_dataloader_tbnn).
+(symmetrize_pointwise, :350-389, the "convsym" model); the tensor-basis
+network (tbnn_net / tbnn, :416-583, with create_dataloader_tbnn); and the
+group-equivariant network (equivariant_net, :102-238, "equi").
 
     x = [∇ū / (|∇ū| + eps), z]         z = standardised log Re_Δ (optional)
     y = symm(W4ᵀ gelu(W3ᵀ gelu(W2ᵀ gelu(W1ᵀ x + b1) + b2) + b3))
@@ -29,6 +30,15 @@ from . import solver as S
 # (default_tiers, src/experiment.jl:50-57)
 TIERS = {"p120": (3, 5, 6), "p400": (6, 10, 14), "p1200": (10, 18, 32), "p3000": (22, 36, 44),
          "p8000": (44, 64, 64), "p16000": (64, 96, 96)}
+
+# hidden widths of every architecture's tiers ("conv" = the mlp, "equi" =
+# equivariant_net in channels per group copy, "tbnn")
+ARCH_TIERS = {
+    "conv": TIERS,
+    "equi": {"p120": (1, 1, 1), "p400": (2, 2, 2), "p1200": (2, 3, 5), "p3000": (4, 5, 8), "p8000": (4, 8, 16)},
+    "tbnn": {"p120": (3, 5, 6), "p400": (6, 10, 14), "p1200": (10, 18, 32), "p3000": (22, 36, 44),
+             "p8000": (46, 64, 64)},
+}
 
 # |G| of the cube's symmetry group: `same_as_equi` multiplies the widths by it
 GROUP_ORDER = {2: 8, 3: 48}
@@ -56,10 +66,10 @@ class _PointwiseNet:
                 raise ValueError(f"{self._create}: D in {{2, 3}}, widths in [1, 1024], nvox_max * width < 2^31")
             raise RuntimeError(f"{self._create} failed: " + lib().sc_pmlp_last_error(None).decode())
         self.net = net
-        info = (ctypes.c_int64 * 10)()
+        info = (ctypes.c_int64 * 12)()
         check_pmlp(lib().sc_pmlp_info(net, info), net)
         (self.D, self.nin, c1, c2, c3, self.nout, self.nparams, self.nvox_max, self.kind,
-         self.row) = list(info)
+         self.row, self.nreg, self.nnet) = list(info)
         self.widths = (c1, c2, c3)
         self.use_redelta = bool(use_redelta)
         self.device = torch.device("cuda", device)
@@ -204,6 +214,30 @@ class TBNN(_PointwiseNet):
 
     def net_outputs(self):
         return self.nbasis
+
+
+class EquivariantNet(_PointwiseNet):
+    """equivariant_net(setup, nchan; use_redelta) (src/nets.jl:102-238): the
+    pointwise network over |G| = 48 (2D: 8) copies of each hidden channel,
+    its weights synthesised from compact trainables (get_weight_synthesis,
+    src/symmetry.jl:156-197) and the sink's D² outputs made symmetric and
+    trace-free; exactly equivariant under the cube's symmetries.  The
+    parameter vector is the compact one (``compact_shapes``)."""
+    _create = "sc_equi_create"
+
+    def __init__(self, D, widths=(4, 5, 8), use_redelta=False, seed=0, nvox_max=1 << 18, device=0):
+        super().__init__(D, widths, use_redelta, seed, nvox_max, device)
+
+    def compact_shapes(self):
+        D, (c1, c2, c3), nreg, nten = self.D, self.widths, self.nreg, self.D * self.D
+        sh = [("lift.weight", (nten, c1)), ("lift.bias", (c1,))]
+        if self.use_redelta:
+            sh.append(("lift.weight_re", (1, c1)))
+        return sh + [("mid_1.weight", (nreg, c2, c1)), ("mid_1.bias", (c2,)), ("mid_2.weight", (nreg, c3, c2)),
+                     ("mid_2.bias", (c3,)), ("sink.weight", (nten, c3))]
+
+    def layer_shapes(self):
+        raise NotImplementedError("the trainables are the compact arrays: compact_shapes()")
 
 
 def redelta_norm(pairs):

@@ -387,3 +387,108 @@ def test_symmetrized_closure_parity_fp64(D, n, use_redelta):
     assert rel(g.download_spectral(du), duo) < 1e-12
     # attaching the plain closure switches the average off again
     assert rel(g.download_physical(plain(ud)), fc(GGo)) < 1e-12
+
+
+# ------------------------------------------------------ equivariant_net
+def equi_of(D, widths=(2, 3, 2), use_redelta=False, seed=1, nvox_max=4096):
+    from symcode.pmlp import EquivariantNet
+
+    return EquivariantNet(D, widths, use_redelta=use_redelta, seed=seed, nvox_max=nvox_max)
+
+
+def flat_f(t):
+    """column-major flatten of a torch tensor"""
+    return t.permute(*reversed(range(t.dim()))).reshape(-1)
+
+
+def test_equi_create_and_init():
+    print_case("equi create", 3)
+    net = equi_of(3, (4, 5, 8), use_redelta=True, seed=3)
+    sh = net.compact_shapes()
+    assert net.nreg == 48 and net.nparams == sum(int(np.prod(s)) for _, s in sh)
+    assert net.nnet == 10 * 192 + 192 + 192 * 240 + 240 + 240 * 384 + 384 + 384 * 9
+    h = net.get_params()
+    o = 0
+    for name, shp in sh:
+        a = h[o:o + int(np.prod(shp))]
+        o += a.size
+        if name.endswith("bias"):
+            assert np.all(a == 0)
+        else:
+            fan = {"lift.weight": 9 + 4, "lift.weight_re": 1 + 4, "mid_1.weight": 48 * 9,
+                   "mid_2.weight": 48 * 13, "sink.weight": 9 + 8}[name]
+            assert np.abs(a).max() <= math.sqrt(6 / fan) and np.abs(a).max() > 0.7 * math.sqrt(6 / fan)
+
+
+@pytest.mark.parametrize("D,use_redelta", [(3, True), (2, False)])
+def test_equi_forward_loss_backward_parity(D, use_redelta):
+    print_case("equi forward backward", D, use_redelta)
+    widths = (2, 3, 2)
+    net = equi_of(D, widths, use_redelta=use_redelta, seed=2)
+    rng = np.random.default_rng(D + 20)
+    GG = rng.standard_normal((D * D, 2000))
+    x, _ = OP.features(GG, None, 0.3, 0.2 if use_redelta else None)
+    x = x.astype(np.float32)
+    t = rng.standard_normal((2000, net.nout)).astype(np.float32)
+    xd, td = torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda")
+    h = net.get_params()
+    ps = OP.equi_unflat(h, D, widths, use_redelta)
+    for v in ps.values():
+        v.requires_grad_(True)
+    layers = OP.equi_project(ps, D, OP.weight_synthesis(D))
+    yo = OP.equi_net(layers, torch.tensor(x.astype(np.float64)), D)
+    tt = torch.tensor(t.astype(np.float64))
+    lo = ((yo - tt) ** 2).sum() / ((tt**2).sum() + float(np.finfo(np.float32).eps))
+    lo.backward()
+    go = torch.cat([flat_f(v.grad) for v in ps.values()]).numpy()
+    net.zero_grad()
+    y = net.forward(xd)
+    assert rel(y.cpu().numpy(), yo.detach().numpy()) < 2e-6
+    loss, dy = net.loss(y, td, "relative")
+    assert loss == pytest.approx(lo.item(), rel=1e-5)
+    net.backward(xd, dy)
+    g = net.get_grads()
+    # per array, against the whole gradient's scale: a bias gradient sums
+    # |G| n_voxel terms that cancel to a small value, and its FP32 rounding is
+    # relative to the terms
+    o = 0
+    for name, shp in net.compact_shapes():
+        n = int(np.prod(shp))
+        assert np.linalg.norm(g[o:o + n] - go[o:o + n]) < 1e-4 * np.linalg.norm(go), name
+        o += n
+    assert rel(g, go) < 1e-4
+    # one clipped AdamW step over the compact arrays (eight with weight_re)
+    w0 = h.astype(np.float64)
+    bounds = [0]
+    for _, shp in net.compact_shapes():
+        bounds.append(bounds[-1] + int(np.prod(shp)))
+    w1 = OO.clipnorm_adamw(w0, g.astype(np.float64), OO.AdamWState(w0.size), bounds, 1e-2, weight_decay=1e-2,
+                           clip=0.05)
+    net.adamw(1e-2, weight_decay=1e-2, clip=0.05)
+    assert np.allclose(net.get_params(), w1, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("D,n,use_redelta", [(3, 16, True), (2, 32, False)])
+def test_equi_closure_parity_fp64(D, n, use_redelta):
+    import symcode as sc
+    from symcode.pmlp import PointwiseClosure
+
+    print_case("equi closure fp64", D, n, use_redelta)
+    go = S.Grid(D, n)
+    g = sc.Grid(D, n, dtype=torch.float64)
+    widths = (2, 3, 2)
+    net = equi_of(D, widths, use_redelta=use_redelta, seed=4)
+    delta, visc, norm = 2 * np.pi / n * 2, 2e-3, (1.1, 0.8)
+    model = PointwiseClosure(net, g, delta, visc, norm if use_redelta else None)
+    u = oracle_solenoidal(go, 41, trunc=True)
+    ud = g.upload_spectral(u)
+    h = net.get_params()
+    re = (visc, *norm) if use_redelta else None
+    fc = lambda G: OP.equi_fullchain(h, D, widths, G, delta, redelta=re)
+    to = fc(OC.gradient_phys(u, go))
+    assert rel(g.download_physical(model(ud)), to) < 1e-12
+    du = sc.les_rhs(g.vectorfield(), ud, g, model, visc)
+    duo = OC.les_rhs(u, go, visc, lambda u_, GG_: fc(GG_))
+    assert rel(g.download_spectral(du), duo) < 1e-12
+    with pytest.raises(ValueError):
+        PointwiseClosure(net, g, delta, visc, norm, symmetrize=True)

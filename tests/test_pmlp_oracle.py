@@ -275,3 +275,68 @@ def test_symmetrize_equivariant_and_idempotent(D):
     pt, _ = rand_params(D, (5, 6, 7), False, 15, tbnn=True)
     tb = lambda G: OP.tbnn_model(pt, G, 0.3, D)
     assert np.allclose(OP.symmetrize(tb, GG, D), tb(GG), atol=1e-13 * np.abs(tb(GG)).max())
+
+
+# ------------------------------------------------------ equivariant_net
+def test_equi_synthesis_in_reynolds_fixed_space_2d():
+    """the synthesis columns are orthonormal fixed points of the Reynolds
+    projectors r / |G| of get_weight_projectors (src/symmetry.jl:92-127),
+    built here by brute force for the 8-element 2D group"""
+    D, nten = 2, 4
+    els = OP.group_elements(D)
+    cayley, _ = OP.cayley_tables(D)
+    n = len(els)
+    s_lift, s_mid, s_sink = OP.weight_synthesis(D)
+    r_lift = np.zeros((n * nten, n * nten))
+    r_sink = np.zeros((nten * n, nten * n))
+    for m in range(n):
+        for x in range(D):
+            for y in range(D):
+                for nn in range(n):
+                    for i in range(D):
+                        for j in range(D):
+                            v = sum(s[x] * s[y] * (cayley[g, nn] == m) * (p[x] == i) * (p[y] == j)
+                                    for g, (p, s) in enumerate(els))
+                            r_lift[m + n * (x + D * y), nn + n * (i + D * j)] = v
+                            r_sink[x + D * y + nten * m, i + D * j + nten * nn] = v
+    r_mid = np.zeros((n * n, n * n))
+    for m in range(n):
+        for nn in range(n):
+            for a in range(n):
+                for b in range(n):
+                    r_mid[m + n * nn, a + n * b] = sum((cayley[g, a] == m) * (cayley[g, b] == nn) for g in range(n))
+    for r, sy in ((r_lift, s_lift), (r_mid, s_mid), (r_sink, s_sink)):
+        assert np.allclose(r / n @ sy, sy, atol=1e-12)
+        assert np.allclose(sy.T @ sy, np.eye(sy.shape[1]), atol=1e-6)
+    # the sum of the projector's fixed-space dimension equals the basis size
+    assert round(np.trace(r_mid / n)) == s_mid.shape[1]
+
+
+def rand_equi(D, widths, use_redelta, seed):
+    sh = OP.equi_shapes(D, widths, use_redelta)
+    rng = np.random.default_rng(seed)
+    return np.concatenate([0.5 * rng.standard_normal(int(np.prod(s))) for _, s in sh])
+
+
+@pytest.mark.parametrize("D,use_redelta", [(2, False), (3, True)])
+def test_equi_network_equivariant(D, use_redelta):
+    widths = (2, 3, 2)
+    h = rand_equi(D, widths, use_redelta, 5)
+    layers = OP.equi_project(OP.equi_unflat(h, D, widths, use_redelta), D, OP.weight_synthesis(D))
+    GG = np.random.default_rng(6).standard_normal((D * D, 7))
+    x, _ = OP.features(GG, None, 0.3, 0.4 if use_redelta else None)
+    y = OP.equi_net(layers, torch.tensor(x), D).numpy()
+    Y = unpack(y, D)
+    assert np.abs(np.trace(Y, axis1=-2, axis2=-1)).max() < 1e-12
+    for p, s in OP.group_elements(D)[1:]:
+        R = OP.element_matrix(p, s)
+        xr, _ = OP.features(rotate_grad(GG, R, D), None, 0.3, 0.4 if use_redelta else None)
+        yr = unpack(OP.equi_net(layers, torch.tensor(xr), D).numpy(), D)
+        assert np.allclose(yr, R @ Y @ R.T, atol=1e-12 * np.abs(Y).max())
+    # a plain mlp of the same widths is not equivariant (the check can fail)
+    p_, _ = rand_params(D, widths, use_redelta, 7)
+    yp = unpack(OP.forward(p_, x, D)[0], D)
+    pe, se = OP.group_elements(D)[1]
+    R = OP.element_matrix(pe, se)
+    xr, _ = OP.features(rotate_grad(GG, R, D), None, 0.3, 0.4 if use_redelta else None)
+    assert not np.allclose(unpack(OP.forward(p_, xr, D)[0], D), R @ yp @ R.T, atol=1e-6)
