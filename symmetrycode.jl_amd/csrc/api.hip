@@ -89,14 +89,12 @@ int sc_destroy(sc_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->tw_n, c->tw_m, c->bufA, c->bufB, c->bufT, c->scr, c->ustart, c->du, c->acc, c->ust,
-                  c->dscr, c->vmax, c->forceA, c->dtdev};
+  void* bufs[] = {c->tw_n, c->tw_m, c->dscr, c->vmax, c->forceA, c->dtdev};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  for (void* b : c->ws)
-    if (b) (void)hipFree(b);
-  for (void* b : c->adjw)
-    if (b) (void)hipFree(b);
+  for (void* b : {c->bufA, c->bufB, c->bufT, c->scr, c->ustart, c->du, c->acc, c->ust}) c->dfree(b);
+  for (void* b : c->ws) c->dfree(b);
+  for (void* b : c->adjw) c->dfree(b);
   if (c->bandadj) (void)hipFree(c->bandadj);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);

@@ -6,6 +6,10 @@
 #include <stdexcept>
 #include "solver_kernels.hpp"
 
+#ifndef SC_STAGGER
+#define SC_STAGGER 1
+#endif
+
 struct sc_cnn;   // learned closure (cnn.hip)
 struct sc_pmlp;  // pointwise MLP closure (pmlp.hip)
 
@@ -91,6 +95,29 @@ struct Ctx {
     SC_HIP(hipEventRecord(prof.back().b, stream));
   }
 
+  // Large buffers start at staggered offsets inside their allocations
+  // (hipMalloc returns them all at the same large alignment, so element o of
+  // every buffer would map to the same HBM channel); dmalloc / dfree
+  std::vector<std::pair<void*, void*>> blocks;  // (buffer, allocation)
+  int nalloc = 0;
+  void dmalloc(void** p, size_t bytes) {
+    const size_t off = SC_STAGGER ? size_t(nalloc % 16) * (size_t(260) << 10) : 0;
+    void* base = nullptr;
+    SC_HIP(hipMalloc(&base, bytes + off));
+    ++nalloc;
+    *p = static_cast<char*>(base) + off;
+    blocks.push_back({*p, base});
+  }
+  void dfree(void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < blocks.size(); ++i)
+      if (blocks[i].first == p) {
+        (void)hipFree(blocks[i].second);
+        blocks.erase(blocks.begin() + long(i));
+        return;
+      }
+  }
+
   int tdim() const { return D == 3 ? 6 : 3; }
   size_t celem() const { return prec == 64 ? 16 : 8; }
   size_t relem() const { return prec == 64 ? 8 : 4; }
@@ -103,16 +130,16 @@ struct Ctx {
   }
   void* ensure_spec(void*& p, int ncomp) {
     if (!p) {
-      SC_HIP(hipMalloc(&p, size_t(ncomp) * spec_elems * celem()));
+      dmalloc(&p, size_t(ncomp) * spec_elems * celem());
       SC_HIP(hipMemsetAsync(p, 0, size_t(ncomp) * spec_elems * celem(), stream));
     }
     return p;
   }
   void* scratch(int ncomp) {
     if (scr_comp < ncomp) {
-      if (scr) SC_HIP(hipFree(scr));
+      dfree(scr);
       scr = nullptr;
-      SC_HIP(hipMalloc(&scr, size_t(ncomp) * spec_elems * celem()));
+      dmalloc(&scr, size_t(ncomp) * spec_elems * celem());
       SC_HIP(hipMemsetAsync(scr, 0, size_t(ncomp) * spec_elems * celem(), stream));
       scr_comp = ncomp;
     }
@@ -120,9 +147,9 @@ struct Ctx {
   }
   void* work(int slot, size_t bytes) {
     if (ws_bytes[slot] < bytes) {
-      if (ws[slot]) SC_HIP(hipFree(ws[slot]));
+      dfree(ws[slot]);
       ws[slot] = nullptr;
-      SC_HIP(hipMalloc(&ws[slot], bytes));
+      dmalloc(&ws[slot], bytes);
       SC_HIP(hipMemsetAsync(ws[slot], 0, bytes, stream));
       ws_bytes[slot] = bytes;
     }
@@ -130,9 +157,9 @@ struct Ctx {
   }
   void* adj(int slot, size_t bytes) {
     if (adjw_bytes[slot] < bytes) {
-      if (adjw[slot]) SC_HIP(hipFree(adjw[slot]));
+      dfree(adjw[slot]);
       adjw[slot] = nullptr;
-      SC_HIP(hipMalloc(&adjw[slot], bytes));
+      dmalloc(&adjw[slot], bytes);
       SC_HIP(hipMemsetAsync(adjw[slot], 0, bytes, stream));
       adjw_bytes[slot] = bytes;
     }

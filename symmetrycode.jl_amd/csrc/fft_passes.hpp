@@ -493,22 +493,27 @@ k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>
     // inside the loop instead of being kept live across it
     int p = p0, lane = lane0;
     asm volatile("" : "+v"(p), "+v"(lane));
+    // only the TPP lanes of the pencil transform; the others would repeat
+    // lane 0's LDS writes, which the LDS serialises as bank conflicts
+    const bool act = lane < TPP;
     C<T> u[D][E];
     C<T> v[PlanInfo<M>::MAXE > PFR::MAXE ? PlanInfo<M>::MAXE : PFR::MAXE];
     const long nxt = row + W;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      PI::load(v, p, [&](int k) { return c2r_pre<T, M>(k, twn_l, [&](int kk) { return sw[kk]; }); });
+      if (act) PI::load(v, p, [&](int k) { return c2r_pre<T, M>(k, twn_l, [&](int kk) { return sw[kk]; }); });
       // the staging row is free: the next component, or the next row's first
       // (after the last row, this row's again: unused, but every wave issues
       // the same loads, so the waits can count them)
       if (c + 1 < D) xp_stage(in + long(c + 1) * g.s_comp_spec + row * g.nhp, zero, sw, kin, lane);
       else xp_stage(in + (nxt < g.nrows ? nxt : row) * g.nhp, zero, sw, kin, lane);
-      PI::run(v, p, ll, 1, twm_l);
+      if (act) {
+        PI::run(v, p, ll, 1, twm_l);
 #pragma unroll
-      for (int e = 0; e < E; ++e) u[c][e] = v[e];
+        for (int e = 0; e < E; ++e) u[c][e] = v[e];
+      }
     }
-    if (vmax_bits != nullptr) {
+    if (vmax_bits != nullptr && act) {
       PI::for_out(p, [&](int e, int) {
         double a = 0, b = 0;
 #pragma unroll
@@ -523,15 +528,17 @@ k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>
     for (int q = 0; q < NP; ++q) {
       const int i = D == 3 ? PA[q] : QA[q];
       const int j = D == 3 ? PB[q] : QB[q];
+      if (act) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
-      if constexpr (DIRECT) {
-        PFR::run(v, p, ll, 1, twm_l);
-        PFR::out_to_lds(v, p, ll, 1);
-      } else {
-        PI::out_to_in(v, p, ll, 1);
-        PF::run(v, p, ll, 1, twm_l);
-        PF::out_to_lds(v, p, ll, 1);
+        for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
+        if constexpr (DIRECT) {
+          PFR::run(v, p, ll, 1, twm_l);
+          PFR::out_to_lds(v, p, ll, 1);
+        } else {
+          PI::out_to_in(v, p, ll, 1);
+          PF::run(v, p, ll, 1, twm_l);
+          PF::out_to_lds(v, p, ll, 1);
+        }
       }
       xsync<true>();
       C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;

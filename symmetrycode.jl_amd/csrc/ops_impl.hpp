@@ -481,7 +481,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     // component at ((tile * n + z) * n + y) * TX + x % TX (in the buffer the
     // forward passes use later): both its writes and the y-pass's reads then
     // stay within few address-translation pages
-    if (!c.bufT) SC_HIP(hipMalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT)));
+    if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT));
     CT* AT = reinterpret_cast<CT*>(c.bufT);
     const long n = c.n, TX = tile_x_rt<T>(c.n), ts = tiled_slot<T>(c);
     StridedGeom gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
