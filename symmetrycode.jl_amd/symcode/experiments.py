@@ -6,6 +6,8 @@ GPU-resident equivalents for the configurations this library targets).
   field and the same time steps; the relative spectrum difference, and the
   HBM rate of every FFT pass and of the whole RHS from the library's
   per-launch timers.
+* ``pointwise_closures``: the pointwise learned closures trained on filtered
+  DNS pairs; a-priori metrics, training and LES-step cost.
 * ``les_classical``: forced DNS, Gaussian-filtered and cut to the LES grid;
   a-priori correlation of the dynamic Smagorinsky and Clark stresses with the
   exact sub-filter stress, then LES rollouts (FP32) with both closures and
@@ -199,18 +201,102 @@ def les_classical(n_dns=512, n_les=128, visc=2e-3, factors=(2, 4), les_steps=500
     return res
 
 
+def pointwise_closures(n_dns=256, n_les=64, visc=2e-3, factor=2, nsnap=6, dt_save=0.1, spinup=0.5, train_steps=300,
+                       lr=3e-3, les_steps=20, cfl=0.3, seed=2, totalenergy=0.2, log=print):
+    """The pointwise learned closures of the reference's study (mlp "conv",
+    its group average "convsym", the TBNN and the equivariant net, at the
+    p1200 / p3000 tiers) trained on the GPU DNS's own filtered pairs with the
+    reference's semantics (create_dataloader / create_dataloader_tbnn pairs,
+    relative loss, ClipNorm(1) + AdamW, warm-up + cosine; one snapshot per
+    step); the a-priori relerr / crosscor of each on a held-out later snapshot
+    against dynamic Smagorinsky and Clark; and the cost of one LES RK4 step
+    (FP64, n_les^3) with each closure.  Returns a dict."""
+    from . import pmlp as PM
+    from .closures import create_smagorinsky
+    from .training import create_data, reference_train
+
+    g_dns = Grid(3, n_dns, dtype=torch.float64)
+    g_les = Grid(3, n_les, dtype=torch.float64)
+    S.set_band_forcing(g_dns, 0.1, 2)
+    u = S.randomfield(S.peak_profile, g_dns, seed=seed, totalenergy=totalenergy, kpeak=4)
+    t = 0.0
+    while t < spinup * (1 - 1e-12):
+        t += S.rk4_cfl(u, g_dns, visc, cfl, spinup - t)
+    pairs = create_data(u, g_dns, g_les, visc, [factor], nsnap, dt_save, cfl=cfl)[factor]
+    train, test = pairs[:-1], pairs[-1]
+    delta = test.delta
+    norm = PM.redelta_norm(pairs)
+    nv = n_les**3
+    tref = S.make_tracefree(S.to_phys(g_les.spacetensorfield(), test.tau, g_les), g_les, spectral=False)
+    res = {"n_dns": n_dns, "n_les": n_les, "visc": visc, "delta": delta, "train_snapshots": len(train),
+           "train_steps": train_steps, "redelta": [p.redelta for p in pairs], "nets": {}}
+
+    def les_cost(model):
+        v = test.ubar.clone()
+        dt = 0.5 * cfl * S.propose_timestep(v, g_les, visc)
+        les_rk4(v, dt, g_les, model, visc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(les_steps):
+            les_rk4(v, dt, g_les, model, visc)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / les_steps * 1e3
+
+    specs = [("conv p3000", PM.PointwiseMLP, PM.ARCH_TIERS["conv"]["p3000"]),
+             ("tbnn p3000", PM.TBNN, PM.ARCH_TIERS["tbnn"]["p3000"]),
+             ("equi p1200", PM.EquivariantNet, PM.ARCH_TIERS["equi"]["p1200"])]
+    for name, cls, widths in specs:
+        net = cls(3, widths, use_redelta=True, seed=0, nvox_max=len(train) * nv)
+        X, Y = PM.pointwise_pairs(net, train, g_les, norm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = reference_train(net, X, Y, train_steps, lr=lr, batchsize=nv)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / train_steps * 1e3
+        del X, Y
+        model = PM.PointwiseClosure(net, g_les, delta, visc, norm)
+        r, c = apriori_metrics(S.make_tracefree(model(test.ubar), g_les, spectral=False), tref)
+        out = {"widths": list(widths), "params": net.nparams, "ms_per_train_step": ms,
+               "loss_first": losses[0], "loss_last": losses[-1], "apriori": {"relerr": r, "crosscor": c},
+               "les_ms_per_step": les_cost(model)}
+        if cls is PM.PointwiseMLP:
+            sym = PM.symmetrize_pointwise(model)
+            rs, cs = apriori_metrics(S.make_tracefree(sym(test.ubar), g_les, spectral=False), tref)
+            res["nets"]["convsym p3000"] = {"apriori": {"relerr": rs, "crosscor": cs},
+                                            "les_ms_per_step": les_cost(sym)}
+        res["nets"][name] = out
+        if log:
+            log(f"{name}: {ms:.1f} ms/train step, loss {losses[0]:.3f} -> {losses[-1]:.3f}, "
+                f"a-priori {r:.3f} / {c:.3f}, LES {out['les_ms_per_step']:.2f} ms/step")
+    tdyn = create_dynamic_smagorinsky(delta, g_les)(test.ubar)
+    tcl = S.make_tracefree(create_clark(delta, g_les)(test.ubar, gradient_phys(test.ubar, g_les)), g_les,
+                           spectral=False)
+    res["classical"] = {
+        "dynsmag": {"apriori": dict(zip(("relerr", "crosscor"), apriori_metrics(tdyn, tref))),
+                    "les_ms_per_step": les_cost(create_dynamic_smagorinsky(delta, g_les))},
+        "clark": {"apriori": dict(zip(("relerr", "crosscor"), apriori_metrics(tcl, tref))),
+                  "les_ms_per_step": les_cost(create_clark(delta, g_les))},
+        "smagorinsky 0.17": {"les_ms_per_step": les_cost(create_smagorinsky(0.17, delta, g_les))},
+        "none": {"les_ms_per_step": les_cost(None)},
+    }
+    return res
+
+
 if __name__ == "__main__":
     import argparse
     import json
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=("fp32_vs_fp64", "les_classical"))
+    ap.add_argument("which", choices=("fp32_vs_fp64", "les_classical", "pointwise_closures"))
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     a = ap.parse_args()
     if a.which == "fp32_vs_fp64":
         kw = {k: v for k, v in (("n", a.n), ("steps", a.steps)) if v is not None}
         r = fp32_vs_fp64(**kw)
+    elif a.which == "pointwise_closures":
+        kw = {k: v for k, v in (("n_dns", a.n), ("train_steps", a.steps)) if v is not None}
+        r = pointwise_closures(**kw)
     else:
         kw = {k: v for k, v in (("n_dns", a.n), ("les_steps", a.steps)) if v is not None}
         r = les_classical(**kw)
