@@ -388,6 +388,35 @@ def test_nonlinearity_810_against_separate_transforms():
     torch.cuda.empty_cache()
 
 
+def test_config1_2d_256_full_length():
+    """The reference's CPU-runnable configuration at full length: 2D 256^2
+    FP64, E(k) ~ k^4 exp(-2 (k/5)^2) at unit energy (seed 0), nu = 1e-3,
+    dt = 1e-3, no closure or forcing; 1000 RK4 steps and 300 Wray3 steps on
+    the GPU against the oracle: the energy every 100 steps within 1e-12 and
+    the final shell spectrum within 1e-11 relative"""
+    import symcode as sc
+
+    print_case("config 1: 2D 256 RK4 1000 steps, Wray3 300 steps")
+    go, gg = grids(2, 256)
+    prof = lambda k: S.peak_profile(k, 5)
+    nu, dt = 1e-3, 1e-3
+    f = lambda v: S.convectiondiffusion(v, go, nu)
+    for ostep, gstep, nsteps in ((S.rk4, sc.rk4, 1000), (S.wray3, sc.wray3, 300)):
+        ud = sc.randomfield(lambda k: prof(k), gg, seed=0, totalenergy=1.0)
+        ref = S.randomfield(prof, go, seed=0, totalenergy=1.0)
+        assert rel(gg.download_spectral(ud), ref) < TOL64
+        for i in range(1, nsteps + 1):
+            ref = ostep(f, ref, dt, go)
+            gstep(ud, dt, gg, nu)
+            if i % 100 == 0:
+                assert sc.energy(ud, gg) == pytest.approx(S.energy(ref, go), rel=1e-12), i
+        k, e = sc.spectrum(ud, gg)
+        ko, eo = S.spectrum(ref, go)
+        assert np.allclose(k, ko, rtol=1e-14)
+        assert rel(e, eo) < 1e-11
+        print(gstep.__name__, "energy", S.energy(ref, go), "spectrum rel", rel(e, eo), flush=True)
+
+
 # ------------------------------------------------------- CFL-driven steppers
 @pytest.mark.parametrize("D,n", [(3, 16), (3, 30), (2, 64)])
 def test_cfl_steppers(D, n):
