@@ -273,6 +273,11 @@ int sc_closure_cnn(sc_ctx* ctx, void* tau, const void* u);
  * W3, b3, W4]; voxel matrices channels-last [nvox][c] FP32; nvox_max bounds
  * every call (activations for that many voxels are kept for backward). */
 typedef struct sc_pmlp sc_pmlp;
+/* A network handle owns its parameters, optimiser state, activations and
+ * inference buffers: calls on one handle are serialised by the caller (its
+ * training calls run on its own stream, closure evaluations on the solver's),
+ * and one handle attached to several contexts must not be evaluated by them
+ * concurrently. */
 int sc_pmlp_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed, long nvox_max, sc_pmlp** out);
 /* The tensor-basis network: tbnn_net (src/nets.jl:535-556) behind tbnn
  * (:558-583).  Inputs are the tb_kernel! invariants of the normalised

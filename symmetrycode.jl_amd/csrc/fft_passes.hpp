@@ -189,6 +189,17 @@ DEV C<T> r2c_post(int k, const C<T>* zl, const C<T>* __restrict__ twn) {
   return scl<T>(r, T(0.5));
 }
 
+// r2c_post for one output from its two entries: X = 1/2 (za + conj zb) -
+// i/2 w (za - conj zb), za = Z[k], zb = Z[M - k] (Z[M] = Z[0]), w = W^k
+template <typename T>
+DEV C<T> r2c_pair(C<T> a, C<T> zb, C<T> w) {
+  C<T> b = conjc<T>(zb);
+  C<T> s = add<T>(a, b), d = sub<T>(a, b);
+  C<T> wd = cmul<T>(w, d);
+  C<T> r = sub<T>(s, muli<1, T>(wd));
+  return scl<T>(r, T(0.5));
+}
+
 template <typename T, int M>
 __global__ void __launch_bounds__((rows_per_block<M>() * Plan<M>::tpp))
 k_x_c2r(const C<T>* __restrict__ in, T* out, const C<T>* __restrict__ twm,
@@ -542,12 +553,15 @@ k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>
       }
       xsync<true>();
       C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
-      // lanes past k = M store X[M] again (same address, same value), so
-      // every store is issued and the waits can count them
+      // X[k] and X[M - k] from the same two entries Z[k], Z[M - k] (Z[M] =
+      // Z[0]); lanes past k = M / 2 store that pair again (same addresses,
+      // same values), so every store is issued and the waits can count them
 #pragma unroll
-      for (int t = 0; t < (M + 64) / 64; ++t) {
-        const int k = min(lane + 64 * t, M);
-        dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn_l), s);
+      for (int t = 0; t < (M / 2 + 64) / 64; ++t) {
+        const int k = min(lane + 64 * t, M / 2);
+        const C<T> za = ll[k], zb = ll[k == 0 ? 0 : M - k];
+        dst[k] = scl<T>(r2c_pair<T>(za, zb, twn_l[k]), s);
+        dst[M - k] = scl<T>(r2c_pair<T>(zb, za, twn_l[M - k]), s);
       }
     }
   }
