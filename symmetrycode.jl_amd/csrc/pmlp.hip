@@ -154,6 +154,156 @@ __global__ void k_bias_gelu_inf(E* z, const E* b, long n, int co) {
   if (i < n) z[i] = gelu(z[i] + b[i % co]);
 }
 
+// ------------------------------------------ fused FP64 inference on MFMA
+// The four layers of the network on v_mfma_f64_16x16x4f64 (D[i][j] +=
+// A[i][k] B[k][j]; lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[l/16 + 4r][l%16], r = 0..3), 16 input rows per row tile: i runs over 16
+// output channels, j over the rows, k over 4 input channels.  Tile m of a
+// layer's result leaves lane (c = l/16, v = l%16) holding channels
+// 16m + c + 4r of row v, which is the B operand of the next layer's K step
+// (tile m, r) when that step's k index c stands for channel 16m + c + 4r; the
+// weights W[co][ci] (the A operand) are read from LDS in that order.  The
+// activations stay in registers from the input row to the output, so only
+// the inputs, the outputs and the weights (once per block) move through
+// memory; every width is at most FT tiles of 16 channels.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+constexpr int FT = 4;  // channel tiles per layer (widths up to 64)
+#ifndef SC_PMLP_FR
+#define SC_PMLP_FR 2
+#endif
+constexpr int FR = SC_PMLP_FR;  // row tiles per wavefront pass
+
+// padded network in LDS: layer l's weights as rows[l] (input channels padded
+// to 4) rows of 16 TOM doubles (output channel co at column co, TOM = the
+// widest layer in tiles of 16) at wofs[l], the biases of the hidden layers
+// (16 to[l]) at bofs[l]; padding is zero, so padded channels stay exactly
+// zero through the layers (gelu(0) = 0).  The fixed row length makes every
+// fragment's LDS offset a constant.
+struct FusedNet {
+  int cin[4], cout[4], ti[4], to[4], rows[4], wofs[4], bofs[3];
+  long wsrc[4], bsrc[3];  // offsets of W (co x ci, ci fastest) and b in pd
+};
+
+// the tanh-form GELU with 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)): the same
+// function as gelu() above, one exp instead of a tanh
+__device__ __forceinline__ double gelu_exp(double z) {
+  const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
+  return z / (1.0 + exp(-2.0 * u));
+}
+
+// acc[rt][m] = sum over the input channels of W[16m + v][ci] in[rt][ci-tile];
+// wv = this lane's (c, v) entry of the layer's first LDS row
+template <int TOM>
+__device__ __forceinline__ void fused_layer(const double* wv, int ti, int cin, int to, const d4_t (&in)[FR][TOM],
+                                            d4_t (&acc)[FR][TOM]) {
+#pragma unroll
+  for (int m = 0; m < TOM; ++m)
+#pragma unroll
+    for (int rt = 0; rt < FR; ++rt) acc[rt][m] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < TOM; ++t) {
+    if (t >= ti) break;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // channels 16t + 4r + (0..3) of this K step; none is live past cin
+      if (16 * t + 4 * r >= cin) break;
+#pragma unroll
+      for (int m = 0; m < TOM; ++m) {
+        if (m >= to) break;
+        const double a = wv[(16 * t + 4 * r) * 16 * TOM + 16 * m];
+#pragma unroll
+        for (int rt = 0; rt < FR; ++rt)
+          acc[rt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, in[rt][t][r], acc[rt][m], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// h = gelu(h + b) on the live tiles; bc = this lane's b[c]
+template <int TOM>
+__device__ __forceinline__ void fused_bias_gelu(d4_t (&h)[FR][TOM], const double* bc, int to) {
+#pragma unroll
+  for (int m = 0; m < TOM; ++m) {
+    if (m >= to) break;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double bb = bc[16 * m + 4 * r];
+#pragma unroll
+      for (int rt = 0; rt < FR; ++rt) h[rt][m][r] = gelu_exp(h[rt][m][r] + bb);
+    }
+  }
+}
+
+// y (nrows x cout[3]) = net(x) for x (nrows rows of leading dimension ldx,
+// the first cin[0] columns read); LDSD doubles of LDS hold the padded network,
+// TOM tiles of 16 channels the widest layer.  Grid-stride over passes of FR
+// row tiles per wavefront.
+template <int TOM, int LDSD>
+__global__ void __launch_bounds__(256) k_mlp_fused(const double* __restrict__ pd, FusedNet f,
+                                                   const double* __restrict__ x, int ldx, long nrows,
+                                                   double* __restrict__ y) {
+  __shared__ double wl[LDSD];
+  constexpr int COLS = 16 * TOM;
+  for (int l = 0; l < 4; ++l) {
+    const int ci = f.cin[l], co = f.cout[l];
+    for (int i = threadIdx.x; i < f.rows[l] * COLS; i += blockDim.x) {
+      const int a = i / COLS, b = i - a * COLS;
+      double w = 0.0;
+      if (a < ci && b < co) w = pd[f.wsrc[l] + a + long(ci) * b];
+      wl[f.wofs[l] + i] = w;
+    }
+    if (l < 3)
+      for (int i = threadIdx.x; i < 16 * f.to[l]; i += blockDim.x) {
+        double b = 0.0;
+        if (i < co) b = pd[f.bsrc[l] + i];
+        wl[f.bofs[l] + i] = b;
+      }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, c = lane >> 4, v = lane & 15;
+  const int nw = blockDim.x >> 6;
+  const long npass = (nrows + 16 * FR - 1) / (16 * FR);
+  const int co3 = f.cout[3];
+  const double* wlv = wl + c * COLS + v;
+  for (long p = long(blockIdx.x) * nw + (threadIdx.x >> 6); p < npass; p += long(gridDim.x) * nw) {
+    d4_t A[FR][TOM], B[FR][TOM];
+#pragma unroll
+    for (int rt = 0; rt < FR; ++rt) {
+      const long row = (p * FR + rt) * 16 + v;
+      const double* xr = x + row * ldx + c;
+#pragma unroll
+      for (int t = 0; t < TOM; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double e = 0.0;
+          if (16 * t + c + 4 * r < f.cin[0] && row < nrows) e = xr[16 * t + 4 * r];
+          A[rt][t][r] = e;
+        }
+    }
+    fused_layer<TOM>(wlv + f.wofs[0], f.ti[0], f.cin[0], f.to[0], A, B);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.to[0]);
+    fused_layer<TOM>(wlv + f.wofs[1], f.ti[1], f.cin[1], f.to[1], B, A);
+    fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.to[1]);
+    fused_layer<TOM>(wlv + f.wofs[2], f.ti[2], f.cin[2], f.to[2], A, B);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.to[2]);
+    fused_layer<TOM>(wlv + f.wofs[3], f.ti[3], f.cin[3], f.to[3], B, A);
+#pragma unroll
+    for (int rt = 0; rt < FR; ++rt) {
+      const long row = (p * FR + rt) * 16 + v;
+      if (row >= nrows) continue;
+#pragma unroll
+      for (int m = 0; m < TOM; ++m) {
+        if (m >= f.to[3]) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int co = 16 * m + c + 4 * r;
+          if (co < co3) y[row * co3 + co] = A[rt][m][r];
+        }
+      }
+    }
+  }
+}
+
 // d *= gelu'(z)
 __global__ void k_gelu_back(float* d, const float* z, long n) {
   const long i = gid();
@@ -676,6 +826,7 @@ struct sc_pmlp {
   // input / ping-pong / output buffers for chunks of up to icap voxels
   double *pd = nullptr, *zdev = nullptr, *xd = nullptr, *hA = nullptr, *hB = nullptr, *yd = nullptr;
   long icap = 0;
+  int ncu = 0;  // compute units of the device (the fused kernel's grid)
   double *part = nullptr, *gnorm = nullptr;
   int adam_t = 0;
   long last_nv = 0;
@@ -849,6 +1000,51 @@ int dim(const sc_pmlp* c) { return c->D; }
 bool uses_redelta(const sc_pmlp* c) { return c->redelta; }
 int arch(const sc_pmlp* c) { return c->arch; }
 
+// the fused kernel's layout of this network (tom = the widest layer in
+// tiles of 16 channels, lds = doubles of LDS); false when a layer is wider
+// than FT tiles or the padded network exceeds 16384 doubles (128 KB)
+bool fused_plan(const sc_pmlp* c, FusedNet& f, int& tom, int& lds) {
+  tom = 1;
+  for (int l = 0; l < 4; ++l) {
+    f.cin[l] = c->cin(l);
+    f.cout[l] = c->cout(l);
+    f.ti[l] = (f.cin[l] + 15) / 16;
+    f.to[l] = (f.cout[l] + 15) / 16;
+    tom = std::max({tom, f.ti[l], f.to[l]});
+    f.rows[l] = (f.cin[l] + 3) / 4 * 4;
+    f.wsrc[l] = c->fo[2 * l];
+  }
+  if (tom > FT) return false;
+  int off = 0;
+  for (int l = 0; l < 4; ++l) {
+    f.wofs[l] = off;
+    off += f.rows[l] * 16 * tom;
+  }
+  for (int l = 0; l < 3; ++l) {
+    f.bofs[l] = off;
+    off += 16 * f.to[l];
+    f.bsrc[l] = c->fo[2 * l + 1];
+  }
+  lds = off;
+  return off <= 16384;
+}
+
+template <int TOM>
+void launch_fused(sc_pmlp* c, const FusedNet& f, int lds, const double* x, long nv, double* out) {
+  const long blocks = (nv + 64 * FR - 1) / (64 * FR);  // 4 wavefronts per block
+  // LDS classes of 32, 72 and 128 KB: 4, 2 and 1 blocks per compute unit
+  if (lds <= 4096)
+    hipLaunchKernelGGL((k_mlp_fused<TOM, 4096>), dim3(unsigned(std::min(blocks, 4L * c->ncu))), dim3(256), 0,
+                       c->stream, (const double*)c->pd, f, x, c->ctot, nv, out);
+  else if (lds <= 9216)
+    hipLaunchKernelGGL((k_mlp_fused<TOM, 9216>), dim3(unsigned(std::min(blocks, 2L * c->ncu))), dim3(256), 0,
+                       c->stream, (const double*)c->pd, f, x, c->ctot, nv, out);
+  else
+    hipLaunchKernelGGL((k_mlp_fused<TOM, 16384>), dim3(unsigned(std::min(blocks, 1L * c->ncu))), dim3(256), 0,
+                       c->stream, (const double*)c->pd, f, x, c->ctot, nv, out);
+  PM_HIP(hipGetLastError());
+}
+
 // one chunk of FP64 inference on nv input rows (leading dimension ctot):
 // y (nt x nv) = symm(mlp(x)) or, for the TBNN, the basis contraction of the
 // coefficients
@@ -856,17 +1052,35 @@ void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
   const double one = 1.0, zero = 0.0;
   const double* in = x;
   double* out = c->hA;
-  for (int l = 0; l < 4; ++l) {
-    const int ci = c->cin(l), co = c->cout(l);
-    double* dst = l == 3 && c->arch == 0 ? y : out;  // the others end in a head
-    PM_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                          c->pd + c->fo[2 * l], ci, in, l == 0 ? c->ctot : ci, &zero, dst, co));
-    if (l < 3) {
-      hipLaunchKernelGGL(k_bias_gelu_inf<double>, dim3(nb(nv * co)), dim3(256), 0, c->stream, dst,
-                         (const double*)c->pd + c->fo[2 * l + 1], nv * co, co);
-      PM_HIP(hipGetLastError());
-      in = dst;
-      out = dst == c->hA ? c->hB : c->hA;
+  FusedNet f;
+  int tom = 0, lds = 0;
+  if (fused_plan(c, f, tom, lds)) {
+    // the whole network in one kernel: the last layer writes y (mlp) or the
+    // head's input (hB)
+    if (!c->ncu) {
+      int dev = 0;
+      PM_HIP(hipGetDevice(&dev));
+      PM_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    out = c->arch == 0 ? y : c->hB;
+    if (tom == 1) launch_fused<1>(c, f, lds, x, nv, out);
+    else if (tom == 2) launch_fused<2>(c, f, lds, x, nv, out);
+    else if (tom == 3) launch_fused<3>(c, f, lds, x, nv, out);
+    else launch_fused<4>(c, f, lds, x, nv, out);
+  } else {
+    // wider networks: one GEMM per layer, the activations through HBM
+    for (int l = 0; l < 4; ++l) {
+      const int ci = c->cin(l), co = c->cout(l);
+      double* dst = l == 3 && c->arch == 0 ? y : out;  // the others end in a head
+      PM_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
+                            c->pd + c->fo[2 * l], ci, in, l == 0 ? c->ctot : ci, &zero, dst, co));
+      if (l < 3) {
+        hipLaunchKernelGGL(k_bias_gelu_inf<double>, dim3(nb(nv * co)), dim3(256), 0, c->stream, dst,
+                           (const double*)c->pd + c->fo[2 * l + 1], nv * co, co);
+        PM_HIP(hipGetLastError());
+        in = dst;
+        out = dst == c->hA ? c->hB : c->hA;
+      }
     }
   }
   if (c->arch == 2)

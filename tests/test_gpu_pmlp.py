@@ -131,15 +131,21 @@ def test_adamw_steps_parity():
         assert np.allclose(net.get_params(), w, rtol=1e-5, atol=1e-7)
 
 
-@pytest.mark.parametrize("D,n,use_redelta", [(3, 16, False), (3, 16, True), (2, 32, True)])
-def test_closure_parity_fp64(D, n, use_redelta):
+# widths up to 64 run the fused MFMA network (one, three and four tiles of 16
+# channels; 30^3 = 27000 rows ends in a partial row tile), wider ones one GEMM
+# per layer
+@pytest.mark.parametrize("D,n,use_redelta,widths", [(3, 16, False, (11, 13, 7)), (3, 16, True, (11, 13, 7)),
+                                                    (2, 32, True, (11, 13, 7)), (3, 30, True, (22, 36, 44)),
+                                                    (2, 30, False, (22, 36, 44)), (3, 16, True, (44, 64, 64)),
+                                                    (3, 16, True, (70, 13, 7))])
+def test_closure_parity_fp64(D, n, use_redelta, widths):
     import symcode as sc
     from symcode.pmlp import PointwiseClosure
 
-    print_case("pmlp closure fp64", D, n, use_redelta)
+    print_case("pmlp closure fp64", D, n, use_redelta, widths)
     go = S.Grid(D, n)
     g = sc.Grid(D, n, dtype=torch.float64)
-    net = net_of(D, (11, 13, 7), use_redelta=use_redelta, seed=4)
+    net = net_of(D, widths, use_redelta=use_redelta, seed=4)
     delta, visc, norm = 2 * np.pi / n * 2, 2e-3, (1.1, 0.8)
     model = PointwiseClosure(net, g, delta, visc, norm if use_redelta else None)
     u = oracle_solenoidal(go, 31, trunc=True)
