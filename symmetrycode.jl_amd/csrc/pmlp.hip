@@ -191,6 +191,13 @@ __device__ __forceinline__ double gelu_exp(double z) {
   return z / (1.0 + exp(-2.0 * u));
 }
 
+// orders a wavefront's LDS stores before its later loads of the same words
+// by other lanes (and the loads before later stores)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // acc[rt][m] = sum over the input channels of W[16m + v][ci] in[rt][ci-tile];
 // wv = this lane's (c, v) entry of the layer's first LDS row
 template <int TOM>
@@ -234,15 +241,9 @@ __device__ __forceinline__ void fused_bias_gelu(d4_t (&h)[FR][TOM], const double
   }
 }
 
-// y (nrows x cout[3]) = net(x) for x (nrows rows of leading dimension ldx,
-// the first cin[0] columns read); LDSD doubles of LDS hold the padded network,
-// TOM tiles of 16 channels the widest layer.  Grid-stride over passes of FR
-// row tiles per wavefront.
-template <int TOM, int LDSD>
-__global__ void __launch_bounds__(256) k_mlp_fused(const double* __restrict__ pd, FusedNet f,
-                                                   const double* __restrict__ x, int ldx, long nrows,
-                                                   double* __restrict__ y) {
-  __shared__ double wl[LDSD];
+// the padded network from pd into LDS (the whole block)
+template <int TOM>
+__device__ __forceinline__ void stage_net(double* wl, const double* __restrict__ pd, const FusedNet& f) {
   constexpr int COLS = 16 * TOM;
   for (int l = 0; l < 4; ++l) {
     const int ci = f.cin[l], co = f.cout[l];
@@ -259,6 +260,19 @@ __global__ void __launch_bounds__(256) k_mlp_fused(const double* __restrict__ pd
         wl[f.bofs[l] + i] = b;
       }
   }
+}
+
+// y (nrows x cout[3]) = net(x) for x (nrows rows of leading dimension ldx,
+// the first cin[0] columns read); LDSD doubles of LDS hold the padded network,
+// TOM tiles of 16 channels the widest layer.  Grid-stride over passes of FR
+// row tiles per wavefront.
+template <int TOM, int LDSD>
+__global__ void __launch_bounds__(256) k_mlp_fused(const double* __restrict__ pd, FusedNet f,
+                                                   const double* __restrict__ x, int ldx, long nrows,
+                                                   double* __restrict__ y) {
+  __shared__ double wl[LDSD];
+  constexpr int COLS = 16 * TOM;
+  stage_net<TOM>(wl, pd, f);
   __syncthreads();
   const int lane = threadIdx.x & 63, c = lane >> 4, v = lane & 15;
   const int nw = blockDim.x >> 6;
@@ -463,6 +477,143 @@ __global__ void k_sym_out(const double* y, const T* GG, long np, long v0, long n
   }
   const double sc = d2 * a2;
   for (int t = 0; t < nt; ++t) tau[t * np + v] = T(acc[t] / double(G) * sc);
+}
+
+// fullchain's closure (the mlp, src/nets.jl:316-331) and its group average
+// (symmetrize_pointwise, :350-389) in one kernel, the work of k_features_inf
+// / k_features_sym, k_mlp_fused, k_tracefree and k_tau_out / k_sym_out: a
+// wavefront takes RW = 16 FR voxels through each of the G group elements
+// (G = 1: the identity alone).  Per element it forms the network input of
+// R_e A R_eᵀ from the voxels' gradients staged in LDS (gb, component-major),
+// runs the four layers, stages the outputs (yb), removes their trace and
+// adds them, conjugated back, to the voxel's sums; then
+// tau_ab = Δ² A2 / G Σ_e qs_a qs_b y_e[q_a q_b] in the field's precision.
+// The norm of the rotated gradient is taken in the unrotated order (the same
+// squares; its rounding may differ from a rotated-order sum in the last bit).
+// Each wavefront owns RW (ng + nt) doubles of LDS at ybase.
+template <int TOM, int LDSD, typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_mlp_closure(const double* __restrict__ pd, FusedNet f, int ybase,
+                                                     const T* __restrict__ GG, long np, int D, int G,
+                                                     const double* __restrict__ zp, double d2, T* __restrict__ tau) {
+  __shared__ double wl[LDSD];
+  constexpr int RW = 16 * FR;
+  constexpr int NSET = 64 / RW;            // output sets per row
+  constexpr int NACC = (6 + NSET - 1) / NSET;
+  stage_net<TOM>(wl, pd, f);
+  __syncthreads();
+  constexpr int COLS = 16 * TOM;
+  const int lane = threadIdx.x & 63, c = lane >> 4, v = lane & 15;
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int ng = D * D, nt = f.cout[3], nin = f.cin[0];
+  double* gb = wl + ybase + wave * RW * (ng + nt);
+  double* yb = gb + RW * ng;
+  const double z = zp ? *zp : 0.0;
+  const double eps = double(std::numeric_limits<T>::epsilon());
+  const double* wlv = wl + c * COLS + v;
+  const int ro = lane % RW, set = lane / RW;  // this lane's output row and set
+  const long npass = (np + RW - 1) / RW;
+  for (long p = long(blockIdx.x) * nw + wave; p < npass; p += long(gridDim.x) * nw) {
+    const long vb = p * RW;
+    for (int i = lane; i < RW * ng; i += 64) {
+      const int k = i / RW, row = i - k * RW;
+      double gv = 0.0;
+      if (vb + row < np) gv = double(GG[long(k) * np + vb + row]);
+      gb[i] = gv;
+    }
+    wave_sync();
+    double den[FR];
+#pragma unroll
+    for (int rt = 0; rt < FR; ++rt) {
+      double a2 = 0.0;
+      for (int k = 0; k < ng; ++k) {
+        const double gv = gb[k * RW + rt * 16 + v];
+        a2 += gv * gv;
+      }
+      den[rt] = sqrt(a2) + eps;
+    }
+    double acc[NACC];
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) acc[j] = 0.0;
+    for (int e = 0; e < G; ++e) {
+      int pe[3] = {0, 1, 2}, se[3] = {1, 1, 1};
+      group_elem(D, e, pe, se);
+      d4_t A[FR][TOM], B[FR][TOM];
+#pragma unroll
+      for (int rt = 0; rt < FR; ++rt)
+#pragma unroll
+        for (int t = 0; t < TOM; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            // input channel ci of row rt*16 + v: (R_e A R_eᵀ)_ab = s_a s_b A_{p_a p_b},
+            // component a + D b; then the Re_Δ input
+            const int ci = 16 * t + c + 4 * r;
+            double xv = 0.0;
+            if (ci < ng) {
+              const int a = ci % D, b = ci / D;
+              const int pa = a == 0 ? pe[0] : (a == 1 ? pe[1] : pe[2]);
+              const int pb = b == 0 ? pe[0] : (b == 1 ? pe[1] : pe[2]);
+              const int sa = a == 0 ? se[0] : (a == 1 ? se[1] : se[2]);
+              const int sb = b == 0 ? se[0] : (b == 1 ? se[1] : se[2]);
+              xv = double(sa * sb) * gb[(pa + D * pb) * RW + rt * 16 + v] / den[rt];
+            } else if (ci < nin) {
+              xv = z;
+            }
+            A[rt][t][r] = xv;
+          }
+      fused_layer<TOM>(wlv + f.wofs[0], f.ti[0], f.cin[0], f.to[0], A, B);
+      fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.to[0]);
+      fused_layer<TOM>(wlv + f.wofs[1], f.ti[1], f.cin[1], f.to[1], B, A);
+      fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.to[1]);
+      fused_layer<TOM>(wlv + f.wofs[2], f.ti[2], f.cin[2], f.to[2], A, B);
+      fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.to[2]);
+      fused_layer<TOM>(wlv + f.wofs[3], f.ti[3], f.cin[3], f.to[3], B, A);
+      // the output tile: channel c + 4r of row rt*16 + v (nt <= 6)
+#pragma unroll
+      for (int rt = 0; rt < FR; ++rt)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+          if (c + 4 * r < nt) yb[(c + 4 * r) * RW + rt * 16 + v] = A[rt][0][r];
+      wave_sync();
+      // trace removal (k_tracefree), then the inverse element's conjugation
+      int q[3] = {0, 1, 2}, qs[3] = {1, 1, 1};
+      for (int a = 0; a < D; ++a) q[pe[a]] = a;
+      for (int a = 0; a < D; ++a) qs[a] = se[q[a]];
+      const double tr = D == 3 ? (yb[ro] + yb[RW + ro] + yb[2 * RW + ro]) / 3.0 : (yb[ro] + yb[RW + ro]) / 2.0;
+#pragma unroll
+      for (int j = 0; j < NACC; ++j) {
+        const int t = set + NSET * j;
+        if (t >= nt) break;
+        // component (a, b) of output t: xx, yy, zz, xy, yz, zx (3D); xx, yy, xy (2D)
+        const int a = D == 3 ? (t < 3 ? t : t - 3) : (t < 2 ? t : 0);
+        const int b = D == 3 ? (t < 3 ? t : (t == 5 ? 0 : t - 2)) : (t < 2 ? t : 1);
+        const int qa = a == 0 ? q[0] : (a == 1 ? q[1] : q[2]);
+        const int qb = b == 0 ? q[0] : (b == 1 ? q[1] : q[2]);
+        const int sa = a == 0 ? qs[0] : (a == 1 ? qs[1] : qs[2]);
+        const int sb = b == 0 ? qs[0] : (b == 1 ? qs[1] : qs[2]);
+        const int k = packed_index(D, qa, qb);
+        double yv = yb[k * RW + ro];
+        if (k < D) yv -= tr;
+        acc[j] += double(sa * sb) * yv;
+      }
+      wave_sync();
+    }
+    const long vox = vb + ro;
+    if (vox < np) {
+      double a2 = 0.0;
+      for (int k = 0; k < ng; ++k) {
+        const double gv = gb[k * RW + ro];
+        a2 += gv * gv;
+      }
+      const double sc = d2 * a2;
+#pragma unroll
+      for (int j = 0; j < NACC; ++j) {
+        const int t = set + NSET * j;
+        if (t >= nt) break;
+        tau[long(t) * np + vox] = T(acc[j] / double(G) * sc);
+      }
+    }
+    wave_sync();
+  }
 }
 
 // ------------------------------------------------------------------ TBNN
@@ -1045,6 +1196,32 @@ void launch_fused(sc_pmlp* c, const FusedNet& f, int lds, const double* x, long 
   PM_HIP(hipGetLastError());
 }
 
+// k_mlp_closure over the field: need = doubles of LDS (network and the
+// wavefronts' staging at ybase)
+template <int TOM, typename T>
+void launch_closure_t(sc_pmlp* c, const FusedNet& f, int need, int ybase, const T* GG, long np, int G,
+                      const double* zp, double d2, T* tau) {
+  const long blocks = (np + 64 * FR - 1) / (64 * FR);  // 4 wavefronts of 16 FR voxels per block
+  if (need <= 4096)
+    hipLaunchKernelGGL((k_mlp_closure<TOM, 4096, T>), dim3(unsigned(std::min(blocks, 4L * c->ncu))), dim3(256), 0,
+                       c->stream, (const double*)c->pd, f, ybase, GG, np, c->D, G, zp, d2, tau);
+  else if (need <= 9216)
+    hipLaunchKernelGGL((k_mlp_closure<TOM, 9216, T>), dim3(unsigned(std::min(blocks, 2L * c->ncu))), dim3(256), 0,
+                       c->stream, (const double*)c->pd, f, ybase, GG, np, c->D, G, zp, d2, tau);
+  else
+    hipLaunchKernelGGL((k_mlp_closure<TOM, 16384, T>), dim3(unsigned(std::min(blocks, 1L * c->ncu))), dim3(256), 0,
+                       c->stream, (const double*)c->pd, f, ybase, GG, np, c->D, G, zp, d2, tau);
+  PM_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_closure(sc_pmlp* c, const FusedNet& f, int tom, int need, int ybase, const T* GG, long np, int G,
+                    const double* zp, double d2, T* tau) {
+  if (tom == 1) launch_closure_t<1, T>(c, f, need, ybase, GG, np, G, zp, d2, tau);
+  else if (tom == 2) launch_closure_t<2, T>(c, f, need, ybase, GG, np, G, zp, d2, tau);
+  else if (tom == 3) launch_closure_t<3, T>(c, f, need, ybase, GG, np, G, zp, d2, tau);
+  else launch_closure_t<4, T>(c, f, need, ybase, GG, np, G, zp, d2, tau);
+}
+
 // one chunk of FP64 inference on nv input rows (leading dimension ctot):
 // y (nt x nv) = symm(mlp(x)) or, for the TBNN, the basis contraction of the
 // coefficients
@@ -1108,10 +1285,17 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
   c->stream = s;
   PM_BLAS(rocblas_set_stream(c->blas, s));
   try {
+    // the mlp whose padded network and per-wavefront staging fit in LDS runs
+    // as one fused kernel over the field; otherwise chunks of features, the
+    // network and the output scaling
+    FusedNet f;
+    int tom = 0, lds = 0;
+    const int ng = c->D * c->D;
+    const bool fused = c->arch == 0 && fused_plan(c, f, tom, lds) && lds + 4 * 16 * FR * (ng + c->nt) <= 16384;
     // voxels per chunk: at most 2^20 input rows and 2^27 activations
     const long vc = std::max(1L, std::min({np, CHUNK / G, CHUNK_ELEMS / (long(c->maxw()) * G)}));
     const long cap = vc * G;                  // input rows per chunk
-    if (cap > c->icap) {
+    if (!fused && cap > c->icap) {
       for (double** q : {&c->xd, &c->hA, &c->hB, &c->yd})
         if (*q) {
           PM_HIP(hipFree(*q));
@@ -1146,26 +1330,35 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
       PM_HIP(hipGetLastError());
     }
     const double* zp = c->redelta ? (const double*)c->zdev : nullptr;
-    for (long v0 = 0; v0 < np; v0 += vc) {
-      const long nv = std::min(vc, np - v0);
-      if (c->arch == 1)
-        hipLaunchKernelGGL(k_tb_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->D, zp, c->xd,
-                           c->ctot, c->ni, c->nt, c->nbas);
-      else if (sym)
-        hipLaunchKernelGGL(k_features_sym<T>, dim3(nb(nv * G)), dim3(256), 0, s, GG, np, v0, nv, c->D, G, zp, c->xd,
-                           c->nin);
-      else
-        hipLaunchKernelGGL(k_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->ng, zp, c->xd,
-                           c->nin);
-      PM_HIP(hipGetLastError());
-      infer_chunk(c, c->xd, c->yd, nv * G);
-      if (sym)
-        hipLaunchKernelGGL(k_sym_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->D,
-                           c->nt, G, d2, tau);
-      else
-        hipLaunchKernelGGL(k_tau_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->ng,
-                           c->nt, d2, tau);
-      PM_HIP(hipGetLastError());
+    if (fused) {
+      if (!c->ncu) {
+        int dev = 0;
+        PM_HIP(hipGetDevice(&dev));
+        PM_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      }
+      launch_closure<T>(c, f, tom, lds + 4 * 16 * FR * (ng + c->nt), lds, GG, np, G, zp, d2, tau);
+    } else {
+      for (long v0 = 0; v0 < np; v0 += vc) {
+        const long nv = std::min(vc, np - v0);
+        if (c->arch == 1)
+          hipLaunchKernelGGL(k_tb_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->D, zp, c->xd,
+                             c->ctot, c->ni, c->nt, c->nbas);
+        else if (sym)
+          hipLaunchKernelGGL(k_features_sym<T>, dim3(nb(nv * G)), dim3(256), 0, s, GG, np, v0, nv, c->D, G, zp, c->xd,
+                             c->nin);
+        else
+          hipLaunchKernelGGL(k_features_inf<T>, dim3(nb(nv)), dim3(256), 0, s, GG, np, v0, nv, c->ng, zp, c->xd,
+                             c->nin);
+        PM_HIP(hipGetLastError());
+        infer_chunk(c, c->xd, c->yd, nv * G);
+        if (sym)
+          hipLaunchKernelGGL(k_sym_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->D,
+                             c->nt, G, d2, tau);
+        else
+          hipLaunchKernelGGL(k_tau_out<T>, dim3(nb(nv)), dim3(256), 0, s, (const double*)c->yd, GG, np, v0, nv, c->ng,
+                             c->nt, d2, tau);
+        PM_HIP(hipGetLastError());
+      }
     }
   } catch (...) {
     c->stream = old;
