@@ -649,7 +649,7 @@ __device__ __forceinline__ void put_dev3(const Mat3& B, double* out, int stride)
 }
 
 template <typename T>
-__device__ double tb_voxel(int D, const T* GG, long np, long v, double* inv, double* bas) {
+__device__ __forceinline__ double tb_voxel(int D, const T* GG, long np, long v, double* inv, double* bas) {
   const int ng = D * D;
   double g[9];
   double a2 = 0;
@@ -760,6 +760,104 @@ __global__ void k_tb_dw(const float* dy, const float* x, long nv, int ctot, int 
     float a = 0.f;
     for (int t = 0; t < nt; ++t) a += dy[v * nt + t] * bx[t + nt * b];
     dw[v * nbas + b] = a;
+  }
+}
+
+// tbnn's closure (src/nets.jl:558-583) in one kernel, the work of
+// k_tb_features_inf, k_mlp_fused, k_tb_contract and k_tau_out: per pass a
+// wavefront's lanes 0..RW-1 stage the invariants [and z] of RW = 16 FR voxels
+// in LDS (xb, channel-major), the four layers give the basis coefficients
+// (staged in yb), and each lane recomputes its voxel's deviatoric basis and
+// contracts it: tau_t = Δ² A2 Σ_b w_b B_b[t].  DD is the grid's dimension
+// (2: 2 invariants, 2 basis tensors, 3 components; 3: 5, 7, 6).  Each
+// wavefront owns RW (cin[0] + nbas) doubles of LDS at ybase.
+template <int TOM, int LDSD, int DD, typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_tb_closure(const double* __restrict__ pd, FusedNet f, int ybase, const T* __restrict__ GG, long np,
+             const double* __restrict__ zp, double d2, T* __restrict__ tau) {
+  __shared__ double wl[LDSD];
+  constexpr int RW = 16 * FR;
+  constexpr int NSET = 64 / RW;
+  constexpr int NI = DD == 3 ? 5 : 2, NT = DD == 3 ? 6 : 3, NB = DD == 3 ? 7 : 2;
+  constexpr int NACC = (NT + NSET - 1) / NSET;
+  stage_net<TOM>(wl, pd, f);
+  __syncthreads();
+  constexpr int COLS = 16 * TOM;
+  const int lane = threadIdx.x & 63, c = lane >> 4, v = lane & 15;
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nin = f.cin[0];
+  double* xb = wl + ybase + wave * RW * (nin + NB);
+  double* yb = xb + RW * nin;
+  const double* wlv = wl + c * COLS + v;
+  const int ro = lane % RW, set = lane / RW;
+  const long npass = (np + RW - 1) / RW;
+  for (long p = long(blockIdx.x) * nw + wave; p < npass; p += long(gridDim.x) * nw) {
+    const long vb = p * RW;
+    if (lane < RW) {
+      double inv[5] = {0, 0, 0, 0, 0}, bas[42];
+      if (vb + lane < np) tb_voxel<T>(DD, GG, np, vb + lane, inv, bas);
+#pragma unroll
+      for (int k = 0; k < NI; ++k) xb[k * RW + lane] = inv[k];
+      if (nin > NI) xb[NI * RW + lane] = *zp;
+    }
+    wave_sync();
+    d4_t A[FR][TOM], B[FR][TOM];
+#pragma unroll
+    for (int rt = 0; rt < FR; ++rt)
+#pragma unroll
+      for (int t = 0; t < TOM; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ci = 16 * t + c + 4 * r;
+          double xv = 0.0;
+          if (ci < nin) xv = xb[ci * RW + rt * 16 + v];
+          A[rt][t][r] = xv;
+        }
+    fused_layer<TOM>(wlv + f.wofs[0], f.ti[0], f.cin[0], f.to[0], A, B);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.to[0]);
+    fused_layer<TOM>(wlv + f.wofs[1], f.ti[1], f.cin[1], f.to[1], B, A);
+    fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.to[1]);
+    fused_layer<TOM>(wlv + f.wofs[2], f.ti[2], f.cin[2], f.to[2], A, B);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.to[2]);
+    fused_layer<TOM>(wlv + f.wofs[3], f.ti[3], f.cin[3], f.to[3], B, A);
+    // the coefficients: channel c + 4r of row rt*16 + v (NB <= 7)
+#pragma unroll
+    for (int rt = 0; rt < FR; ++rt)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        if (c + 4 * r < NB) yb[(c + 4 * r) * RW + rt * 16 + v] = A[rt][0][r];
+    wave_sync();
+    const long vox = vb + ro;
+    if (vox < np) {
+      double inv[5], bas[42];
+      const double sc = d2 * tb_voxel<T>(DD, GG, np, vox, inv, bas);
+      double w[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) w[b] = yb[b * RW + ro];
+#pragma unroll
+      for (int j = 0; j < NACC; ++j) {
+        // component t = set + NSET j, selected among the compile-time
+        // candidates so that bas stays in registers
+        double m[NSET];
+#pragma unroll
+        for (int s = 0; s < NSET; ++s) {
+          const int t = s + NSET * j;
+          double a = 0.0;
+          if (t < NT) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) a += bas[t + NT * b] * w[b];
+          }
+          m[s] = a;
+        }
+        double mm = m[0];
+#pragma unroll
+        for (int s = 1; s < NSET; ++s)
+          if (set == s) mm = m[s];
+        const int t = set + NSET * j;
+        if (t < NT) tau[long(t) * np + vox] = T(mm * sc);
+      }
+    }
+    wave_sync();
   }
 }
 
@@ -1222,6 +1320,33 @@ void launch_closure(sc_pmlp* c, const FusedNet& f, int tom, int need, int ybase,
   else launch_closure_t<4, T>(c, f, need, ybase, GG, np, G, zp, d2, tau);
 }
 
+// k_tb_closure over the field (need = doubles of LDS; at most 9216)
+template <int TOM, int DD, typename T>
+void launch_tb_closure_t(sc_pmlp* c, const FusedNet& f, int need, int ybase, const T* GG, long np,
+                         const double* zp, double d2, T* tau) {
+  const long blocks = (np + 64 * FR - 1) / (64 * FR);
+  if (need <= 4096)
+    hipLaunchKernelGGL((k_tb_closure<TOM, 4096, DD, T>), dim3(unsigned(std::min(blocks, 4L * c->ncu))), dim3(256),
+                       0, c->stream, (const double*)c->pd, f, ybase, GG, np, zp, d2, tau);
+  else
+    hipLaunchKernelGGL((k_tb_closure<TOM, 9216, DD, T>), dim3(unsigned(std::min(blocks, 2L * c->ncu))), dim3(256),
+                       0, c->stream, (const double*)c->pd, f, ybase, GG, np, zp, d2, tau);
+  PM_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_tb_closure(sc_pmlp* c, const FusedNet& f, int tom, int need, int ybase, const T* GG, long np,
+                       const double* zp, double d2, T* tau) {
+  if (c->D == 3) {
+    if (tom == 1) launch_tb_closure_t<1, 3, T>(c, f, need, ybase, GG, np, zp, d2, tau);
+    else if (tom == 2) launch_tb_closure_t<2, 3, T>(c, f, need, ybase, GG, np, zp, d2, tau);
+    else launch_tb_closure_t<3, 3, T>(c, f, need, ybase, GG, np, zp, d2, tau);
+  } else {
+    if (tom == 1) launch_tb_closure_t<1, 2, T>(c, f, need, ybase, GG, np, zp, d2, tau);
+    else if (tom == 2) launch_tb_closure_t<2, 2, T>(c, f, need, ybase, GG, np, zp, d2, tau);
+    else launch_tb_closure_t<3, 2, T>(c, f, need, ybase, GG, np, zp, d2, tau);
+  }
+}
+
 // one chunk of FP64 inference on nv input rows (leading dimension ctot):
 // y (nt x nv) = symm(mlp(x)) or, for the TBNN, the basis contraction of the
 // coefficients
@@ -1291,7 +1416,9 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
     FusedNet f;
     int tom = 0, lds = 0;
     const int ng = c->D * c->D;
-    const bool fused = c->arch == 0 && fused_plan(c, f, tom, lds) && lds + 4 * 16 * FR * (ng + c->nt) <= 16384;
+    const bool planned = (c->arch == 0 || c->arch == 1) && fused_plan(c, f, tom, lds);
+    const int need = c->arch == 0 ? lds + 4 * 16 * FR * (ng + c->nt) : lds + 4 * 16 * FR * (c->nin + c->nbas);
+    const bool fused = planned && (c->arch == 0 ? need <= 16384 : (tom <= 3 && need <= 9216));
     // voxels per chunk: at most 2^20 input rows and 2^27 activations
     const long vc = std::max(1L, std::min({np, CHUNK / G, CHUNK_ELEMS / (long(c->maxw()) * G)}));
     const long cap = vc * G;                  // input rows per chunk
@@ -1336,7 +1463,10 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
         PM_HIP(hipGetDevice(&dev));
         PM_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
       }
-      launch_closure<T>(c, f, tom, lds + 4 * 16 * FR * (ng + c->nt), lds, GG, np, G, zp, d2, tau);
+      if (c->arch == 0)
+        launch_closure<T>(c, f, tom, need, lds, GG, np, G, zp, d2, tau);
+      else
+        launch_tb_closure<T>(c, f, tom, need, lds, GG, np, zp, d2, tau);
     } else {
       for (long v0 = 0; v0 < np; v0 += vc) {
         const long nv = std::min(vc, np - v0);

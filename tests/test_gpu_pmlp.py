@@ -322,15 +322,19 @@ def test_tbnn_forward_loss_backward_parity(D, use_redelta):
         assert rel(gg[name], go_[name]) < 2e-5, name
 
 
-@pytest.mark.parametrize("D,n,use_redelta", [(3, 16, True), (2, 32, False)])
-def test_tbnn_closure_parity_fp64(D, n, use_redelta):
+# up to three tiles of 16 channels: one fused kernel (30^3 ends in a partial
+# row tile); 64 channels: the chunked path
+@pytest.mark.parametrize("D,n,use_redelta,widths", [(3, 16, True, (11, 13, 7)), (2, 32, False, (11, 13, 7)),
+                                                    (3, 30, True, (22, 36, 44)), (2, 30, True, (22, 36, 44)),
+                                                    (3, 16, False, (46, 64, 64))])
+def test_tbnn_closure_parity_fp64(D, n, use_redelta, widths):
     import symcode as sc
     from symcode.pmlp import PointwiseClosure
 
-    print_case("tbnn closure fp64", D, n, use_redelta)
+    print_case("tbnn closure fp64", D, n, use_redelta, widths)
     go = S.Grid(D, n)
     g = sc.Grid(D, n, dtype=torch.float64)
-    net = tbnn_of(D, (11, 13, 7), use_redelta=use_redelta, seed=4)
+    net = tbnn_of(D, widths, use_redelta=use_redelta, seed=4)
     delta, visc, norm = 2 * np.pi / n * 2, 2e-3, (1.1, 0.8)
     model = PointwiseClosure(net, g, delta, visc, norm if use_redelta else None)
     u = oracle_solenoidal(go, 37, trunc=True)
@@ -344,6 +348,23 @@ def test_tbnn_closure_parity_fp64(D, n, use_redelta):
     assert rel(g.download_spectral(du), duo) < 1e-12
     with pytest.raises(ValueError):
         PointwiseClosure(net, g, delta, visc, norm, symmetrize=True)
+
+
+def test_tbnn_closure_parity_fp32():
+    import symcode as sc
+    from symcode.pmlp import PointwiseClosure
+
+    print_case("tbnn closure fp32", 3, 16)
+    go = S.Grid(3, 16)
+    g = sc.Grid(3, 16, dtype=torch.float32)
+    net = tbnn_of(3, (11, 13, 7), use_redelta=True, seed=6)
+    delta, visc, norm = 2 * np.pi / 16 * 2, 2e-3, (1.1, 0.8)
+    model = PointwiseClosure(net, g, delta, visc, norm)
+    u = oracle_solenoidal(go, 41, trunc=True)
+    tau = g.download_physical(model(g.upload_spectral(u)))
+    to = OP.tbnn_model(oracle_params(net), OC.gradient_phys(u, go), delta, 3, eps=np.finfo(np.float32).eps,
+                       redelta=(visc, *norm))
+    assert rel(tau, to) < 1e-5
 
 
 def test_tbnn_training_teacher():
