@@ -185,10 +185,16 @@ struct FusedNet {
 };
 
 // the tanh-form GELU with 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)): the same
-// function as gelu() above, one exp instead of a tanh
+// function as gelu() above, one exp instead of a tanh; 1 / (1 + e) by the
+// hardware reciprocal and two Newton steps (within an ulp or two of the
+// division), the exponent capped so that 1 + e stays finite
 __device__ __forceinline__ double gelu_exp(double z) {
   const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
-  return z / (1.0 + exp(-2.0 * u));
+  const double d = 1.0 + exp(fmin(-2.0 * u, 708.0));
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return z * r;
 }
 
 // orders a wavefront's LDS stores before its later loads of the same words
@@ -226,14 +232,16 @@ __device__ __forceinline__ void fused_layer(const double* wv, int ti, int cin, i
   }
 }
 
-// h = gelu(h + b) on the live tiles; bc = this lane's b[c]
+// h = gelu(h + b) on the live channels (co < cout); bc = this lane's b[c].
+// Channel groups 16m + 4r + (0..3) wholly past cout hold exact zeros
+// (zero weights and bias) and gelu(0) = 0, so they are skipped.
 template <int TOM>
-__device__ __forceinline__ void fused_bias_gelu(d4_t (&h)[FR][TOM], const double* bc, int to) {
+__device__ __forceinline__ void fused_bias_gelu(d4_t (&h)[FR][TOM], const double* bc, int cout) {
 #pragma unroll
   for (int m = 0; m < TOM; ++m) {
-    if (m >= to) break;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
+      if (16 * m + 4 * r >= cout) break;
       const double bb = bc[16 * m + 4 * r];
 #pragma unroll
       for (int rt = 0; rt < FR; ++rt) h[rt][m][r] = gelu_exp(h[rt][m][r] + bb);
@@ -295,11 +303,11 @@ __global__ void __launch_bounds__(256) k_mlp_fused(const double* __restrict__ pd
         }
     }
     fused_layer<TOM>(wlv + f.wofs[0], f.ti[0], f.cin[0], f.to[0], A, B);
-    fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.to[0]);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.cout[0]);
     fused_layer<TOM>(wlv + f.wofs[1], f.ti[1], f.cin[1], f.to[1], B, A);
-    fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.to[1]);
+    fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.cout[1]);
     fused_layer<TOM>(wlv + f.wofs[2], f.ti[2], f.cin[2], f.to[2], A, B);
-    fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.to[2]);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.cout[2]);
     fused_layer<TOM>(wlv + f.wofs[3], f.ti[3], f.cin[3], f.to[3], B, A);
 #pragma unroll
     for (int rt = 0; rt < FR; ++rt) {
@@ -561,11 +569,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             A[rt][t][r] = xv;
           }
       fused_layer<TOM>(wlv + f.wofs[0], f.ti[0], f.cin[0], f.to[0], A, B);
-      fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.to[0]);
+      fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.cout[0]);
       fused_layer<TOM>(wlv + f.wofs[1], f.ti[1], f.cin[1], f.to[1], B, A);
-      fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.to[1]);
+      fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.cout[1]);
       fused_layer<TOM>(wlv + f.wofs[2], f.ti[2], f.cin[2], f.to[2], A, B);
-      fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.to[2]);
+      fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.cout[2]);
       fused_layer<TOM>(wlv + f.wofs[3], f.ti[3], f.cin[3], f.to[3], B, A);
       // the output tile: channel c + 4r of row rt*16 + v (nt <= 6)
 #pragma unroll
@@ -814,11 +822,11 @@ k_tb_closure(const double* __restrict__ pd, FusedNet f, int ybase, const T* __re
           A[rt][t][r] = xv;
         }
     fused_layer<TOM>(wlv + f.wofs[0], f.ti[0], f.cin[0], f.to[0], A, B);
-    fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.to[0]);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[0] + c, f.cout[0]);
     fused_layer<TOM>(wlv + f.wofs[1], f.ti[1], f.cin[1], f.to[1], B, A);
-    fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.to[1]);
+    fused_bias_gelu<TOM>(A, wl + f.bofs[1] + c, f.cout[1]);
     fused_layer<TOM>(wlv + f.wofs[2], f.ti[2], f.cin[2], f.to[2], A, B);
-    fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.to[2]);
+    fused_bias_gelu<TOM>(B, wl + f.bofs[2] + c, f.cout[2]);
     fused_layer<TOM>(wlv + f.wofs[3], f.ti[3], f.cin[3], f.to[3], B, A);
     // the coefficients: channel c + 4r of row rt*16 + v (NB <= 7)
 #pragma unroll

@@ -282,18 +282,54 @@ def pointwise_closures(n_dns=256, n_les=64, visc=2e-3, factor=2, nsnap=6, dt_sav
     return res
 
 
+def pointwise_les_cost(n=64, steps=10, seed=1, visc=2e-3, dt=1e-3, log=print):
+    """The cost of one FP64 LES RK4 step at n^3 with each pointwise closure at
+    the reference's tiers (mlp and TBNN p3000, the mlp's group average,
+    equivariant net p1200) on a random solenoidal field: the network weights
+    are the initialisation (the cost does not depend on them).  Returns
+    {closure: ms per RK4 step}."""
+    from . import pmlp as PM
+
+    g = Grid(3, n, dtype=torch.float64)
+    u = S.randomfield(S.peak_profile, g, seed=seed, totalenergy=0.2, kpeak=4)
+    delta = 2 * 2 * math.pi / n
+    specs = [("mlp p3000", PM.PointwiseMLP, PM.ARCH_TIERS["conv"]["p3000"], False),
+             ("mlp p3000 group average", PM.PointwiseMLP, PM.ARCH_TIERS["conv"]["p3000"], True),
+             ("tbnn p3000", PM.TBNN, PM.ARCH_TIERS["tbnn"]["p3000"], False),
+             ("equi p1200", PM.EquivariantNet, PM.ARCH_TIERS["equi"]["p1200"], False)]
+    res = {"n": n, "steps": steps, "ms_per_step": {}}
+    for name, cls, widths, sym in specs:
+        net = cls(3, widths, use_redelta=True, seed=0, nvox_max=n**3)
+        model = PM.PointwiseClosure(net, g, delta, visc, (0.5, 0.3), symmetrize=sym)
+        v = u.clone()
+        les_rk4(v, dt, g, model, visc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            les_rk4(v, dt, g, model, visc)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        res["ms_per_step"][name] = ms
+        if log:
+            log(f"{name}: {ms:.2f} ms per LES RK4 step at {n}^3")
+    return res
+
+
 if __name__ == "__main__":
     import argparse
     import json
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=("fp32_vs_fp64", "les_classical", "pointwise_closures"))
+    ap.add_argument("which", choices=("fp32_vs_fp64", "les_classical", "pointwise_closures", "pointwise_les_cost"))
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     a = ap.parse_args()
     if a.which == "fp32_vs_fp64":
         kw = {k: v for k, v in (("n", a.n), ("steps", a.steps)) if v is not None}
         r = fp32_vs_fp64(**kw)
+    elif a.which == "pointwise_les_cost":
+        kw = {k: v for k, v in (("n", a.n), ("steps", a.steps)) if v is not None}
+        r = pointwise_les_cost(**kw)
     elif a.which == "pointwise_closures":
         kw = {k: v for k, v in (("n_dns", a.n), ("train_steps", a.steps)) if v is not None}
         r = pointwise_closures(**kw)
