@@ -583,9 +583,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
           if (c + 4 * r < nt) yb[(c + 4 * r) * RW + rt * 16 + v] = A[rt][0][r];
       wave_sync();
       // trace removal (k_tracefree), then the inverse element's conjugation
-      int q[3] = {0, 1, 2}, qs[3] = {1, 1, 1};
-      for (int a = 0; a < D; ++a) q[pe[a]] = a;
-      for (int a = 0; a < D; ++a) qs[a] = se[q[a]];
+      // q[pe[a]] = a, qs[a] = se[q[a]], by selects (no register-indexed stores)
+      int q[3], qs[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        q[k] = pe[0] == k ? 0 : (pe[1] == k ? 1 : 2);
+        qs[k] = q[k] == 0 ? se[0] : (q[k] == 1 ? se[1] : se[2]);
+      }
       const double tr = D == 3 ? (yb[ro] + yb[RW + ro] + yb[2 * RW + ro]) / 3.0 : (yb[ro] + yb[RW + ro]) / 2.0;
 #pragma unroll
       for (int j = 0; j < NACC; ++j) {
