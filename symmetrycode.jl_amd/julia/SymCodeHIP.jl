@@ -6,7 +6,7 @@
 # device buffers in the library's device layout, created by the allocators
 # below (the Julia GC owns them through a finalizer).
 #
-# Usage (see INTEGRATION.md):
+# Usage:
 #     using SymmetryCode, SymCodeHIP
 #     g = SymmetryCode.Grid{3}(; l = 2π, n = 810, backend = SymCodeHIP.HIPBackend())
 #     u = SymmetryCode.randomfield(SymmetryCode.peak_profile, g; kpeak = 4)
