@@ -431,43 +431,49 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   }
 }
 
-// Persistent form of k_x_products_w for FP64 rows whose inputs are band-limited
-// to kx <= kin < 64 XP_CHUNKS: wavefront gw walks rows gw, gw + W, ... (W waves
-// in the grid).  Each velocity row is staged in LDS with 16-byte direct
+// Persistent form of k_x_products_w for rows whose inputs are band-limited to
+// kx <= kin < xp_entries<T>(): wavefront gw walks rows gw, gw + W, ... (W
+// waves in the grid).  Each velocity row is staged in LDS with 16-byte direct
 // global->LDS loads issued one transform ahead of its use (the next component
 // while the current one is transformed, the next row's first component while
 // the current row's products are transformed), so no register holds an
 // in-flight load and the loads' latency overlaps the FFTs.  The staging row is
 // zero beyond kin (entries above kin are loaded from a zero source, those past
-// 64 XP_CHUNKS are cleared once), so the C2R pre-processing reads it without
-// bounds tests.
+// xp_entries<T>() are cleared once; FP32 loads carry two entries, so entry
+// kin + 1 may hold the row's own value and is masked where it is read), so
+// the C2R pre-processing reads it without bounds tests.
 #ifndef SC_XP_WAVES
 #define SC_XP_WAVES 4
 #endif
-constexpr int XP_CHUNKS = 5;  // direct loads of 64 entries per staged row
+// entries per 16-byte load, chunks of 64 loads per staged row, entries staged
+template <typename T> constexpr int xp_per_load() { return 16 / int(sizeof(C<T>)); }
+template <typename T> constexpr int xp_chunks() { return sizeof(T) == 8 ? 5 : 3; }
+template <typename T> constexpr int xp_entries() { return 64 * xp_chunks<T>() * xp_per_load<T>(); }
 
-// stg[k] = k <= kin ? src[k] : 0 for k < 64 XP_CHUNKS; lane l of chunk j fills
-// stg[64 j + l] (the destination of a direct load is the wave's base + 16 l)
-DEV void xp_stage(const C<double>* src, const C<double>* zero, C<double>* stg, int kin, int lane) {
+// stg[k] = k <= kin ? src[k] : 0 for k < xp_entries<T>() (FP32: the pair
+// holding kin is loaded whole); lane l of chunk j fills the 16 bytes at
+// stg + per-load (64 j + l) (a direct load writes the wave's base + 16 l)
+template <typename T>
+DEV void xp_stage(const C<T>* src, const C<T>* zero, C<T>* stg, int kin, int lane) {
+  constexpr int PL = xp_per_load<T>();
   // the wave's earlier LDS reads of stg have returned (lgkmcnt(0))
   __builtin_amdgcn_s_waitcnt(0xC07F);
 #pragma unroll
-  for (int j = 0; j < XP_CHUNKS; ++j) {
-    const int k = j * 64 + lane;
-    const C<double>* a = k <= kin ? src + k : zero;
-    __builtin_amdgcn_global_load_lds((const void*)a, (__attribute__((address_space(3))) void*)(stg + j * 64), 16, 0,
-                                     0);
+  for (int j = 0; j < xp_chunks<T>(); ++j) {
+    const int k = (j * 64 + lane) * PL;
+    const C<T>* a = k <= kin ? src + k : zero;
+    __builtin_amdgcn_global_load_lds((const void*)a, (__attribute__((address_space(3))) void*)(stg + j * 64 * PL), 16,
+                                     0, 0);
   }
 }
 
-template <int M, int D>
+template <typename T, int M, int D>
 __global__ void __launch_bounds__(64 * SC_XP_WAVES, 8 / SC_XP_WAVES)
-k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>* __restrict__ twm,
-               const C<double>* __restrict__ twn, RowGeom g, unsigned long long* vmax_bits) {
-  using T = double;
+k_x_products_p(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm, const C<T>* __restrict__ twn,
+               RowGeom g, unsigned long long* vmax_bits) {
   constexpr int TPP = Plan<M>::tpp;
   static_assert(TPP <= 64, "one wavefront per row");
-  static_assert(64 * XP_CHUNKS <= M + 1, "staged chunks fit the row");
+  static_assert(xp_entries<T>() <= (M + 1 + 7) / 8 * 8, "staged chunks fit the staging row");
   constexpr int NW = SC_XP_WAVES;
   constexpr int NP = D == 3 ? 6 : 3;
   constexpr int XS = (M + 1 + 7) / 8 * 8;  // staging row: entries 0..M
@@ -492,7 +498,7 @@ k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>
   const long W = long(gridDim.x) * NW;
   const int kin = g.kin;
   long row = long(blockIdx.x) * NW + w;
-  if (row < g.nrows) xp_stage(in + row * g.nhp, zero, sw, kin, lane0);
+  if (row < g.nrows) xp_stage<T>(in + row * g.nhp, zero, sw, kin, lane0);
   double vm = 0.0;
   constexpr int PA[6] = {0, 1, 2, 0, 1, 2};
   constexpr int PB[6] = {0, 1, 2, 1, 2, 0};
@@ -512,12 +518,18 @@ k_x_products_p(const C<double>* __restrict__ in, C<double>* out, const C<double>
     const long nxt = row + W;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      if (act) PI::load(v, p, [&](int k) { return c2r_pre<T, M>(k, twn_l, [&](int kk) { return sw[kk]; }); });
+      if (act)
+        PI::load(v, p, [&](int k) {
+          return c2r_pre<T, M>(k, twn_l, [&](int kk) {
+            if constexpr (xp_per_load<T>() > 1) return kk <= kin ? sw[kk] : mk<T>(T(0), T(0));
+            else return sw[kk];
+          });
+        });
       // the staging row is free: the next component, or the next row's first
       // (after the last row, this row's again: unused, but every wave issues
       // the same loads, so the waits can count them)
-      if (c + 1 < D) xp_stage(in + long(c + 1) * g.s_comp_spec + row * g.nhp, zero, sw, kin, lane);
-      else xp_stage(in + (nxt < g.nrows ? nxt : row) * g.nhp, zero, sw, kin, lane);
+      if (c + 1 < D) xp_stage<T>(in + long(c + 1) * g.s_comp_spec + row * g.nhp, zero, sw, kin, lane);
+      else xp_stage<T>(in + (nxt < g.nrows ? nxt : row) * g.nhp, zero, sw, kin, lane);
       if (act) {
         PI::run(v, p, ll, 1, twm_l);
 #pragma unroll

@@ -194,21 +194,19 @@ static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale,
   auto twm = reinterpret_cast<const C<T>*>(c.tw_m);
   auto twn = reinterpret_cast<const C<T>*>(c.tw_n);
   c.prof_begin("x_products");
-  if constexpr (std::is_same<T, double>::value) {
-    if (SC_XPROD_PREFETCH && M == 405 && g.kin < XP_CHUNKS * 64) {
-      // persistent: two blocks of SC_XP_WAVES wavefronts per compute unit
-      constexpr int NW = SC_XP_WAVES;
-      const long blocks = std::min<long>((g.nrows + NW - 1) / NW, 2L * c.ncu);
-      if (c.D == 3)
-        hipLaunchKernelGGL((k_x_products_p<405, 3>), dim3(unsigned(blocks)), dim3(64 * NW), 0, c.stream, in, out,
-                           twm, twn, g, vmax);
-      else
-        hipLaunchKernelGGL((k_x_products_p<405, 2>), dim3(unsigned(blocks)), dim3(64 * NW), 0, c.stream, in, out,
-                           twm, twn, g, vmax);
-      c.prof_end();
-      SC_HIP(hipGetLastError());
-      return;
-    }
+  if (SC_XPROD_PREFETCH && M == 405 && g.kin < xp_entries<T>()) {
+    // persistent: two blocks of SC_XP_WAVES wavefronts per compute unit
+    constexpr int NW = SC_XP_WAVES;
+    const long blocks = std::min<long>((g.nrows + NW - 1) / NW, 2L * c.ncu);
+    if (c.D == 3)
+      hipLaunchKernelGGL((k_x_products_p<T, 405, 3>), dim3(unsigned(blocks)), dim3(64 * NW), 0, c.stream, in, out,
+                         twm, twn, g, vmax);
+    else
+      hipLaunchKernelGGL((k_x_products_p<T, 405, 2>), dim3(unsigned(blocks)), dim3(64 * NW), 0, c.stream, in, out,
+                         twm, twn, g, vmax);
+    c.prof_end();
+    SC_HIP(hipGetLastError());
+    return;
   }
   if (SC_XPROD_WAVE && tpp_rt(M) <= 64) {
     constexpr int NW = SC_XW_WAVES;

@@ -354,17 +354,18 @@ def test_rk4_810_fused_matches_unfused(prec):
     torch.cuda.empty_cache()
 
 
-def test_nonlinearity_810_against_separate_transforms():
-    """810^3 FP64: nonlinearity! through the fused x-pass (C2R, products and
-    R2C of every row in one kernel, persistent with staged rows) against
+@pytest.mark.parametrize("D,prec", [(3, 64), (3, 32), (2, 32)])
+def test_nonlinearity_810_against_separate_transforms(D, prec):
+    """810^D: nonlinearity! through the fused x-pass (C2R, products and R2C
+    of every row in one kernel, persistent with staged rows) against
     to_spec(v_i v_j) of v = to_phys(twothirds(u)) through the separate x
     transforms; and the CFL step's fused max |v| against propose_timestep."""
     import gc
 
     import symcode as sc
 
-    print_case("nonlinearity 810 fused x-pass vs separate transforms")
-    g = sc.Grid(3, 810, dtype=torch.float64)
+    print_case("nonlinearity 810 fused x-pass vs separate transforms", D, prec)
+    g = sc.Grid(D, 810, dtype=torch.float64 if prec == 64 else torch.float32)
     u = sc.randomfield(sc.peak_profile, g, seed=5, totalenergy=0.2, kpeak=4)
     sig = sc.nonlinearity(g.tensorfield(), u, g)
     t = u.clone()
@@ -372,17 +373,18 @@ def test_nonlinearity_810_against_separate_transforms():
     v = sc.to_phys(g.spacevectorfield(), t, g)
     del t
     prod, spec = g.spacescalarfield(), g.scalarfield()
-    for q, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]):
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)] if D == 3 else [(0, 0), (1, 1), (0, 1)]
+    for q, (i, j) in enumerate(pairs):
         torch.mul(v[i:i + 1], v[j:j + 1], out=prod)
         sc.to_spec(spec, prod, g)
         err = float(torch.linalg.norm((sig[q:q + 1] - spec).flatten()) / torch.linalg.norm(spec.flatten()))
         print("component", q, "rel diff", err, flush=True)
-        assert err < 1e-13, q
+        assert err < (1e-13 if prec == 64 else 5e-6), q
     del sig, v, prod, spec
     nu, cfl = 2.7e-4, 0.3
     dto = cfl * sc.propose_timestep(u, g, nu)
     dtg = sc.rk4_cfl(u, g, nu, cfl)
-    assert dtg == pytest.approx(dto, rel=1e-12)
+    assert dtg == pytest.approx(dto, rel=1e-12 if prec == 64 else 1e-5)
     del u, g
     gc.collect()
     torch.cuda.empty_cache()

@@ -416,6 +416,23 @@ def test_symmetrized_closure_parity_fp64(D, n, use_redelta):
     assert rel(g.download_physical(plain(ud)), fc(GGo)) < 1e-12
 
 
+def test_symmetrized_closure_parity_fp32():
+    import symcode as sc
+    from symcode.pmlp import PointwiseClosure, symmetrize_pointwise
+
+    print_case("symmetrized mlp closure fp32", 3, 16)
+    go = S.Grid(3, 16)
+    g = sc.Grid(3, 16, dtype=torch.float32)
+    net = net_of(3, (22, 36, 44), use_redelta=True, seed=5)
+    delta, visc, norm = 2 * np.pi / 16 * 2, 2e-3, (1.1, 0.8)
+    model = symmetrize_pointwise(PointwiseClosure(net, g, delta, visc, norm))
+    u = oracle_solenoidal(go, 43, trunc=True)
+    tau = g.download_physical(model(g.upload_spectral(u)))
+    p = oracle_params(net)
+    fc = lambda G: OP.fullchain(p, G, delta, 3, eps=np.finfo(np.float32).eps, redelta=(visc, *norm))
+    assert rel(tau, OP.symmetrize(fc, OC.gradient_phys(u, go), 3)) < 1e-5
+
+
 # ------------------------------------------------------ equivariant_net
 def equi_of(D, widths=(2, 3, 2), use_redelta=False, seed=1, nvox_max=4096):
     from symcode.pmlp import EquivariantNet
