@@ -567,8 +567,10 @@ k_x_products_p(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
       C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
       // X[k] and X[M - k] from the same two entries Z[k], Z[M - k] (Z[M] =
       // Z[0]); lanes past k = M / 2 store that pair again (same addresses,
-      // same values), so every store is issued and the waits can count them
-#pragma unroll
+      // same values), so every store is issued and the waits can count them.
+      // Not unrolled: the unrolled form needs more registers than the
+      // wavefront has next to the three live velocities (FP64: 11 spilled)
+#pragma unroll 1
       for (int t = 0; t < (M / 2 + 64) / 64; ++t) {
         const int k = min(lane + 64 * t, M / 2);
         const C<T> za = ll[k], zb = ll[k == 0 ? 0 : M - k];
