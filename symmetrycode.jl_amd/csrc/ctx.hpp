@@ -9,6 +9,10 @@
 #ifndef SC_STAGGER
 #define SC_STAGGER 1
 #endif
+// choose the tiled buffer among candidate allocations by the y-pass's rate
+#ifndef SC_TUNE_TILED
+#define SC_TUNE_TILED 1
+#endif
 
 struct sc_cnn;   // learned closure (cnn.hip)
 struct sc_pmlp;  // pointwise MLP closure (pmlp.hip)
@@ -41,6 +45,7 @@ struct Ctx {
   void* bufA = nullptr;   // D spectral comps: inverse pipeline
   void* bufB = nullptr;   // 6 spectral comps: forward pipeline / stress
   void* bufT = nullptr;   // 6 slots, z-pencil tiled: forward y-pass output
+  bool tiled_tuned = false;  // bufT chosen among candidate allocations (tune_tiled)
   void* scr = nullptr;    // general spectral scratch
   int scr_comp = 0;
   void* ustart = nullptr; // RK storage (D comps each)

@@ -462,6 +462,58 @@ const double* Ops<T>::band_coeff(Ctx& c, const CT* u) {
   return c.forceA;
 }
 
+// The forward y-pass writes the z-pencil-tiled buffer at a rate that depends
+// on where that buffer's pages landed: on MI355X at 810^3 FP64 the same pass
+// runs 11.5-11.9 ms per launch into some allocations and 13.5-14.1 ms into
+// others, whatever its input, and an allocation keeps its rate.  So at the
+// first RHS whose tiled buffer is large, the pass is timed into the buffer it
+// has and into up to three fresh allocations (as many as fit with 8 GB to
+// spare), and the fastest is kept; the others are freed.  The pass only
+// writes the candidate, so the results are the same whichever is kept.
+template <typename T>
+static void tune_tiled(Ctx& c, const C<T>* B) {
+  c.tiled_tuned = true;
+  const size_t bytes = size_t(6) * tiled_slot<T>(c) * sizeof(C<T>);
+  if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return;
+  size_t freeb = 0, totalb = 0;
+  SC_HIP(hipMemGetInfo(&freeb, &totalb));
+  const size_t spare = size_t(8) << 30;
+  const int extra = int(std::min<size_t>(3, freeb > spare ? (freeb - spare) / (bytes + (size_t(5) << 20)) : 0));
+  if (extra == 0) return;
+  std::vector<void*> cand{c.bufT};
+  for (int i = 0; i < extra; ++i) {
+    void* q = nullptr;
+    c.dmalloc(&q, bytes);
+    cand.push_back(q);
+  }
+  const bool prof = c.prof_on;
+  c.prof_on = false;
+  hipEvent_t e0, e1;
+  SC_HIP(hipEventCreate(&e0));
+  SC_HIP(hipEventCreate(&e1));
+  float best = 0.f;
+  size_t ib = 0;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    y_fwd_div<T>(c, B, reinterpret_cast<C<T>*>(cand[i]));  // first touch
+    SC_HIP(hipEventRecord(e0, c.stream));
+    y_fwd_div<T>(c, B, reinterpret_cast<C<T>*>(cand[i]));
+    SC_HIP(hipEventRecord(e1, c.stream));
+    SC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (i == 0 || ms < best) {
+      best = ms;
+      ib = i;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.prof_on = prof;
+  c.bufT = cand[ib];
+  for (size_t i = 0; i < cand.size(); ++i)
+    if (i != ib) c.dfree(cand[i]);
+}
+
 // Right-hand side of the time steppers: the five-pass pipeline with the
 // divergence, viscous term and forcing fused into the forward passes.  The
 // gradient part -nu k (k.u) of the viscous term is omitted (removed by the
@@ -513,6 +565,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     rp.forceA = c.forceA;
   }
   if (c.D == 3) {
+    if (!c.tiled_tuned) tune_tiled<T>(c, B);
     CT* BT = reinterpret_cast<CT*>(c.bufT);
     y_fwd_div<T>(c, B, BT);
     fwd_rhs_last<T>(c, BT, u, du, rp, lin);
