@@ -9,7 +9,8 @@
 #ifndef SC_STAGGER
 #define SC_STAGGER 1
 #endif
-// choose the tiled buffer among candidate allocations by the y-pass's rate
+// choose the tiled buffer and the RK4 state buffers among candidate
+// allocations by the rates of the passes that write them
 #ifndef SC_TUNE_TILED
 #define SC_TUNE_TILED 1
 #endif
@@ -46,6 +47,7 @@ struct Ctx {
   void* bufB = nullptr;   // 6 spectral comps: forward pipeline / stress
   void* bufT = nullptr;   // 6 slots, z-pencil tiled: forward y-pass output
   bool tiled_tuned = false;  // bufT chosen among candidate allocations (tune_tiled)
+  bool state_tuned = false;  // ust, acc chosen among candidate allocations (tune_state)
   void* scr = nullptr;    // general spectral scratch
   int scr_comp = 0;
   void* ustart = nullptr; // RK storage (D comps each)
