@@ -100,12 +100,12 @@ int sc_convectiondiffusion(sc_ctx* ctx, void* du, const void* u, double visc);
 int sc_wray3(sc_ctx* ctx, void* u, double dt, double visc);
 /* classic RK4 with projected stages (no reference counterpart; configs use it).
  * Buffer placement: on a 3D context whose buffers exceed 1 GB, the first
- * RHS and the first RK4 step each time their scratch passes into up to three
- * extra allocations of the pass's output buffers (the tiled y-pass buffer;
- * the RK4 stage state and accumulator), keep the fastest and free the rest:
- * a one-time peak of up to 3 x 6 C (first RHS) or 3 x 2 x 3 C (first RK4
- * step) more device memory, C = one spectral field, bounded to leave 8 GB
- * free; results do not depend on the choice. */
+ * RHS and the first RK4 step time a pass into extra allocations of the
+ * buffers it writes (the tiled y-pass buffer: up to three more; the RK4
+ * stage state and accumulator: up to two more of each), keep the fastest and
+ * free the rest: a one-time peak of up to 3 x 6 C (first RHS) or 2 x 2 x 3 C
+ * (first RK4 step) more device memory, C = one spectral field, bounded to
+ * leave 8 GB free; results do not depend on the choice. */
 int sc_rk4(sc_ctx* ctx, void* u, double dt, double visc);
 /* one step with dt = min(cfl * propose_timestep(u), dtmax), the velocity
  * maximum taken from the first stage's own C2R (the caller's

@@ -626,11 +626,13 @@ double Ops<T>::wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int le
 // classic RK4 with a projection of every stage value
 // The RK4 update's rate likewise depends on the pages of the two buffers it
 // both reads and writes, the stage input ust and the accumulator acc (810^3
-// FP64: 11.5-11.8 ms per stage-1 launch with some allocations, 12.5-13.0 ms
-// with others; the RHS buffer makes no difference).  At the first large 3D
-// RK4 step a stage-1 update (which writes only ust and acc, both rewritten by
-// stage 0 before they are read) is timed into the pair the context has and
-// into up to three fresh pairs, and the fastest pair is kept.
+// FP64: 11.5-11.8 ms per launch on average with some allocations, 12.5-13.0
+// ms with others; replacing either buffer alone by a fresh allocation was
+// enough, the RHS buffer makes no difference).  At the first large 3D RK4
+// step a stage-1 update (which writes only ust and acc, both rewritten by
+// stage 0 before they are read) is timed for every pairing of the context's
+// ust or one of up to two fresh allocations with its acc or one of up to two
+// fresh ones, and the fastest pairing is kept.
 template <typename T>
 static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   c.state_tuned = true;
@@ -638,15 +640,16 @@ static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return;
   size_t freeb = 0, totalb = 0;
   SC_HIP(hipMemGetInfo(&freeb, &totalb));
-  const size_t spare = size_t(8) << 30, pair = 2 * (bytes + (size_t(5) << 20));
-  const int extra = int(std::min<size_t>(3, freeb > spare ? (freeb - spare) / pair : 0));
+  const size_t spare = size_t(8) << 30, one = bytes + (size_t(5) << 20);
+  const int extra = int(std::min<size_t>(2, freeb > spare ? (freeb - spare) / (2 * one) : 0));
   if (extra == 0) return;
-  std::vector<std::pair<void*, void*>> cand{{c.ust, c.acc}};
+  std::vector<void*> us{c.ust}, ac{c.acc};
   for (int i = 0; i < extra; ++i) {
     void *a = nullptr, *b = nullptr;
     c.dmalloc(&a, bytes);
     c.dmalloc(&b, bytes);
-    cand.push_back({a, b});
+    us.push_back(a);
+    ac.push_back(b);
   }
   const bool prof = c.prof_on;
   c.prof_on = false;
@@ -656,35 +659,38 @@ static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   const SpecGeom g = c.geom();
   const double* dtp = c.dtdev;  // its value does not matter: the outputs are overwritten
   float best = 0.f;
-  size_t ib = 0;
-  for (size_t i = 0; i < cand.size(); ++i) {
-    C<T>* us = reinterpret_cast<C<T>*>(cand[i].first);
-    C<T>* ac = reinterpret_cast<C<T>*>(cand[i].second);
-    for (int rep = 0; rep < 2; ++rep) {  // first touch, then timed
-      if (rep == 1) SC_HIP(hipEventRecord(e0, c.stream));
-      hipLaunchKernelGGL((k_rk4_update_lin<T, 3>), dim3(nblk(g.total)), dim3(256), 0, c.stream,
-                         const_cast<C<T>*>(u), us, ac, du, dtp, 0.5, 1, g, 0.0, (const double*)nullptr, 0);
-      SC_HIP(hipGetLastError());
+  size_t bi = 0, bj = 0;
+  bool first = true;
+  for (size_t i = 0; i < us.size(); ++i)
+    for (size_t j = 0; j < ac.size(); ++j) {
+      C<T>* s_ = reinterpret_cast<C<T>*>(us[i]);
+      C<T>* a_ = reinterpret_cast<C<T>*>(ac[j]);
+      for (int rep = 0; rep < 2; ++rep) {  // first touch, then timed
+        if (rep == 1) SC_HIP(hipEventRecord(e0, c.stream));
+        hipLaunchKernelGGL((k_rk4_update_lin<T, 3>), dim3(nblk(g.total)), dim3(256), 0, c.stream,
+                           const_cast<C<T>*>(u), s_, a_, du, dtp, 0.5, 1, g, 0.0, (const double*)nullptr, 0);
+        SC_HIP(hipGetLastError());
+      }
+      SC_HIP(hipEventRecord(e1, c.stream));
+      SC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (first || ms < best) {
+        best = ms;
+        bi = i;
+        bj = j;
+        first = false;
+      }
     }
-    SC_HIP(hipEventRecord(e1, c.stream));
-    SC_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    SC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (i == 0 || ms < best) {
-      best = ms;
-      ib = i;
-    }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c.prof_on = prof;
-  c.ust = cand[ib].first;
-  c.acc = cand[ib].second;
-  for (size_t i = 0; i < cand.size(); ++i)
-    if (i != ib) {
-      c.dfree(cand[i].first);
-      c.dfree(cand[i].second);
-    }
+  c.ust = us[bi];
+  c.acc = ac[bj];
+  for (size_t i = 0; i < us.size(); ++i)
+    if (i != bi) c.dfree(us[i]);
+  for (size_t j = 0; j < ac.size(); ++j)
+    if (j != bj) c.dfree(ac[j]);
 }
 
 template <typename T>
