@@ -465,20 +465,22 @@ const double* Ops<T>::band_coeff(Ctx& c, const CT* u) {
 // The forward y-pass writes the z-pencil-tiled buffer at a rate that depends
 // on where that buffer's pages landed: on MI355X at 810^3 FP64 the same pass
 // runs 11.5-11.9 ms per launch into some allocations and 13.5-14.1 ms into
-// others, whatever its input, and an allocation keeps its rate.  So at the
-// first RHS whose tiled buffer is large, the pass is timed into the buffer it
-// has and into up to three fresh allocations (as many as fit with 8 GB to
-// spare), and the fastest is kept; the others are freed.  The pass only
-// writes the candidate, so the results are the same whichever is kept.
-template <typename T>
-static void tune_tiled(Ctx& c, const C<T>* B) {
+// others, whatever its input, and an allocation keeps its rate (about half
+// of the allocations are slow; the inverse z-pass, which writes the same
+// buffer, varies with it, 2.3 against 2.8 ms).  So at the first RHS whose
+// tiled buffer is large, the two passes are timed into the buffer it has and
+// into up to five fresh allocations (as many as fit with 8 GB to spare), and
+// the fastest is kept; the others are freed.  The passes only write the
+// candidate, so the results are the same whichever is kept.
+template <typename T, class ZInv>
+static void tune_tiled(Ctx& c, const C<T>* B, ZInv zinv) {
   c.tiled_tuned = true;
   const size_t bytes = size_t(6) * tiled_slot<T>(c) * sizeof(C<T>);
   if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return;
   size_t freeb = 0, totalb = 0;
   SC_HIP(hipMemGetInfo(&freeb, &totalb));
   const size_t spare = size_t(8) << 30;
-  const int extra = int(std::min<size_t>(3, freeb > spare ? (freeb - spare) / (bytes + (size_t(5) << 20)) : 0));
+  const int extra = int(std::min<size_t>(5, freeb > spare ? (freeb - spare) / (bytes + (size_t(5) << 20)) : 0));
   if (extra == 0) return;
   std::vector<void*> cand{c.bufT};
   for (int i = 0; i < extra; ++i) {
@@ -494,9 +496,11 @@ static void tune_tiled(Ctx& c, const C<T>* B) {
   float best = 0.f;
   size_t ib = 0;
   for (size_t i = 0; i < cand.size(); ++i) {
-    y_fwd_div<T>(c, B, reinterpret_cast<C<T>*>(cand[i]));  // first touch
+    C<T>* ct = reinterpret_cast<C<T>*>(cand[i]);
+    y_fwd_div<T>(c, B, ct);  // first touch
     SC_HIP(hipEventRecord(e0, c.stream));
-    y_fwd_div<T>(c, B, reinterpret_cast<C<T>*>(cand[i]));
+    zinv(ct);
+    y_fwd_div<T>(c, B, ct);
     SC_HIP(hipEventRecord(e1, c.stream));
     SC_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -526,6 +530,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
   CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, c.D));
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
   const int kc = c.n / 3;
+  StridedGeom gz{};
   if (c.D == 3) {
     // the inverse z-pass writes y-pencil tiles, element (x, y, z) of a
     // component at ((tile * n + z) * n + y) * TX + x % TX (in the buffer the
@@ -534,7 +539,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT));
     CT* AT = reinterpret_cast<CT*>(c.bufT);
     const long n = c.n, TX = tile_x_rt<T>(c.n), ts = tiled_slot<T>(c);
-    StridedGeom gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
+    gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
     gz.o_tile = n * n * TX;
     gz.o_axis = n * TX;
     gz.o_other = TX;
@@ -565,7 +570,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     rp.forceA = c.forceA;
   }
   if (c.D == 3) {
-    if (!c.tiled_tuned) tune_tiled<T>(c, B);
+    if (!c.tiled_tuned) tune_tiled<T>(c, B, [&](CT* dst) { c2c_launch<T>(c, gz, 2, +1, u, dst, 3); });
     CT* BT = reinterpret_cast<CT*>(c.bufT);
     y_fwd_div<T>(c, B, BT);
     fwd_rhs_last<T>(c, BT, u, du, rp, lin);
@@ -631,8 +636,8 @@ double Ops<T>::wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int le
 // enough, the RHS buffer makes no difference).  At the first large 3D RK4
 // step a stage-1 update (which writes only ust and acc, both rewritten by
 // stage 0 before they are read) is timed for every pairing of the context's
-// ust or one of up to two fresh allocations with its acc or one of up to two
-// fresh ones, and the fastest pairing is kept.
+// ust or one of up to three fresh allocations with its acc or one of up to
+// three fresh ones, and the fastest pairing is kept.
 template <typename T>
 static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   c.state_tuned = true;
@@ -641,7 +646,7 @@ static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   size_t freeb = 0, totalb = 0;
   SC_HIP(hipMemGetInfo(&freeb, &totalb));
   const size_t spare = size_t(8) << 30, one = bytes + (size_t(5) << 20);
-  const int extra = int(std::min<size_t>(2, freeb > spare ? (freeb - spare) / (2 * one) : 0));
+  const int extra = int(std::min<size_t>(3, freeb > spare ? (freeb - spare) / (2 * one) : 0));
   if (extra == 0) return;
   std::vector<void*> us{c.ust}, ac{c.acc};
   for (int i = 0; i < extra; ++i) {
