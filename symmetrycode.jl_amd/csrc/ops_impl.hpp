@@ -462,6 +462,34 @@ const double* Ops<T>::band_coeff(Ctx& c, const CT* u) {
   return c.forceA;
 }
 
+// timing scope of the placement choices below: profiling off (their launches
+// are not the caller's), two events; restored / destroyed on every exit
+struct TuneScope {
+  Ctx& c;
+  bool prof;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  explicit TuneScope(Ctx& c_) : c(c_), prof(c_.prof_on) {
+    SC_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      throw Error(3, "hipEventCreate failed");
+    }
+    c.prof_on = false;
+  }
+  ~TuneScope() {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.prof_on = prof;
+  }
+  // ms between the two events once e1 has completed
+  float elapsed() {
+    SC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+  }
+};
+
 // The forward y-pass writes the z-pencil-tiled buffer at a rate that depends
 // on where that buffer's pages landed: on MI355X at 810^3 FP64 the same pass
 // runs 11.5-11.9 ms per launch into some allocations and 13.5-14.1 ms into
@@ -483,36 +511,32 @@ static void tune_tiled(Ctx& c, const C<T>* B, ZInv zinv) {
   const int extra = int(std::min<size_t>(5, freeb > spare ? (freeb - spare) / (bytes + (size_t(5) << 20)) : 0));
   if (extra == 0) return;
   std::vector<void*> cand{c.bufT};
-  for (int i = 0; i < extra; ++i) {
-    void* q = nullptr;
-    c.dmalloc(&q, bytes);
-    cand.push_back(q);
-  }
-  const bool prof = c.prof_on;
-  c.prof_on = false;
-  hipEvent_t e0, e1;
-  SC_HIP(hipEventCreate(&e0));
-  SC_HIP(hipEventCreate(&e1));
-  float best = 0.f;
   size_t ib = 0;
-  for (size_t i = 0; i < cand.size(); ++i) {
-    C<T>* ct = reinterpret_cast<C<T>*>(cand[i]);
-    y_fwd_div<T>(c, B, ct);  // first touch
-    SC_HIP(hipEventRecord(e0, c.stream));
-    zinv(ct);
-    y_fwd_div<T>(c, B, ct);
-    SC_HIP(hipEventRecord(e1, c.stream));
-    SC_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    SC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (i == 0 || ms < best) {
-      best = ms;
-      ib = i;
+  try {
+    for (int i = 0; i < extra; ++i) {
+      void* q = nullptr;
+      c.dmalloc(&q, bytes);
+      cand.push_back(q);
     }
+    TuneScope ts(c);
+    float best = 0.f;
+    for (size_t i = 0; i < cand.size(); ++i) {
+      C<T>* ct = reinterpret_cast<C<T>*>(cand[i]);
+      y_fwd_div<T>(c, B, ct);  // first touch
+      SC_HIP(hipEventRecord(ts.e0, c.stream));
+      zinv(ct);
+      y_fwd_div<T>(c, B, ct);
+      SC_HIP(hipEventRecord(ts.e1, c.stream));
+      const float ms = ts.elapsed();
+      if (i == 0 || ms < best) {
+        best = ms;
+        ib = i;
+      }
+    }
+  } catch (...) {
+    for (size_t i = 1; i < cand.size(); ++i) c.dfree(cand[i]);
+    throw;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  c.prof_on = prof;
   c.bufT = cand[ib];
   for (size_t i = 0; i < cand.size(); ++i)
     if (i != ib) c.dfree(cand[i]);
@@ -649,47 +673,44 @@ static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   const int extra = int(std::min<size_t>(3, freeb > spare ? (freeb - spare) / (2 * one) : 0));
   if (extra == 0) return;
   std::vector<void*> us{c.ust}, ac{c.acc};
-  for (int i = 0; i < extra; ++i) {
-    void *a = nullptr, *b = nullptr;
-    c.dmalloc(&a, bytes);
-    c.dmalloc(&b, bytes);
-    us.push_back(a);
-    ac.push_back(b);
-  }
-  const bool prof = c.prof_on;
-  c.prof_on = false;
-  hipEvent_t e0, e1;
-  SC_HIP(hipEventCreate(&e0));
-  SC_HIP(hipEventCreate(&e1));
-  const SpecGeom g = c.geom();
-  const double* dtp = c.dtdev;  // its value does not matter: the outputs are overwritten
-  float best = 0.f;
   size_t bi = 0, bj = 0;
-  bool first = true;
-  for (size_t i = 0; i < us.size(); ++i)
-    for (size_t j = 0; j < ac.size(); ++j) {
-      C<T>* s_ = reinterpret_cast<C<T>*>(us[i]);
-      C<T>* a_ = reinterpret_cast<C<T>*>(ac[j]);
-      for (int rep = 0; rep < 2; ++rep) {  // first touch, then timed
-        if (rep == 1) SC_HIP(hipEventRecord(e0, c.stream));
-        hipLaunchKernelGGL((k_rk4_update_lin<T, 3>), dim3(nblk(g.total)), dim3(256), 0, c.stream,
-                           const_cast<C<T>*>(u), s_, a_, du, dtp, 0.5, 1, g, 0.0, (const double*)nullptr, 0);
-        SC_HIP(hipGetLastError());
-      }
-      SC_HIP(hipEventRecord(e1, c.stream));
-      SC_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (first || ms < best) {
-        best = ms;
-        bi = i;
-        bj = j;
-        first = false;
-      }
+  try {
+    for (int i = 0; i < extra; ++i) {
+      void *a = nullptr, *b = nullptr;
+      c.dmalloc(&a, bytes);
+      us.push_back(a);
+      c.dmalloc(&b, bytes);
+      ac.push_back(b);
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  c.prof_on = prof;
+    TuneScope ts(c);
+    const SpecGeom g = c.geom();
+    const double* dtp = c.dtdev;  // its value does not matter: the outputs are overwritten
+    float best = 0.f;
+    bool first = true;
+    for (size_t i = 0; i < us.size(); ++i)
+      for (size_t j = 0; j < ac.size(); ++j) {
+        C<T>* s_ = reinterpret_cast<C<T>*>(us[i]);
+        C<T>* a_ = reinterpret_cast<C<T>*>(ac[j]);
+        for (int rep = 0; rep < 2; ++rep) {  // first touch, then timed
+          if (rep == 1) SC_HIP(hipEventRecord(ts.e0, c.stream));
+          hipLaunchKernelGGL((k_rk4_update_lin<T, 3>), dim3(nblk(g.total)), dim3(256), 0, c.stream,
+                             const_cast<C<T>*>(u), s_, a_, du, dtp, 0.5, 1, g, 0.0, (const double*)nullptr, 0);
+          SC_HIP(hipGetLastError());
+        }
+        SC_HIP(hipEventRecord(ts.e1, c.stream));
+        const float ms = ts.elapsed();
+        if (first || ms < best) {
+          best = ms;
+          bi = i;
+          bj = j;
+          first = false;
+        }
+      }
+  } catch (...) {
+    for (size_t i = 1; i < us.size(); ++i) c.dfree(us[i]);
+    for (size_t j = 1; j < ac.size(); ++j) c.dfree(ac[j]);
+    throw;
+  }
   c.ust = us[bi];
   c.acc = ac[bj];
   for (size_t i = 0; i < us.size(); ++i)
