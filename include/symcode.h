@@ -101,9 +101,10 @@ int sc_wray3(sc_ctx* ctx, void* u, double dt, double visc);
 /* classic RK4 with projected stages (no reference counterpart; configs use it).
  * Buffer placement: on a 3D context whose buffers exceed 1 GB, the first
  * RHS and the first RK4 step time a pass into extra allocations of the
- * buffers it writes (the tiled y-pass buffer: up to five more; the RK4
- * stage state and accumulator: up to three more of each), keep the fastest
- * and free the rest: a one-time peak of up to 5 x 6 C (first RHS) or 3 x 2 x 3 C
+ * buffers it writes (the inverse pipeline's buffer: up to three more; the
+ * tiled y-pass buffer: up to five more; the RK4 stage state and accumulator:
+ * up to three more of each), keep the fastest and free the rest: a one-time
+ * peak of up to 5 x 6 C (first RHS) or 3 x 2 x 3 C
  * (first RK4 step) more device memory, C = one spectral field, bounded to
  * leave 8 GB free; results do not depend on the choice. */
 int sc_rk4(sc_ctx* ctx, void* u, double dt, double visc);

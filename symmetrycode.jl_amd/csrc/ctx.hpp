@@ -46,7 +46,8 @@ struct Ctx {
   void* bufA = nullptr;   // D spectral comps: inverse pipeline
   void* bufB = nullptr;   // 6 spectral comps: forward pipeline / stress
   void* bufT = nullptr;   // 6 slots, z-pencil tiled: forward y-pass output
-  bool tiled_tuned = false;  // bufT chosen among candidate allocations (tune_tiled)
+  bool tiled_tuned = false;  // bufT chosen among candidate allocations (choose_placement)
+  bool inv_tuned = false;    // bufA likewise
   bool state_tuned = false;  // ust, acc chosen among candidate allocations (tune_state)
   void* scr = nullptr;    // general spectral scratch
   int scr_comp = 0;

@@ -490,27 +490,29 @@ struct TuneScope {
   }
 };
 
-// The forward y-pass writes the z-pencil-tiled buffer at a rate that depends
-// on where that buffer's pages landed: on MI355X at 810^3 FP64 the same pass
-// runs 11.5-11.9 ms per launch into some allocations and 13.5-14.1 ms into
-// others, whatever its input, and an allocation keeps its rate (about half
-// of the allocations are slow; the inverse z-pass, which writes the same
-// buffer, varies with it, 2.3 against 2.8 ms).  So at the first RHS whose
-// tiled buffer is large, the two passes are timed into the buffer it has and
-// into up to five fresh allocations (as many as fit with 8 GB to spare), and
-// the fastest is kept; the others are freed.  The passes only write the
-// candidate, so the results are the same whichever is kept.
-template <typename T, class ZInv>
-static void tune_tiled(Ctx& c, const C<T>* B, ZInv zinv) {
-  c.tiled_tuned = true;
-  const size_t bytes = size_t(6) * tiled_slot<T>(c) * sizeof(C<T>);
-  if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return;
+// Passes that write a large buffer run at a rate that depends on where its
+// pages landed, and an allocation keeps its rate.  On MI355X at 810^3 FP64
+// the forward y-pass ran 11.5-11.9 ms per launch into some allocations of the
+// z-pencil-tiled buffer and 13.5-14.1 ms into others, whatever its input
+// (the inverse z-pass, which writes the same buffer, 2.3 against 2.8 ms),
+// and the inverse y-pass 2.9 or 3.1 ms by its output buffer.  So at the
+// first RHS whose buffers are large, launch(candidate) is timed (after one
+// untimed launch) into the buffer the context has and into up to max_extra
+// fresh allocations of the same size (as many as fit with 8 GB to spare),
+// and the fastest is kept; the others are freed.  Each launch writes all of
+// its candidate, so the kept buffer holds the launch's output, and the
+// results are the same whichever is kept.  Returns false, having launched
+// nothing, when the buffer is small or no candidate fits.
+template <class Launch>
+static bool choose_placement(Ctx& c, void*& buf, size_t bytes, int max_extra, Launch launch) {
+  if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return false;
   size_t freeb = 0, totalb = 0;
   SC_HIP(hipMemGetInfo(&freeb, &totalb));
   const size_t spare = size_t(8) << 30;
-  const int extra = int(std::min<size_t>(5, freeb > spare ? (freeb - spare) / (bytes + (size_t(5) << 20)) : 0));
-  if (extra == 0) return;
-  std::vector<void*> cand{c.bufT};
+  const int extra = int(std::min<size_t>(size_t(max_extra),
+                                         freeb > spare ? (freeb - spare) / (bytes + (size_t(5) << 20)) : 0));
+  if (extra == 0) return false;
+  std::vector<void*> cand{buf};
   size_t ib = 0;
   try {
     for (int i = 0; i < extra; ++i) {
@@ -521,11 +523,9 @@ static void tune_tiled(Ctx& c, const C<T>* B, ZInv zinv) {
     TuneScope ts(c);
     float best = 0.f;
     for (size_t i = 0; i < cand.size(); ++i) {
-      C<T>* ct = reinterpret_cast<C<T>*>(cand[i]);
-      y_fwd_div<T>(c, B, ct);  // first touch
+      launch(cand[i]);  // first touch
       SC_HIP(hipEventRecord(ts.e0, c.stream));
-      zinv(ct);
-      y_fwd_div<T>(c, B, ct);
+      launch(cand[i]);
       SC_HIP(hipEventRecord(ts.e1, c.stream));
       const float ms = ts.elapsed();
       if (i == 0 || ms < best) {
@@ -537,9 +537,10 @@ static void tune_tiled(Ctx& c, const C<T>* B, ZInv zinv) {
     for (size_t i = 1; i < cand.size(); ++i) c.dfree(cand[i]);
     throw;
   }
-  c.bufT = cand[ib];
+  buf = cand[ib];
   for (size_t i = 0; i < cand.size(); ++i)
     if (i != ib) c.dfree(cand[i]);
+  return true;
 }
 
 // Right-hand side of the time steppers: the five-pass pipeline with the
@@ -574,7 +575,14 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     gy.s_axis = TX;
     gy.s_other = n * TX;
     gy.s_comp = ts;
-    c2c_launch<T>(c, gy, 1, +1, AT, A, 3);
+    auto inv_y = [&](void* dst) { c2c_launch<T>(c, gy, 1, +1, AT, reinterpret_cast<CT*>(dst), 3); };
+    bool done = false;
+    if (!c.inv_tuned) {
+      c.inv_tuned = true;
+      done = choose_placement(c, c.bufA, size_t(c.D) * c.spec_elems * c.celem(), 3, inv_y);
+      A = reinterpret_cast<CT*>(c.bufA);  // if chosen, it holds the inverse y-pass's output
+    }
+    if (!done) inv_y(A);
   } else {
     c2c_axis<T>(c, 1, +1, u, A, 2, kc + 1, false, kc, true, 1.0);
   }
@@ -594,7 +602,13 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     rp.forceA = c.forceA;
   }
   if (c.D == 3) {
-    if (!c.tiled_tuned) tune_tiled<T>(c, B, [&](CT* dst) { c2c_launch<T>(c, gz, 2, +1, u, dst, 3); });
+    if (!c.tiled_tuned) {
+      c.tiled_tuned = true;
+      choose_placement(c, c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT), 5, [&](void* dst) {
+        c2c_launch<T>(c, gz, 2, +1, u, reinterpret_cast<CT*>(dst), 3);
+        y_fwd_div<T>(c, B, reinterpret_cast<CT*>(dst));
+      });
+    }
     CT* BT = reinterpret_cast<CT*>(c.bufT);
     y_fwd_div<T>(c, B, BT);
     fwd_rhs_last<T>(c, BT, u, du, rp, lin);
