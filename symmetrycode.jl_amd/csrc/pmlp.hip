@@ -123,11 +123,16 @@ __global__ void k_a2_partials(const T* GG, long np, int ng, double* part) {
 // the live standardised log Re_Δ of fullchain: Re_Δ = Δ² sqrt(mean A2) / ν,
 // z = (log Re_Δ - μ) / σ, rounded to the field precision (append_redelta,
 // src/data.jl:689-694)
+// one wavefront: lane l adds partials l, l + 64, ..., then an xor butterfly
+// (every lane ends with the same sum, a + b = b + a exactly)
 __global__ void k_redelta_z(const double* part, int nblk, long np, double d2, double visc, double mu, double sigma,
                             int fp32, double* z) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+  const int l = threadIdx.x;
   double s = 0;
-  for (int i = 0; i < nblk; ++i) s += part[i];
+  for (int i = l; i < nblk; i += 64) s += part[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (l != 0) return;
   const double re = d2 * sqrt(s / double(np)) / visc;
   const double zz = (log(re) - mu) / sigma;
   *z = fp32 ? double(float(zz)) : zz;
