@@ -32,6 +32,9 @@ ALG_C = {
     "c2c_y_fwd": 12, "c2c_z_fwd": 12, "(k_div_visc<T, 3>)": 12,
     # RK4 update with the viscous and forcing terms: 12, 18, 18, 15 C over the stages
     "(k_rk4_update_lin<T, 3>)": 15.75,
+    # last forward pass with the RK4 update fused (the pass model's 12 C plus
+    # the update's 9 C in stages 0-2 and 6 C in stage 3): 21, 21, 21, 18 C
+    "z_fwd_rk4": 20.25,
 }
 RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates)
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at
@@ -40,9 +43,11 @@ RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates
 MEASURED_GB_810_FP64 = {"x_products": 34.15, "y_fwd_div": 51.35, "z_fwd_div": 38.87, "(k_rk4_update_lin<T, 3>)": 67.35}
 
 
-def cpu_baseline(n_sample=256, threads=None):
+def cpu_baseline(n_sample=256, threads=None, reps=3):
     """Oracle (numpy / scipy.fft) RK4 step at n_sample^3 FP64 on the host
-    cores, extrapolated to 810^3 with an N log N model."""
+    cores: one warm-up step, then the median of ``reps`` timed steps,
+    extrapolated to 810^3 with an N log N model (810^3 FP64 needs ~90 GB of
+    host buffers)."""
     import numpy as np
 
     from oracle import spectral as S
@@ -53,9 +58,13 @@ def cpu_baseline(n_sample=256, threads=None):
     u = S.randomfield(lambda k: S.peak_profile(k, 4), g, seed=0, totalenergy=0.2)
     f = lambda v: S.convectiondiffusion(v, g, 2.7e-4, forcing=(0.1, 2))
     dt = 0.3 * S.propose_timestep(u, g, 2.7e-4)
-    t0 = time.perf_counter()
-    S.rk4(f, u, dt, g)
-    t = time.perf_counter() - t0
+    u = S.rk4(f, u, dt, g)  # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        u = S.rk4(f, u, dt, g)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
     N1, N2 = 810.0**3, float(n_sample) ** 3
     scale = (N1 * math.log(N1)) / (N2 * math.log(N2))
     return {
@@ -63,8 +72,9 @@ def cpu_baseline(n_sample=256, threads=None):
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one RK4 step of the CPU oracle at {n_sample}^3 FP64 ({t:.2f} s, scipy.fft "
-                  f"workers={threads}), extrapolated to 810^3 by N log N (x{scale:.1f})",
+        "sample": f"RK4 steps of the CPU oracle at {n_sample}^3 FP64 (scipy.fft workers={threads}): one "
+                  f"warm-up step, then the median of {reps} ({', '.join(f'{x:.2f}' for x in ts)} s), "
+                  f"extrapolated to 810^3 by N log N (x{scale:.1f})",
     }
 
 
@@ -171,6 +181,7 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     t_grad = (time.perf_counter() - t0) / grad_steps
     rollouts = {}
     ref_norm = sum(sc.spectraldot(r, r, g32).real for r in refs) / (2 * nsteps)
+    assert ref_norm > 0, "the filtered-DNS reference trajectory is zero"
     for name, m in (("learned", model), ("dynsmag", sc.create_dynamic_smagorinsky(delta, g32)), ("none", None)):
         T.rollout_loss(u0, refs, dt, g32, visc, m)
         torch.cuda.synchronize()
@@ -179,7 +190,7 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
         Lm = T.rollout_loss(u0, refs, dt, g32, visc, m, states)
         torch.cuda.synchronize()
         rollouts[name] = {"rollouts_per_s": 1.0 / (time.perf_counter() - t0),
-                          "rel_traj_mse": Lm / ref_norm if ref_norm > 0 else None,
+                          "rel_traj_mse": Lm / ref_norm,
                           "spectrum_err_final": spectrum_error(states[-1], refs[-1], g32),
                           "e_post_final": T.e_post(states[-1], refs[-1], g32)}
     les_grad = {
@@ -198,7 +209,7 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=810)
     ap.add_argument("--prec", type=int, default=64, choices=(32, 64))
@@ -225,11 +236,11 @@ def main():
     sc.set_band_forcing(g, 0.1, 2)
     L = sc.lib()
 
-    def step():
-        return sc.rk4_cfl(u, g, nu, cfl)
-
-    for _ in range(args.warmup):
-        step()
+    # K steps = one call of the multi-step entry point (the state stays in
+    # the z-pencil-tiled layout between steps; the conversions of u into and
+    # out of it are inside the timed region)
+    if args.warmup > 0:
+        sc.rk4_steps(u, g, nu, args.warmup, cfl=cfl)
     if args.copy_calib_gb > 0:
         nb = int(args.copy_calib_gb * 1e9) // 16
         a = torch.empty(nb, dtype=torch.complex128, device="cuda")
@@ -244,8 +255,7 @@ def main():
     torch.cuda.synchronize()
     e0.record(stream)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    dts = sc.rk4_steps(u, g, nu, args.steps, cfl=cfl)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -296,6 +306,7 @@ def main():
         },
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1][1])},
         "energy_after": e,
+        "dt_per_step": [float(d) for d in dts],
         "wall_s": wall,
     }
     if not args.no_closure:

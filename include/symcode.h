@@ -113,6 +113,14 @@ int sc_rk4(sc_ctx* ctx, void* u, double dt, double visc);
  * `Δt = min(cfl * propose_timestep(...), tstop - t)`, src/data.jl:61-62,
  * src/les.jl:209-210); *dt receives the step taken */
 int sc_rk4_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, double* dt);
+/* nsteps steps of sc_rk4_cfl (cfl > 0: dt = min(cfl * propose_timestep(u),
+ * dtmax) per step) or of sc_rk4 (cfl <= 0: dt = dtmax) in one call, the
+ * reference's march loop without the per-step host work (src/data.jl:57-66).
+ * 3D: the Runge-Kutta state stays in z-pencil tiles between the passes
+ * (converted from and back to the device layout of u once per call) and the
+ * stage update, viscous term, forcing and projection are fused into the last
+ * forward pass.  dts (nullable) receives the nsteps step sizes. */
+int sc_rk4_steps(sc_ctx* ctx, void* u, int nsteps, double cfl, double visc, double dtmax, double* dts);
 int sc_wray3_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, double* dt);
 int sc_les_wray3_cfl(sc_ctx* ctx, void* u, double cfl, double visc, double dtmax, int closure, double p1,
                      double p2, double* dt);
@@ -133,6 +141,10 @@ int sc_dissipation(sc_ctx* ctx, const void* u, double visc, double* eps);  /* ge
  * uses it (src/les.jl:97) */
 int sc_norm2(sc_ctx* ctx, const void* u, int ncomp, double* s);
 int sc_palinstrophy(sc_ctx* ctx, const void* u, double visc, double* p);   /* get_palinstrophy!, :813-827 */
+/* filter_reynolds(ubar, g, visc, Delta) (:768-774): Re_Delta = Delta^2
+ * sqrt(sum_ij getenergy(2/3 (i k_j ubar_i))) / visc, the gradient truncated
+ * to the 2/3 box whatever ubar holds */
+int sc_filter_reynolds(sc_ctx* ctx, const void* ubar, double visc, double delta, double* re);
 /* spectrum (:664-699): s[i-1] = shell i energy, i = 1..nshell */
 int sc_spectrum(sc_ctx* ctx, const void* u, int ncomp, int nshell, double* s);
 /* energy of integer shells 0..nshell-1 (getshells, :1080-1131) and their rescaling */

@@ -114,12 +114,14 @@ def dynamic_coefficient(sum_ml, sum_mm):
     return 0.0 if sum_mm == 0 else max(-sum_ml / sum_mm, 0.0)
 
 
-def dynamic_smagorinsky(u, delta, g: Grid, return_coeff=False):
+def dynamic_smagorinsky(u, delta, g: Grid, return_coeff=False, return_parts=False):
     """create_dynamic_smagorinsky's model (src/closures.jl:188-271) with the
     INTENDED test-filtered strain S̃ = to_phys(2/3 G̃ 2/3 Ŝ).  The reference
     computes S̃ from a buffer its own preceding C2R has destroyed
     (src/closures.jl:234-239); that value depends on FFT-library scratch and
-    is not reproducible (parity unpinned against the reference's output)."""
+    is not reproducible (parity unpinned against the reference's output).
+    ``return_parts``: also return the intermediates L, m1 (dealiased), m2, M
+    and c in a dict."""
     D = g.D
     dt = 2 * delta
     dd = np.sqrt(dt**2 + delta**2)
@@ -142,6 +144,8 @@ def dynamic_smagorinsky(u, delta, g: Grid, return_coeff=False):
     ml, mm = ml_mm(M, L, D)
     c = dynamic_coefficient(float(np.sum(ml)), float(np.sum(mm)))
     tau = c * m1
+    if return_parts:
+        return tau, dict(L=L, m1=m1, m2=m2, M=M, c=c)
     return (tau, c) if return_coeff else tau
 
 

@@ -3,7 +3,7 @@ src/filtering.jl and src/data.jl:614-642).  TEST INFRASTRUCTURE ONLY -- see
 ``oracle/__init__.py``."""
 import numpy as np
 
-from .spectral import Grid, nonlinearity, twothirds
+from .spectral import Grid, nonlinearity, twothirds, vectorgradient
 
 
 def cutoff_index(nbar, n, i, is1):
@@ -67,3 +67,18 @@ def sfs(u, g_dns: Grid, g_les: Grid, delta):
     s2 = nonlinearity(ubar, g_les)
     tau = np.stack([twothirds(s1[i] - s2[i], g_les) for i in range(g_dns.tensordim)])
     return ubar, make_tracefree(tau, g_les.D)
+
+
+def getenergy(u, g: Grid):
+    """getenergy (src/solver.jl:701): Σ|û|² over the RFFT array with the
+    interior x-modes counted twice."""
+    return float(np.sum(g.rfft_weight() * np.abs(u) ** 2))
+
+
+def filter_reynolds(ubar, g: Grid, visc, delta):
+    """filter_reynolds (src/solver.jl:768-774): Re_Δ = Δ² √(Σ_ij
+    getenergy(2/3(i k_j ū_i))) / ν.  The gradient is truncated to the 2/3
+    box (and loses k = 0) whatever ū holds."""
+    A, _ = vectorgradient(ubar, g)
+    a2 = sum(getenergy(twothirds(a, g), g) for a in A)
+    return delta**2 * np.sqrt(a2) / visc

@@ -1,4 +1,5 @@
 // C ABI of libsymcode_hip.so (declared in include/symcode.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -96,6 +97,7 @@ int sc_destroy(sc_ctx* c) {
   for (void* b : c->ws) c->dfree(b);
   for (void* b : c->adjw) c->dfree(b);
   if (c->bandadj) (void)hipFree(c->bandadj);
+  if (c->dtlog) (void)hipFree(c->dtlog);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -155,6 +157,30 @@ int sc_profile_report(sc_ctx* c, char* buf, int buflen) {
   });
 }
 
+// Host <-> device copies go through the context's pinned staging buffer in
+// chunks of at most 32 MB: the runtime is never handed pageable memory, so
+// every transfer is a DMA ordered on the context's stream after the work
+// queued before it (a torch allocation's zero fill, a kernel), whatever path
+// the runtime takes for pageable copies.  rows x width bytes, pitches in bytes.
+static void staged_2d(sc_ctx* c, char* dst, size_t dpitch, const char* src, size_t spitch, size_t width,
+                      size_t rows, bool h2d) {
+  const size_t chunk = size_t(32) << 20;
+  const size_t R = std::max<size_t>(1, chunk / width);
+  char* pin = reinterpret_cast<char*>(c->hostbuf(long((std::min(R, rows) * width + 7) / 8)));
+  for (size_t r0 = 0; r0 < rows; r0 += R) {
+    const size_t nr = std::min(R, rows - r0);
+    if (h2d) {
+      for (size_t r = 0; r < nr; ++r) std::memcpy(pin + r * width, src + (r0 + r) * spitch, width);
+      SC_HIP(hipMemcpy2DAsync(dst + r0 * dpitch, dpitch, pin, width, width, nr, hipMemcpyHostToDevice, c->stream));
+      SC_HIP(hipStreamSynchronize(c->stream));
+    } else {
+      SC_HIP(hipMemcpy2DAsync(pin, width, src + r0 * spitch, spitch, width, nr, hipMemcpyDeviceToHost, c->stream));
+      SC_HIP(hipStreamSynchronize(c->stream));
+      for (size_t r = 0; r < nr; ++r) std::memcpy(dst + (r0 + r) * dpitch, pin + r * width, width);
+    }
+  }
+}
+
 int sc_upload_spectral(sc_ctx* c, void* dev, const void* host, int ncomp) {
   return guard(c, [&] {
     need(dev && host && ncomp > 0, "null pointer or ncomp");
@@ -163,7 +189,7 @@ int sc_upload_spectral(sc_ctx* c, void* dev, const void* host, int ncomp) {
     for (int k = 0; k < ncomp; ++k) {
       char* d = (char*)dev + size_t(k) * c->spec_elems * e;
       const char* h = (const char*)host + size_t(k) * c->nh * rows * e;
-      SC_HIP(hipMemcpy2DAsync(d, c->nhp * e, h, c->nh * e, c->nh * e, rows, hipMemcpyHostToDevice, c->stream));
+      staged_2d(c, d, c->nhp * e, h, c->nh * e, c->nh * e, rows, true);
     }
   });
 }
@@ -175,22 +201,22 @@ int sc_download_spectral(sc_ctx* c, void* host, const void* dev, int ncomp) {
     for (int k = 0; k < ncomp; ++k) {
       const char* d = (const char*)dev + size_t(k) * c->spec_elems * e;
       char* h = (char*)host + size_t(k) * c->nh * rows * e;
-      SC_HIP(hipMemcpy2DAsync(h, c->nh * e, d, c->nhp * e, c->nh * e, rows, hipMemcpyDeviceToHost, c->stream));
+      staged_2d(c, h, c->nh * e, d, c->nhp * e, c->nh * e, rows, false);
     }
-    SC_HIP(hipStreamSynchronize(c->stream));
   });
 }
 int sc_upload_physical(sc_ctx* c, void* dev, const void* host, int ncomp) {
   return guard(c, [&] {
     need(dev && host && ncomp > 0, "null pointer or ncomp");
-    SC_HIP(hipMemcpyAsync(dev, host, size_t(ncomp) * c->phys_elems * c->relem(), hipMemcpyHostToDevice, c->stream));
+    const size_t row = size_t(c->n) * c->relem(), rows = size_t(ncomp) * c->phys_elems / c->n;
+    staged_2d(c, (char*)dev, row, (const char*)host, row, row, rows, true);
   });
 }
 int sc_download_physical(sc_ctx* c, void* host, const void* dev, int ncomp) {
   return guard(c, [&] {
     need(dev && host && ncomp > 0, "null pointer or ncomp");
-    SC_HIP(hipMemcpyAsync(host, dev, size_t(ncomp) * c->phys_elems * c->relem(), hipMemcpyDeviceToHost, c->stream));
-    SC_HIP(hipStreamSynchronize(c->stream));
+    const size_t row = size_t(c->n) * c->relem(), rows = size_t(ncomp) * c->phys_elems / c->n;
+    staged_2d(c, (char*)host, row, (const char*)dev, row, row, rows, false);
   });
 }
 
@@ -278,6 +304,13 @@ int sc_rk4_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, double* 
     if (dt) *dt = d;
   });
 }
+int sc_rk4_steps(sc_ctx* c, void* u, int nsteps, double cfl, double nu, double dtmax, double* dts) {
+  return guard(c, [&] {
+    need_fft(c);
+    need(nsteps >= 0 && dtmax > 0, "nsteps must be >= 0 and dtmax positive");
+    TYPED(c, sc::Ops<T>::rk4_steps(*c, (CT*)u, nsteps, cfl, nu, dtmax, dts));
+  });
+}
 int sc_wray3_cfl(sc_ctx* c, void* u, double cfl, double nu, double dtmax, double* dt) {
   return guard(c, [&] {
     need_fft(c);
@@ -349,6 +382,14 @@ int sc_norm2(sc_ctx* c, const void* u, int ncomp, double* s) {
 }
 int sc_dissipation(sc_ctx* c, const void* u, double nu, double* eps) {
   return guard(c, [&] { TYPED(c, *eps = sc::Ops<T>::sum_modes(*c, (const CT*)u, c->D, 1, nu)); });
+}
+int sc_filter_reynolds(sc_ctx* c, const void* ubar, double visc, double delta, double* re) {
+  return guard(c, [&] {
+    need(visc > 0, "visc must be positive");
+    double a2 = 0;
+    TYPED(c, a2 = sc::Ops<T>::sum_modes(*c, (const CT*)ubar, c->D, 4, 1.0));
+    *re = delta * delta * std::sqrt(a2) / visc;
+  });
 }
 int sc_palinstrophy(sc_ctx* c, const void* u, double nu, double* p) {
   return guard(c, [&] { TYPED(c, *p = sc::Ops<T>::sum_modes(*c, (const CT*)u, c->D, 2, nu)); });

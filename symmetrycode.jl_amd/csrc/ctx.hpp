@@ -59,6 +59,8 @@ struct Ctx {
   unsigned long long* vmax = nullptr;
   double* forceA = nullptr;
   double* dtdev = nullptr;  // time step of the current step (device)
+  double* dtlog = nullptr;  // step sizes of a multi-step call (device)
+  long dtlog_n = 0;
   void* ws[16] = {};       // named workspaces (closures, LES, adjoint RHS)
   size_t ws_bytes[16] = {};
   void* adjw[8] = {};      // adjoint step workspaces

@@ -201,6 +201,17 @@ template <int N, int TPP_ = Plan<N>::tpp, class PL = Plan<N>> struct PlanInfo {
 // (WAVE) only the calling wavefront when one wavefront owns the pencil.  LDS
 // instructions of one wavefront execute in order, so a wavefront-scope fence
 // that keeps the compiler from reordering them is sufficient there.
+// Twiddles exp(-2 pi i k / N), k < N, from a table of the first N/2
+// (w^(k + N/2) = -w^k, N even): half the LDS of a full table.
+template <typename T, int N> struct HalfTwiddles {
+  const C<T>* t;
+  DEV C<T> operator[](int k) const {
+    if (k < N / 2) return t[k];
+    const C<T> w = t[k - N / 2];
+    return mk<T>(-w.x, -w.y);
+  }
+};
+
 template <bool WAVE> DEV void xsync() {
   if constexpr (WAVE) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -277,7 +288,9 @@ template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp, bool WAVE = false
     }
   }
 
-  template <int s> static DEV void compute(C<T>* v, int p, const C<T>* __restrict__ tw) {
+  // tw: the forward twiddles exp(-2 pi i k / N), k < N, as a pointer or an
+  // accessor with operator[] (HalfTwiddles)
+  template <int s, class TW> static DEV void compute(C<T>* v, int p, const TW& tw) {
     constexpr int R = PI::R(s);
     constexpr int Ns = PI::Ns(s);
     constexpr int NB = PI::NB(s);
@@ -320,7 +333,7 @@ template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp, bool WAVE = false
       }
   }
 
-  template <int s> static DEV void run_from(C<T>* v, int p, C<T>* lds, int LS, const C<T>* tw) {
+  template <int s, class TW> static DEV void run_from(C<T>* v, int p, C<T>* lds, int LS, const TW& tw) {
     compute<s>(v, p, tw);
     if constexpr (s + 1 < NS) {
       xsync<WAVE>();  // previous users of lds are done
@@ -331,7 +344,9 @@ template <typename T, int N, int SGN, int TPP_ = Plan<N>::tpp, bool WAVE = false
     }
   }
   // Run all stages.  All threads of the block must call this (barriers).
-  static DEV void run(C<T>* v, int p, C<T>* lds, int LS, const C<T>* tw) { run_from<0>(v, p, lds, LS, tw); }
+  template <class TW> static DEV void run(C<T>* v, int p, C<T>* lds, int LS, const TW& tw) {
+    run_from<0>(v, p, lds, LS, tw);
+  }
 
   // Move the final-stage outputs (natural order) to the stage-0 input
   // layout of another transform of the same length, through LDS.

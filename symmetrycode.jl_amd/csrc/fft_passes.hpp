@@ -19,6 +19,32 @@ DEV int xcd_remap(int b, int nb) {
   return (b & 7) * q + (b >> 3);
 }
 
+// Logical block b -> (tile, other index o).  pair = 0: tile fastest.
+// pair = 1: adjacent other indices side by side (b = 2 (o/2 ntile + tile) +
+// o%2; an odd last index runs alone at the end).  A pass whose output is
+// tiled with the other index next to the x tile (element (.., o, x%TX), a
+// 64-byte run per o) then has the two blocks that write the halves of one
+// 128-byte line running together, so the line is completed in L2 instead of
+// reaching HBM as two partial writes far apart in time.
+#ifndef SC_PAIR_OTHER
+#define SC_PAIR_OTHER 1
+#endif
+DEV void block_tile_other(int b, int ntile, int nother, int pair, int& tile, int& o) {
+  if (!(SC_PAIR_OTHER && pair)) {
+    tile = b % ntile;
+    o = b / ntile;
+    return;
+  }
+  const int full = (nother / 2) * 2 * ntile;
+  if (b < full) {
+    tile = (b >> 1) % ntile;
+    o = 2 * ((b >> 1) / ntile) + (b & 1);
+  } else {
+    tile = b - full;
+    o = nother - 1;
+  }
+}
+
 // pencils per block along x for the strided passes: a 128-byte row segment,
 // halved until the LDS tile fits in 64 KiB
 #ifndef SC_TILE_BYTES
@@ -51,6 +77,7 @@ struct StridedGeom {
   int mask23;         // apply the 2/3 box mask (and zero mode) on load
   int kcut;           // n / 3, for mask23
   double scale;       // output scale
+  int pair = 0;       // block order of block_tile_other (1: the output is tiled along the other axis)
 };
 
 // minimum waves per SIMD requested from the register allocator (0 = none)
@@ -92,9 +119,8 @@ k_c2c_strided(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ t
   for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
   __syncthreads();
   const int nb = g.ntile * g.nother;
-  const int b = xcd_remap(blockIdx.x, nb);
-  const int tile = b % g.ntile;
-  const int o = b / g.ntile;
+  int tile, o;
+  block_tile_other(xcd_remap(blockIdx.x, nb), g.ntile, g.nother, g.pair, tile, o);
   const int oi = o < g.other_lo ? o : o + g.other_gap;
   const int t = threadIdx.x;
   const int xl = t % TX, p = t / TX;

@@ -37,6 +37,7 @@ struct Ops {
   static double rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_closure = -1, double p1 = 0,
                         double p2 = 0);
   static double read_dt(Ctx& c);
+  static void rk4_steps(Ctx& c, CT* u, int nsteps, double cfl, double nu, double dtmax, double* dts);
   // adjoint
   static void lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double b, int ncomp);
   // closure < 0: the DNS right-hand side; 0 or 4: les_rhs with that closure

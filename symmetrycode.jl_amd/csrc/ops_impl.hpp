@@ -457,7 +457,7 @@ const double* Ops<T>::band_coeff(Ctx& c, const CT* u) {
   if (c.force_kind != 1) return nullptr;
   if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
   hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, u, c.D, c.geom(), c.kband, c.injection,
-                     c.forceA);
+                     c.forceA, 0);
   SC_HIP(hipGetLastError());
   return c.forceA;
 }
@@ -569,6 +569,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     gz.o_axis = n * TX;
     gz.o_other = TX;
     gz.o_comp = ts;
+    gz.pair = 1;
     c2c_launch<T>(c, gz, 2, +1, u, AT, 3);
     StridedGeom gy = axis_geom<T>(c, 1, kc + 1, false, kc, false, 1.0);
     gy.s_tile = n * n * TX;
@@ -596,7 +597,7 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
     c.prof_begin("band_coeff");
     hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, u, c.D, c.geom(), c.kband, c.injection,
-                       c.forceA);
+                       c.forceA, 0);
     c.prof_end();
     SC_HIP(hipGetLastError());
     rp.forceA = c.forceA;
@@ -761,6 +762,162 @@ double Ops<T>::rk4_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int les_
     else SPEC_LAUNCH((k_rk4_update<T, 2>), u, ust, acc, du, (const double*)dtp, frac[s], s, g_);
   }
   return read_dt(c);
+}
+
+// nsteps RK4 steps with the Runge-Kutta state held in z-pencil tiles (3D):
+// u is converted to the tiled layout once on entry and back on exit, and each
+// stage runs the inverse z-pass (reading the tiled stage input), the inverse
+// y-pass, the x-pass, the forward y-pass and k_z_fwd_rk4, the last forward
+// pass fused with the viscous and forcing terms, the RK4 accumulation and the
+// projection.  The stage arithmetic is that of rk4_dev.  cfl > 0: every step
+// takes min(cfl * propose_timestep, dtmax) from its first stage's velocity
+// maximum; otherwise dt = dtmax.  dts (host, nullable) receives the steps.
+template <typename T>
+static void ztile_convert(Ctx& c, const C<T>* src, C<T>* dst, int ncomp, bool to_tiles) {
+  const int n = c.n, TX = tile_x_rt<T>(n);
+  const int ntile = (c.nh + TX - 1) / TX;
+  const unsigned nblocks = unsigned(ntile) * unsigned((n + 7) / 8) * unsigned((n + 31) / 32);
+  const long ts = tiled_slot<T>(c);
+  c.prof_begin(to_tiles ? "ztile_in" : "ztile_out");
+#define SC_CONV(TXV)                                                                                          \
+  if (TX == TXV) {                                                                                            \
+    if (to_tiles)                                                                                             \
+      hipLaunchKernelGGL((k_ztile_convert<T, TXV, true>), dim3(nblocks, unsigned(ncomp)), dim3(256), 0, c.stream, \
+                         src, dst, n, c.nhp, c.spec_elems, ts);                                               \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_ztile_convert<T, TXV, false>), dim3(nblocks, unsigned(ncomp)), dim3(256), 0,       \
+                         c.stream, src, dst, n, c.nhp, c.spec_elems, ts);                                     \
+  }
+  SC_CONV(2) SC_CONV(4) SC_CONV(8) SC_CONV(16)
+#undef SC_CONV
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void z_fwd_rk4(Ctx& c, const C<T>* BT, C<T>* U, C<T>* US, C<T>* AC, RhsParams rp, const double* dtp,
+                      double frac, int stage) {
+  StridedGeom g = full_geom<T>(c, 2);
+  dim3 grid(unsigned(g.ntile) * unsigned(g.nother));
+  const C<T>* tw = reinterpret_cast<const C<T>*>(c.tw_n);
+  c.prof_begin("z_fwd_rk4");
+  switch (c.n) {
+#define X(NN)                                                                                                  \
+  case NN:                                                                                                     \
+    if constexpr (z_rk4_fits<T, NN>())                                                                         \
+      hipLaunchKernelGGL((k_z_fwd_rk4<T, NN>), grid, dim3(wide_threads<T, NN>()), 0, c.stream, BT,             \
+                         tiled_slot<T>(c), U, US, AC, tw, g, rp, dtp, frac, stage);                             \
+    else                                                                                                       \
+      throw Error(2, "no fused RK4 stage at this size");                                                       \
+    break;
+    SC_FOR_N(X)
+#undef X
+    default: throw Error(2, "unsupported grid size");
+  }
+  c.prof_end();
+  SC_HIP(hipGetLastError());
+}
+
+template <typename T>
+static bool z_rk4_fits_rt(int n) {
+  switch (n) {
+#define X(NN) case NN: return z_rk4_fits<T, NN>();
+    SC_FOR_N(X)
+#undef X
+  }
+  return false;
+}
+
+template <typename T>
+void Ops<T>::rk4_steps(Ctx& c, CT* u, int nsteps, double cfl, double nu, double dtmax, double* dts) {
+  if (c.D != 3 || !z_rk4_fits_rt<T>(c.n)) {
+    for (int i = 0; i < nsteps; ++i) {
+      const double d = rk4_dev(c, u, dtmax, nu, cfl);
+      if (dts) dts[i] = d;
+    }
+    return;
+  }
+  const int n = c.n, kc = n / 3;
+  const long TX = tile_x_rt<T>(n), ts = tiled_slot<T>(c);
+  CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, 3));
+  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, 6));
+  if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * ts * sizeof(CT));
+  CT* U = reinterpret_cast<CT*>(c.ensure_spec(c.du, 3));   // u0, tiled
+  CT* US = reinterpret_cast<CT*>(c.ensure_spec(c.ust, 3)); // stage value, tiled
+  CT* AC = reinterpret_cast<CT*>(c.ensure_spec(c.acc, 3)); // accumulator, tiled
+  double* dtp = dt_slot<T>(c);
+  // the placement choice of the pipeline's buffers happens at the first RHS
+  // (its output goes to the stage buffer, which stage 0 overwrites)
+  if (!c.inv_tuned || !c.tiled_tuned) {
+    rhs_fused(c, US, u, nu, nullptr, false);
+    A = reinterpret_cast<CT*>(c.bufA);
+  }
+  if (c.dtlog_n < nsteps) {
+    if (c.dtlog) SC_HIP(hipFree(c.dtlog));
+    c.dtlog = nullptr;
+    SC_HIP(hipMalloc(&c.dtlog, size_t(nsteps) * sizeof(double)));
+    c.dtlog_n = nsteps;
+  }
+  double* dlog = c.dtlog;
+  ztile_convert<T>(c, u, U, 3, true);
+  StridedGeom gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
+  gz.s_tile = n * n * TX;
+  gz.s_axis = TX;
+  gz.s_other = n * TX;
+  gz.s_comp = ts;
+  gz.o_tile = n * n * TX;
+  gz.o_axis = n * TX;
+  gz.o_other = TX;
+  gz.o_comp = ts;
+  gz.pair = 1;
+  StridedGeom gy = axis_geom<T>(c, 1, kc + 1, false, kc, false, 1.0);
+  gy.s_tile = n * n * TX;
+  gy.s_axis = TX;
+  gy.s_other = n * TX;
+  gy.s_comp = ts;
+  SpecGeom gt = c.geom();
+  gt.s_comp = ts;
+  RhsParams rp;
+  rp.kf = c.geom().kf;
+  rp.nu = nu;
+  rp.forceA = nullptr;
+  rp.kband2 = c.kband * c.kband;
+  if (c.force_kind == 1) {
+    if (!c.forceA) SC_HIP(hipMalloc(&c.forceA, sizeof(double)));
+    rp.forceA = c.forceA;
+  }
+  CT* AT = reinterpret_cast<CT*>(c.bufT);
+  const double frac[4] = {0.5, 0.5, 1.0, 1.0 / 6.0};
+  const double fac = 1.0 / std::pow(double(n), 3);
+  for (int step = 0; step < nsteps; ++step) {
+    if (cfl > 0) zero_vmax<T>(c);
+    else hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(1), 0, c.stream, dtp, dtmax);
+    for (int s = 0; s < 4; ++s) {
+      const CT* sin = s == 0 ? U : US;
+      c2c_launch<T>(c, gz, 2, +1, sin, AT, 3);
+      c2c_launch<T>(c, gy, 1, +1, AT, A, 3);
+      x_products<T>(c, B, A, kc, fac, (cfl > 0 && s == 0) ? c.vmax : nullptr);
+      if (cfl > 0 && s == 0) cfl_dt<T>(c, cfl, nu, dtmax);
+      if (c.force_kind == 1) {
+        c.prof_begin("band_coeff");
+        hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, sin, 3, gt, c.kband, c.injection,
+                           c.forceA, int(TX));
+        c.prof_end();
+        SC_HIP(hipGetLastError());
+      }
+      y_fwd_div<T>(c, B, AT);
+      z_fwd_rk4<T>(c, AT, U, US, AC, rp, dtp, frac[s], s);
+    }
+    hipLaunchKernelGGL(k_copy_f64, dim3(1), dim3(1), 0, c.stream, dlog + step, (const double*)dtp);
+    SC_HIP(hipGetLastError());
+  }
+  ztile_convert<T>(c, U, u, 3, false);
+  if (dts && nsteps > 0) {
+    double* h = c.hostbuf(nsteps);
+    SC_HIP(hipMemcpyAsync(h, dlog, size_t(nsteps) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipStreamSynchronize(c.stream));
+    std::memcpy(dts, h, size_t(nsteps) * sizeof(double));
+  }
 }
 
 template <typename T>

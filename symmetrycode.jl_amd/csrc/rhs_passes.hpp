@@ -9,7 +9,9 @@
 // projection of every Runge-Kutta stage removes exactly; the stepper path
 // therefore omits it (the stand-alone convectiondiffusion keeps it).
 #pragma once
+#include <type_traits>
 #include "fft_passes.hpp"
+#include "solver_kernels.hpp"
 
 namespace sc {
 
@@ -63,9 +65,8 @@ k_y_fwd_div(const C<T>* buf, C<T>* outT, long s_slot, const C<T>* __restrict__ t
   __shared__ C<T> twl[N];
   stage_twiddles<T>(twl, tw, N);
   const int nb = g.ntile * g.nother;
-  const int b = xcd_remap(blockIdx.x, nb);
-  const int tile = b % g.ntile;
-  const int oi = b / g.ntile;  // z index
+  int tile, oi;  // oi: z index; z pairs side by side (the output is z-tiled)
+  block_tile_other(xcd_remap(blockIdx.x, nb), g.ntile, g.nother, 1, tile, oi);
   const int t = threadIdx.x;
   const int xl = t % TX, p = t / TX;
   const int x = tile * TX + xl;
@@ -196,6 +197,188 @@ k_z_fwd_rhs(const C<T>* bufT, long s_slot, const C<T>* __restrict__ u, C<T>* du,
   }
 }
 
+// LDS of k_z_fwd_rk4 (the transform exchange, half a twiddle table and two
+// waiting components): the fused stage exists where it fits one block
+template <typename T, int N> constexpr bool z_rk4_fits() {
+  return (3L * tile_x<T, N>() * N + N / 2) * long(sizeof(C<T>)) <= 163840;
+}
+
+// RK4 stage as the last forward pass, for a Runge-Kutta state kept in
+// z-pencil tiles like the y-pass output (element (x, y, z) of a component at
+// ((x/TX n + y) n + z) TX + x%TX, component stride s_slot, so one block's
+// share of every state component is one contiguous run).  For each component
+// i the divergence -i (A_i + kz s_iz) of the six transformed slots gives
+// k_i = div_i - nu |k|^2 us_i (+ A_f us_i on the band 0 < |k|^2 <= kband2),
+// us the stage input (u0 at stage 0, ust after), and then the update of
+// k_rk4_update_lin: stage 0 acc = k, stages 1-2 acc += 2k, the stage value
+// v = u0 + c dt k (stage 3: u0 + dt/6 (acc + k)).  The first two components'
+// stage values wait in LDS for the third (each thread reads back its own
+// entries); then the three are projected with the FP64 |k|^2 division and
+// written to ust (stage 3: u0).  kz s_iz of components 0 and 1 is parked in
+// the LDS slot that the component's stage value overwrites afterwards; the
+// third component is peeled off the loop and holds it in registers, which the
+// absent prefetch leaves free.  LDS: the transform exchange, half a twiddle
+// table and the two waiting components (158 KB at n = 810 FP64).  Each state
+// element is read and written by the one thread that owns its (x, z)
+// position, so the update runs in place.
+template <typename T, int N>
+__global__ void SC_LB_WIDE((wide_threads<T, N>()))
+k_z_fwd_rk4(const C<T>* bufT, long s_slot, C<T>* u0, C<T>* ust, C<T>* acc, const C<T>* __restrict__ tw,
+            StridedGeom g, RhsParams rp, const double* dtp, double frac, int stage) {
+  constexpr int TX = tile_x<T, N>();
+  constexpr int TPP = tpp_wide<N>();
+  using PE = Pencil<T, N, -1, TPP>;
+  constexpr int E = PE::EL;
+  static_assert(z_rk4_fits<T, N>(), "k_z_fwd_rk4: LDS");
+  __shared__ C<T> lds[TX * N];
+  __shared__ C<T> twh[N / 2];
+  __shared__ C<T> vw[2][TX * N];  // kz s_iz, then the stage values of components 0 and 1 (natural order)
+  for (int i = threadIdx.x; i < N / 2; i += blockDim.x) twh[i] = tw[i];
+  __syncthreads();
+  const HalfTwiddles<T, N> twl{twh};
+  const int nb = g.ntile * g.nother;
+  const int b = xcd_remap(blockIdx.x, nb);
+  const int tile = b % g.ntile;
+  const int oi = b / g.ntile;  // y index
+  const int t = threadIdx.x;
+  const int xl = t % TX, p = t / TX;
+  const int x = tile * TX + xl;
+  const bool xok = x < g.xs;
+  const long toff = (long(tile) * N + oi) * N * TX + xl;  // (tile, y = oi, z = 0) of a slot or component
+  const T kf = T(rp.kf);
+  const int kyi = freq(oi, g.n_other);
+  const T kx = kf * T(x), ky = kf * T(kyi);
+  const T nu = T(rp.nu);
+  const T cdt = T(*dtp * frac);
+  T fA = T(0);
+  if (rp.forceA) fA = T(*rp.forceA);
+  SpecGeom pg{};
+  pg.kf = rp.kf;
+  C<T> v[PE::MAXE];
+  C<T> nx[PE::E0];
+  C<T> kzs[E];  // kz * FFT(s_zz) of the third component
+  const C<T>* tsrc = bufT + toff;
+  auto fetch = [&](int slot) __attribute__((always_inline)) {
+    const C<T>* src = tsrc + long(slot) * s_slot;
+    PE::load(nx, p, [&](int j) { return xok ? src[j * TX] : mk<T>(T(0), T(0)); });
+  };
+  auto fft = [&](int next) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < PE::E0; ++e) v[e] = nx[e];
+    if (next >= 0) fetch(next);
+    PE::run(v, p, lds + xl, TX, twl);
+  };
+  // slots of A_i (A_x, A_y, A_z) and of s_iz (s_zx, s_yz, s_zz)
+  auto SA = [](int i) { return i < 2 ? i : 3; };
+  auto SZ = [](int i) { return i < 2 ? 5 - i : 2; };
+  // the two transforms of component i; v holds FFT(A_i) afterwards
+  auto transforms = [&](int i, auto last) __attribute__((always_inline)) {
+    fft(SA(i));
+    if constexpr (decltype(last)::value)
+      PE::for_out(p, [&](int e, int k) { kzs[e] = scl<T>(v[e], kf * T(freq(k, N))); });
+    else
+      PE::for_out(p, [&](int e, int k) { vw[i][k * TX + xl] = scl<T>(v[e], kf * T(freq(k, N))); });
+    fft(decltype(last)::value ? -1 : SZ(i + 1));
+  };
+  // the update of component i; sink(k, v) receives its stage value at z index k
+  auto update = [&](int i, auto last, auto&& sink) __attribute__((always_inline)) {
+    const long ci = long(i) * s_slot + toff;
+    PE::for_out(p, [&](int e, int k) {
+      const int kzi = freq(k, N);
+      const T kz = kf * T(kzi);
+      C<T> a;
+      if constexpr (decltype(last)::value) a = add<T>(kzs[e], v[e]);
+      else a = add<T>(vw[i][k * TX + xl], v[e]);
+      C<T> kk = mk<T>(a.y, -a.x);  // -i a
+      const long o = ci + long(k) * TX;
+      const C<T> b0 = u0[o];
+      const C<T> uu = stage == 0 ? b0 : ust[o];
+      const T k2 = kx * kx + ky * ky + kz * kz;
+      kk = mk<T>(kk.x - nu * k2 * uu.x, kk.y - nu * k2 * uu.y);
+      if (rp.forceA) {
+        const int k2i = x * x + kyi * kyi + kzi * kzi;
+        if (k2i > 0 && k2i <= rp.kband2) kk = add<T>(kk, scl<T>(uu, fA));
+      }
+      C<T> s_;
+      if (stage == 0) s_ = kk;
+      else if (stage == 3) s_ = add<T>(acc[o], kk);
+      else s_ = add<T>(acc[o], scl<T>(kk, T(2)));
+      if (stage < 3) acc[o] = s_;
+      sink(k, add<T>(b0, scl<T>(stage < 3 ? kk : s_, cdt)));
+    });
+  };
+  fetch(SZ(0));
+#pragma unroll 1
+  for (int i = 0; i < 2; ++i) {
+    transforms(i, std::false_type{});
+    if (xok) update(i, std::false_type{}, [&](int k, C<T> vv) { vw[i][k * TX + xl] = vv; });
+  }
+  transforms(2, std::true_type{});
+  if (xok)
+    update(2, std::true_type{}, [&](int k, C<T> vv) {
+      Mode m;
+      m.kx = x;
+      m.ky = kyi;
+      m.kz = freq(k, N);
+      C<T> w[3] = {vw[0][k * TX + xl], vw[1][k * TX + xl], vv};
+      project_mode<T, 3>(m, pg, w);
+      C<T>* dst = (stage < 3 ? ust : u0) + toff + long(k) * TX;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dst[long(c) * s_slot] = w[c];
+    });
+}
+
+// Layout conversion of ncomp spectral components between the x-contiguous
+// layout (element (x, y, z) at (z n + y) nhp + x, component stride s_spec)
+// and z-pencil tiles (((x/TX n + y) n + z) TX + x%TX, component stride ts).
+// A block moves a tile of TX x by 8 y by 32 z through LDS, so the spectral
+// side is accessed in runs of TX elements and the tiled side in runs of
+// 32 TX.  Grid: (ntile * ceil(n/8) * ceil(n/32), ncomp).
+template <typename T, int TX, bool TO_TILES>
+__global__ void __launch_bounds__(256) k_ztile_convert(const C<T>* __restrict__ src, C<T>* dst, int n, int nhp,
+                                                       long s_spec, long ts) {
+  constexpr int YB = 8, ZB = 32;
+  constexpr int NE = YB * ZB * TX;
+  __shared__ C<T> buf[YB][ZB][TX + 1];
+  const int nyb = (n + YB - 1) / YB, nzb = (n + ZB - 1) / ZB;
+  long bid = blockIdx.x;
+  const int zb = int(bid % nzb);
+  bid /= nzb;
+  const int yb = int(bid % nyb);
+  const int tile = int(bid / nyb);
+  const long x0 = long(tile) * TX;
+  const C<T>* sp = src + long(blockIdx.y) * (TO_TILES ? s_spec : ts);
+  C<T>* dp = dst + long(blockIdx.y) * (TO_TILES ? ts : s_spec);
+  // spectral order: xl fastest, then y, then z; tiled order: xl, z, y
+  auto spec_at = [&](int e, int& yy, int& zz, long& off) {
+    const int xl = e % TX;
+    yy = (e / TX) % YB;
+    zz = e / (TX * YB);
+    off = (long(zb * ZB + zz) * n + (yb * YB + yy)) * nhp + x0 + xl;
+  };
+  auto tile_at = [&](int e, int& yy, int& zz, long& off) {
+    const int xl = e % TX;
+    zz = (e / TX) % ZB;
+    yy = e / (TX * ZB);
+    off = ((long(tile) * n + (yb * YB + yy)) * n + (zb * ZB + zz)) * TX + xl;
+  };
+  for (int e = threadIdx.x; e < NE; e += 256) {
+    int yy, zz;
+    long off;
+    if (TO_TILES) spec_at(e, yy, zz, off);
+    else tile_at(e, yy, zz, off);
+    if (yb * YB + yy < n && zb * ZB + zz < n) buf[yy][zz][e % TX] = sp[off];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NE; e += 256) {
+    int yy, zz;
+    long off;
+    if (TO_TILES) tile_at(e, yy, zz, off);
+    else spec_at(e, yy, zz, off);
+    if (yb * YB + yy < n && zb * ZB + zz < n) dp[off] = buf[yy][zz][e % TX];
+  }
+}
+
 // 2D final y-pass over the packed stress (xx, yy, xy).
 template <typename T, int N>
 __global__ void SC_LB_WIDE((wide_threads<T, N>()))
@@ -261,6 +444,7 @@ static __global__ void k_cfl_dt(const unsigned long long* vmax_bits, double cfl,
   *dt = fmin(d, dtmax);
 }
 static __global__ void k_set_f64(double* p, double v) { *p = v; }
+static __global__ void k_copy_f64(double* dst, const double* src) { *dst = *src; }
 static __global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
 }  // namespace sc

@@ -531,7 +531,8 @@ __global__ void k_sum_modes(const C<T>* u, int ncomp, SpecGeom g, int kind, doub
       a += double(v.x) * double(v.x) + double(v.y) * double(v.y);
     }
     double w = kind == 3 ? 1.0 : rfft_w(m, g);
-    if (kind == 1 || kind == 2) {
+    if (kind == 4 && !in23(m, g.n / 3)) a = 0;  // filter_reynolds: gradient inside the 2/3 box
+    if (kind == 1 || kind == 2 || kind == 4) {
       double kx = g.kf * m.kx, ky = g.kf * m.ky, kz = g.kf * m.kz;
       double k2 = kx * kx + ky * ky + kz * kz;
       a *= nu * (kind == 2 ? k2 * k2 : k2);
@@ -655,7 +656,8 @@ __global__ void k_shell_scale(C<T>* u, int ncomp, SpecGeom g, int nshell, const 
 // Linear band forcing coefficient A = injection / sum_{0<|k|^2<=kb2} w |u|^2,
 // one block over the (small) band.
 template <typename T>
-__global__ void k_band_coeff(const C<T>* u, int ncomp, SpecGeom g, int kband, double injection, double* A) {
+__global__ void k_band_coeff(const C<T>* u, int ncomp, SpecGeom g, int kband, double injection, double* A,
+                             int tx = 0) {
   __shared__ double sm[256];
   double acc = 0.0;
   const int kb2 = kband * kband;
@@ -670,7 +672,8 @@ __global__ void k_band_coeff(const C<T>* u, int ncomp, SpecGeom g, int kband, do
     if (k2 == 0 || k2 > kb2) continue;
     int y = ky < 0 ? ky + g.n : ky;
     int z = kz < 0 ? kz + g.n : kz;
-    long off = kx + long(g.nhp) * (y + long(g.n) * z);
+    // tx > 0: u in z-pencil tiles of tx x-values (component stride g.s_comp)
+    long off = tx > 0 ? ((long(kx / tx) * g.n + y) * g.n + z) * tx + kx % tx : kx + long(g.nhp) * (y + long(g.n) * z);
     double w = (kx >= 1 && kx <= g.nh - 2) ? 2.0 : 1.0;
     for (int c = 0; c < ncomp; ++c) {
       C<T> v = u[c * g.s_comp + off];

@@ -53,11 +53,13 @@ SIGNATURES = {
     "sc_propose_timestep": [P, P, c_double, DP],
     "sc_rk4_cfl": [P, P, c_double, c_double, c_double, DP],
     "sc_wray3_cfl": [P, P, c_double, c_double, c_double, DP],
+    "sc_rk4_steps": [P, P, c_int, c_double, c_double, c_double, DP],
     "sc_les_wray3_cfl": [P, P, c_double, c_double, c_double, c_int, c_double, c_double, DP],
     "sc_set_band_forcing": [P, c_double, c_int],
     "sc_energy": [P, P, c_int, DP],
     "sc_spectraldot": [P, P, P, c_int, DP, DP],
     "sc_dissipation": [P, P, c_double, DP],
+    "sc_filter_reynolds": [P, P, c_double, c_double, DP],
     "sc_palinstrophy": [P, P, c_double, DP],
     "sc_spectrum": [P, P, c_int, c_int, DP],
     "sc_shell_energies": [P, P, c_int, c_int, DP],

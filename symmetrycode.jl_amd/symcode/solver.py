@@ -155,6 +155,19 @@ def rk4_cfl(u, g: Grid, visc, cfl, dtmax=float("inf")):
     return dt.value
 
 
+def rk4_steps(u, g: Grid, visc, nsteps, cfl=0.0, dtmax=float("inf")):
+    """nsteps RK4 steps in one call (the march loop of create_dns,
+    src/data.jl:57-66, without its per-step host work): cfl > 0 takes
+    min(cfl * propose_timestep, dtmax) every step, else dt = dtmax.  Returns
+    the step sizes."""
+
+    if cfl <= 0 and not math.isfinite(dtmax):
+        raise ValueError("a fixed step needs a finite dtmax")
+    dts = (ctypes.c_double * max(int(nsteps), 1))()
+    _c(lib().sc_rk4_steps(g.ctx, _p(u), int(nsteps), float(cfl), visc, float(min(dtmax, 1e300)), dts), g)
+    return np.array(dts[:nsteps])
+
+
 def wray3_cfl(u, g: Grid, visc, cfl, dtmax=float("inf")):
     """wray3! with Δt = min(cfl · propose_timestep(u), dtmax) (the callers'
     step-size rule, src/data.jl:61-62).  Returns Δt."""
@@ -208,6 +221,14 @@ def get_dissipation(u, visc, g: Grid):
     """get_dissipation! (src/solver.jl:796-811)."""
     e = _d()
     _c(lib().sc_dissipation(g.ctx, _p(u), visc, ctypes.byref(e)), g)
+    return e.value
+
+
+def filter_reynolds(ubar, g: Grid, visc, delta):
+    """filter_reynolds (src/solver.jl:768-774): Re_Δ = Δ² √(Σ_ij
+    getenergy(2/3 (i k_j ū_i))) / ν."""
+    e = _d()
+    _c(lib().sc_filter_reynolds(g.ctx, _p(ubar), visc, delta, ctypes.byref(e)), g)
     return e.value
 
 

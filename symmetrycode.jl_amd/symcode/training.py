@@ -43,8 +43,9 @@ class Pair:
 
 def filter_reynolds(ubar, g: Grid, visc, delta):
     """filter_reynolds (src/solver.jl:768-774): Re_Δ = Δ² √⟨|∇ū|²⟩ / ν with
-    ⟨|∇ū|²⟩ = Σ_w k²|û|² of the truncated field."""
-    return delta**2 * math.sqrt(S.get_dissipation(ubar, 1.0, g)) / visc
+    ⟨|∇ū|²⟩ = Σ_ij getenergy(2/3 (i k_j ū_i)): the gradient is truncated to
+    the 2/3 box on the device, whatever ū holds."""
+    return S.filter_reynolds(ubar, g, visc, delta)
 
 
 def create_data(u, g_dns: Grid, g_les: Grid, visc, filters, nsnap, dt_save, cfl=0.3, shells=None):
