@@ -133,6 +133,14 @@ int sc_set_band_forcing(sc_ctx* ctx, double injection, int kband);
 
 /* FP64 reductions */
 int sc_energy(sc_ctx* ctx, const void* u, int ncomp, double* e);          /* energy, :643-646 */
+/* sc_energy without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device or pinned memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid it).  Its partial sums live in
+ * a fixed scratch array of 532 KB per GPU that the library holds until the
+ * process ends; fields of at most 1.7e10 entries per component (error 1
+ * beyond).  The sum runs in chunks, so it equals sc_energy to rounding, not
+ * bit for bit.  Calls that overlap on one GPU share the scratch and race. */
+int sc_energy_noalloc(sc_ctx* ctx, const void* u, int ncomp, double* e);
 int sc_spectraldot(sc_ctx* ctx, const void* u, const void* v, int ncomp,
                    double* re, double* im);                                /* :708-709 */
 int sc_dissipation(sc_ctx* ctx, const void* u, double visc, double* eps);  /* get_dissipation!, :796-811 */

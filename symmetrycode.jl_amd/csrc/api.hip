@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include "../../include/symcode.h"
 #include "ops.hpp"
@@ -12,6 +13,26 @@ struct sc_ctx : sc::Ctx {};
 
 namespace {
 thread_local std::string g_create_err;
+
+// Scratch of sc_energy_noalloc: one copy per GPU, part of this code object
+// (no allocation), kept until the process ends.  Host code never uses the
+// array's name as an address: its device address on each device comes from
+// hipGetSymbolAddress, taken once per device (with that device current) by
+// the first sc_create there.  Calls that overlap on one GPU share the array
+// and race.
+static __device__ __align__(256) double scratch[sc::NOALLOC_DOUBLES];
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+void init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw sc::Error(1, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  SC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_dev[device] = static_cast<double*>(p);
+}
 
 template <class F>
 int guard(sc_ctx* c, F&& f) {
@@ -69,6 +90,7 @@ int sc_create(int D, int n, double l, int precision, int device, sc_ctx** out) {
   c->phys_elems = long(n) * n * c->nz;
   try {
     SC_HIP(hipSetDevice(device));
+    init_scratch(device);
     SC_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     SC_HIP(hipStreamCreate(&c->stream));
     c->own_stream = true;
@@ -373,6 +395,16 @@ int sc_set_band_forcing(sc_ctx* c, double injection, int kband) {
 
 int sc_energy(sc_ctx* c, const void* u, int ncomp, double* e) {
   return guard(c, [&] { TYPED(c, *e = sc::Ops<T>::sum_modes(*c, (const CT*)u, ncomp, 0, 0.0) / 2); });
+}
+int sc_energy_noalloc(sc_ctx* c, const void* u, int ncomp, double* e) {
+  return guard(c, [&] {
+    need(u && e && ncomp > 0, "null pointer or ncomp");
+    double* scr = g_scratch_dev[c->device];
+    need(scr != nullptr, "no scratch on this device");
+    double s = 0;
+    TYPED(c, sc::Ops<T>::sum_modes_noalloc(*c, (const CT*)u, ncomp, 0, 0.0, scr, &s));
+    *e = s / 2;
+  });
 }
 int sc_spectraldot(sc_ctx* c, const void* u, const void* v, int ncomp, double* re, double* im) {
   return guard(c, [&] { TYPED(c, sc::Ops<T>::dot_modes(*c, (const CT*)u, (const CT*)v, ncomp, re, im)); });

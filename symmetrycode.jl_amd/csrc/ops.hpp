@@ -47,6 +47,7 @@ struct Ops {
   static void stage_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure);
   // reductions (return to host)
   static double sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu);
+  static void sum_modes_noalloc(Ctx& c, const CT* u, int ncomp, int kind, double nu, double* scr, double* out);
   static void dot_modes(Ctx& c, const CT* u, const CT* v, int ncomp, double* re, double* im);
   static void shell_energy(Ctx& c, const CT* u, int ncomp, int nshell, double* host_out);
   static void shell_scale(Ctx& c, CT* u, int ncomp, int nshell, const double* host_factor);

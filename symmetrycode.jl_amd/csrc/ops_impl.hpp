@@ -965,13 +965,41 @@ double Ops<T>::sum_modes(Ctx& c, const CT* u, int ncomp, int kind, double nu) {
   SpecGeom g = c.geom();
   unsigned nb = nblk(g.total);
   double* part = c.dscratch(long(nb) + 1);
-  hipLaunchKernelGGL(k_sum_modes<T>, dim3(nb), dim3(RB), 0, c.stream, u, ncomp, g, kind, nu, part);
+  hipLaunchKernelGGL(k_sum_modes<T>, dim3(nb), dim3(RB), 0, c.stream, u, ncomp, g, kind, nu, part, 0L);
   SC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part, long(nb), part + nb);
   double* h = c.hostbuf(1);
   SC_HIP(hipMemcpyAsync(h, part + nb, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   SC_HIP(hipStreamSynchronize(c.stream));
   return h[0];
+}
+
+// sum_modes through the caller's scratch of NOALLOC_DOUBLES doubles, with no
+// allocation: k_sum_modes runs in chunks of at most NOALLOC_PARTIALS blocks,
+// each chunk's partials are summed into its slot before the next chunk
+// reuses them, then the slots are summed; the total goes to the host double
+// *out.  Every scratch entry read is written earlier in the same call.
+template <typename T>
+void Ops<T>::sum_modes_noalloc(Ctx& c, const CT* u, int ncomp, int kind, double nu, double* scr, double* out) {
+  const SpecGeom g = c.geom();
+  const long nb = (g.total + RB - 1) / RB;
+  const long nch = (nb + NOALLOC_PARTIALS - 1) / NOALLOC_PARTIALS;
+  if (nch > NOALLOC_CHUNKS)
+    throw Error(1, "field too large for the no-allocation reduction (at most " +
+                       std::to_string(NOALLOC_PARTIALS * NOALLOC_CHUNKS * RB) + " entries per component)");
+  double* sums = scr + NOALLOC_PARTIALS;
+  double* total = sums + NOALLOC_CHUNKS;
+  for (long ch = 0; ch < nch; ++ch) {
+    const long b0 = ch * NOALLOC_PARTIALS;
+    const long blocks = std::min(NOALLOC_PARTIALS, nb - b0);
+    hipLaunchKernelGGL(k_sum_modes<T>, dim3(unsigned(blocks)), dim3(RB), 0, c.stream, u, ncomp, g, kind, nu, scr,
+                       b0 * RB);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, (const double*)scr, blocks, sums + ch);
+  }
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, (const double*)sums, nch, total);
+  SC_HIP(hipGetLastError());
+  SC_HIP(hipMemcpyAsync(out, total, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
 }
 
 template <typename T>

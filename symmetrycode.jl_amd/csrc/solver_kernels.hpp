@@ -502,6 +502,14 @@ __global__ void k_band_adj(const C<T>* u, const C<T>* wv, int ncomp, SpecGeom g,
 // over blocks is done by k_finish_sum (fixed order: deterministic).
 constexpr int RB = 256;
 
+// Scratch of the no-allocation reduction (sc_energy_noalloc), in doubles:
+// the partials of one chunk of at most NOALLOC_PARTIALS blocks, the sums of
+// at most NOALLOC_CHUNKS chunks, and the total.  The largest field it takes
+// has NOALLOC_PARTIALS * NOALLOC_CHUNKS * RB entries per component (1.7e10).
+constexpr long NOALLOC_PARTIALS = 65536;
+constexpr long NOALLOC_CHUNKS = 1024;
+constexpr long NOALLOC_DOUBLES = NOALLOC_PARTIALS + NOALLOC_CHUNKS + 1;
+
 template <class F>
 DEV void block_sum_store(double v, double* partial) {
   __shared__ double sm[RB];
@@ -520,11 +528,12 @@ DEV double rfft_w(const Mode& m, const SpecGeom& g) { return (m.x >= 1 && m.x <=
 // kind 1: nu * sum_c w k^2 |u_c|^2 (get_dissipation!)
 // kind 2: nu * sum_c w k^4 |u_c|^2 (get_palinstrophy!)
 // kind 3: sum_c |u_c|^2 over the stored half spectrum, unweighted (LinearAlgebra.norm^2)
+// (base: index of the block's first mode, for launches over part of the field)
 template <typename T>
-__global__ void k_sum_modes(const C<T>* u, int ncomp, SpecGeom g, int kind, double nu, double* partial) {
+__global__ void k_sum_modes(const C<T>* u, int ncomp, SpecGeom g, int kind, double nu, double* partial, long base) {
   Mode m;
   double acc = 0.0;
-  if (mode_of(gid(), g, m)) {
+  if (mode_of(base + gid(), g, m)) {
     double a = 0;
     for (int c = 0; c < ncomp; ++c) {
       C<T> v = u[c * g.s_comp + m.off];

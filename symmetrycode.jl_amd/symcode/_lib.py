@@ -57,6 +57,7 @@ SIGNATURES = {
     "sc_les_wray3_cfl": [P, P, c_double, c_double, c_double, c_int, c_double, c_double, DP],
     "sc_set_band_forcing": [P, c_double, c_int],
     "sc_energy": [P, P, c_int, DP],
+    "sc_energy_noalloc": [P, P, c_int, DP],
     "sc_spectraldot": [P, P, P, c_int, DP, DP],
     "sc_dissipation": [P, P, c_double, DP],
     "sc_filter_reynolds": [P, P, c_double, c_double, DP],
