@@ -324,12 +324,14 @@ def test_fp32_rk4_3d(n):
 
 @pytest.mark.parametrize("prec", [64, 32])
 def test_rk4_810_fused_matches_unfused(prec):
-    """810^3: the RK4 step whose last forward pass carries the projection and
-    the stage update (k_z_fwd_rk4, half twiddle table) against the unfused
-    path (the LES step without a closure: generic nonlinearity, divergence
-    and viscous pass, separate projected update), band forcing on.  The
-    oracle cannot run this size in test time; the two paths share no pass
-    after the inverse transforms."""
+    """810^3: the per-step RK4 entry point (fused five-pass right-hand side
+    writing the divergence, then k_rk4_update_lin adding the viscous and
+    forcing terms, accumulating and projecting) against the unfused path (the
+    LES step without a closure: generic nonlinearity, divergence and viscous
+    pass, separate projected update), band forcing on.  The oracle cannot run
+    this size in test time; the two paths share no pass after the inverse
+    transforms.  The multi-step path (k_z_fwd_rk4) is compared with this one
+    in test_gpu_rk4_steps.py."""
     import gc
 
     import symcode as sc
