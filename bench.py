@@ -40,7 +40,8 @@ RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at
 # 810^3 FP64, calibrated against a 4 GB device copy (FETCH x 2, WRITE x 1);
 # profiles/rk4_810_fp64_counters.md holds the table and the commands
-MEASURED_GB_810_FP64 = {"x_products": 34.15, "y_fwd_div": 51.35, "z_fwd_div": 38.87, "(k_rk4_update_lin<T, 3>)": 67.35}
+MEASURED_GB_810_FP64 = {"x_products": 34.15, "y_fwd_div": 52.24, "z_fwd_rk4": 81.20, "z_fwd_div": 38.87,
+                        "(k_rk4_update_lin<T, 3>)": 67.35}
 
 
 def cpu_baseline(n_sample=256, threads=None, reps=3):

@@ -1,10 +1,17 @@
 # SymCodeHIP.jl -- Julia binding of libsymcode_hip.so for SymmetryCode.jl.
 #
 # Adds methods of the reference's own functions for a new backend tag, so the
-# existing call sites (create_dns, create_data, solve_les!, sfs!) run on an
-# MI355X through the C ABI in include/symcode.h.  Fields are `HIPField`s:
-# device buffers in the library's device layout, created by the allocators
-# below (the Julia GC owns them through a finalizer).
+# existing call sites run on an MI355X through the C ABI in
+# include/symcode.h.  Fields are `HIPField`s: device buffers in the library's
+# device layout, created by the allocators below (the Julia GC owns them
+# through a finalizer).  A field iterates over its components (views that
+# share its memory), as the reference's NamedTuples of arrays do, so the
+# reference's own create_dns march loop, turbulence_statistics, sfs! and
+# solve_les! run unchanged on them; their broadcasts reduce to `dest .= a`,
+# `dest .= a .± b` and `dest .= c .* a` on spectral fields.  Artefact saving
+# (`u |> cpu_device()`, JLD2) is not covered: `download` returns the host
+# arrays.  Julia is not installed where this binding was written, so it has
+# not been run.
 #
 # Usage:
 #     using SymmetryCode, SymCodeHIP
@@ -57,12 +64,17 @@ function context(g::S.Grid{D,T,HIPBackend}) where {D,T}
 end
 
 # -------------------------------------------------------------------- fields
-"A device field of `ncomp` spectral (complex) or physical (real) scalars."
+"""
+A device field of `ncomp` spectral (complex) or physical (real) scalars.  A
+field owns its memory (`parent === nothing`) or is a view of one component of
+another field (`parent` keeps that field alive).
+"""
 mutable struct HIPField{T}
     ptr::Ptr{Cvoid}
     ncomp::Int
     spectral::Bool
     ctx::Context
+    parent::Any
 end
 
 function HIPField{T}(ctx::Context, ncomp, spectral) where {T}
@@ -73,9 +85,61 @@ function HIPField{T}(ctx::Context, ncomp, spectral) where {T}
     rc = ccall((:hipMalloc, "libamdhip64"), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, bytes)
     rc == 0 || error("hipMalloc failed ($rc)")
     ccall((:hipMemset, "libamdhip64"), Cint, (Ptr{Cvoid}, Cint, Csize_t), p[], 0, bytes)
-    f = HIPField{T}(p[], ncomp, spectral, ctx)
+    f = HIPField{T}(p[], ncomp, spectral, ctx, nothing)
     finalizer(x -> ccall((:hipFree, "libamdhip64"), Cint, (Ptr{Cvoid},), x.ptr), f)
     f
+end
+
+# components: f[i] is a one-component view (component stride = spec_elems or
+# phys_elems scalars); iteration yields the components in the reference's
+# NamedTuple order (x, y, z; xx, yy, zz, xy, yz, zx)
+component_bytes(f::HIPField{T}) where {T} =
+    (f.spectral ? 2 * f.ctx.spec_elems : f.ctx.phys_elems) * sizeof(T)
+function Base.getindex(f::HIPField{T}, i::Integer) where {T}
+    1 <= i <= f.ncomp || throw(BoundsError(f, i))
+    f.ncomp == 1 && return f
+    HIPField{T}(f.ptr + (i - 1) * component_bytes(f), 1, f.spectral, f.ctx, f)
+end
+Base.length(f::HIPField) = f.ncomp
+Base.iterate(f::HIPField, i = 1) = i > f.ncomp ? nothing : (f[i], i + 1)
+
+# Broadcasting on spectral fields, for the reference's `dest .= a`,
+# `dest .= a .+ b`, `dest .= a .- b`, `dest .= c .* a` and `dest .*= c`
+# (sc_lincomb: dest = α x + β y); anything else raises.
+struct HIPStyle <: Base.Broadcast.BroadcastStyle end
+Base.BroadcastStyle(::Type{<:HIPField}) = HIPStyle()
+Base.BroadcastStyle(s::HIPStyle, ::Base.Broadcast.DefaultArrayStyle{0}) = s
+Base.broadcastable(f::HIPField) = f
+Base.axes(f::HIPField) = (Base.OneTo(f.ncomp),)
+Base.Broadcast.instantiate(bc::Base.Broadcast.Broadcasted{HIPStyle}) = bc
+function lincomb!(dest::HIPField, x::HIPField, α, y::HIPField, β)
+    (dest.spectral && x.spectral && y.spectral) || error("broadcasts are supported on spectral fields only")
+    check(ccall((:sc_lincomb, libsymcode), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Cdouble, Cint),
+                dest.ctx.ptr, dest.ptr, x.ptr, α, y.ptr, β, dest.ncomp), dest.ctx.ptr)
+    dest
+end
+# α x for a field or a product of a number and a field
+scaled(x::HIPField) = (1.0, x)
+function scaled(bc::Base.Broadcast.Broadcasted)
+    if bc.f === (*) && length(bc.args) == 2
+        a, b = bc.args
+        a isa Number && b isa HIPField && return (Float64(a), b)
+        b isa Number && a isa HIPField && return (Float64(b), a)
+    end
+    error("unsupported broadcast on HIPField: $(bc.f)")
+end
+Base.copyto!(dest::HIPField, x::HIPField) = lincomb!(dest, x, 1.0, x, 0.0)
+function Base.copyto!(dest::HIPField, bc::Base.Broadcast.Broadcasted{HIPStyle})
+    if bc.f === identity && length(bc.args) == 1
+        return copyto!(dest, bc.args[1])
+    elseif (bc.f === (+) || bc.f === (-)) && length(bc.args) == 2
+        α, x = scaled(bc.args[1])
+        β, y = scaled(bc.args[2])
+        return lincomb!(dest, x, α, y, bc.f === (-) ? -β : β)
+    end
+    α, x = scaled(bc)
+    lincomb!(dest, x, α, x, 0.0)
 end
 
 const HIPGrid{D,T} = S.Grid{D,T,HIPBackend}
@@ -107,9 +171,13 @@ function download(f::HIPField{T}) where {T}
 end
 
 # ------------------------------------------------------------------ solver
-# cache: the library owns all scratch; getcache only binds the context
-S.getcache(g::HIPGrid) = (; ctx = context(g))
-S.statscache(g::HIPGrid) = (; ctx = context(g))
+# cache: the library owns the solver's scratch; getcache binds the context and
+# allocates the stress σ that sfs! reads from the DNS cache (the reference's
+# getcache holds it too); the other entries exist for the call sites that
+# name them
+S.getcache(g::HIPGrid) = (; ctx = context(g), σ = S.tensorfield(g), vi_vj = nothing, v = nothing, plan = nothing,
+                          ustart = nothing, du = nothing)
+S.statscache(g::HIPGrid) = (; ctx = context(g), field = nothing)
 
 S.apply!(::typeof(S.twothirds!), g::HIPGrid, (u, _)) =
     check(ccall((:sc_twothirds, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint), u.ctx.ptr, u.ptr, u.ncomp), u.ctx.ptr)
@@ -158,6 +226,34 @@ function S.energy(u::HIPField)
     e[]
 end
 
+# getenergy (src/solver.jl:701) of one spectral array (a component)
+function S.getenergy(u::HIPField)
+    e = Ref{Cdouble}(0)
+    check(ccall((:sc_energy, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Cdouble}),
+                u.ctx.ptr, u.ptr, u.ncomp, e), u.ctx.ptr)
+    2 * e[]
+end
+
+function S.get_palinstrophy!(p, u::HIPField, visc, g::HIPGrid)
+    e = Ref{Cdouble}(0)
+    check(ccall((:sc_palinstrophy, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ref{Cdouble}),
+                u.ctx.ptr, u.ptr, visc, e), u.ctx.ptr)
+    e[]
+end
+
+function S.make_tracefree!(τ::HIPField, g::HIPGrid)
+    f = τ.spectral ? :sc_make_tracefree_spectral : :sc_make_tracefree_physical
+    check(ccall((f, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), τ.ctx.ptr, τ.ptr), τ.ctx.ptr)
+    τ
+end
+
+function S.filter_reynolds(ubar::HIPField, g::HIPGrid, visc, Δ)
+    r = Ref{Cdouble}(0)
+    check(ccall((:sc_filter_reynolds, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Ref{Cdouble}),
+                ubar.ctx.ptr, ubar.ptr, visc, Δ, r), ubar.ctx.ptr)
+    r[]
+end
+
 function S.get_dissipation!(diss, u::HIPField, visc, g::HIPGrid)
     e = Ref{Cdouble}(0)
     check(ccall((:sc_dissipation, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ref{Cdouble}),
@@ -188,7 +284,13 @@ function S.maintain_shell_energy!(u::HIPField, shells)
     u
 end
 
+# The device draws its normals from a counter-based stream keyed by a seed
+# (Julia's Xoshiro stream cannot be reproduced on the device).  The
+# reference's callers pass `rng` (create_dns: `rng = Xoshiro(dns.seed)`,
+# src/data.jl:39-46): the seed is then drawn from it, so distinct rngs give
+# distinct fields and equal rngs equal ones; without rng, `seed` is used.
 function S.randomfield(profile, grid::HIPGrid; totalenergy = 1, rng = nothing, seed = 0, kwargs...)
+    seed = rng === nothing ? UInt64(seed) : rand(rng, UInt64)
     kdiag = floor(Int, sqrt(S.dim(grid)) * div(grid.n, 2))
     prof = [profile(k; kwargs...) for k in 0:kdiag]
     E0 = totalenergy .* prof ./ sum(prof)
@@ -213,18 +315,6 @@ function S.create_verstappen(C, Δ, g::HIPGrid{D}) where {D}
 end
 S.create_bardina(s, Δ, g::HIPGrid) = BuiltinClosure(6, s, Δ)
 
-function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::BuiltinClosure, visc)
-    check(ccall((:sc_les_rhs, libsymcode), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
-                u.ctx.ptr, du.ptr, u.ptr, visc, model.kind, model.p1, model.p2), u.ctx.ptr)
-end
-
-function S.wray3!(::typeof(S.les!), u::HIPField, Δt, grid::HIPGrid, cache; model::BuiltinClosure, visc)
-    check(ccall((:sc_les_wray3, libsymcode), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cdouble, Cdouble),
-                u.ctx.ptr, u.ptr, Δt, visc, model.kind, model.p1, model.p2), u.ctx.ptr)
-    u
-end
 
 # ------------------------------------------------------------ learned closure
 "The BF16-MFMA 3x3x3 circular CNN (sc_cnn_*); parameters stay on the device."
@@ -273,12 +363,6 @@ attach(m::CNNClosure, ctx) = check(ccall((:sc_set_cnn_closure, libsymcode), Cint
                                          (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cdouble),
                                          ctx.ptr, m.net.ptr, m.s_in, m.s_out, m.extra), ctx.ptr)
 
-function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::CNNClosure, visc)
-    attach(model, u.ctx)
-    check(ccall((:sc_les_rhs, libsymcode), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
-                u.ctx.ptr, du.ptr, u.ptr, visc, 4, 0, 0), u.ctx.ptr)
-end
 
 # ------------------------------------------------------------ pointwise MLP closure
 "The reference's pointwise `mlp` (src/nets.jl:247-298) on the device (sc_pmlp_*)."
@@ -349,11 +433,26 @@ function attach(m::MLPClosure, ctx)
     m.symmetrize && check(ccall((:sc_set_pmlp_symmetrize, libsymcode), Cint, (Ptr{Cvoid}, Cint), ctx.ptr, 1), ctx.ptr)
 end
 
-function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model::MLPClosure, visc)
+# les! and wray3!(les!, …) (src/les.jl:4-33, src/solver.jl:968-1012): one
+# method each (keyword types do not take part in dispatch), the model's kind
+# and parameters passed to the library, a network attached first
+params(m::BuiltinClosure) = (m.p1, m.p2)
+params(m) = (0.0, 0.0)
+function S.les!(du::HIPField, u::HIPField, grid::HIPGrid, cache; model, visc)
     attach(model, u.ctx)
+    p1, p2 = params(model)
     check(ccall((:sc_les_rhs, libsymcode), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Cdouble, Cdouble),
-                u.ctx.ptr, du.ptr, u.ptr, visc, 7, 0, 0), u.ctx.ptr)
+                u.ctx.ptr, du.ptr, u.ptr, visc, kind(model), p1, p2), u.ctx.ptr)
+end
+
+function S.wray3!(::typeof(S.les!), u::HIPField, Δt, grid::HIPGrid, cache; model, visc)
+    attach(model, u.ctx)
+    p1, p2 = params(model)
+    check(ccall((:sc_les_wray3, libsymcode), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cdouble, Cdouble),
+                u.ctx.ptr, u.ptr, Δt, visc, kind(model), p1, p2), u.ctx.ptr)
+    u
 end
 
 # ------------------------------------------------------------ adjoint
@@ -379,8 +478,9 @@ function rk4_step(u::HIPField, Δt, grid::HIPGrid; visc, model = nothing)
                     u.ctx.ptr, v.ptr, Δt, visc), u.ctx.ptr)
     else
         attach(model, u.ctx)
+        p1, p2 = params(model)
         check(ccall((:sc_les_rk4, libsymcode), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cdouble, Cdouble),
-                    u.ctx.ptr, v.ptr, Δt, visc, kind(model), 0, 0), u.ctx.ptr)
+                    u.ctx.ptr, v.ptr, Δt, visc, kind(model), p1, p2), u.ctx.ptr)
     end
     v
 end
