@@ -95,7 +95,7 @@ def spectrum_error(u, ref, g):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, log=print):
+def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, nroll=100, pretrain=150, log=print):
     """The learned-closure workloads.
 
     Data: a 512^3 FP64 forced DNS (nu = 2e-3, band forcing 0.1 on |k| <= 2,
@@ -103,10 +103,14 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     cut to 128^3 at 8 save times: 16 (u_bar, tau) pairs.
     Training: one optimiser step on the batch of 16 with the reference's
     semantics = forward (trace-free output), relative loss, backward (data
-    and weight gradients of all four 3x3x3 layers), ClipNorm(1) + AdamW.
-    A-posteriori: 128^3 FP32 LES with the learned closure, 20 RK4 steps from
-    the filtered DNS, trajectory MSE against the filtered DNS; one gradient
-    step = rollout + hand-written adjoint (stages recomputed) + Adam."""
+    and weight gradients of all four 3x3x3 layers), ClipNorm(1) + AdamW;
+    timed, then ``pretrain`` more steps (warm-up + cosine) before the
+    a-posteriori part.
+    A-posteriori: 128^3 FP32 LES with the trained closure from the filtered
+    DNS, the filtered DNS advanced alongside as the reference trajectory.
+    One gradient step = 20 RK4 steps + trajectory MSE + hand-written adjoint
+    (stages recomputed) + Adam.  The closures' accuracy: shell-spectrum error
+    and e_post against the filtered DNS after 20 and ``nroll`` steps."""
     import torch
 
     import symcode as sc
@@ -142,12 +146,14 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     conv_flop = sum(v[2] for v in conv.values()) / train_steps
     nvox = 16 * n_les**3
     useful = 3 * 2.0 * nvox * 27 * (4 * 64 + 64 * 64 * 2 + 64 * 6)  # fwd + data grad + weight grad
+    hist = T.reference_train(net, x, y, pretrain, lr=1e-3)
     cnn_train = {
         "value": 1.0 / t_train, "unit": "training steps/s", "ms_per_step": t_train * 1e3,
         "workload": f"3x3x3 circular CNN 4->64->64->64->6 (GELU), batch 16 x {n_les}^3 filtered "
                     f"{n_dns}^3 DNS pairs, BF16 MFMA / FP32 accumulate, trace-free output, relative loss, "
                     "ClipNorm(1) + AdamW lr 1e-3",
         "loss": loss,
+        "pretrain": {"steps": pretrain, "relative_loss_first": hist[0], "relative_loss_last": hist[-1]},
         "conv_ms_per_step": conv_ms,
         "conv_tflops_executed": conv_flop / (conv_ms / 1e3) / 1e12,
         "mfma_util_executed": conv_flop / (conv_ms / 1e3) / PEAK_BF16,
@@ -155,9 +161,10 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
         "mfma_peak_tflops": PEAK_BF16 / 1e12,
         "kernels_ms_per_step": {k: v[1] / train_steps for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])},
     }
-    log(f"cnn train: {cnn_train['ms_per_step']:.1f} ms/step")
-    # ---- a-posteriori gradient steps, 128^3 FP32
+    log(f"cnn train: {cnn_train['ms_per_step']:.1f} ms/step, pretrain loss {hist[0]:.3f} -> {hist[-1]:.3f}")
+    # ---- a-posteriori, 128^3 FP32
     g32 = sc.Grid(3, n_les, dtype=torch.float32)
+    sc.set_band_forcing(g32, 0.1, 2)
     delta = 2 * g_les.l / n_les
     # fixed LES step: half the CFL-0.3 step of the DNS scaled to the 4x coarser grid
     dt = 0.5 * cfl * sc.propose_timestep(u, g_dns, visc) * (n_dns / n_les)
@@ -165,7 +172,7 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     u0 = T.regrid(ub, g_les, g32)
     re0 = T.filter_reynolds(ub, g_les, visc, delta)
     refs = []
-    for _ in range(nsteps):
+    for _ in range(max(nsteps, nroll)):
         # the DNS advances with sub-steps of its own CFL limit to each LES time
         tt = 0.0
         while tt < dt * (1 - 1e-12):
@@ -173,34 +180,46 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
         refs.append(T.regrid(sc.sfs(u, g_dns, g_les, delta)[0], g_les, g32))
     del g_dns, u
     model = nrm.closure(net, g32, delta, re0)
-    T.aposteriori_step(net, u0, refs, dt, g32, visc, model)
+    window = refs[:nsteps]
+    theta = net.get_params()  # the rollouts below use the pre-trained network
+    T.aposteriori_step(net, u0, window, dt, g32, visc, model, lr=1e-4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(grad_steps):
-        Lc = T.aposteriori_step(net, u0, refs, dt, g32, visc, model)
+        Lc = T.aposteriori_step(net, u0, window, dt, g32, visc, model, lr=1e-4)
     torch.cuda.synchronize()
     t_grad = (time.perf_counter() - t0) / grad_steps
+    net.set_params(theta)
     rollouts = {}
-    ref_norm = sum(sc.spectraldot(r, r, g32).real for r in refs) / (2 * nsteps)
+    ref_norm = sum(sc.spectraldot(r, r, g32).real for r in window) / (2 * nsteps)
     assert ref_norm > 0, "the filtered-DNS reference trajectory is zero"
-    for name, m in (("learned", model), ("dynsmag", sc.create_dynamic_smagorinsky(delta, g32)), ("none", None)):
-        T.rollout_loss(u0, refs, dt, g32, visc, m)
+    closures = (("learned", model), ("dynsmag", sc.create_dynamic_smagorinsky(delta, g32)), ("none", None),
+                ("clark", sc.create_clark(delta, g32)))
+    for name, m in closures:
+        T.rollout_loss(u0, window, dt, g32, visc, m)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        states = []
-        Lm = T.rollout_loss(u0, refs, dt, g32, visc, m, states)
+        Lm = T.rollout_loss(u0, window, dt, g32, visc, m)
         torch.cuda.synchronize()
-        rollouts[name] = {"rollouts_per_s": 1.0 / (time.perf_counter() - t0),
-                          "rel_traj_mse": Lm / ref_norm,
-                          "spectrum_err_final": spectrum_error(states[-1], refs[-1], g32),
-                          "e_post_final": T.e_post(states[-1], refs[-1], g32)}
+        rate = 1.0 / (time.perf_counter() - t0)
+        v = u0.clone()
+        acc = {}
+        for i, r in enumerate(refs[:nroll]):
+            sc.les_rk4(v, dt, g32, m, visc)
+            if i + 1 in (nsteps, nroll):
+                acc[i + 1] = {"spectrum_err": spectrum_error(v, r, g32), "e_post": T.e_post(v, r, g32)}
+        rollouts[name] = {"rollouts_per_s": rate, "rel_traj_mse": Lm / ref_norm, "after_steps": acc}
+    sc.set_band_forcing(g32, 0.0)
+    le, dy = rollouts["learned"]["after_steps"], rollouts["dynsmag"]["after_steps"]
     les_grad = {
         "value": 1.0 / t_grad, "unit": "gradient steps/s", "ms_per_grad_step": t_grad * 1e3,
-        "workload": f"{n_les}^3 FP32 LES with the learned closure, {nsteps} RK4 steps (dt={dt:.3e}) from "
+        "workload": f"{n_les}^3 FP32 LES with the trained learned closure, {nsteps} RK4 steps (dt={dt:.3e}) from "
                     f"filtered {n_dns}^3 DNS, trajectory MSE, hand-written adjoint with per-step "
                     "checkpoints and stage recompute, Adam",
         "loss": Lc,
         "rollouts": rollouts,
+        "spectrum_err_learned_vs_dynsmag": {k: [le[k]["spectrum_err"], dy[k]["spectrum_err"]] for k in le},
+        "learned_at_most_dynsmag_spectrum_err": all(le[k]["spectrum_err"] <= dy[k]["spectrum_err"] for k in le),
         "grad_steps_vs_dynsmag_rollouts": (1.0 / t_grad) / rollouts["dynsmag"]["rollouts_per_s"],
     }
     log(f"les grad: {les_grad['ms_per_grad_step']:.1f} ms/step")

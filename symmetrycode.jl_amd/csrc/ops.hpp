@@ -77,7 +77,6 @@ struct Ops {
   static void les_wray3(Ctx& c, CT* u, double dt, double nu, int closure, double p1, double p2);
   static void launch_phys(Ctx& c, T* y, const T* x, T a, T b, long n);
   static void smag_tensor(Ctx& c, T* m, const T* S, double delta);
-  static void ml_mm(Ctx& c, const T* M, const T* L, double* ml, double* mm);
 };
 
 }  // namespace sc

@@ -1215,11 +1215,11 @@ void Ops<T>::les_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure, double
   if (closure < 0 || closure > 7) throw Error(1, "unknown closure kind");
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, nt));
   nonlinearity(c, B, u, nullptr);
-  if (closure != 0) {
+  if (closure == 3) {
+    dynsmag_add_spec<T>(c, B, u, p1);
+  } else if (closure != 0) {
     T* tau = reinterpret_cast<T*>(c.work(0, size_t(nt) * c.phys_elems * sizeof(T)));
-    if (closure == 3) {
-      dynsmag(c, tau, u, p1);
-    } else if (closure == 4) {
+    if (closure == 4) {
       cnn_tau(c, tau, u);
     } else if (closure == 6) {
       bardina(c, tau, u, p1, p2);
@@ -1299,62 +1299,91 @@ void Ops<T>::bardina(Ctx& c, T* tau, const CT* u, double s, double delta) {
 
 // create_dynamic_smagorinsky's model (src/closures.jl:188-271), with the
 // test-filtered strain computed from the truncated strain (the reference
-// reads a buffer its own C2R destroyed; see the oracle).
+// reads a buffer its own C2R destroyed; see the oracle), arranged so that
+// nothing is transformed twice and nothing returns to the host:
+//   u~ = 2/3 G~ u, s~ = nonlinearity(u~); L = to_phys(2/3 G~ s - s~) in one
+//   C2R (to_phys is linear); S = to_phys(2/3 S^), S~ = to_phys(2/3 G~ S^);
+//   m1 = smagorinsky_tensor(S, Delta) and M1 = rfft(m1).  The dealiased m1 is
+//   irfft(2/3 M1), so its test filter is irfft(2/3 G~ M1) and the stress's
+//   own spectrum is c 2/3 M1 / n^D: no second forward transform.
+//   <M:L_dev> and <M:M> with M = that test filter - smagorinsky_tensor(S~,
+//   Delta^) formed inside the sums; c = max(-<ML>/<MM>, 0) on the device.
+// sig: nonlinearity(u) when the caller has it (les!), else null.  Returns
+// M1 = rfft(m1) (unnormalised) and the device address of c.
 template <typename T>
-double Ops<T>::dynsmag(Ctx& c, T* tau, const CT* u, double delta) {
+static void dynsmag_core(Ctx& c, const C<T>* u, const C<T>* sig, double delta, C<T>*& M1, double*& coef_dev) {
+  using CT = C<T>;
   const int D = c.D, nt = c.tdim();
   const long sp = c.spec_elems, np = c.phys_elems;
   const double dt = 2 * delta;
   const double dd = std::sqrt(dt * dt + delta * delta);
   const double inv = 1.0 / std::pow(double(c.n), D);
   CT* UT = reinterpret_cast<CT*>(c.work(3, size_t(D) * sp * sizeof(CT)));
-  CT* SIG = reinterpret_cast<CT*>(c.work(4, size_t(nt) * sp * sizeof(CT)));
-  CT* SIGT = reinterpret_cast<CT*>(c.work(5, size_t(nt) * sp * sizeof(CT)));
-  CT* SH = reinterpret_cast<CT*>(c.work(6, size_t(nt) * sp * sizeof(CT)));
+  CT* SIGT = reinterpret_cast<CT*>(c.work(5, size_t(nt) * sp * sizeof(CT)));  // s~, then L^
+  CT* SH = reinterpret_cast<CT*>(c.work(6, size_t(nt) * sp * sizeof(CT)));    // S^, then 2/3 G~ M1
   T* L = reinterpret_cast<T*>(c.work(7, size_t(nt) * np * sizeof(T)));
-  T* P = reinterpret_cast<T*>(c.work(8, size_t(nt) * np * sizeof(T)));
-  T* S = reinterpret_cast<T*>(c.work(9, size_t(nt) * np * sizeof(T)));
+  T* S = reinterpret_cast<T*>(c.work(9, size_t(nt) * np * sizeof(T)));        // S, then m1
   T* St = reinterpret_cast<T*>(c.work(10, size_t(nt) * np * sizeof(T)));
-  T* m2 = reinterpret_cast<T*>(c.work(11, size_t(nt) * np * sizeof(T)));
-  T* M = reinterpret_cast<T*>(c.work(12, size_t(nt) * np * sizeof(T)));
-  // test-filtered velocity
-  SC_HIP(hipMemcpyAsync(UT, u, size_t(D) * sp * sizeof(CT), hipMemcpyDeviceToDevice, c.stream));
-  gaussian(c, UT, D, dt);
-  twothirds(c, UT, D);
-  // Germano L = tilde(uu) - tilde(u) tilde(u)
-  nonlinearity(c, SIG, u, nullptr);
-  nonlinearity(c, SIGT, UT, nullptr);
-  gaussian(c, SIG, nt, dt);
-  twothirds(c, SIG, nt);
-  c2r(c, L, SIG, nt, 1.0, false);
-  c2r(c, P, SIGT, nt, 1.0, false);
-  launch_phys(c, L, P, T(-1), T(1), long(nt) * np);
-  // strain at grid (S) and combined (St) filter levels
-  strainrate(c, SH, u);
-  twothirds(c, SH, nt);
-  c2r(c, S, SH, nt, 1.0, false);
-  gaussian(c, SH, nt, dt);
-  twothirds(c, SH, nt);
-  c2r(c, St, SH, nt, 1.0, false);
-  // Smagorinsky tensors; m1 goes straight into tau
-  smag_tensor(c, tau, S, delta);
-  smag_tensor(c, m2, St, dd);
-  // m1 = dealias_phys!(m1); M = test_filter_phys!(copy(m1)) - m2
-  r2c(c, SH, tau, nt, 1.0);
-  twothirds(c, SH, nt);
-  c2r(c, tau, SH, nt, inv, false);
-  r2c(c, SH, tau, nt, 1.0);
-  gaussian(c, SH, nt, dt);
-  twothirds(c, SH, nt);
-  c2r(c, M, SH, nt, inv, false);
-  launch_phys(c, M, m2, T(-1), T(1), long(nt) * np);
-  // Lilly average c = max(-<ML>/<MM>, 0)
-  double ml = 0, mm = 0;
-  ml_mm(c, M, L, &ml, &mm);
-  const double coef = mm == 0.0 ? 0.0 : std::fmax(-ml / mm, 0.0);
-  hipLaunchKernelGGL(k_phys_scale<T>, dim3(nblk(long(nt) * np)), dim3(256), 0, c.stream, tau, coef, long(nt) * np);
+  M1 = reinterpret_cast<CT*>(c.work(11, size_t(nt) * sp * sizeof(CT)));
+  T* Mt = reinterpret_cast<T*>(c.work(12, size_t(nt) * np * sizeof(T)));
+  if (!sig) {
+    CT* SIG = reinterpret_cast<CT*>(c.work(4, size_t(nt) * sp * sizeof(CT)));
+    Ops<T>::nonlinearity(c, SIG, u, nullptr);
+    sig = SIG;
+  }
+  // Germano L
+  SPEC_LAUNCH(k_gauss23<T>, UT, u, (const CT*)nullptr, dt, g_, D);
+  Ops<T>::nonlinearity(c, SIGT, UT, nullptr);
+  SPEC_LAUNCH(k_gauss23<T>, SIGT, sig, (const CT*)SIGT, dt, g_, nt);
+  Ops<T>::c2r(c, L, SIGT, nt, 1.0, false);
+  // strain at the grid and combined filter levels (the 2/3 mask on load)
+  Ops<T>::strainrate(c, SH, u);
+  Ops<T>::c2r(c, S, SH, nt, 1.0, true);
+  Ops<T>::gaussian(c, SH, nt, dt);
+  Ops<T>::c2r(c, St, SH, nt, 1.0, true);
+  // m1 and its spectrum; the test-filtered dealiased m1
+  Ops<T>::smag_tensor(c, S, S, delta);
+  Ops<T>::r2c(c, M1, S, nt, 1.0);
+  SPEC_LAUNCH(k_gauss23<T>, SH, M1, (const CT*)nullptr, dt, g_, nt);
+  Ops<T>::c2r(c, Mt, SH, nt, inv, true);
+  // Lilly average on the device
+  const unsigned nb = nblk(np);
+  double* part = c.dscratch(2 * long(nb) + 3);
+  if (D == 3) hipLaunchKernelGGL((k_ml_mm_fused<T, 3>), dim3(nb), dim3(RB), 0, c.stream, Mt, St, L, dd, np, part);
+  else hipLaunchKernelGGL((k_ml_mm_fused<T, 2>), dim3(nb), dim3(RB), 0, c.stream, Mt, St, L, dd, np, part);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part, long(nb), part + 2 * nb);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part + nb, long(nb), part + 2 * nb + 1);
+  hipLaunchKernelGGL(k_dyn_coef, dim3(1), dim3(1), 0, c.stream, (const double*)(part + 2 * nb), part + 2 * nb + 2);
   SC_HIP(hipGetLastError());
-  return coef;
+  coef_dev = part + 2 * nb + 2;
+}
+
+// the model's physical stress c dealias(m1); returns c
+template <typename T>
+double Ops<T>::dynsmag(Ctx& c, T* tau, const CT* u, double delta) {
+  const int nt = c.tdim();
+  const double inv = 1.0 / std::pow(double(c.n), c.D);
+  CT* M1 = nullptr;
+  double* cd = nullptr;
+  dynsmag_core<T>(c, u, nullptr, delta, M1, cd);
+  CT* X = reinterpret_cast<CT*>(c.work(6, size_t(nt) * c.spec_elems * sizeof(CT)));
+  SPEC_LAUNCH(k_coef23<T>, X, (const CT*)M1, (const double*)cd, inv, 0, g_, nt);
+  c2r(c, tau, X, nt, 1.0, true);
+  double* h = c.hostbuf(1);
+  SC_HIP(hipMemcpyAsync(h, cd, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SC_HIP(hipStreamSynchronize(c.stream));
+  return h[0];
+}
+
+// les!'s stress with the dynamic model: sig += c to_spec(dealias(m1)), sig
+// holding nonlinearity(u) on entry (shared with the model's Germano L)
+template <typename T>
+static void dynsmag_add_spec(Ctx& c, C<T>* sig, const C<T>* u, double delta) {
+  const double inv = 1.0 / std::pow(double(c.n), c.D);
+  C<T>* M1 = nullptr;
+  double* cd = nullptr;
+  dynsmag_core<T>(c, u, sig, delta, M1, cd);
+  SPEC_LAUNCH(k_coef23<T>, sig, (const C<T>*)M1, (const double*)cd, inv, 1, g_, c.tdim());
 }
 
 template <typename T>
@@ -1368,22 +1397,6 @@ void Ops<T>::smag_tensor(Ctx& c, T* m, const T* S, double delta) {
   if (c.D == 3) hipLaunchKernelGGL((k_smag_tensor<T, 3>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, m, S, delta, c.phys_elems);
   else hipLaunchKernelGGL((k_smag_tensor<T, 2>), dim3(nblk(c.phys_elems)), dim3(256), 0, c.stream, m, S, delta, c.phys_elems);
   SC_HIP(hipGetLastError());
-}
-
-template <typename T>
-void Ops<T>::ml_mm(Ctx& c, const T* M, const T* L, double* ml, double* mm) {
-  unsigned nb = nblk(c.phys_elems);
-  double* part = c.dscratch(2 * long(nb) + 2);
-  if (c.D == 3) hipLaunchKernelGGL((k_ml_mm<T, 3>), dim3(nb), dim3(RB), 0, c.stream, M, L, c.phys_elems, part);
-  else hipLaunchKernelGGL((k_ml_mm<T, 2>), dim3(nb), dim3(RB), 0, c.stream, M, L, c.phys_elems, part);
-  SC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part, long(nb), part + 2 * nb);
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(1024), 0, c.stream, part + nb, long(nb), part + 2 * nb + 1);
-  double* h = c.hostbuf(2);
-  SC_HIP(hipMemcpyAsync(h, part + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  SC_HIP(hipStreamSynchronize(c.stream));
-  *ml = h[0];
-  *mm = h[1];
 }
 
 // wray3!(les!, u, dt, grid, cache; model, visc)

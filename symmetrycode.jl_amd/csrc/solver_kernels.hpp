@@ -956,33 +956,88 @@ __global__ void k_smag_tensor(T* m, const T* S, double delta, long np) {
   for (int c = 0; c < NT; ++c) m[c * np + id] = -2 * nu * s[c];
 }
 
-// smagorinsky_ml_mm! (src/closures.jl:152-179) with the FP64 block sums of
-// M:L_dev (partial[b]) and M:M (partial[nblocks + b]).
+// The dynamic model's spectral steps.  dst = 2/3 G(src) (- minus when minus
+// is non-null): the Gaussian filter of width delta with the forced-shell
+// carve-out, then the 2/3 mask (twothirds! after gaussianfilter!, one pass);
+// dst may alias src or minus.
+template <typename T>
+__global__ void k_gauss23(C<T>* dst, const C<T>* src, const C<T>* minus, double delta, SpecGeom g, int ncomp) {
+  Mode m;
+  if (!mode_of(gid(), g, m)) return;
+  const bool keep = in23(m, g.n / 3);
+  const double kx = g.kf * m.kx, ky = g.kf * m.ky, kz = g.kf * m.kz;
+  const int k2i = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+  const T w = k2i < 9 ? T(1) : T(exp(-delta * delta * (kx * kx + ky * ky + kz * kz) / 24));
+  for (int c = 0; c < ncomp; ++c) {
+    const long o = c * g.s_comp + m.off;
+    C<T> v = keep ? scl<T>(src[o], w) : czero<T>();
+    if (minus) v = sub<T>(v, minus[o]);
+    dst[o] = v;
+  }
+}
+
+// smagorinsky_ml_mm! (src/closures.jl:152-179) for the dynamic model with M
+// formed on the fly: M = Mt - m2, Mt the test-filtered m1 and m2 =
+// smagorinsky_tensor(St, dd) of the combined-filter strain St; FP64 block
+// sums of M:L_dev (partial[b]) and M:M (partial[nblocks + b]).
 template <typename T, int D>
-__global__ void k_ml_mm(const T* M, const T* L, long np, double* partial) {
-  long id = gid();
+__global__ void k_ml_mm_fused(const T* Mt, const T* St, const T* L, double dd, long np, double* partial) {
+  constexpr int NT = D == 3 ? 6 : 3;
+  const long id = gid();
   double ml = 0, mm = 0;
   if (id < np) {
-    if constexpr (D == 2) {
-      T l0 = L[id], l1 = L[np + id], l2 = L[2 * np + id];
-      T tr = (l0 + l1) / 2;
-      l0 -= tr; l1 -= tr;
-      T m0 = M[id], m1 = M[np + id], m2 = M[2 * np + id];
-      ml = double(m0 * l0 + m1 * l1 + 2 * m2 * l2);
-      mm = double(m0 * m0 + m1 * m1 + 2 * m2 * m2);
-    } else {
-      T l[6], m[6];
+    T s[NT], l[NT], M[NT];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) { l[c] = L[c * np + id]; m[c] = M[c * np + id]; }
-      T tr = (l[0] + l[1] + l[2]) / 3;
-      l[0] -= tr; l[1] -= tr; l[2] -= tr;
-      ml = double(m[0] * l[0] + m[1] * l[1] + m[2] * l[2] + 2 * m[3] * l[3] + 2 * m[4] * l[4] + 2 * m[5] * l[5]);
-      mm = double(m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + 2 * m[3] * m[3] + 2 * m[4] * m[4] + 2 * m[5] * m[5]);
+    for (int c = 0; c < NT; ++c) {
+      s[c] = St[c * np + id];
+      l[c] = L[c * np + id];
+    }
+    T S2;
+    if constexpr (D == 2) S2 = s[0] * s[0] + s[1] * s[1] + 2 * s[2] * s[2];
+    else S2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2 * s[3] * s[3] + 2 * s[4] * s[4] + 2 * s[5] * s[5];
+    const T nu = T(dd * dd) * sqrt(2 * S2);
+#pragma unroll
+    for (int c = 0; c < NT; ++c) M[c] = Mt[c * np + id] - (-2 * nu * s[c]);
+    if constexpr (D == 2) {
+      const T tr = (l[0] + l[1]) / 2;
+      l[0] -= tr;
+      l[1] -= tr;
+      ml = double(M[0] * l[0] + M[1] * l[1] + 2 * M[2] * l[2]);
+      mm = double(M[0] * M[0] + M[1] * M[1] + 2 * M[2] * M[2]);
+    } else {
+      const T tr = (l[0] + l[1] + l[2]) / 3;
+      l[0] -= tr;
+      l[1] -= tr;
+      l[2] -= tr;
+      ml = double(M[0] * l[0] + M[1] * l[1] + M[2] * l[2] + 2 * M[3] * l[3] + 2 * M[4] * l[4] + 2 * M[5] * l[5]);
+      mm = double(M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + 2 * M[3] * M[3] + 2 * M[4] * M[4] + 2 * M[5] * M[5]);
     }
   }
   block_sum_store<int>(ml, partial);
   __syncthreads();
   block_sum_store<int>(mm, partial + gridDim.x);
+}
+
+// the Lilly coefficient c = max(-<ML>/<MM>, 0), 0 when <MM> = 0, from the
+// two sums (sums[0] = <ML>, sums[1] = <MM>), kept on the device
+static __global__ void k_dyn_coef(const double* sums, double* coef) {
+  *coef = sums[1] == 0.0 ? 0.0 : fmax(-sums[0] / sums[1], 0.0);
+}
+
+// y = c scale 2/3(x) (accumulate: y += ...), c read from the device: the
+// dynamic model's stress c to_spec(dealias_phys(m1)) from x = rfft(m1)
+template <typename T>
+__global__ void k_coef23(C<T>* y, const C<T>* x, const double* coef, double scale, int accumulate, SpecGeom g,
+                         int ncomp) {
+  Mode m;
+  if (!mode_of(gid(), g, m)) return;
+  const bool keep = in23(m, g.n / 3);
+  const T a = T(*coef * scale);
+  for (int c = 0; c < ncomp; ++c) {
+    const long o = c * g.s_comp + m.off;
+    const C<T> v = keep ? scl<T>(x[o], a) : czero<T>();
+    y[o] = accumulate ? add<T>(y[o], v) : v;
+  }
 }
 
 // physical helpers
