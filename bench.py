@@ -226,6 +226,35 @@ def bench_closure(train_steps=3, grad_steps=2, n_dns=512, n_les=128, nsteps=20, 
     return cnn_train, les_grad
 
 
+def bench_cnn_only(steps, warmup, n=128, batch=16):
+    """training steps of the learned closure (4->64->64->64->6, batch 16 x
+    128^3) on seeded synthetic input and target, with the reference's
+    semantics; per-kernel HIP-event times"""
+    import torch
+
+    import symcode as sc
+
+    net = sc.cnn.ConvClosureNet(n, batch_max=batch, cin=4, width=64, cout=6, seed=0, tracefree=True)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    x = net.new_input(batch)
+    x[..., :4] = torch.randn((batch, net.nvox, 4), generator=gen, device="cuda").to(torch.bfloat16)
+    y = torch.randn((batch, net.nvox, net.cout), generator=gen, device="cuda")
+    for _ in range(max(warmup, 1)):
+        net.train_step(x, y, 1e-3, loss="relative", clip=1.0)
+    torch.cuda.synchronize()
+    net.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.train_step(x, y, 1e-3, loss="relative", clip=1.0)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    rep = net.profile_report()
+    net.profile(False)
+    return {"workload": f"learned closure training step, batch {batch} x {n}^3, synthetic", "steps": steps,
+            "ms_per_step": t * 1e3,
+            "kernels_ms_per_step": {k: v[1] / steps for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +266,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the learned-closure training and a-posteriori gradient benchmarks")
+    ap.add_argument("--cnn-only", action="store_true",
+                    help="time only --steps training steps of the batch-16 x 128^3 learned closure on seeded "
+                         "synthetic input (the kernels the counter runs look at)")
     ap.add_argument("--copy-calib-gb", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GB (counter calibration)")
     args = ap.parse_args()
@@ -249,6 +281,9 @@ def main():
 
     import symcode as sc
 
+    if args.cnn_only:
+        print(json.dumps(bench_cnn_only(args.steps, args.warmup)), flush=True)
+        return
     n, nu, cfl = args.n, 2.7e-4, 0.3
     dtype = torch.float64 if args.prec == 64 else torch.float32
     g = sc.Grid(3, n, dtype=dtype, device=0)
