@@ -332,146 +332,299 @@ __device__ __forceinline__ void transpose8x8(const uint32_t in[8][4], uint32_t o
 // Weight gradient of one layer:
 //   dW[t][co][ci] += sum_v dout[v][co] * in[v + (dz,dy,dx)][ci]
 // as GEMMs with M = co, N = ci, K = the voxels of an x row.  Persistent
-// blocks: blockIdx.y = dz + 1, and block p takes the (batch, z) planes p,
-// p + gridDim.x, ... whole, walking y with a sliding window of the input
-// plane z + dz, so each block keeps all nine (dy, dx) taps of its dz in
+// blocks: each plane group of three blocks (dz = -1, 0, 1) takes the (batch,
+// z) planes g, g + ngroups, ... whole, walking y with a sliding window of
+// input rows, so each block keeps all nine (dy, dx) taps of its dz in
 // registers.  Block = 8 wavefronts: wavefronts 0-3 compute (wavefront w owns
-// input channels 16 w .. 16 w + 15), wavefronts 4-7 stage the next dout row
-// and the next input row into the free LDS slots meanwhile (one barrier per
-// row).  Rows are staged transposed ([channel][x], 8 x 8 register
-// transposes); input rows carry an 8-voxel periodic halo on both sides, and
-// the x shifts dx = -1, +1 are formed in registers from aligned LDS reads.
-// Block sums go to part[p][t][co][ci] (k_reduce_partials adds them up); with
+// the input channels 16 (w % NW) .. + 15 of the NW 16-channel tiles, and one
+// 4 / NW share of the output-channel tiles, so a narrow input still uses the
+// four SIMDs), wavefronts 4-7 stage rows.  Staging is software-pipelined:
+// the producers load row sets PD rows ahead into registers and, while row y
+// is computed, write the set for row y + 1 (dout row y + 1, input row y + 2)
+// into the LDS slots not in use (one barrier per row).
+//
+// LDS layout: a row of a 16-channel tile is n / 8 blocks of 8 voxels x 16
+// channels; a block is 16 chunks of 16 bytes, chunk = one channel's 8
+// voxels, at chunk index swz(channel, block) = channel ^ ((4 (channel >> 3))
+// ^ block) & 7); tiles start 16 bytes apart modulo 256.  An MFMA operand read
+// (16 channels of one block per 16 lanes) is then one 256-byte block, which
+// ds_read_b128 serves without bank conflicts, and the staging stores (8
+// lanes = 8 channel blocks of one voxel block, or 8 voxel blocks of one
+// channel block) hit 8 distinct 16-byte slots modulo 128 bytes, which
+// ds_write_b128 needs.  Input rows carry one halo block on each side; the x
+// shifts dx = -1, +1 are formed in registers from the neighbouring blocks.
+// Block sums go to part[g][t][co][ci] (k_reduce_partials adds them up); with
 // db set, the dz = 0 blocks also add the bias gradient sum_v dout[v][co].
-template <int CIN, int COUT>
+__device__ __forceinline__ int wg_swz(int ch, int blk) { return ch ^ (((4 * (ch >> 3)) ^ blk) & 7); }
+
+template <int CIN, int COUT, int PD, bool SHARE_IN>
 __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ in, const bf16* __restrict__ dout,
                                                        float* part, float* db, int n, long nplanes) {
-  // row strides in bf16 elements: 136 (272 B) for dout rows (n <= 128) and
-  // 152 (304 B) for input rows with their halos
-  constexpr int XD = 128 + 8, XL = 128 + 16 + 8;
+  constexpr int NBM = 16;                      // x blocks per row (n <= 128)
   constexpr int CP = CIN < 16 ? 16 : CIN;      // staged input channels (zero-padded)
   constexpr int MT = COUT / 16;
-  constexpr int NW = CP / 16;                  // compute wavefronts with input channels
-  __shared__ __attribute__((aligned(16))) bf16 dT[2 * COUT * XD];  // dout rows y, y + 1
-  __shared__ __attribute__((aligned(16))) bf16 xT[4 * CP * XL];    // input rows y - 1 .. y + 2
-  const int dz = int(blockIdx.y) - 1;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  constexpr int NW = CP / 16;                  // 16-channel input tiles
+  // one input tile (CIN <= 16): compute wavefront w takes input row a = w
+  // (three wavefronts) and every output tile, so each set of shifted B
+  // fragments feeds 3 MT MFMAs; otherwise wavefront w takes input tile w % NW,
+  // every input row and a 4 / NW share of the output tiles
+  constexpr bool ROWW = NW == 1;
+  constexpr int MS = ROWW ? 1 : ((4 / NW) < MT ? (4 / NW) : MT);  // compute wavefronts per input tile
+  constexpr int MTW = MT / MS;                 // output-channel tiles per compute wavefront
+  constexpr int NCW = ROWW ? 3 : NW * MS;      // compute wavefronts in use
+  constexpr int RA = ROWW ? 1 : 3;             // input rows per compute wavefront
+  static_assert(4 % NW == 0 && MT % MS == 0, "wgrad tiling");
+  // tile strides in 16-byte chunks (+1: tiles start 16 bytes apart)
+  constexpr int DTS = NBM * 16 + 1, XTS = (NBM + 2) * 16 + 1;
+  // one staging job (8 voxels x 8 channels) per producer thread and row
+  // (n <= 128): the dout row's blocks, then the input row's; the halos are
+  // written by the jobs of x blocks 0 and n / 8 - 1
+  static_assert(16 * (COUT / 8) + 16 * (CIN / 8) <= 256, "wgrad staging jobs");
+  static_assert(64 % PD == 0, "row sets in flight divide n");
+  __shared__ uint4 dT[2 * MT * DTS];  // dout rows y, y + 1
+  __shared__ uint4 xT[4 * NW * XTS];  // input rows y - 1 .. y + 2
+  // block -> (plane group pg, dz): the three dz blocks of a plane group are
+  // consecutive multiples of 8 apart in the launch order, so they run on one
+  // XCD (workgroups are dealt to the 8 XCDs round-robin) and share its L2
+  const int ngroups = int(gridDim.x / 3);
+  const int xcd = int(blockIdx.x % 8), kx = int(blockIdx.x / 8);
+  const int pg = xcd * (ngroups / 8) + kx / 3;
+  const int dz = kx % 3 - 1;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // w: wave-uniform
   const int li = l & 15, lk = l >> 4;
+  const int wc = ROWW ? 0 : w % NW, wm = ROWW ? 0 : w / NW;  // compute wavefront's input tile and output-tile share
   const bool producer = w >= 4;
-  const int pt = threadIdx.x - 256;  // producer thread index
-  floatx4 acc[3][3][MT];
+  const int pt = int(threadIdx.x) - 256;  // producer thread index
+  floatx4 acc[RA][3][MTW];
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int m = 0; m < MT; ++m) acc[a][d][m] = floatx4{0, 0, 0, 0};
-  if (CIN < CP) {  // zero channel rows that are never loaded
-    for (int i = threadIdx.x; i < 4 * (CP - CIN) * XL; i += 512) {
-      const int s = i / ((CP - CIN) * XL), rem = i % ((CP - CIN) * XL);
-      xT[(s * CP + CIN) * XL + rem] = bf16(0.f);
-    }
+      for (int m = 0; m < MTW; ++m) acc[a][d][m] = floatx4{0, 0, 0, 0};
+  if (CIN < CP) {  // the zero-padded channels are never written: clear all input slots once
+    for (int i = threadIdx.x; i < 4 * NW * XTS; i += 512) xT[i] = uint4{0, 0, 0, 0};
   }
   float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bias partial sums of this producer's dout channel block
   const bool dobias = db != nullptr && dz == 0;
   const int nb8 = n / 8;
   const int nd_row = nb8 * (COUT / 8);
-  // transposed staging by the producer threads: dout row drow (if >= 0) into
-  // dT slot ds, and input row xrow (if >= 0) into xT slot xs
-  auto stage = [&](long drow, int ds, long xrow, int xs) {
-    const int nd = drow >= 0 ? nd_row : 0;
-    const int nx = xrow >= 0 ? (nb8 + 2) * (CIN / 8) : 0;
-    for (int blk = pt; blk < nd + nx; blk += 256) {
-      uint32_t v[8][4], o[8][4];
-      bf16* dst;
-      int rs;
-      if (blk < nd) {
-        const int xb = blk % nb8, cb = blk / nb8;
+  const int nx_row = nb8 * (CIN / 8);
+  uint32_t v[PD][8][4];  // blocks loaded by this producer thread, not yet staged
+  // this producer thread's staging job: a dout block (pt < nd_row) or an
+  // input block, channel block fastest (8 lanes read one voxel's 128 bytes);
+  // threads without a job re-read input block 0.  The global loads are issued
+  // by every producer thread on every row, so the compiler can count the
+  // loads in flight exactly (waiting for one row set while the PD - 1 later
+  // sets stay in flight)
+  const bool jd = pt < nd_row, jx = !jd && pt < nd_row + nx_row;
+  const int jj = jx ? pt - nd_row : 0;
+  const int cb = jd ? pt % (COUT / 8) : jj % (CIN / 8);
+  const int xb = jd ? pt / (COUT / 8) : jj / (CIN / 8);
+  // global loads of this thread's block of dout row drow or input row xrow into v[p]
+  auto load = [&](int p, long drow, long xrow) __attribute__((always_inline)) {
+    const bf16* src = jd ? &dout[(drow * n + xb * 8) * COUT + cb * 8] : &in[(xrow * n + xb * 8) * CIN + cb * 8];
+    const int stride = jd ? COUT : CIN;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const uint4 q = *reinterpret_cast<const uint4*>(&dout[(drow * n + xb * 8 + e) * COUT + cb * 8]);
-          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
-        }
-        dst = &dT[(ds * COUT + cb * 8) * XD + xb * 8];
-        rs = XD;
-        if (dobias) {
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-              bs[c] += __uint_as_float((c & 1) ? (v[e][c >> 1] & 0xffff0000u) : (v[e][c >> 1] << 16));
-        }
-      } else {
-        const int j = blk - nd;
-        const int xb = j % (nb8 + 2), cb = j / (nb8 + 2);  // x from 8 (xb - 1)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int xsrc = wrap(8 * (xb - 1) + e, n);
-          const uint4 q = *reinterpret_cast<const uint4*>(&in[(xrow * n + xsrc) * CIN + cb * 8]);
-          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
-        }
-        dst = &xT[(xs * CP + cb * 8) * XL + xb * 8];
-        rs = XL;
-      }
-      transpose8x8(v, o);
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-        *reinterpret_cast<uint4*>(dst + c * rs) = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+    for (int e = 0; e < 8; ++e) {
+      const uint4 q = *reinterpret_cast<const uint4*>(src + e * stride);
+      v[p][e][0] = q.x; v[p][e][1] = q.y; v[p][e][2] = q.z; v[p][e][3] = q.w;
     }
   };
-  // slot of input row r (r in [-1, n]): (r + 1) & 3
-#pragma unroll 1
-  for (long q = blockIdx.x; q < nplanes; q += gridDim.x) {
-    const int z = int(q % n);
-    const long b = q / n;
-    const long pin = b * n + wrap(z + dz, n);  // input plane
-    __syncthreads();
-    if (producer) {
-      stage(q * n + 0, 0, pin * n + (n - 1), 0);  // dout row 0; input row -1
-      stage(-1, 0, pin * n + 0, 1);               // input row 0
-      stage(-1, 0, pin * n + (1 % n), 2);         // input row 1
-    }
-    __syncthreads();
-#pragma unroll 1
-    for (int y = 0; y < n; ++y) {
-      if (producer) {
-        // next dout row and input row y + 2 into the slots not in use
-        if (y + 1 < n) stage(q * n + y + 1, (y + 1) & 1, pin * n + (y + 2) % n, (y + 3) & 3);
-      } else if (w < NW) {
-        const int sl[3] = {y & 3, (y + 1) & 3, (y + 2) & 3};  // input rows y - 1, y, y + 1
-        const bf16* drow = &dT[((y & 1) * COUT) * XD];
-#pragma unroll 1
-        for (int k0 = 0; k0 < n; k0 += 32) {
-          const int x0 = k0 + 8 * lk;
-          bf16x8 av[MT];
+  // transposed LDS stores of v[p]: a dout block (if dvalid) into dT slot ds,
+  // an input block (if xvalid) into xT slot xs and, for x blocks 0 and
+  // n / 8 - 1, into the periodic halo block on the other side
+  auto store = [&](int p, bool dvalid, int ds, bool xvalid, int xs) __attribute__((always_inline)) {
+    uint32_t o[8][4];
+    transpose8x8(v[p], o);
+    const int ch0 = (cb % 2) * 8;  // channel of o[0] within its 16-channel tile
+    if (jd && dvalid) {
+      uint4* tile = &dT[(ds * MT + cb / 2) * DTS + xb * 16];
 #pragma unroll
-          for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8*>(&drow[(m * 16 + li) * XD + x0]);
+      for (int c = 0; c < 8; ++c) tile[wg_swz(ch0 + c, xb)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      if (dobias) {  // o[c] holds channel cb * 8 + c at the block's 8 voxels, two per dword
+        typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+        const bf16x2_t one2 = {__bf16(1.0f), __bf16(1.0f)};
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const bf16* xr = &xT[(sl[a] * CP + 16 * w + li) * XL + x0];
-            // dwords: m = (x0 - 2, x0 - 1), L0 = x0 .. x0 + 7, p = (x0 + 8, x0 + 9)
-            const uint32_t mm = *reinterpret_cast<const uint32_t*>(xr + 6);
-            const uint4 L0 = *reinterpret_cast<const uint4*>(xr + 8);
-            const uint32_t pp = *reinterpret_cast<const uint32_t*>(xr + 16);
-            uint4 S[3];
-            // dx = -1: x0 - 1 .. x0 + 6;  dx = 0: x0 .. x0 + 7;  dx = +1: x0 + 1 .. x0 + 8
-            S[0] = uint4{__builtin_amdgcn_alignbyte(L0.x, mm, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
-                         __builtin_amdgcn_alignbyte(L0.z, L0.y, 2), __builtin_amdgcn_alignbyte(L0.w, L0.z, 2)};
-            S[1] = L0;
-            S[2] = uint4{__builtin_amdgcn_alignbyte(L0.y, L0.x, 2), __builtin_amdgcn_alignbyte(L0.z, L0.y, 2),
-                         __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(pp, L0.w, 2)};
+        for (int c = 0; c < 8; ++c)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              bf16x8 bv;
-              __builtin_memcpy(&bv, &S[d], 16);
-#pragma unroll
-              for (int m = 0; m < MT; ++m)
-                acc[a][d][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[a][d][m], 0, 0, 0);
-            }
+          for (int j = 0; j < 4; ++j) {
+            bf16x2_t pr;
+            __builtin_memcpy(&pr, &o[c][j], 4);
+            bs[c] = __builtin_amdgcn_fdot2_f32_bf16(pr, one2, bs[c], false);  // exact products, fp32 sums
           }
+      }
+    } else if (jx && xvalid) {
+      uint4* tile = &xT[(xs * NW + cb / 2) * XTS];
+      const int bk = xb + 1;  // LDS block of x block xb (block 0 is the left halo)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) tile[bk * 16 + wg_swz(ch0 + c, bk)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      if (xb == 0 || xb == nb8 - 1) {
+        const int hb = xb == 0 ? nb8 + 1 : 0;  // x + n or x - n
+#pragma unroll
+        for (int c = 0; c < 8; ++c) tile[hb * 16 + wg_swz(ch0 + c, hb)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      }
+    }
+  };
+  // one row of MFMAs: dout row y against input rows y - 1, y, y + 1, in
+  // steps of (32 voxels, one input row).  The LDS reads of the next step are
+  // issued before the MFMAs of the current one (two register sets).  A lane
+  // reads x block bx = k0 / 8 + lk; with k0 a multiple of 64 in the loop,
+  // bx & 7 and so the chunk swizzle are fixed per lane and x step, and the
+  // lane parts of the addresses are computed once.
+  struct BOps {
+    u32x4 p, c, x;  // input blocks bx, bx + 1, bx + 2 (x - 8 .., x .., x + 8 ..)
+  };
+  int offa[2], offb[2][3];  // chunk offsets of this lane's reads in x steps k0 and k0 + 32
+#pragma unroll
+  for (int s2 = 0; s2 < 2; ++s2) {
+    const int b0 = lk + 4 * s2;
+    offa[s2] = b0 * 16 + wg_swz(li, b0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) offb[s2][j] = (b0 + j) * 16 + wg_swz(li, b0 + j);
+  }
+  const u32x4* dTv = reinterpret_cast<const u32x4*>(dT);
+  const u32x4* xTv = reinterpret_cast<const u32x4*>(xT);
+  // with four output tiles per wavefront the registers are short: the
+  // offsets are then recomputed per read
+  constexpr bool PRE = MTW < 4;
+  auto oa = [&](int s2) __attribute__((always_inline)) {
+    const int b0 = lk + 4 * s2;
+    return PRE ? offa[s2] : b0 * 16 + wg_swz(li, b0);
+  };
+  auto ob = [&](int s2, int j) __attribute__((always_inline)) {
+    const int b0 = lk + 4 * s2 + j;
+    return PRE ? offb[s2][j] : b0 * 16 + wg_swz(li, b0);
+  };
+  auto read_a = [&](u32x4* av, int drow0, int k0, int s2) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < MTW; ++m) av[m] = dTv[drow0 + m * DTS + 2 * k0 + oa(s2)];
+  };
+  auto read_b = [&](BOps& o, int slot, int k0, int s2) __attribute__((always_inline)) {
+    const int base = (slot * NW + wc) * XTS + 2 * k0;  // 2 k0 = (k0 / 8) * 16 chunks
+    o.p = xTv[base + ob(s2, 0)];
+    o.c = xTv[base + ob(s2, 1)];
+    o.x = xTv[base + ob(s2, 2)];
+  };
+  auto mfma_row = [&](const u32x4* am, const BOps& o, floatx4 (&accr)[3][MTW]) __attribute__((always_inline)) {
+    // only p.w and x.x enter the MFMAs; using the whole chunks keeps their
+    // LDS reads 16 bytes wide, which these lane patterns serve without bank
+    // conflicts (dword reads would conflict 4-way).  No instruction.
+    asm volatile("" : : "v"(o.p), "v"(o.x));
+    const u32x4 P = o.p, L0 = o.c, N = o.x;
+    u32x4 S[3];
+    // dx = -1: x0 - 1 .. x0 + 6;  dx = 0: x0 .. x0 + 7;  dx = +1: x0 + 1 .. x0 + 8
+    S[0] = u32x4{__builtin_amdgcn_alignbyte(L0.x, P.w, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
+                 __builtin_amdgcn_alignbyte(L0.z, L0.y, 2), __builtin_amdgcn_alignbyte(L0.w, L0.z, 2)};
+    S[1] = L0;
+    S[2] = u32x4{__builtin_amdgcn_alignbyte(L0.y, L0.x, 2), __builtin_amdgcn_alignbyte(L0.z, L0.y, 2),
+                 __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(N.x, L0.w, 2)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      bf16x8 bv;
+      __builtin_memcpy(&bv, &S[d], 16);
+#pragma unroll
+      for (int m = 0; m < MTW; ++m) {
+        bf16x8 av;
+        __builtin_memcpy(&av, &am[m], 16);
+        accr[d][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, accr[d][m], 0, 0, 0);
+      }
+    }
+  };
+  auto compute_row = [&](int y) __attribute__((always_inline)) {
+    const int sl[3] = {y & 3, (y + 1) & 3, (y + 2) & 3};  // input rows y - 1, y, y + 1
+    const int drow0 = ((y & 1) * MT + wm * MTW) * DTS;
+    u32x4 a0[MTW], a1[MTW];
+    BOps b0, b1;
+    if constexpr (ROWW) {
+      const int sw = (y + w) & 3;  // this wavefront's input row y - 1 + w
+      read_a(a0, drow0, 0, 0);
+      read_b(b0, sw, 0, 0);
+#pragma unroll 1
+      for (int k0 = 0; k0 < n; k0 += 64) {
+        read_a(a1, drow0, k0, 1);
+        read_b(b1, sw, k0, 1);
+        mfma_row(a0, b0, acc[0]);
+        if (k0 + 64 < n) {
+          read_a(a0, drow0, k0 + 64, 0);
+          read_b(b0, sw, k0 + 64, 0);
+        }
+        mfma_row(a1, b1, acc[0]);
+      }
+      return;
+    }
+    if constexpr (!ROWW) {
+      read_a(a0, drow0, 0, 0);
+      read_b(b0, sl[0], 0, 0);
+#pragma unroll 1
+      for (int k0 = 0; k0 < n; k0 += 64) {  // n is a multiple of 64: two x steps per iteration
+        read_b(b1, sl[1], k0, 0);
+        mfma_row(a0, b0, acc[0]);
+        read_b(b0, sl[2], k0, 0);
+        mfma_row(a0, b1, acc[RA - 2]);
+        read_a(a1, drow0, k0, 1);
+        read_b(b1, sl[0], k0, 1);
+        mfma_row(a0, b0, acc[RA - 1]);
+        read_b(b0, sl[1], k0, 1);
+        mfma_row(a1, b1, acc[0]);
+        read_b(b1, sl[2], k0, 1);
+        mfma_row(a1, b0, acc[RA - 2]);
+        if (k0 + 64 < n) {
+          read_a(a0, drow0, k0 + 64, 0);
+          read_b(b0, sl[0], k0 + 64, 0);
+        }
+        mfma_row(a1, b1, acc[RA - 1]);
+      }
+    }
+  };
+  // Row set s = dout row s + 1 and input row s + 2; it is loaded into
+  // v[s % PD] during row s - PD (or before the plane's first row) and staged
+  // during row s.  Slot of input row r (r in [-1, n]): (r + 1) & 3.
+  // Producers and compute wavefronts run separate loops (the role is
+  // wave-uniform) with the same barriers: two per plane and one per row.
+  if (producer) {
+#pragma unroll 1
+    for (long q = pg; q < nplanes; q += ngroups) {
+      // the plane group's three blocks read the same plane of the wider
+      // tensor at the same time: input plane q (SHARE_IN) or dout plane q
+      const int z = int(q % n);
+      const long b = q / n;
+      const long pin = SHARE_IN ? q : b * n + wrap(z + dz, n);    // input plane
+      const long pout = SHARE_IN ? b * n + wrap(z - dz, n) : q;  // dout plane
+      __syncthreads();
+      load(0, pout * n + 0, pin * n + (n - 1));  // dout row 0; input row -1
+      store(0, true, 0, true, 0);
+      load(0, pout * n + 0, pin * n + 0);  // input row 0
+      store(0, false, 0, true, 1);
+      load(0, pout * n + 0, pin * n + (1 % n));  // input row 1
+      store(0, false, 0, true, 2);
+#pragma unroll
+      for (int s = 0; s < PD; ++s) load(s, pout * n + min(s + 1, n - 1), pin * n + (s + 2) % n);
+      __syncthreads();
+#pragma unroll 1
+      for (int y0 = 0; y0 < n; y0 += PD) {
+#pragma unroll
+        for (int p = 0; p < PD; ++p) {
+          const int y = y0 + p;
+          // set y: dout row y + 1, input row y + 2.  Set n - 1 (dout row n
+          // does not exist) is loaded like the others and its input block is
+          // written to the free slots, which the next plane overwrites.
+          store(p, y + 1 < n, (y + 1) & 1, true, (y + 3) & 3);
+          load(p, pout * n + min(y + PD + 1, n - 1), pin * n + (y + PD + 2) % n);
+          __syncthreads();
         }
       }
+    }
+  } else {
+#pragma unroll 1
+    for (long q = pg; q < nplanes; q += ngroups) {
       __syncthreads();
+      __syncthreads();
+#pragma unroll 1
+      for (int y = 0; y < n; ++y) {
+        if (w < NCW) compute_row(y);
+        __syncthreads();
+      }
     }
   }
   if (dobias) {
@@ -485,27 +638,27 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
     }
     __syncthreads();
     if (threadIdx.x < COUT) {
-      const int cb = threadIdx.x / 8, c = threadIdx.x % 8;
+      const int cbt = threadIdx.x / 8, c = threadIdx.x % 8;
       float t = 0.f;
-      for (int xb = 0; xb < nb8; ++xb) t += red[(cb * nb8 + xb) * 8 + c];
+      for (int x8 = 0; x8 < nb8; ++x8) t += red[(x8 * (COUT / 8) + cbt) * 8 + c];
       atomicAdd(&db[threadIdx.x], t);
     }
   }
-  if (producer || w >= NW) return;
-  // D[co = 16 m + 4 lk + r][ci = 16 w + li]
-  const int ci = 16 * w + li;
+  if (producer || w >= NCW) return;
+  // D[co = 16 (wm MTW + m) + 4 lk + r][ci = 16 wc + li]
+  const int ci = 16 * wc + li;
   if (ci >= CIN) return;
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int t = (dz + 1) * 9 + a * 3 + d;
+      const int t = (dz + 1) * 9 + (ROWW ? w : a) * 3 + d;
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
+      for (int m = 0; m < MTW; ++m)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int co = m * 16 + 4 * lk + r;
-          part[((long(blockIdx.x) * 27 + t) * COUT + co) * CIN + ci] = acc[a][d][m][r];
+          const int co = (wm * MTW + m) * 16 + 4 * lk + r;
+          part[((long(pg) * 27 + t) * COUT + co) * CIN + ci] = acc[a][d][m][r];
         }
     }
 }
@@ -769,21 +922,33 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
   throw std::runtime_error("unsupported conv shape");
 }
 
+// row sets in flight per producer thread (registers: 32 VGPRs each)
+#ifndef SC_WG_PD_WIDE
+#define SC_WG_PD_WIDE 2
+#endif
+#ifndef SC_WG_PD_NARROW
+#define SC_WG_PD_NARROW 4
+#endif
+
 void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* dout, float* grad, float* db,
                   int batch) {
   const long nplanes = long(batch) * c->n;
-  const long G = std::min<long>(nplanes, std::max(1, c->ncu / 3));  // one block per CU
+  // plane groups: a multiple of 8 (one XCD holds a group's three blocks), at
+  // most one block per CU
+  const long G = 8 * std::max(1, c->ncu / 24);
   const long len = 27L * coutp * cinp;
   if (c->part_len < G * len) {
     if (c->part) CNN_HIP(hipFree(c->part));
     CNN_HIP(hipMalloc(&c->part, G * len * sizeof(float)));
     c->part_len = G * len;
   }
-  dim3 grid(unsigned(G), 3);
+  dim3 grid(unsigned(3 * G));
   c->pbeg("conv_wgrad", 2.0 * double(nplanes) * c->n * c->n * 27.0 * ((cinp + 15) / 16 * 16) * coutp);
 #define WG(CI, CO)                                                                                              \
   if (cinp == CI && coutp == CO) {                                                                               \
-    hipLaunchKernelGGL((k_conv_wgrad<CI, CO>), grid, dim3(512), 0, c->stream, in, dout, c->part, db, c->n, nplanes); \
+    constexpr bool WIDE = CI >= 64 && CO >= 64;                                                                  \
+    hipLaunchKernelGGL((k_conv_wgrad<CI, CO, WIDE ? SC_WG_PD_WIDE : SC_WG_PD_NARROW, (CI > CO)>), grid,           \
+                       dim3(512), 0, c->stream, in, dout, c->part, db, c->n, nplanes);                            \
     CNN_HIP(hipGetLastError());                                                                                  \
   } else
   WG(8, 64) WG(64, 64) WG(64, 16) { throw std::runtime_error("unsupported wgrad shape"); }
