@@ -329,16 +329,16 @@ __device__ __forceinline__ void transpose8x8(const uint32_t in[8][4], uint32_t o
     }
 }
 
-// Weight gradient of one layer:
+// Weight gradient of a 64 -> 64 layer:
 //   dW[t][co][ci] += sum_v dout[v][co] * in[v + (dz,dy,dx)][ci]
 // as GEMMs with M = co, N = ci, K = the voxels of an x row.  Persistent
 // blocks: each plane group of three blocks (dz = -1, 0, 1) takes the (batch,
 // z) planes g, g + ngroups, ... whole, walking y with a sliding window of
 // input rows, so each block keeps all nine (dy, dx) taps of its dz in
-// registers.  Block = 8 wavefronts: wavefronts 0-3 compute (wavefront w owns
-// the input channels 16 (w % NW) .. + 15 of the NW 16-channel tiles, and one
-// 4 / NW share of the output-channel tiles, so a narrow input still uses the
-// four SIMDs), wavefronts 4-7 stage rows.  Staging is software-pipelined:
+// registers (36 accumulator tiles per wavefront: all taps of a block in one
+// block would not fit).  Block = 8 wavefronts: wavefronts 0-3 compute
+// (wavefront w owns input channels 16 w .. 16 w + 15 and all output
+// channels), wavefronts 4-7 stage rows.  Staging is software-pipelined:
 // the producers load row sets PD rows ahead into registers and, while row y
 // is computed, write the set for row y + 1 (dout row y + 1, input row y + 2)
 // into the LDS slots not in use (one barrier per row).
@@ -357,23 +357,13 @@ __device__ __forceinline__ void transpose8x8(const uint32_t in[8][4], uint32_t o
 // db set, the dz = 0 blocks also add the bias gradient sum_v dout[v][co].
 __device__ __forceinline__ int wg_swz(int ch, int blk) { return ch ^ (((4 * (ch >> 3)) ^ blk) & 7); }
 
-template <int CIN, int COUT, int PD, bool SHARE_IN>
+template <int CIN, int COUT, int PD>
 __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ in, const bf16* __restrict__ dout,
                                                        float* part, float* db, int n, long nplanes) {
   constexpr int NBM = 16;                      // x blocks per row (n <= 128)
-  constexpr int CP = CIN < 16 ? 16 : CIN;      // staged input channels (zero-padded)
-  constexpr int MT = COUT / 16;
-  constexpr int NW = CP / 16;                  // 16-channel input tiles
-  // one input tile (CIN <= 16): compute wavefront w takes input row a = w
-  // (three wavefronts) and every output tile, so each set of shifted B
-  // fragments feeds 3 MT MFMAs; otherwise wavefront w takes input tile w % NW,
-  // every input row and a 4 / NW share of the output tiles
-  constexpr bool ROWW = NW == 1;
-  constexpr int MS = ROWW ? 1 : ((4 / NW) < MT ? (4 / NW) : MT);  // compute wavefronts per input tile
-  constexpr int MTW = MT / MS;                 // output-channel tiles per compute wavefront
-  constexpr int NCW = ROWW ? 3 : NW * MS;      // compute wavefronts in use
-  constexpr int RA = ROWW ? 1 : 3;             // input rows per compute wavefront
-  static_assert(4 % NW == 0 && MT % MS == 0, "wgrad tiling");
+  constexpr int MT = COUT / 16;                // output-channel tiles, all per compute wavefront
+  constexpr int NW = CIN / 16;                 // 16-channel input tiles, one per compute wavefront
+  static_assert(CIN == 64 && COUT == 64, "the 64 x 64 layer (the narrow layers use k_conv_wgrad3)");
   // tile strides in 16-byte chunks (+1: tiles start 16 bytes apart)
   constexpr int DTS = NBM * 16 + 1, XTS = (NBM + 2) * 16 + 1;
   // one staging job (8 voxels x 8 channels) per producer thread and row
@@ -392,19 +382,16 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   const int dz = kx % 3 - 1;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // w: wave-uniform
   const int li = l & 15, lk = l >> 4;
-  const int wc = ROWW ? 0 : w % NW, wm = ROWW ? 0 : w / NW;  // compute wavefront's input tile and output-tile share
+  const int wc = w;  // compute wavefront's input tile
   const bool producer = w >= 4;
   const int pt = int(threadIdx.x) - 256;  // producer thread index
-  floatx4 acc[RA][3][MTW];
+  floatx4 acc[3][3][MT];
 #pragma unroll
-  for (int a = 0; a < RA; ++a)
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int m = 0; m < MTW; ++m) acc[a][d][m] = floatx4{0, 0, 0, 0};
-  if (CIN < CP) {  // the zero-padded channels are never written: clear all input slots once
-    for (int i = threadIdx.x; i < 4 * NW * XTS; i += 512) xT[i] = uint4{0, 0, 0, 0};
-  }
+      for (int m = 0; m < MT; ++m) acc[a][d][m] = floatx4{0, 0, 0, 0};
   float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // bias partial sums of this producer's dout channel block
   const bool dobias = db != nullptr && dz == 0;
   const int nb8 = n / 8;
@@ -469,44 +456,25 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   // one row of MFMAs: dout row y against input rows y - 1, y, y + 1, in
   // steps of (32 voxels, one input row).  The LDS reads of the next step are
   // issued before the MFMAs of the current one (two register sets).  A lane
-  // reads x block bx = k0 / 8 + lk; with k0 a multiple of 64 in the loop,
-  // bx & 7 and so the chunk swizzle are fixed per lane and x step, and the
-  // lane parts of the addresses are computed once.
+  // reads x block bx = k0 / 8 + lk.
   struct BOps {
     u32x4 p, c, x;  // input blocks bx, bx + 1, bx + 2 (x - 8 .., x .., x + 8 ..)
   };
-  int offa[2], offb[2][3];  // chunk offsets of this lane's reads in x steps k0 and k0 + 32
-#pragma unroll
-  for (int s2 = 0; s2 < 2; ++s2) {
-    const int b0 = lk + 4 * s2;
-    offa[s2] = b0 * 16 + wg_swz(li, b0);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) offb[s2][j] = (b0 + j) * 16 + wg_swz(li, b0 + j);
-  }
   const u32x4* dTv = reinterpret_cast<const u32x4*>(dT);
   const u32x4* xTv = reinterpret_cast<const u32x4*>(xT);
-  // with four output tiles per wavefront the registers are short: the
-  // offsets are then recomputed per read
-  constexpr bool PRE = MTW < 4;
-  auto oa = [&](int s2) __attribute__((always_inline)) {
-    const int b0 = lk + 4 * s2;
-    return PRE ? offa[s2] : b0 * 16 + wg_swz(li, b0);
-  };
-  auto ob = [&](int s2, int j) __attribute__((always_inline)) {
-    const int b0 = lk + 4 * s2 + j;
-    return PRE ? offb[s2][j] : b0 * 16 + wg_swz(li, b0);
-  };
-  auto read_a = [&](u32x4* av, int drow0, int k0, int s2) __attribute__((always_inline)) {
+  auto chunk = [&](int blk) __attribute__((always_inline)) { return blk * 16 + wg_swz(li, blk); };
+  auto read_a = [&](u32x4* av, int drow0, int k0) __attribute__((always_inline)) {
 #pragma unroll
-    for (int m = 0; m < MTW; ++m) av[m] = dTv[drow0 + m * DTS + 2 * k0 + oa(s2)];
+    for (int m = 0; m < MT; ++m) av[m] = dTv[drow0 + m * DTS + chunk(k0 / 8 + lk)];
   };
-  auto read_b = [&](BOps& o, int slot, int k0, int s2) __attribute__((always_inline)) {
-    const int base = (slot * NW + wc) * XTS + 2 * k0;  // 2 k0 = (k0 / 8) * 16 chunks
-    o.p = xTv[base + ob(s2, 0)];
-    o.c = xTv[base + ob(s2, 1)];
-    o.x = xTv[base + ob(s2, 2)];
+  auto read_b = [&](BOps& o, int slot, int k0) __attribute__((always_inline)) {
+    const u32x4* xr = &xTv[(slot * NW + wc) * XTS];
+    const int bx = k0 / 8 + lk;
+    o.p = xr[chunk(bx)];
+    o.c = xr[chunk(bx + 1)];
+    o.x = xr[chunk(bx + 2)];
   };
-  auto mfma_row = [&](const u32x4* am, const BOps& o, floatx4 (&accr)[3][MTW]) __attribute__((always_inline)) {
+  auto mfma_row = [&](const u32x4* am, const BOps& o, floatx4 (&accr)[3][MT]) __attribute__((always_inline)) {
     // only p.w and x.x enter the MFMAs; using the whole chunks keeps their
     // LDS reads 16 bytes wide, which these lane patterns serve without bank
     // conflicts (dword reads would conflict 4-way).  No instruction.
@@ -524,7 +492,7 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
       bf16x8 bv;
       __builtin_memcpy(&bv, &S[d], 16);
 #pragma unroll
-      for (int m = 0; m < MTW; ++m) {
+      for (int m = 0; m < MT; ++m) {
         bf16x8 av;
         __builtin_memcpy(&av, &am[m], 16);
         accr[d][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, accr[d][m], 0, 0, 0);
@@ -533,48 +501,29 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   };
   auto compute_row = [&](int y) __attribute__((always_inline)) {
     const int sl[3] = {y & 3, (y + 1) & 3, (y + 2) & 3};  // input rows y - 1, y, y + 1
-    const int drow0 = ((y & 1) * MT + wm * MTW) * DTS;
-    u32x4 a0[MTW], a1[MTW];
+    const int drow0 = (y & 1) * MT * DTS;
+    u32x4 a0[MT], a1[MT];
     BOps b0, b1;
-    if constexpr (ROWW) {
-      const int sw = (y + w) & 3;  // this wavefront's input row y - 1 + w
-      read_a(a0, drow0, 0, 0);
-      read_b(b0, sw, 0, 0);
+    read_a(a0, drow0, 0);
+    read_b(b0, sl[0], 0);
 #pragma unroll 1
-      for (int k0 = 0; k0 < n; k0 += 64) {
-        read_a(a1, drow0, k0, 1);
-        read_b(b1, sw, k0, 1);
-        mfma_row(a0, b0, acc[0]);
-        if (k0 + 64 < n) {
-          read_a(a0, drow0, k0 + 64, 0);
-          read_b(b0, sw, k0 + 64, 0);
-        }
-        mfma_row(a1, b1, acc[0]);
+    for (int k0 = 0; k0 < n; k0 += 64) {  // n is a multiple of 64: two x steps per iteration
+      read_b(b1, sl[1], k0);
+      mfma_row(a0, b0, acc[0]);
+      read_b(b0, sl[2], k0);
+      mfma_row(a0, b1, acc[1]);
+      read_a(a1, drow0, k0 + 32);
+      read_b(b1, sl[0], k0 + 32);
+      mfma_row(a0, b0, acc[2]);
+      read_b(b0, sl[1], k0 + 32);
+      mfma_row(a1, b1, acc[0]);
+      read_b(b1, sl[2], k0 + 32);
+      mfma_row(a1, b0, acc[1]);
+      if (k0 + 64 < n) {
+        read_a(a0, drow0, k0 + 64);
+        read_b(b0, sl[0], k0 + 64);
       }
-      return;
-    }
-    if constexpr (!ROWW) {
-      read_a(a0, drow0, 0, 0);
-      read_b(b0, sl[0], 0, 0);
-#pragma unroll 1
-      for (int k0 = 0; k0 < n; k0 += 64) {  // n is a multiple of 64: two x steps per iteration
-        read_b(b1, sl[1], k0, 0);
-        mfma_row(a0, b0, acc[0]);
-        read_b(b0, sl[2], k0, 0);
-        mfma_row(a0, b1, acc[RA - 2]);
-        read_a(a1, drow0, k0, 1);
-        read_b(b1, sl[0], k0, 1);
-        mfma_row(a0, b0, acc[RA - 1]);
-        read_b(b0, sl[1], k0, 1);
-        mfma_row(a1, b1, acc[0]);
-        read_b(b1, sl[2], k0, 1);
-        mfma_row(a1, b0, acc[RA - 2]);
-        if (k0 + 64 < n) {
-          read_a(a0, drow0, k0 + 64, 0);
-          read_b(b0, sl[0], k0 + 64, 0);
-        }
-        mfma_row(a1, b1, acc[RA - 1]);
-      }
+      mfma_row(a1, b1, acc[2]);
     }
   };
   // Row set s = dout row s + 1 and input row s + 2; it is loaded into
@@ -585,12 +534,11 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   if (producer) {
 #pragma unroll 1
     for (long q = pg; q < nplanes; q += ngroups) {
-      // the plane group's three blocks read the same plane of the wider
-      // tensor at the same time: input plane q (SHARE_IN) or dout plane q
+      // the plane group's three blocks read the same dout plane q at the same time
       const int z = int(q % n);
       const long b = q / n;
-      const long pin = SHARE_IN ? q : b * n + wrap(z + dz, n);    // input plane
-      const long pout = SHARE_IN ? b * n + wrap(z - dz, n) : q;  // dout plane
+      const long pin = b * n + wrap(z + dz, n);  // input plane
+      const long pout = q;                       // dout plane
       __syncthreads();
       load(0, pout * n + 0, pin * n + (n - 1));  // dout row 0; input row -1
       store(0, true, 0, true, 0);
@@ -622,7 +570,7 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
       __syncthreads();
 #pragma unroll 1
       for (int y = 0; y < n; ++y) {
-        if (w < NCW) compute_row(y);
+        compute_row(y);
         __syncthreads();
       }
     }
@@ -644,23 +592,293 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
       atomicAdd(&db[threadIdx.x], t);
     }
   }
-  if (producer || w >= NCW) return;
-  // D[co = 16 (wm MTW + m) + 4 lk + r][ci = 16 wc + li]
+  if (producer) return;
+  // D[co = 16 m + 4 lk + r][ci = 16 wc + li]
   const int ci = 16 * wc + li;
-  if (ci >= CIN) return;
 #pragma unroll
-  for (int a = 0; a < RA; ++a)
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int t = (dz + 1) * 9 + (ROWW ? w : a) * 3 + d;
+      const int t = (dz + 1) * 9 + a * 3 + d;
 #pragma unroll
-      for (int m = 0; m < MTW; ++m)
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int co = (wm * MTW + m) * 16 + 4 * lk + r;
+          const int co = m * 16 + 4 * lk + r;
           part[((long(pg) * 27 + t) * COUT + co) * CIN + ci] = acc[a][d][m][r];
         }
     }
+}
+
+// Weight gradient of the two narrow layers with all 27 taps in one block, so
+// each staged row reaches one compute unit once (the three-block form above
+// moves every row into three compute units, which bounds these layers by
+// the bytes delivered per unit, not by MFMAs).  Block p takes the (batch, z)
+// planes p, p + gridDim.x, ... whole.  Staging (wavefronts NCWV .. NCWV + 3),
+// LDS layout and row pipeline are those of k_conv_wgrad.
+//  INPACK (CIN <= 8, COUT = 64): the input planes z - 1, z, z + 1 are staged
+//    and (tap, input channel) pairs go on M: compute wavefront w owns the
+//    taps s = 2 w, 2 w + 1 of s = 3 (dz + 1) + dy + 1 (lanes 0-7 and 8-15,
+//    one channel each), reads their x blocks once and forms the dx = -1, 0, 1
+//    fragments, each against the 4 output tiles, and tap s = 8 (lanes 0-7;
+//    lanes 8-15 repeat it and their rows are dropped) against output tile w:
+//    15 MFMAs per 32 voxels.
+//  otherwise (CIN = 64, COUT = 16): the input plane z and the dout planes
+//    z + 1, z, z - 1 (dz = -1, 0, 1) are staged; compute wavefront w owns
+//    input tile w and all 27 taps (27 MFMAs per 32 voxels).
+template <int CIN, int COUT, int PD>
+__global__ void __launch_bounds__(512, 1)
+    k_conv_wgrad3(const bf16* __restrict__ in, const bf16* __restrict__ dout, float* part, float* db, int n,
+                  long nplanes) {
+  constexpr bool INPACK = CIN <= 8;
+  constexpr int NCWV = 4;  // compute wavefronts
+  constexpr int NBM = 16;               // x blocks per row (n <= 128)
+  constexpr int CP = CIN < 16 ? 16 : CIN;
+  constexpr int MT = COUT / 16;
+  constexpr int NW = CP / 16;
+  constexpr int NPI = INPACK ? 3 : 1;  // input planes per row step
+  constexpr int NPO = INPACK ? 1 : 3;  // dout planes per row step
+  constexpr int DTS = NBM * 16 + 1, XTS = (NBM + 2) * 16 + 1;
+  static_assert(NPO * 16 * (COUT / 8) + NPI * 16 * (CIN / 8) <= 256, "staging jobs");
+  static_assert(INPACK ? (MT == 4 && NW == 1) : (MT == 1 && NW == 4), "wgrad3 shapes");
+  static_assert(64 % PD == 0, "row sets in flight divide n");
+  __shared__ uint4 dT[NPO * 2 * MT * DTS];  // [dout plane][row slot y & 1][tile]
+  __shared__ uint4 xT[NPI * 4 * NW * XTS];  // [input plane][row slot (r + 1) & 3][tile]
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // w: wave-uniform
+  const int li = l & 15, lk = l >> 4;
+  const bool producer = w >= NCWV;
+  const int pt = int(threadIdx.x) - 64 * NCWV;  // producer thread index
+  constexpr int NACC = INPACK ? 3 * MT + 3 : 27;
+  floatx4 acc[NACC];
+#pragma unroll
+  for (int i = 0; i < NACC; ++i) acc[i] = floatx4{0, 0, 0, 0};
+  if (CIN < CP) {  // the zero-padded channels are never written: clear all input slots once
+    for (int i = threadIdx.x; i < NPI * 4 * NW * XTS; i += 64 * (NCWV + 4)) xT[i] = uint4{0, 0, 0, 0};
+  }
+  float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int nb8 = n / 8;
+  const int ndp = nb8 * (COUT / 8), nxp = nb8 * (CIN / 8);  // jobs per dout / input plane row
+  const int nd_row = NPO * ndp, nx_row = NPI * nxp;
+  const bool jd = pt < nd_row, jx = !jd && pt < nd_row + nx_row;
+  const int jj = jd ? pt : (jx ? pt - nd_row : 0);
+  const int jpl = jd ? jj / ndp : jj / nxp;          // the job's plane index (po or pi)
+  const int jr = jd ? jj % ndp : jj % nxp;
+  const int cb = jd ? jr % (COUT / 8) : jr % (CIN / 8);
+  const int xb = jd ? jr / (COUT / 8) : jr / (CIN / 8);
+  // the bias gradient: every dout row once (INPACK: the one dout plane;
+  // otherwise the dz = 0 plane)
+  const bool dobias = db != nullptr && (INPACK || jpl == 1);
+  uint32_t v[PD][8][4];
+  long myplane = 0;  // this job's plane, set per block plane
+  auto load = [&](int p, int yd, int yx) __attribute__((always_inline)) {
+    const bf16* src = jd ? &dout[((myplane * n + yd) * n + xb * 8) * COUT + cb * 8]
+                         : &in[((myplane * n + yx) * n + xb * 8) * CIN + cb * 8];
+    const int stride = jd ? COUT : CIN;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const uint4 q = *reinterpret_cast<const uint4*>(src + e * stride);
+      v[p][e][0] = q.x; v[p][e][1] = q.y; v[p][e][2] = q.z; v[p][e][3] = q.w;
+    }
+  };
+  auto store = [&](int p, bool dvalid, int ds, bool xvalid, int xs) __attribute__((always_inline)) {
+    uint32_t o[8][4];
+    transpose8x8(v[p], o);
+    const int ch0 = (cb % 2) * 8;
+    if (jd && dvalid) {
+      uint4* tile = &dT[((jpl * 2 + ds) * MT + cb / 2) * DTS + xb * 16];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) tile[wg_swz(ch0 + c, xb)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      if (dobias) {
+        typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+        const bf16x2_t one2 = {__bf16(1.0f), __bf16(1.0f)};
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            bf16x2_t pr;
+            __builtin_memcpy(&pr, &o[c][j], 4);
+            bs[c] = __builtin_amdgcn_fdot2_f32_bf16(pr, one2, bs[c], false);
+          }
+      }
+    } else if (jx && xvalid) {
+      uint4* tile = &xT[((jpl * 4 + xs) * NW + cb / 2) * XTS];
+      const int bk = xb + 1;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) tile[bk * 16 + wg_swz(ch0 + c, bk)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      if (xb == 0 || xb == nb8 - 1) {
+        const int hb = xb == 0 ? nb8 + 1 : 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) tile[hb * 16 + wg_swz(ch0 + c, hb)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      }
+    }
+  };
+  const u32x4* dTv = reinterpret_cast<const u32x4*>(dT);
+  const u32x4* xTv = reinterpret_cast<const u32x4*>(xT);
+  // the x shifts: from the chunks of x blocks bx, bx + 1, bx + 2 (P, C, N)
+  auto shifts = [&](const u32x4& P, const u32x4& C, const u32x4& N, u32x4* S) __attribute__((always_inline)) {
+    asm volatile("" : : "v"(P), "v"(N));  // keeps the reads 16 bytes wide (conflict-free); no instruction
+    S[0] = u32x4{__builtin_amdgcn_alignbyte(C.x, P.w, 2), __builtin_amdgcn_alignbyte(C.y, C.x, 2),
+                 __builtin_amdgcn_alignbyte(C.z, C.y, 2), __builtin_amdgcn_alignbyte(C.w, C.z, 2)};
+    S[1] = C;
+    S[2] = u32x4{__builtin_amdgcn_alignbyte(C.y, C.x, 2), __builtin_amdgcn_alignbyte(C.z, C.y, 2),
+                 __builtin_amdgcn_alignbyte(C.w, C.z, 2), __builtin_amdgcn_alignbyte(N.x, C.w, 2)};
+  };
+  auto mfma = [&](const u32x4& a, const u32x4& b, floatx4& d) __attribute__((always_inline)) {
+    bf16x8 av, bv;
+    __builtin_memcpy(&av, &a, 16);
+    __builtin_memcpy(&bv, &b, 16);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, d, 0, 0, 0);
+  };
+  // this lane's rows: INPACK: tap s = min(2 w + li / 8, 8) (plane dz + 1 = s / 3,
+  // row y - 1 + s % 3), channel li % 8; otherwise the input tile w rows
+  const int sl_tap = INPACK ? 2 * w + li / 8 : 0;
+  auto compute_row = [&](int y) __attribute__((always_inline)) {
+#pragma unroll 1
+    for (int k0 = 0; k0 < n; k0 += 32) {
+      const int bx = k0 / 8 + lk;
+      if constexpr (INPACK) {
+        const int pi = sl_tap / 3, rs = (y + sl_tap % 3) & 3;
+        const u32x4* xr = &xTv[(pi * 4 + rs) * XTS];
+        const int ci = li & 7;
+        const u32x4 P = xr[bx * 16 + wg_swz(ci, bx)];
+        const u32x4 C = xr[(bx + 1) * 16 + wg_swz(ci, bx + 1)];
+        const u32x4 N = xr[(bx + 2) * 16 + wg_swz(ci, bx + 2)];
+        u32x4 B[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) B[m] = dTv[((y & 1) * MT + m) * DTS + bx * 16 + wg_swz(li, bx)];
+        u32x4 S[3];
+        shifts(P, C, N, S);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int m = 0; m < MT; ++m) mfma(S[d], B[m], acc[d * MT + m]);
+        // tap 8 (plane z + 1, row y + 1) against output tile w
+        const u32x4* xr8 = &xTv[(2 * 4 + ((y + 2) & 3)) * XTS];
+        const u32x4 P8 = xr8[bx * 16 + wg_swz(ci, bx)];
+        const u32x4 C8 = xr8[(bx + 1) * 16 + wg_swz(ci, bx + 1)];
+        const u32x4 N8 = xr8[(bx + 2) * 16 + wg_swz(ci, bx + 2)];
+        shifts(P8, C8, N8, S);
+        const u32x4 Bw = dTv[((y & 1) * MT + w) * DTS + bx * 16 + wg_swz(li, bx)];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) mfma(S[d], Bw, acc[3 * MT + d]);
+      } else {
+        u32x4 A[3];  // dout planes dz = -1, 0, 1 (output tile 0)
+#pragma unroll
+        for (int po = 0; po < 3; ++po) A[po] = dTv[(po * 2 + (y & 1)) * DTS + bx * 16 + wg_swz(li, bx)];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const u32x4* xr = &xTv[(((y + a) & 3) * NW + w) * XTS];
+          const u32x4 P = xr[bx * 16 + wg_swz(li, bx)];
+          const u32x4 C = xr[(bx + 1) * 16 + wg_swz(li, bx + 1)];
+          const u32x4 N = xr[(bx + 2) * 16 + wg_swz(li, bx + 2)];
+          u32x4 S[3];
+          shifts(P, C, N, S);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int po = 0; po < 3; ++po) mfma(A[po], S[d], acc[po * 9 + a * 3 + d]);
+        }
+      }
+    }
+  };
+  if (producer) {
+#pragma unroll 1
+    for (long q = blockIdx.x; q < nplanes; q += gridDim.x) {
+      const int z = int(q % n);
+      const long b = q / n;
+      // INPACK: dout plane q, input planes z - 1, z, z + 1; otherwise input
+      // plane q, dout planes z + 1, z, z - 1 (dz = -1, 0, 1: dout plane z - dz)
+      myplane = jd ? (INPACK ? q : b * n + wrap(z + 1 - jpl, n)) : (INPACK ? b * n + wrap(z + jpl - 1, n) : q);
+      __syncthreads();
+      load(0, 0, n - 1);  // dout row 0; input row -1
+      store(0, true, 0, true, 0);
+      load(0, 0, 0);  // input row 0
+      store(0, false, 0, true, 1);
+      load(0, 0, 1 % n);  // input row 1
+      store(0, false, 0, true, 2);
+#pragma unroll
+      for (int s2 = 0; s2 < PD; ++s2) load(s2, min(s2 + 1, n - 1), (s2 + 2) % n);
+      __syncthreads();
+#pragma unroll 1
+      for (int y0 = 0; y0 < n; y0 += PD) {
+#pragma unroll
+        for (int p = 0; p < PD; ++p) {
+          const int y = y0 + p;
+          store(p, y + 1 < n, (y + 1) & 1, true, (y + 3) & 3);
+          load(p, min(y + PD + 1, n - 1), (y + PD + 2) % n);
+          __syncthreads();
+        }
+      }
+    }
+  } else {
+#pragma unroll 1
+    for (long q = blockIdx.x; q < nplanes; q += gridDim.x) {
+      __syncthreads();
+      __syncthreads();
+#pragma unroll 1
+      for (int y = 0; y < n; ++y) {
+        compute_row(y);
+        __syncthreads();
+      }
+    }
+  }
+  if (db != nullptr) {
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(xT);  // >= 8 KB
+    if (producer) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) red[pt * 8 + c] = (jd && dobias) ? bs[c] : 0.f;
+    }
+    __syncthreads();
+    if (threadIdx.x < COUT) {
+      const int cbt = threadIdx.x / 8, c = threadIdx.x % 8;
+      const int p0 = INPACK ? 0 : ndp;  // the plane whose rows carry the bias sums
+      float t = 0.f;
+      for (int x8 = 0; x8 < nb8; ++x8) t += red[(p0 + x8 * (COUT / 8) + cbt) * 8 + c];
+      atomicAdd(&db[threadIdx.x], t);
+    }
+  }
+  if (producer) return;
+  const long pbase = long(blockIdx.x) * 27;
+  if constexpr (INPACK) {
+    // D[row = 4 lk + r][co = 16 m + li]; row -> (tap 2 w + row / 8, channel row % 8)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 4 * lk + r, s = 2 * w + row / 8, ci = row % 8;
+          const int t = s * 3 + d;  // (dz + 1) * 9 + (dy + 1) * 3 + dx + 1
+          const int co = 16 * m + li;
+          part[((pbase + t) * COUT + co) * CIN + ci] = acc[d * MT + m][r];
+        }
+    // tap 8: rows 0-7 (channels), output tile w
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * lk + r;
+        if (row >= 8) continue;
+        const int co = 16 * w + li;
+        part[((pbase + 24 + d) * COUT + co) * CIN + row] = acc[3 * MT + d][r];
+      }
+  } else {
+    // D[co = 4 lk + r][ci = 16 w + li]
+    const int ci = 16 * w + li;
+#pragma unroll
+    for (int po = 0; po < 3; ++po)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int co = 4 * lk + r;
+            const int t = po * 9 + a * 3 + d;
+            part[((pbase + t) * COUT + co) * CIN + ci] = acc[po * 9 + a * 3 + d][r];
+          }
+  }
 }
 
 __global__ void k_reduce_partials(const float* part, float* out, long nparts, long len) {
@@ -923,36 +1141,42 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
 }
 
 // row sets in flight per producer thread (registers: 32 VGPRs each)
-#ifndef SC_WG_PD_WIDE
-#define SC_WG_PD_WIDE 2
+#ifndef SC_WG_PD
+#define SC_WG_PD 2
 #endif
-#ifndef SC_WG_PD_NARROW
-#define SC_WG_PD_NARROW 4
+#ifndef SC_WG3_PD
+#define SC_WG3_PD 2
 #endif
+
 
 void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* dout, float* grad, float* db,
                   int batch) {
   const long nplanes = long(batch) * c->n;
-  // plane groups: a multiple of 8 (one XCD holds a group's three blocks), at
-  // most one block per CU
-  const long G = 8 * std::max(1, c->ncu / 24);
+  // the narrow layers take all 27 taps per block (one block per CU); the
+  // 64 x 64 layer uses plane groups of three blocks: a multiple of 8 (one XCD
+  // holds a group's three blocks), at most one block per CU
+  const bool narrow = !(cinp >= 64 && coutp >= 64);
+  const long G = narrow ? std::min<long>(nplanes, std::max(1, c->ncu)) : 8 * std::max(1, c->ncu / 24);
   const long len = 27L * coutp * cinp;
   if (c->part_len < G * len) {
     if (c->part) CNN_HIP(hipFree(c->part));
     CNN_HIP(hipMalloc(&c->part, G * len * sizeof(float)));
     c->part_len = G * len;
   }
-  dim3 grid(unsigned(3 * G));
   c->pbeg("conv_wgrad", 2.0 * double(nplanes) * c->n * c->n * 27.0 * ((cinp + 15) / 16 * 16) * coutp);
-#define WG(CI, CO)                                                                                              \
-  if (cinp == CI && coutp == CO) {                                                                               \
-    constexpr bool WIDE = CI >= 64 && CO >= 64;                                                                  \
-    hipLaunchKernelGGL((k_conv_wgrad<CI, CO, WIDE ? SC_WG_PD_WIDE : SC_WG_PD_NARROW, (CI > CO)>), grid,           \
-                       dim3(512), 0, c->stream, in, dout, c->part, db, c->n, nplanes);                            \
-    CNN_HIP(hipGetLastError());                                                                                  \
-  } else
-  WG(8, 64) WG(64, 64) WG(64, 16) { throw std::runtime_error("unsupported wgrad shape"); }
-#undef WG
+  if (cinp == 64 && coutp == 64) {
+    hipLaunchKernelGGL((k_conv_wgrad<64, 64, SC_WG_PD>), dim3(unsigned(3 * G)), dim3(512), 0, c->stream, in, dout,
+                       c->part, db, c->n, nplanes);
+  } else if (cinp == 8 && coutp == 64) {
+    hipLaunchKernelGGL((k_conv_wgrad3<8, 64, SC_WG3_PD>), dim3(unsigned(G)), dim3(64 * 8), 0, c->stream, in, dout,
+                       c->part, db, c->n, nplanes);
+  } else if (cinp == 64 && coutp == 16) {
+    hipLaunchKernelGGL((k_conv_wgrad3<64, 16, SC_WG3_PD>), dim3(unsigned(G)), dim3(64 * 8), 0, c->stream, in, dout,
+                       c->part, db, c->n, nplanes);
+  } else {
+    throw std::runtime_error("unsupported wgrad shape");
+  }
+  CNN_HIP(hipGetLastError());
   c->pend();
   c->pbeg("wgrad_reduce", 0);
   hipLaunchKernelGGL(k_reduce_partials, dim3(unsigned((len + 255) / 256)), dim3(256), 0, c->stream, c->part, grad, G,
