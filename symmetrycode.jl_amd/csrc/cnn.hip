@@ -623,9 +623,10 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
 //    fragments, each against the 4 output tiles, and tap s = 8 (lanes 0-7;
 //    lanes 8-15 repeat it and their rows are dropped) against output tile w:
 //    15 MFMAs per 32 voxels.
-//  otherwise (CIN = 64, COUT = 16): the input plane z and the dout planes
+//  otherwise (CIN = 64, COUT = 16 or 8): the input plane z and the dout planes
 //    z + 1, z, z - 1 (dz = -1, 0, 1) are staged; compute wavefront w owns
-//    input tile w and all 27 taps (27 MFMAs per 32 voxels).
+//    input tile w and all 27 taps (27 MFMAs per 32 voxels); COUT = 8 (the
+//    6-channel output layer) uses one output tile with channels 8-15 zero.
 template <int CIN, int COUT, int PD>
 __global__ void __launch_bounds__(512, 1)
     k_conv_wgrad3(const bf16* __restrict__ in, const bf16* __restrict__ dout, float* part, float* db, int n,
@@ -634,13 +635,14 @@ __global__ void __launch_bounds__(512, 1)
   constexpr int NCWV = 4;  // compute wavefronts
   constexpr int NBM = 16;               // x blocks per row (n <= 128)
   constexpr int CP = CIN < 16 ? 16 : CIN;
-  constexpr int MT = COUT / 16;
+  constexpr int MT = (COUT + 15) / 16;  // output tiles (COUT = 8: one tile, channels 8-15 zero)
   constexpr int NW = CP / 16;
   constexpr int NPI = INPACK ? 3 : 1;  // input planes per row step
   constexpr int NPO = INPACK ? 1 : 3;  // dout planes per row step
   constexpr int DTS = NBM * 16 + 1, XTS = (NBM + 2) * 16 + 1;
   static_assert(NPO * 16 * (COUT / 8) + NPI * 16 * (CIN / 8) <= 256, "staging jobs");
   static_assert(INPACK ? (MT == 4 && NW == 1) : (MT == 1 && NW == 4), "wgrad3 shapes");
+  static_assert(COUT % 8 == 0, "dout blocks of 8 channels");
   static_assert(64 % PD == 0, "row sets in flight divide n");
   __shared__ uint4 dT[NPO * 2 * MT * DTS];  // [dout plane][row slot y & 1][tile]
   __shared__ uint4 xT[NPI * 4 * NW * XTS];  // [input plane][row slot (r + 1) & 3][tile]
@@ -654,6 +656,9 @@ __global__ void __launch_bounds__(512, 1)
   for (int i = 0; i < NACC; ++i) acc[i] = floatx4{0, 0, 0, 0};
   if (CIN < CP) {  // the zero-padded channels are never written: clear all input slots once
     for (int i = threadIdx.x; i < NPI * 4 * NW * XTS; i += 64 * (NCWV + 4)) xT[i] = uint4{0, 0, 0, 0};
+  }
+  if (COUT < 16 * MT) {  // likewise the zero-padded output channels
+    for (int i = threadIdx.x; i < NPO * 2 * MT * DTS; i += 64 * (NCWV + 4)) dT[i] = uint4{0, 0, 0, 0};
   }
   float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int nb8 = n / 8;
@@ -866,6 +871,7 @@ __global__ void __launch_bounds__(512, 1)
   } else {
     // D[co = 4 lk + r][ci = 16 w + li]
     const int ci = 16 * w + li;
+    if (4 * lk >= COUT) return;
 #pragma unroll
     for (int po = 0; po < 3; ++po)
 #pragma unroll
@@ -1134,8 +1140,8 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
     c->pend();                                                                                            \
     return;                                                                                               \
   }
-  CONV(8, 64, 64, HIDDEN) CONV(64, 64, 64, HIDDEN) CONV(64, 16, 16, LAST)
-  CONV(16, 64, 64, BWD_HIDDEN) CONV(64, 64, 64, BWD_HIDDEN) CONV(64, 16, 8, BWD_INPUT)
+  CONV(8, 64, 64, HIDDEN) CONV(64, 64, 64, HIDDEN) CONV(64, 16, 16, LAST) CONV(64, 16, 8, LAST)
+  CONV(16, 64, 64, BWD_HIDDEN) CONV(8, 64, 64, BWD_HIDDEN) CONV(64, 64, 64, BWD_HIDDEN) CONV(64, 16, 8, BWD_INPUT)
 #undef CONV
   throw std::runtime_error("unsupported conv shape");
 }
@@ -1172,6 +1178,9 @@ void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* do
                        c->part, db, c->n, nplanes);
   } else if (cinp == 64 && coutp == 16) {
     hipLaunchKernelGGL((k_conv_wgrad3<64, 16, SC_WG3_PD>), dim3(unsigned(G)), dim3(64 * 8), 0, c->stream, in, dout,
+                       c->part, db, c->n, nplanes);
+  } else if (cinp == 64 && coutp == 8) {
+    hipLaunchKernelGGL((k_conv_wgrad3<64, 8, SC_WG3_PD>), dim3(unsigned(G)), dim3(64 * 8), 0, c->stream, in, dout,
                        c->part, db, c->n, nplanes);
   } else {
     throw std::runtime_error("unsupported wgrad shape");
@@ -1314,6 +1323,10 @@ int sc_cnn_create(int n, int batch_max, int cin, int width, int cout, uint64_t s
   (void)hipGetLastError();
   sc_cnn* c = new sc_cnn();
   c->n = n; c->bmax = batch_max; c->cin = cin; c->width = width; c->cout = cout;
+  // output channel pitch: 8 when it suffices, so the loss gradient and the
+  // last layer's data and weight gradients carry 8 channels, not 16 (the
+  // forward's MFMA tile stays 16 wide and stores 8)
+  c->cout_p = cout <= 8 ? 8 : 16;
   try {
     CNN_HIP(hipStreamCreate(&c->stream));
     c->own_stream = true;
