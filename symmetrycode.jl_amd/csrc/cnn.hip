@@ -65,7 +65,17 @@ __device__ __forceinline__ float gelu_grad(float z) {
 enum Mode { HIDDEN = 0, LAST = 1, BWD_HIDDEN = 2, BWD_INPUT = 3 };
 
 constexpr int VPB = 64;   // output voxels per x segment (n must be a multiple)
-constexpr int TROWS = 8;  // output y rows per block
+#ifndef SC_CNN_PREFETCH
+#define SC_CNN_PREFETCH 0
+#endif
+#ifndef SC_CNN_TROWS
+#define SC_CNN_TROWS 4
+#endif
+// output y rows per block: 4 rows (4 wavefronts, 57 KB of LDS for a 64-channel
+// input plane) let two blocks share a compute unit, so one block's plane
+// fetch, plane commits and epilogue overlap the other's MFMAs; with 8 rows
+// (95 KB) a unit holds one block and those stalls are exposed
+constexpr int TROWS = SC_CNN_TROWS;
 // y rows per wavefront.  RPW = 2 halves the weight-fragment traffic per flop
 // but leaves one wavefront per SIMD (the LDS tile allows one block per CU):
 // batch-16 x 128^3 training steps took 87.8 ms against 80.3 ms with RPW = 1
@@ -81,6 +91,9 @@ constexpr int PADC = SC_CNN_PADC;  // LDS channel padding (multiple of 8) agains
 
 constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 
+#ifndef SC_CNN_SPLIT
+#define SC_CNN_SPLIT 1  // two rows x half the output tiles per wavefront (see k_conv)
+#endif
 #ifndef SC_CNN_BDIST
 #define SC_CNN_BDIST 4  // k steps between a B-fragment request and its use (2 or 4)
 #endif
@@ -104,7 +117,7 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 //  BWD_HIDDEN: out_a = conv * gelu'(zprev)                            (bf16)
 //  BWD_INPUT:  out_f = conv (first COUTS channels)                    (fp32)
 template <int CIN, int COUTT, int COUTS, int MODE>
-__global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
+__global__ void __launch_bounds__(CONV_THREADS, 8 / (CONV_THREADS / 64)) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
                                                  const float* __restrict__ bias, bf16* out_a, bf16* out_z,
                                                  float* out_f, const bf16* __restrict__ zprev, int n, long ntiles,
                                                  int tracefree) {
@@ -113,9 +126,15 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
   constexpr bool BYPLANE = CIN >= 32;
   constexpr int NPL = BYPLANE ? 1 : 3;
   constexpr int NKC = kpad(CIN) / 32;
-  constexpr int NT = COUTT / 16;
+  // with 4 output tiles (and one row per wavefront otherwise) a wavefront
+  // takes two rows and half the output tiles: the same MFMAs per k step,
+  // half the weight-fragment traffic through L1 (8 wavefronts of a block
+  // request the same fragments), twice the A reads from LDS
+  constexpr bool SPLIT = SC_CNN_SPLIT && COUTT == 64 && RPW == 1;
+  constexpr int NT = SPLIT ? 2 : COUTT / 16;   // output tiles per wavefront
+  constexpr int NTA = COUTT / 16;              // output tiles in the weight layout
   constexpr int CH8 = CIN / 8;
-  constexpr int MT = 4 * RPW;  // M tiles of 16 voxels: 4 per row
+  constexpr int MT = SPLIT ? 8 : 4 * RPW;      // M tiles of 16 voxels: 4 per row
   constexpr int NPH = BYPLANE ? 3 : 1;
   constexpr int NCH = NPL * RW * XW * CH8;
   constexpr int NPT = (NCH + CONV_THREADS - 1) / CONV_THREADS;
@@ -123,6 +142,8 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
   const int segs = n / VPB, ytiles = n / TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
+  const int wr = SPLIT ? 2 * (w / 2) : w * RPW;  // the wavefront's first row
+  const int wn = SPLIT ? 2 * (w % 2) : 0;        // and first output tile
   struct Tile { long b; int z, y0, x0; };
   auto decode = [&](long t) {
     Tile T;
@@ -146,7 +167,7 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
     if constexpr (CIN >= 32) {  // one tap per k step: the offsets are uniform
       const int t = (kc * 32) / CIN, cb = (kc * 32) % CIN;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16* base = lane_a + ((w * RPW + 1 + dy) * XW + 1 + dx) * CL + cb;
+      const bf16* base = lane_a + ((wr + 1 + dy) * XW + 1 + dx) * CL + cb;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
       return;
@@ -156,7 +177,7 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
     if (t < 27) {
       const int pl = BYPLANE ? 0 : t / 9;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16* base = &win[((pl * RW + w * RPW + 1 + dy) * XW + 1 + li + dx) * CL + c0];
+      const bf16* base = &win[((pl * RW + wr + 1 + dy) * XW + 1 + li + dx) * CL + c0];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
     } else {
@@ -169,7 +190,7 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
   auto load_b = [&](int kc, bf16x8* bv) {
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      bv[nt] = *reinterpret_cast<const bf16x8*>(&W[(long(kc * NT + nt) * 64 + l) * 8]);
+      bv[nt] = *reinterpret_cast<const bf16x8*>(&W[(long(kc * NTA + wn + nt) * 64 + l) * 8]);
   };
   // staging of (tile T, phase PH): all of a phase's 16-byte chunks are fetched
   // into registers at once, and written to LDS between two barriers
@@ -199,8 +220,10 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
   __syncthreads();
 #pragma unroll 1
   for (int ph = 0; ph < NPH; ++ph) {
-    // prefetch this tile's next input plane while this one is used
-    if (ph + 1 < NPH) {
+    // prefetch this tile's next input plane while this one is used (without
+    // it, the plane is fetched at the commit, and the other block on the
+    // compute unit covers the wait)
+    if (SC_CNN_PREFETCH && ph + 1 < NPH) {
       CONV_FETCH(cur, ph + 1)
     }
     const int kc0 = BYPLANE ? ph * (NKC / 3) : 0;
@@ -258,6 +281,9 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
     }
 #endif
     if (ph + 1 < NPH) {
+      if (!SC_CNN_PREFETCH) {
+        CONV_FETCH(cur, ph + 1)
+      }
       __syncthreads();
       CONV_COMMIT()
       __syncthreads();
@@ -267,10 +293,10 @@ __global__ void __launch_bounds__(CONV_THREADS, 1) k_conv(const bf16* __restrict
   // D[co = 16 nt + 4 lk + r][voxel = 16 mt + li]: four consecutive channels of
   // one voxel, stored (and, for the data gradient, the pre-activation read)
   // as one 8- or 16-byte access
-  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + w * RPW) * unsigned(n) + cur.x0 + li;
+  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + wr) * unsigned(n) + cur.x0 + li;
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
-    const int co0 = nt * 16 + 4 * lk;
+    const int co0 = (wn + nt) * 16 + 4 * lk;
     if (co0 >= COUTS) continue;
     float bco[4] = {0.f, 0.f, 0.f, 0.f};
     if constexpr (MODE == HIDDEN) {
