@@ -71,11 +71,13 @@ constexpr int VPB = 64;   // output voxels per x segment (n must be a multiple)
 #ifndef SC_CNN_TROWS
 #define SC_CNN_TROWS 4
 #endif
-// output y rows per block: 4 rows (4 wavefronts, 57 KB of LDS for a 64-channel
-// input plane) let two blocks share a compute unit, so one block's plane
-// fetch, plane commits and epilogue overlap the other's MFMAs; with 8 rows
-// (95 KB) a unit holds one block and those stalls are exposed
-constexpr int TROWS = SC_CNN_TROWS;
+// output y rows per block: for 64 output channels 4 rows (4 wavefronts, 57 KB
+// of LDS for a 64-channel input plane) let two blocks share a compute unit,
+// so one block's plane fetch, plane commits and epilogue overlap the other's
+// MFMAs (with 8 rows, 95 KB, a unit holds one block and those stalls are
+// exposed); the 16-column output layers, short on MFMAs, keep 8 rows and
+// the smaller y halo (10 staged rows per 8 instead of 6 per 4)
+constexpr int conv_trows(int coutt) { return coutt == 16 ? 8 : SC_CNN_TROWS; }
 // y rows per wavefront.  RPW = 2 halves the weight-fragment traffic per flop
 // but leaves one wavefront per SIMD (the LDS tile allows one block per CU):
 // batch-16 x 128^3 training steps took 87.8 ms against 80.3 ms with RPW = 1
@@ -83,7 +85,7 @@ constexpr int TROWS = SC_CNN_TROWS;
 #define SC_CNN_RPW 1
 #endif
 constexpr int RPW = SC_CNN_RPW;
-constexpr int CONV_THREADS = 64 * (TROWS / RPW);      // threads per convolution block
+constexpr int conv_threads(int coutt) { return 64 * (conv_trows(coutt) / RPW); }  // threads per block
 #ifndef SC_CNN_PADC
 #define SC_CNN_PADC 8
 #endif
@@ -117,11 +119,12 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 //  BWD_HIDDEN: out_a = conv * gelu'(zprev)                            (bf16)
 //  BWD_INPUT:  out_f = conv (first COUTS channels)                    (fp32)
 template <int CIN, int COUTT, int COUTS, int MODE>
-__global__ void __launch_bounds__(CONV_THREADS, 8 / (CONV_THREADS / 64)) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
+__global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) / 64)) k_conv(const bf16* __restrict__ in, const bf16* __restrict__ W,
                                                  const float* __restrict__ bias, bf16* out_a, bf16* out_z,
                                                  float* out_f, const bf16* __restrict__ zprev, int n, long ntiles,
                                                  int tracefree) {
   constexpr int CL = CIN + PADC;
+  constexpr int TROWS = conv_trows(COUTT), CONV_THREADS = conv_threads(COUTT);
   constexpr int RW = TROWS + 2, XW = VPB + 2;
   constexpr bool BYPLANE = CIN >= 32;
   constexpr int NPL = BYPLANE ? 1 : 3;
@@ -1153,15 +1156,15 @@ int cguard(sc_cnn* c, F&& f) {
 void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const bf16* W, const float* bias,
                  bf16* oa, bf16* oz, float* of, const bf16* zp, int batch) {
   const long rows = long(batch) * c->n * c->n;
-  const long ntiles = long(batch) * c->n * (c->n / TROWS) * (c->n / VPB);
-  const dim3 grid{unsigned(ntiles)};
+
   static const char* names[4] = {"conv_fwd", "conv_last", "conv_bwd_data", "conv_bwd_input"};
   const double flop = 2.0 * double(rows) * c->n * kpad(cinp) * ((coutp + 15) / 16 * 16);
 #define CONV(CI, COT, COS, MO)                                                                             \
   if (cinp == CI && coutp == COS && mode == MO) {                                                         \
     c->pbeg(names[MO], flop);                                                                             \
-    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), grid, dim3(CONV_THREADS), 0, c->stream, in, W, bias, oa, oz, of, zp, \
-                       c->n, ntiles, c->tracefree ? 1 : 0);                                              \
+    const long ntiles = long(batch) * c->n * (c->n / conv_trows(COT)) * (c->n / VPB);                    \
+    hipLaunchKernelGGL((k_conv<CI, COT, COS, MO>), dim3(unsigned(ntiles)), dim3(conv_threads(COT)), 0, c->stream, in, W, \
+                       bias, oa, oz, of, zp, c->n, ntiles, c->tracefree ? 1 : 0);                        \
     CNN_HIP(hipGetLastError());                                                                           \
     c->pend();                                                                                            \
     return;                                                                                               \
