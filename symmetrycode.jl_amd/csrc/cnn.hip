@@ -86,18 +86,24 @@ constexpr int conv_trows(int coutt) { return coutt == 16 ? 8 : SC_CNN_TROWS; }
 #endif
 constexpr int RPW = SC_CNN_RPW;
 constexpr int conv_threads(int coutt) { return 64 * (conv_trows(coutt) / RPW); }  // threads per block
-#ifndef SC_CNN_PADC
-#define SC_CNN_PADC 8
-#endif
-constexpr int PADC = SC_CNN_PADC;  // LDS channel padding (multiple of 8) against bank conflicts
+// Staged input rows, chunk-major: for each (plane, row, 8-channel chunk c8)
+// the row's voxels are 16-byte entries, voxel x at position x ^ (c8 & 7) of
+// a run of XWP = 80 entries (1280 bytes, a multiple of 256).  Measured on
+// this chip with a one-wavefront probe and the bank-conflict counter,
+// ds_read_b128 is conflict-free when the four 16-lane groups of the wave
+// start at the same offset modulo 256 bytes (aligned or not, contiguous or
+// strided within a group), and conflicts 2-way when they do not (groups
+// 1168 or 272 bytes apart).  An A-fragment read (group lk = chunk c8 of 16
+// consecutive voxels) covers the 16 slots of a 256-byte window once (the
+// XOR permutes within aligned 8-entry blocks) at congruent offsets, and the
+// staging stores (8 consecutive lanes = the 8 chunks of one voxel) land in 8
+// distinct 16-byte slots modulo 128 bytes, as ds_write_b128 needs.
+constexpr int XWP = 80;
 
 constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 
 #ifndef SC_CNN_SPLIT
 #define SC_CNN_SPLIT 1  // two rows x half the output tiles per wavefront (see k_conv)
-#endif
-#ifndef SC_CNN_BDIST
-#define SC_CNN_BDIST 4  // k steps between a B-fragment request and its use (2 or 4)
 #endif
 
 // One convolution layer (or its data gradient) as an implicit GEMM with the
@@ -123,9 +129,9 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
                                                  const float* __restrict__ bias, bf16* out_a, bf16* out_z,
                                                  float* out_f, const bf16* __restrict__ zprev, int n, long ntiles,
                                                  int tracefree) {
-  constexpr int CL = CIN + PADC;
   constexpr int TROWS = conv_trows(COUTT), CONV_THREADS = conv_threads(COUTT);
   constexpr int RW = TROWS + 2, XW = VPB + 2;
+  static_assert(XWP >= (XW | 7) + 1 && XWP % 16 == 0, "staged row width");
   constexpr bool BYPLANE = CIN >= 32;
   constexpr int NPL = BYPLANE ? 1 : 3;
   constexpr int NKC = kpad(CIN) / 32;
@@ -141,7 +147,7 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   constexpr int NPH = BYPLANE ? 3 : 1;
   constexpr int NCH = NPL * RW * XW * CH8;
   constexpr int NPT = (NCH + CONV_THREADS - 1) / CONV_THREADS;
-  __shared__ __attribute__((aligned(16))) bf16 win[NPL * RW * XW * CL];
+  __shared__ __attribute__((aligned(16))) bf16 win[NPL * RW * CH8 * XWP * 8];
   const int segs = n / VPB, ytiles = n / TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
@@ -163,14 +169,18 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0, 0, 0, 0};
-  // A fragment of M tile mt (row w RPW + mt / 4, x 16 (mt % 4) + li) at k step kc
-  const bf16* lane_a = &win[li * CL + 8 * lk];
-  auto mt_off = [](int mt) { return ((mt / 4) * XW + 16 * (mt % 4)) * CL; };
+  // A fragment of M tile mt (row wr + mt / 4, x 16 (mt % 4) + li) at k step
+  // kc: lane (li, lk) reads chunk lk of the k step's 32 channels; the
+  // swizzle of x only touches its low three bits, so the M tiles' offsets
+  // stay constants
+  const bf16* lane_a = &win[lk * XWP * 8];
+  auto mt_off = [](int mt) { return ((mt / 4) * CH8 * XWP + 16 * (mt % 4)) * 8; };
   auto load_a = [&](int kc, bf16x8* a) {
-    if constexpr (CIN >= 32) {  // one tap per k step: the offsets are uniform
+    if constexpr (CIN >= 32) {  // one tap per k step: the row offsets are uniform
       const int t = (kc * 32) / CIN, cb = (kc * 32) % CIN;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16* base = lane_a + ((wr + 1 + dy) * XW + 1 + dx) * CL + cb;
+      const int c8 = cb / 8 + lk;
+      const bf16* base = lane_a + (((wr + 1 + dy) * CH8 + cb / 8) * XWP + ((1 + dx + li) ^ (c8 & 7))) * 8;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
       return;
@@ -180,7 +190,8 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
     if (t < 27) {
       const int pl = BYPLANE ? 0 : t / 9;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16* base = &win[((pl * RW + wr + 1 + dy) * XW + 1 + li + dx) * CL + c0];
+      const int c8 = c0 / 8;
+      const bf16* base = &win[(((pl * RW + wr + 1 + dy) * CH8 + c8) * XWP + ((1 + li + dx) ^ (c8 & 7))) * 8];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
     } else {
@@ -214,7 +225,8 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
 #define CONV_COMMIT()                                                                                       \
   _Pragma("clang loop unroll(full)") for (int j = 0; j < NPT; ++j) {                                        \
     const int i = threadIdx.x + CONV_THREADS * j;                                                           \
-    if (i < NCH) *reinterpret_cast<u32x4*>(&win[(i / CH8) * CL + (i % CH8) * 8]) = pre[j];                  \
+    if (i < NCH)                                                                                            \
+      *reinterpret_cast<u32x4*>(&win[(((i / CH8) / XW * CH8 + i % CH8) * XWP + ((i / CH8) % XW ^ (i % CH8 & 7))) * 8]) = pre[j]; \
   }
   if (long(blockIdx.x) >= ntiles) return;
   const Tile cur = decode(blockIdx.x);
@@ -230,59 +242,31 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
       CONV_FETCH(cur, ph + 1)
     }
     const int kc0 = BYPLANE ? ph * (NKC / 3) : 0;
-    const int kc1 = BYPLANE ? kc0 + NKC / 3 : NKC;
-#if SC_CNN_BDIST == 4
-    // software pipeline over k steps, four at a time: B fragments are
-    // requested four steps ahead, A fragments one step ahead
-    bf16x8 aE[MT], aO[MT], b0[NT], b1[NT], b2[NT], b3[NT];
-    load_b(kc0, b0);
-    if (kc0 + 1 < kc1) load_b(kc0 + 1, b1);
-    if (kc0 + 2 < kc1) load_b(kc0 + 2, b2);
-    if (kc0 + 3 < kc1) load_b(kc0 + 3, b3);
-    load_a(kc0, aE);
-#define CONV_STEP(A_CUR, A_NXT, B_CUR, J)                                                                   \
-    if (kc + J < kc1) {                                                                                       \
-      if (kc + J + 1 < kc1) load_a(kc + J + 1, A_NXT);                                                       \
-      _Pragma("unroll") for (int mt = 0; mt < MT; ++mt)                                                      \
-          _Pragma("unroll") for (int nt = 0; nt < NT; ++nt)                                                  \
-              acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B_CUR[nt], A_CUR[mt], acc[mt][nt], 0, 0, 0); \
-      if (kc + J + 4 < kc1) load_b(kc + J + 4, B_CUR);                                                       \
-    }
-#pragma unroll 1
-    for (int kc = kc0; kc < kc1; kc += 4) {
-      CONV_STEP(aE, aO, b0, 0)
-      CONV_STEP(aO, aE, b1, 1)
-      CONV_STEP(aE, aO, b2, 2)
-      CONV_STEP(aO, aE, b3, 3)
-    }
-#undef CONV_STEP
-#else
-    // software pipeline over k steps, two at a time (E = even, O = odd):
-    // B fragments are requested two steps ahead
-    bf16x8 aE[MT], bE[NT], bO[NT];
-    load_b(kc0, bE);
-    if (kc0 + 1 < kc1) load_b(kc0 + 1, bO);
-#pragma unroll 1
-    for (int kc = kc0; kc < kc1; kc += 2) {
-      const bool h1 = kc + 1 < kc1, h2 = kc + 2 < kc1, h3 = kc + 3 < kc1;
-      load_a(kc, aE);
+    // software pipeline over the phase's k steps, fully unrolled (the step
+    // count is a compile-time constant, so the code is straight-line): B
+    // fragments are requested four steps ahead, A fragments one step ahead.
+    // With the loads under run-time conditions the compiler either waits
+    // with vmcnt(0) (draining the fragments requested for later steps too)
+    // or sinks a load into the block that uses it
+    constexpr int NS = BYPLANE ? NKC / 3 : NKC;
+    bf16x8 af[2][MT], bf[4][NT];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (j < NS) load_b(kc0 + j, bf[j]);
+    load_a(kc0, af[0]);
+#pragma unroll
+    for (int kk = 0; kk < NS; ++kk) {
+      // the scheduler keeps each step's loads in that step (left to itself
+      // it moves every load next to its first use, which undoes the pipeline)
+      __builtin_amdgcn_sched_barrier(0);
+      if (kk + 1 < NS) load_a(kc0 + kk + 1, af[(kk + 1) & 1]);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bE[nt], aE[mt], acc[mt][nt], 0, 0, 0);
-      if (h2) load_b(kc + 2, bE);
-      if (h1) {
-        load_a(kc + 1, aE);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-          for (int nt = 0; nt < NT; ++nt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bO[nt], aE[mt], acc[mt][nt], 0, 0, 0);
-        if (h3) load_b(kc + 3, bO);
-      }
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[kk & 3][nt], af[kk & 1][mt], acc[mt][nt], 0, 0, 0);
+      if (kk + 4 < NS) load_b(kc0 + kk + 4, bf[kk & 3]);
     }
-#endif
     if (ph + 1 < NPH) {
       if (!SC_CNN_PREFETCH) {
         CONV_FETCH(cur, ph + 1)
