@@ -358,14 +358,20 @@ k_x_products(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ tw
 }
 
 // ------------------------------------------------------------------
-// The same fused x-pass with one wavefront per row: every LDS exchange is
-// wavefront-local (no block barrier after the twiddle staging), so the
-// wavefronts of a CU progress independently and hide each other's latency.
-// Lanes beyond the pencil's thread count duplicate lane 0's work and store
-// nothing.
+// The same fused x-pass with each row held by one wavefront: every LDS
+// exchange is wavefront-local (no block barrier after the twiddle staging),
+// so the wavefronts of a CU progress independently and hide each other's
+// latency.  When the pencil's thread count divides 64, a wavefront holds
+// 64 / TPP rows side by side (lane = row slot * TPP + p): all its lanes
+// transform (at M = 64, 128, 256 one row would leave 56, 48 or 32 of them
+// idle); otherwise it holds one row, and lanes beyond TPP stay idle in the
+// transforms and share the R2C post-processing.
 #ifndef SC_XW_WAVES
 #define SC_XW_WAVES 2
 #endif
+template <int M> constexpr int xw_rows_per_wave() {
+  return 64 % Plan<M>::tpp == 0 ? 64 / Plan<M>::tpp : 1;
+}
 template <typename T, int M, int D>
 __global__ void __launch_bounds__(64 * SC_XW_WAVES)
 k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ twm,
@@ -373,6 +379,8 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   constexpr int TPP = Plan<M>::tpp;
   static_assert(TPP <= 64, "one wavefront per row");
   constexpr int NW = SC_XW_WAVES;
+  constexpr int RPW = xw_rows_per_wave<M>();
+  constexpr int LPR = 64 / RPW;  // lanes per row slot
   constexpr int NP = D == 3 ? 6 : 3;
   using PI = Pencil<T, M, +1, TPP, true>;
   using PF = Pencil<T, M, -1, TPP, true>;
@@ -382,7 +390,7 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   using PFR = Pencil<T, M, -1, TPP, true, PlanRev<M>>;
   constexpr bool DIRECT = M / PlanInfo<M>::R(PlanInfo<M>::NS - 1) == TPP;
   constexpr int E = PlanInfo<M>::E_LAST;
-  __shared__ C<T> lds[NW * M];
+  __shared__ C<T> lds[NW * RPW * M];
   __shared__ C<T> twm_l[M];
   __shared__ C<T> twn_l[M + 1];
   __shared__ double red[NW];
@@ -390,11 +398,12 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   for (int i = threadIdx.x; i <= M; i += blockDim.x) twn_l[i] = twn[i];
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const bool act = lane < TPP;
-  const int p = act ? lane : 0;
-  const long row = long(blockIdx.x) * NW + w;
+  const int slot = lane / LPR, q = lane % LPR;  // row slot, lane within it
+  const bool act = q < TPP;
+  const int p = act ? q : 0;
+  const long row = (long(blockIdx.x) * NW + w) * RPW + slot;
   const bool rok = row < g.nrows;
-  C<T>* ll = lds + w * M;
+  C<T>* ll = lds + (w * RPW + slot) * M;
   C<T> u[D][E];
   C<T> v[PlanInfo<M>::MAXE > PFR::MAXE ? PlanInfo<M>::MAXE : PFR::MAXE];
 #pragma unroll
@@ -435,9 +444,9 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
   constexpr int QB[3] = {0, 1, 1};
   const T s = T(g.scale);
 #pragma unroll
-  for (int q = 0; q < NP; ++q) {
-    const int i = D == 3 ? PA[q] : QA[q];
-    const int j = D == 3 ? PB[q] : QB[q];
+  for (int qq = 0; qq < NP; ++qq) {
+    const int i = D == 3 ? PA[qq] : QA[qq];
+    const int j = D == 3 ? PB[qq] : QB[qq];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = mk<T>(u[i][e].x * u[j][e].x, u[i][e].y * u[j][e].y);
     if constexpr (DIRECT) {
@@ -451,8 +460,8 @@ k_x_products_w(const C<T>* __restrict__ in, C<T>* out, const C<T>* __restrict__ 
     }
     xsync<true>();
     if (rok) {
-      C<T>* dst = out + long(q) * g.s_comp_spec + row * g.nhp;
-      for (int k = lane; k <= M; k += 64) dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn_l), s);
+      C<T>* dst = out + long(qq) * g.s_comp_spec + row * g.nhp;
+      for (int k = q; k <= M; k += LPR) dst[k] = scl<T>(r2c_post<T, M>(k, ll, twn_l), s);
     }
   }
 }

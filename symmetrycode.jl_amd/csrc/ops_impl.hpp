@@ -210,11 +210,12 @@ static void x_products(Ctx& c, C<T>* out, const C<T>* in, int kin, double scale,
   }
   if (SC_XPROD_WAVE && tpp_rt(M) <= 64) {
     constexpr int NW = SC_XW_WAVES;
-    dim3 grid(unsigned((g.nrows + NW - 1) / NW), 1);
     switch (M) {
 #define X(MM)                                                                                            \
   case MM:                                                                                               \
     if constexpr (Plan<MM>::tpp <= 64) {                                                                 \
+      constexpr long RB = long(NW) * xw_rows_per_wave<MM>();  /* rows per block */                       \
+      const dim3 grid(unsigned((g.nrows + RB - 1) / RB), 1);                                             \
       if (c.D == 3)                                                                                      \
         hipLaunchKernelGGL((k_x_products_w<T, MM, 3>), grid, dim3(64 * NW), 0, c.stream, in, out, twm, twn, g, vmax); \
       else                                                                                               \
