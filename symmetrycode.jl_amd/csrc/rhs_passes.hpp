@@ -209,12 +209,14 @@ template <typename T, int N> constexpr bool z_rk4_fits() {
 // share of every state component is one contiguous run).  For each component
 // i the divergence -i (A_i + kz s_iz) of the six transformed slots gives
 // k_i = div_i - nu |k|^2 us_i (+ A_f us_i on the band 0 < |k|^2 <= kband2),
-// us the stage input (u0 at stage 0, ust after), and then the update of
-// k_rk4_update_lin: stage 0 acc = k, stages 1-2 acc += 2k, the stage value
-// v = u0 + c dt k (stage 3: u0 + dt/6 (acc + k)).  The first two components'
-// stage values wait in LDS for the third (each thread reads back its own
-// entries); then the three are projected with the FP64 |k|^2 division and
-// written to ust (stage 3: u0).  kz s_iz of components 0 and 1 is parked in
+// us the stage input (u0 at stage 0, ust after), and then the RK4 update with
+// acc holding the running new state: stage 0 acc = u0 + dt/6 k, stages 1-2
+// acc += dt/3 k, the stage value v = u0 + c dt k; stage 3's value is the new
+// state acc + dt/6 k, so the last stage reads no u0 (three streams instead of
+// four).  The first two components' stage values wait in LDS for the third
+// (each thread reads back its own entries); then the three are projected with
+// the FP64 |k|^2 division (acc carries the unprojected k, and P(u0 + c dt k)
+// = u0 + c dt P(k) for the projected u0) and written to ust (stage 3: u0).  kz s_iz of components 0 and 1 is parked in
 // the LDS slot that the component's stage value overwrites afterwards; the
 // third component is peeled off the loop and holds it in registers, which the
 // absent prefetch leaves free.  LDS: the transform exchange, half a twiddle
@@ -250,6 +252,7 @@ k_z_fwd_rk4(const C<T>* bufT, long s_slot, C<T>* u0, C<T>* ust, C<T>* acc, const
   const T kx = kf * T(x), ky = kf * T(kyi);
   const T nu = T(rp.nu);
   const T cdt = T(*dtp * frac);
+  const T wdt = T(*dtp * (stage == 0 || stage == 3 ? 1.0 / 6.0 : 1.0 / 3.0));  // RK4 weight x dt
   T fA = T(0);
   if (rp.forceA) fA = T(*rp.forceA);
   SpecGeom pg{};
@@ -291,20 +294,25 @@ k_z_fwd_rk4(const C<T>* bufT, long s_slot, C<T>* u0, C<T>* ust, C<T>* acc, const
       else a = add<T>(vw[i][k * TX + xl], v[e]);
       C<T> kk = mk<T>(a.y, -a.x);  // -i a
       const long o = ci + long(k) * TX;
-      const C<T> b0 = u0[o];
-      const C<T> uu = stage == 0 ? b0 : ust[o];
+      const C<T> uu = stage == 0 ? u0[o] : ust[o];
       const T k2 = kx * kx + ky * ky + kz * kz;
       kk = mk<T>(kk.x - nu * k2 * uu.x, kk.y - nu * k2 * uu.y);
       if (rp.forceA) {
         const int k2i = x * x + kyi * kyi + kzi * kzi;
         if (k2i > 0 && k2i <= rp.kband2) kk = add<T>(kk, scl<T>(uu, fA));
       }
-      C<T> s_;
-      if (stage == 0) s_ = kk;
-      else if (stage == 3) s_ = add<T>(acc[o], kk);
-      else s_ = add<T>(acc[o], scl<T>(kk, T(2)));
-      if (stage < 3) acc[o] = s_;
-      sink(k, add<T>(b0, scl<T>(stage < 3 ? kk : s_, cdt)));
+      // separate branches: with the accumulator's load and store in one
+      // statement the compiler may issue the load at stage 0 too
+      if (stage == 3) {
+        sink(k, add<T>(acc[o], scl<T>(kk, wdt)));
+      } else if (stage == 0) {
+        acc[o] = add<T>(uu, scl<T>(kk, wdt));
+        sink(k, add<T>(uu, scl<T>(kk, cdt)));
+      } else {
+        const C<T> b0 = u0[o];
+        acc[o] = add<T>(acc[o], scl<T>(kk, wdt));
+        sink(k, add<T>(b0, scl<T>(kk, cdt)));
+      }
     });
   };
   fetch(SZ(0));
