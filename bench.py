@@ -34,13 +34,15 @@ ALG_C = {
     "(k_rk4_update_lin<T, 3>)": 15.75,
     # last forward pass with the RK4 update fused (the pass model's 12 C plus
     # the update's 9 C in stages 0-2 and 6 C in stage 3): 21, 21, 21, 18 C
+    # (the kernel moves 15, 21, 21, 15 C: no accumulator read at stage 0, and
+    # the accumulator holds the running new state, so stage 3 reads no u0)
     "z_fwd_rk4": 20.25,
 }
 RK4_STEP_C = 213  # fused-pipeline model of one RK4 step (45 C per RHS + updates)
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at
 # 810^3 FP64, calibrated against a 4 GB device copy (FETCH x 2, WRITE x 1);
 # profiles/rk4_810_fp64_counters.md holds the table and the commands
-MEASURED_GB_810_FP64 = {"x_products": 34.15, "y_fwd_div": 52.24, "z_fwd_rk4": 81.20, "z_fwd_div": 38.87,
+MEASURED_GB_810_FP64 = {"x_products": 34.15, "y_fwd_div": 52.13, "z_fwd_rk4": 79.46, "z_fwd_div": 38.87,
                         "(k_rk4_update_lin<T, 3>)": 67.35}
 
 
