@@ -4,7 +4,6 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
-#include <hipcub/hipcub.hpp>
 #include "ops.hpp"
 #include "fft_passes.hpp"
 #include "rhs_passes.hpp"
@@ -1091,7 +1090,8 @@ double Ops<T>::sample_mean(Ctx& c, const T* x, long n) {
 // moments (src/analysis.jl:9-18) and the backscatter fraction
 // (:132) of the device samples x[0, n): mean, median, std (n - 1), skewness
 // (third standardised moment, 0 when std is 0), fraction of x < 0.  The
-// median comes from a device radix sort of a copy.
+// median's two middle order statistics come from a device radix selection
+// (k_radix_hist / k_radix_pick: 8-bit digits of order-preserving keys).
 template <typename T>
 void Ops<T>::sample_moments(Ctx& c, const T* x, long n, double* out) {
   if (n < 2) throw Error(1, "moments need at least two samples");
@@ -1100,17 +1100,38 @@ void Ops<T>::sample_moments(Ctx& c, const T* x, long n, double* out) {
   const double m = sum / double(n);
   sample_sums<T>(c, x, nullptr, n, 1, m, 0, 3, cm);
   const double sd = std::sqrt(cm[0] / double(n - 1));
-  T* sorted = reinterpret_cast<T*>(c.work(6, size_t(n) * sizeof(T)));
-  size_t tb = 0;
-  SC_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, x, sorted, int(n), 0, int(8 * sizeof(T)), c.stream));
-  void* tmp = c.work(7, tb);
-  SC_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tb, x, sorted, int(n), 0, int(8 * sizeof(T)), c.stream));
-  T mid[2];
-  const long i0 = (n - 1) / 2;
-  SC_HIP(hipMemcpyAsync(mid, sorted + i0, 2 * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+  auto* sel = reinterpret_cast<RadixSel*>(c.work(7, 2 * sizeof(RadixSel) + 512 * sizeof(unsigned long long)));
+  auto* hist = reinterpret_cast<unsigned long long*>(sel + 2);
+  hipLaunchKernelGGL(k_radix_init, dim3(1), dim3(256), 0, c.stream, sel, (long long)((n - 1) / 2),
+                     (long long)(n / 2), hist);
+  const unsigned nb = unsigned(std::min<long>(1024, (n + 255) / 256));
+  for (int shift = 8 * int(sizeof(T)) - 8; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_radix_hist<T>, dim3(nb), dim3(256), 0, c.stream, x, n, sel, shift, hist);
+    hipLaunchKernelGGL(k_radix_pick, dim3(1), dim3(256), 0, c.stream, sel, shift, hist);
+  }
+  SC_HIP(hipGetLastError());
+  static_assert(2 * sizeof(RadixSel) <= 6 * sizeof(double), "host staging");
+  RadixSel* hs = reinterpret_cast<RadixSel*>(c.hostbuf(6));
+  SC_HIP(hipMemcpyAsync(hs, sel, 2 * sizeof(RadixSel), hipMemcpyDeviceToHost, c.stream));
   SC_HIP(hipStreamSynchronize(c.stream));
+  // the keys back to values (the inverse of okey)
+  auto value = [](unsigned long long key) -> double {
+    if constexpr (sizeof(T) == 8) {
+      const unsigned long long b = (key >> 63) ? (key & ~(1ull << 63)) : ~key;
+      double v;
+      std::memcpy(&v, &b, sizeof v);
+      return v;
+    } else {
+      const unsigned k32 = unsigned(key);
+      const unsigned b = (k32 >> 31) ? (k32 & 0x7fffffffu) : ~k32;
+      float v;
+      std::memcpy(&v, &b, sizeof v);
+      return double(v);
+    }
+  };
+  const double mid0 = value(hs[0].prefix), mid1 = value(hs[1].prefix);
   out[0] = m;
-  out[1] = n % 2 ? double(mid[0]) : (double(mid[0]) + double(mid[1])) / 2;
+  out[1] = n % 2 ? mid0 : (mid0 + mid1) / 2;
   out[2] = sd;
   out[3] = sd == 0.0 ? 0.0 : cm[1] / double(n) / (sd * sd * sd);
   out[4] = cm[2] / double(n);

@@ -939,6 +939,73 @@ __global__ void k_sample_stats(const T* x, const T* y, long n, int kind, double 
   }
 }
 
+// ---------------------------------------------------------- radix selection
+// Order statistics of floating-point samples without sorting: the value of
+// rank k is found digit by digit (8 bits at a time, most significant first)
+// on order-preserving unsigned keys (negative values bit-inverted, the sign
+// bit set on the others), each pass counting the digit over the samples whose
+// higher digits match the prefix found so far.  Two ranks are selected in the
+// same passes (the two middle elements of a median).
+DEV unsigned long long okey(float x) {
+  const unsigned b = __float_as_uint(x);
+  return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+DEV unsigned long long okey(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+struct RadixSel {
+  unsigned long long prefix, mask;  // digits found so far and their bit mask
+  long long k;                      // rank among the samples matching the prefix
+};
+static __global__ void k_radix_init(RadixSel* sel, long long r0, long long r1, unsigned long long* hist) {
+  if (threadIdx.x == 0) {
+    sel[0] = RadixSel{0, 0, r0};
+    sel[1] = RadixSel{0, 0, r1};
+  }
+  for (int b = threadIdx.x; b < 512; b += blockDim.x) hist[b] = 0;
+}
+// hist[256 j + d]: samples matching selection j's prefix whose digit at
+// `shift` is d
+template <typename T>
+__global__ void __launch_bounds__(256) k_radix_hist(const T* x, long n, const RadixSel* sel, int shift,
+                                                    unsigned long long* hist) {
+  __shared__ unsigned h[512];
+  for (int b = threadIdx.x; b < 512; b += 256) h[b] = 0;
+  __syncthreads();
+  const RadixSel s0 = sel[0], s1 = sel[1];
+  for (long i = gid(); i < n; i += long(gridDim.x) * blockDim.x) {
+    const unsigned long long key = okey(x[i]);
+    const unsigned d = unsigned(key >> shift) & 255u;
+    if ((key & s0.mask) == s0.prefix) atomicAdd(&h[d], 1u);
+    if ((key & s1.mask) == s1.prefix) atomicAdd(&h[256 + d], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 512; b += 256)
+    if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+}
+// the digit holding each selection's rank extends its prefix; the
+// histogram is cleared for the next pass
+static __global__ void k_radix_pick(RadixSel* sel, int shift, unsigned long long* hist) {
+  if (threadIdx.x < 2) {
+    RadixSel s = sel[threadIdx.x];
+    const unsigned long long* h = hist + 256 * threadIdx.x;
+    unsigned long long below = 0;
+    for (int b = 0; b < 256; ++b) {
+      if ((unsigned long long)s.k < below + h[b]) {
+        s.prefix |= (unsigned long long)b << shift;
+        s.mask |= 255ull << shift;
+        s.k -= (long long)below;
+        break;
+      }
+      below += h[b];
+    }
+    sel[threadIdx.x] = s;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 512; b += blockDim.x) hist[b] = 0;
+}
+
 // smagorinsky_tensor! (src/closures.jl:117-150), optional output scale c
 template <typename T, int D>
 __global__ void k_smag_tensor(T* m, const T* S, double delta, long np) {

@@ -41,6 +41,29 @@ def test_dissipation_transfer_parity(D, n):
     assert A.sample_mean(eps, gg) == pytest.approx(float(OA.sfs_dissipation(tau, oGG, D).mean()), rel=1e-12)
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n", [2, 3, 1000, 1001, (1 << 20) + 3])
+def test_median_radix_selection(n, prec):
+    """The median's device radix selection against numpy on samples with
+    repeated values, both signs and signed zeros (exact equality)."""
+    import symcode as sc
+    from symcode import analysis as A
+
+    print_case("median", n, prec)
+    dt = np.float64 if prec == 64 else np.float32
+    gg = sc.Grid(3, 16, dtype=torch.float64 if prec == 64 else torch.float32)
+    r = np.random.default_rng(n + prec)
+    x = np.round(r.standard_normal(n) * 50) / 8  # many repeats
+    x[: n // 7] = -0.0
+    x[n // 7 : n // 5] = 0.0
+    x = x.astype(dt)
+    r.shuffle(x)
+    m = A.moments(torch.tensor(x, device="cuda"), gg)
+    xs = np.sort(x.astype(np.float64))
+    ref = xs[(n - 1) // 2] if n % 2 else (xs[n // 2 - 1] + xs[n // 2]) / 2
+    assert m["median"] == ref
+
+
 @pytest.mark.parametrize("n", [1001, 4096])
 def test_moments_and_apriori_parity(n):
     import symcode as sc
