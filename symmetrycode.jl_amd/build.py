@@ -47,7 +47,7 @@ def build(jobs=8, verbose=False, defines=(), out=None, bdir=None):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(one, SOURCES))
     tmp = out + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lrocblas", "-o", tmp]
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")

@@ -9,14 +9,16 @@
 //
 // with |∇ū|² = Σ_ij G_ij² pointwise and eps = eps(Float64).  Training uses the
 // dataloader's pairs (x, y = τ / (Δ² |∇ū|² + eps)) in FP32.  Layout: voxel
-// matrices are channels-fastest, [nvox][c] (= column-major c x nvox, the
-// layout rocBLAS sees); weights are Lux's (c_in, c_out) column-major
-// (c_in fastest), so the flat parameter vector [W1, b1, W2, b2, W3, b3, W4]
-// is Lux's own parameter order.  Every layer is a plain GEMM over the voxels
-// (rocBLAS); bias + GELU, GELU', the output trace removal, the feature
-// normalisation, the loss and the optimiser are this file's kernels.
+// matrices are channels-fastest, [nvox][c] (= column-major c x nvox); weights
+// are Lux's (c_in, c_out) column-major (c_in fastest), so the flat parameter
+// vector [W1, b1, W2, b2, W3, b3, W4] is Lux's own parameter order.  Training
+// runs every layer as a GEMM over the voxels (k_dense: bias + GELU fused into
+// the forward's store, GELU' into the data gradient's, the weight gradient
+// split over voxel chunks and summed in order); the FP64 inference of the
+// closure runs the fused MFMA kernels below, or k_dense per layer for
+// networks too wide for them.  The output trace removal, the feature
+// normalisation, the loss and the optimiser are this file's kernels too.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 #include <algorithm>
 #include <array>
 #include <cfloat>
@@ -36,12 +38,6 @@ namespace pmlp {
   do {                                                                                  \
     hipError_t e_ = (call);                                                             \
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-#define PM_BLAS(call)                                                                   \
-  do {                                                                                  \
-    rocblas_status s_ = (call);                                                         \
-    if (s_ != rocblas_status_success)                                                   \
-      throw std::runtime_error(std::string(#call) + ": " + rocblas_status_to_string(s_)); \
   } while (0)
 
 namespace {
@@ -143,21 +139,6 @@ __global__ void k_widen(const float* p, double* q, long n) {
   if (i < n) q[i] = double(p[i]);
 }
 
-// z += b (per channel); a = gelu(z)
-__global__ void k_bias_gelu(float* z, const float* b, float* a, long nv, int co) {
-  const long i = gid();
-  if (i >= nv * co) return;
-  const float t = z[i] + b[i % co];
-  z[i] = t;
-  a[i] = gelu(t);
-}
-
-// inference: z = gelu(z + b) in place
-template <typename E>
-__global__ void k_bias_gelu_inf(E* z, const E* b, long n, int co) {
-  const long i = gid();
-  if (i < n) z[i] = gelu(z[i] + b[i % co]);
-}
 
 // ------------------------------------------ fused FP64 inference on MFMA
 // The four layers of the network on v_mfma_f64_16x16x4f64 (D[i][j] +=
@@ -331,11 +312,198 @@ __global__ void __launch_bounds__(256) k_mlp_fused(const double* __restrict__ pd
   }
 }
 
-// d *= gelu'(z)
-__global__ void k_gelu_back(float* d, const float* z, long n) {
-  const long i = gid();
-  if (i >= n) return;
-  d[i] *= gelu_grad(z[i]);
+
+// ------------------------------------------------------- dense layers
+// C[n ldc + m] = epilogue(sum_k A(m, k) B(k, n)) for m < M, n < N, over a
+// k range: a register-blocked GEMM on the VALU, one 64 x 64 tile of C per
+// block of 256 threads (4 x 4 per thread), the A and B tiles of 16 k staged
+// in LDS.  The layer shapes are thin (M, K at most a few hundred channels,
+// N the voxels), so the layers are bound by their activations' HBM traffic
+// and the epilogue that a separate pass would add; fusing the bias and the
+// activation into the store removes that pass.
+//   A(m, k) = A[m lda + k] (AT false) or A[k lda + m] (AT true)
+//   B(k, n) = B[n ldb + k] (BT false) or B[k ldb + n] (BT true)
+// Grid: (N tiles, M tiles, k splits).  Split over k: block z takes k in
+// [z kc, min(K, (z + 1) kc)) and writes the partial tile at C + z N ldc
+// (DN_PLAIN only).
+enum DenseMode { DN_PLAIN = 0, DN_BIAS_GELU_TRAIN = 1, DN_BIAS_GELU = 2, DN_GELU_BACK = 3 };
+//   DN_BIAS_GELU_TRAIN: z = sum + b[m] to C, gelu(z) to C2
+//   DN_BIAS_GELU:       gelu(sum + b[m]) to C
+//   DN_GELU_BACK:       sum * gelu'(zp[n ldc + m]) to C
+constexpr int DN_BM = 64, DN_BN = 64, DN_BK = 16;
+template <typename E, int MODE, bool AT, bool BT>
+__global__ void __launch_bounds__(256) k_dense(const E* __restrict__ A, long lda, const E* __restrict__ B, long ldb,
+                                              E* C, long ldc, int M, long N, long K, long kc,
+                                              const E* __restrict__ bias, E* C2, const E* __restrict__ zp) {
+  __shared__ __attribute__((aligned(16))) E As[DN_BK][DN_BM + 4];
+  __shared__ __attribute__((aligned(16))) E Bs[DN_BK][DN_BN + 4];
+  const int t = threadIdx.x;
+  const int tm = t % 16, tn = t / 16;  // consecutive threads on consecutive m: coalesced stores
+  const int m0 = blockIdx.y * DN_BM;
+  const long n0 = long(blockIdx.x) * DN_BN;
+  const long k0 = long(blockIdx.z) * kc, k1 = K < k0 + kc ? K : k0 + kc;
+  E acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = E(0);
+  for (long kb = k0; kb < k1; kb += DN_BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < DN_BM * DN_BK / 256; ++r) {
+      const int e = t + 256 * r;
+      // the index along the operand's contiguous direction varies fastest
+      const int kk = AT ? e / DN_BM : e % DN_BK, mm = AT ? e % DN_BM : e / DN_BK;
+      const long k = kb + kk;
+      const int m = m0 + mm;
+      As[kk][mm] = (k < k1 && m < M) ? (AT ? A[k * lda + m] : A[long(m) * lda + k]) : E(0);
+    }
+#pragma unroll
+    for (int r = 0; r < DN_BN * DN_BK / 256; ++r) {
+      const int e = t + 256 * r;
+      const int kk = BT ? e / DN_BN : e % DN_BK, nn = BT ? e % DN_BN : e / DN_BK;
+      const long k = kb + kk, n = n0 + nn;
+      Bs[kk][nn] = (k < k1 && n < N) ? (BT ? B[k * ldb + n] : B[n * ldb + k]) : E(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < DN_BK; ++kk) {
+      E a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][4 * tm + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][4 * tn + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+  }
+  E* Cz = C + long(blockIdx.z) * N * ldc;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const long n = n0 + 4 * tn + j;
+    if (n >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + 4 * tm + i;
+      if (m >= M) continue;
+      const long o = n * ldc + m;
+      if constexpr (MODE == DN_PLAIN) {
+        Cz[o] = acc[i][j];
+      } else if constexpr (MODE == DN_BIAS_GELU_TRAIN) {
+        const E zz = acc[i][j] + bias[m];
+        C[o] = zz;
+        C2[o] = gelu(zz);
+      } else if constexpr (MODE == DN_BIAS_GELU) {
+        C[o] = gelu(acc[i][j] + bias[m]);
+      } else {
+        C[o] = acc[i][j] * gelu_grad(zp[o]);
+      }
+    }
+  }
+}
+
+// FP64 dense layer of the inference on v_mfma_f64_16x16x4f64 (lane l holds
+// A[l%16][l/16], B[l/16][l%16], D[l/16 + 4r][l%16]): rows of the product are
+// voxels, columns output channels, so each D row is 16 consecutive channels
+// of one voxel (128-byte stores).  out[v ldo + o] = act(sum_k x[v ldx + k]
+// W[o ldw + k] + b[o]) for v < nv, o < co, k < ci; act = gelu on hidden
+// layers (GELU) or the identity.  Block: 128 voxels x 32 channels, 4
+// wavefronts of 32 voxels x 32 channels (2 x 2 MFMA tiles); 32-channel k
+// slices of the voxels and the weights staged in LDS, the next slice loaded
+// into registers while the current one is multiplied.  Grid: (channel
+// blocks, voxel blocks), so the blocks sharing a voxel tile run side by side
+// and read it from L2 once.
+constexpr int DM_BV = 128, DM_BO = 32, DM_BK = 32, DM_PAD = DM_BK + 1;
+template <bool GELU>
+__global__ void __launch_bounds__(256) k_dense_mfma64(const double* __restrict__ x, long ldx,
+                                                     const double* __restrict__ W, long ldw,
+                                                     const double* __restrict__ bias, double* out, long ldo,
+                                                     long nv, int co, int ci) {
+  __shared__ double Xs[DM_BV][DM_PAD];
+  __shared__ double Ws[DM_BO][DM_PAD];
+  constexpr int NX = DM_BV * DM_BK / 256, NW = DM_BO * DM_BK / 256;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int o0 = blockIdx.x * DM_BO;
+  const long v0 = long(blockIdx.y) * DM_BV;
+  const int wv = 32 * w;  // the wavefront's first voxel within the block
+  // staging of the k slice from k0: entry e = t + 256 r, k fastest
+  double px[NX], pw[NW];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NX; ++r) {
+      const int e = t + 256 * r, kk = e % DM_BK;
+      const long v = v0 + e / DM_BK;
+      px[r] = (v < nv && k0 + kk < ci) ? x[v * ldx + k0 + kk] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < NW; ++r) {
+      const int e = t + 256 * r, kk = e % DM_BK, o = o0 + e / DM_BK;
+      pw[r] = (o < co && k0 + kk < ci) ? W[long(o) * ldw + k0 + kk] : 0.0;
+    }
+  };
+  d4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
+  fetch(0);
+  for (int k0 = 0; k0 < ci; k0 += DM_BK) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NX; ++r) Xs[(t + 256 * r) / DM_BK][(t + 256 * r) % DM_BK] = px[r];
+#pragma unroll
+    for (int r = 0; r < NW; ++r) Ws[(t + 256 * r) / DM_BK][(t + 256 * r) % DM_BK] = pw[r];
+    __syncthreads();
+    if (k0 + DM_BK < ci) fetch(k0 + DM_BK);
+#pragma unroll
+    for (int ks = 0; ks < DM_BK; ks += 4) {
+      double av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = Xs[wv + 16 * a + (l & 15)][ks + (l >> 4)];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = Ws[16 * b + (l & 15)][ks + (l >> 4)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int o = o0 + 16 * b + (l & 15);
+    if (o >= co) continue;
+    const double bo = GELU ? bias[o] : 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long v = v0 + wv + 16 * a + (l >> 4) + 4 * r;
+        if (v >= nv) continue;
+        const double s = acc[a][b][r];
+        out[v * ldo + o] = GELU ? gelu(s + bo) : s;
+      }
+  }
+}
+
+// out[j] += sum over z of part[z len + j], z in order (deterministic)
+__global__ void k_dense_reduce(const float* part, long nz, long len, float* out) {
+  const long j = gid();
+  if (j >= len) return;
+  float s = 0.f;
+  for (long z = 0; z < nz; ++z) s += part[z * len + j];
+  out[j] += s;
+}
+
+// part[z co + o] = sum of d[v co + o] over the voxels v of chunk z
+__global__ void k_colsum_partials(const float* d, int co, long nv, long chunk, float* part) {
+  const long v0 = long(blockIdx.x) * chunk, v1 = nv < v0 + chunk ? nv : v0 + chunk;
+  for (int o = threadIdx.x; o < co; o += blockDim.x) {
+    float s = 0.f;
+    for (long v = v0; v < v1; ++v) s += d[v * co + o];
+    part[long(blockIdx.x) * co + o] = s;
+  }
 }
 
 // trace removal of packed (xx, yy[, zz], ...) rows, in place: the `symm`
@@ -1073,7 +1241,6 @@ struct sc_pmlp {
   long nvmax = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  rocblas_handle blas = nullptr;
   std::string err;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   long plen = 0;
@@ -1086,8 +1253,10 @@ struct sc_pmlp {
   long flen = 0;
   int* gt = nullptr;  // arch 2 group tables (device): perm, sign, pinv [nreg][3], relidx, ninv [nreg][nreg]
   // activations of the last forward (z pre-activation, a = gelu(z)) per
-  // hidden layer, two gradient scratch buffers, a vector of ones (bias sums)
-  float *z[3] = {}, *a[3] = {}, *d0 = nullptr, *d1 = nullptr, *ones = nullptr;
+  // hidden layer, two gradient scratch buffers, the parameter gradients'
+  // partial sums (dpart_len floats)
+  float *z[3] = {}, *a[3] = {}, *d0 = nullptr, *d1 = nullptr, *dpart = nullptr;
+  long dpart_len = 0;
   // FP64 inference (the closure): widened parameters, the live Re_Δ input,
   // input / ping-pong / output buffers for chunks of up to icap voxels
   double *pd = nullptr, *zdev = nullptr, *xd = nullptr, *hA = nullptr, *hB = nullptr, *yd = nullptr;
@@ -1171,12 +1340,38 @@ void synthesise_T(sc_pmlp* c) {
   PM_HIP(hipGetLastError());
 }
 
-// Z (co x nv) = Wᵀ (co x ci) X (ci x nv, leading dimension ldin(l))
-void gemm_fwd(sc_pmlp* c, int l, const float* X, float* Z, long nv) {
-  const float one = 1.f, zero = 0.f;
+// C (ldc x N, M rows used) = A B through k_dense, k split over nz blocks
+template <typename E, int MODE, bool AT, bool BT>
+void dense(hipStream_t s, const E* A, long lda, const E* B, long ldb, E* C, long ldc, int M, long N, long K,
+           long nz = 1, const E* bias = nullptr, E* C2 = nullptr, const E* zp = nullptr) {
+  const long kc = (K + nz - 1) / nz;
+  const dim3 grid(unsigned((N + DN_BN - 1) / DN_BN), unsigned((M + DN_BM - 1) / DN_BM), unsigned(nz));
+  hipLaunchKernelGGL((k_dense<E, MODE, AT, BT>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, kc, bias,
+                     C2, zp);
+  PM_HIP(hipGetLastError());
+}
+
+// layer l of the training network on nv voxels: Z (co x nv) = W (co x ci) X
+// (ci x nv, leading dimension ldin(l)); hidden layers add the bias and
+// store z in Z and gelu(z) in act
+void layer_fwd(sc_pmlp* c, int l, const float* X, float* Z, float* act, long nv) {
   const int ci = c->cin(l), co = c->cout(l);
-  PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                        c->pf + c->fo[2 * l], ci, X, c->ldin(l), &zero, Z, co));
+  if (l < 3)
+    dense<float, DN_BIAS_GELU_TRAIN, false, false>(c->stream, c->pf + c->fo[2 * l], ci, X, c->ldin(l), Z, co, co, nv,
+                                                  ci, 1, c->pf + c->fo[2 * l + 1], act);
+  else
+    dense<float, DN_PLAIN, false, false>(c->stream, c->pf + c->fo[2 * l], ci, X, c->ldin(l), Z, co, co, nv, ci);
+}
+
+// partial-sum scratch of the parameter gradients (floats)
+float* grad_scratch(sc_pmlp* c, long n) {
+  if (c->dpart_len < n) {
+    if (c->dpart) PM_HIP(hipFree(c->dpart));
+    c->dpart = nullptr;
+    PM_HIP(hipMalloc(&c->dpart, n * sizeof(float)));
+    c->dpart_len = n;
+  }
+  return c->dpart;
 }
 
 void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
@@ -1184,23 +1379,20 @@ void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
   if (c->arch == 2) synthesise<float>(c, c->pf, float(c->alpha()));
   const float* in = x;
   for (int l = 0; l < 3; ++l) {
-    gemm_fwd(c, l, in, c->z[l], nv);
-    hipLaunchKernelGGL(k_bias_gelu, dim3(nb(nv * c->wf[l])), dim3(256), 0, c->stream, c->z[l],
-                       c->pf + c->fo[2 * l + 1], c->a[l], nv, c->wf[l]);
-    PM_HIP(hipGetLastError());
+    layer_fwd(c, l, in, c->z[l], c->a[l], nv);
     in = c->a[l];
   }
   if (c->arch == 2) {
     // the sink's D^2 outputs, then the symmetrising trace removal
-    gemm_fwd(c, 3, in, c->d1, nv);
+    layer_fwd(c, 3, in, c->d1, nullptr, nv);
     hipLaunchKernelGGL(k_symm_sink<float>, dim3(nb(nv)), dim3(256), 0, c->stream, (const float*)c->d1, y, nv, c->D);
   } else if (c->arch == 1) {
     // basis coefficients, then the contraction with the rows' basis tensors
-    gemm_fwd(c, 3, in, c->d1, nv);
+    layer_fwd(c, 3, in, c->d1, nullptr, nv);
     hipLaunchKernelGGL(k_tb_contract<float>, dim3(nb(nv)), dim3(256), 0, c->stream, (const float*)c->d1, x, nv, c->ctot,
                        c->nin, c->nt, c->nbas, y);
   } else {
-    gemm_fwd(c, 3, in, y, nv);
+    layer_fwd(c, 3, in, y, nullptr, nv);
     hipLaunchKernelGGL(k_tracefree<float>, dim3(nb(nv)), dim3(256), 0, c->stream, y, nv, c->nt, c->D);
   }
   PM_HIP(hipGetLastError());
@@ -1210,7 +1402,6 @@ void forward_impl(sc_pmlp* c, const float* x, float* y, long nv) {
 // accumulate parameter gradients of the last forward for dy = dL/dy
 void backward_impl(sc_pmlp* c, const float* x, const float* dy, long nv) {
   if (nv != c->last_nv) throw std::runtime_error("backward needs the voxel count of the last forward");
-  const float one = 1.f, zero = 0.f;
   float* d = c->d0;
   if (c->arch == 2) {
     hipLaunchKernelGGL(k_symm_sink_T, dim3(nb(nv)), dim3(256), 0, c->stream, dy, d, nv, c->D);
@@ -1227,21 +1418,29 @@ void backward_impl(sc_pmlp* c, const float* x, const float* dy, long nv) {
   for (int l = 3; l >= 0; --l) {
     const int ci = c->cin(l), co = c->cout(l);
     const float* A = l == 0 ? x : c->a[l - 1];
-    // dW (ci x co) += A (ci x nv) dZᵀ (nv x co)
-    PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, ci, co, int(nv), &one, A,
-                          c->ldin(l), d, co, &one, c->gf + c->fo[2 * l], ci));
+    // dW (co x ci, stored [o][i]) += sum over the voxels of dZ Aᵀ: the voxels
+    // split over nz blocks per 64 x 64 tile, the partials added in order
+    const long tiles = long((ci + DN_BM - 1) / DN_BM) * ((co + DN_BN - 1) / DN_BN);
+    const long nz = std::max(1L, std::min((nv + 255) / 256, 1024 / tiles));
+    float* part = grad_scratch(c, nz * long(ci) * co + nz * co);
+    dense<float, DN_PLAIN, true, true>(c->stream, A, c->ldin(l), d, co, part, ci, ci, co, nv, nz);
+    hipLaunchKernelGGL(k_dense_reduce, dim3(nb(long(ci) * co)), dim3(256), 0, c->stream, (const float*)part, nz,
+                       long(ci) * co, c->gf + c->fo[2 * l]);
     if (l < 3) {
-      // db (co) += dZ (co x nv) 1
-      PM_BLAS(rocblas_sgemv(c->blas, rocblas_operation_none, co, int(nv), &one, d, co, c->ones, 1, &one,
-                            c->gf + c->fo[2 * l + 1], 1));
+      // db (co) += sum over the voxels of dZ
+      float* pb = part + nz * long(ci) * co;
+      const long chunk = (nv + nz - 1) / nz;
+      hipLaunchKernelGGL(k_colsum_partials, dim3(unsigned((nv + chunk - 1) / chunk)), dim3(256), 0, c->stream,
+                         (const float*)d, co, nv, chunk, pb);
+      hipLaunchKernelGGL(k_dense_reduce, dim3(nb(co)), dim3(256), 0, c->stream, (const float*)pb,
+                         (nv + chunk - 1) / chunk, long(co), c->gf + c->fo[2 * l + 1]);
     }
+    PM_HIP(hipGetLastError());
     if (l > 0) {
-      // dA (ci x nv) = W (ci x co) dZ (co x nv), then dZ_{l-1} = dA * gelu'(z_{l-1})
+      // dZ_{l-1} (ci x nv) = (Wᵀ dZ) * gelu'(z_{l-1})
       float* dn = d == c->d0 ? c->d1 : c->d0;
-      PM_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, ci, int(nv), co, &one,
-                            c->pf + c->fo[2 * l], ci, d, co, &zero, dn, ci));
-      hipLaunchKernelGGL(k_gelu_back, dim3(nb(nv * ci)), dim3(256), 0, c->stream, dn, c->z[l - 1], nv * ci);
-      PM_HIP(hipGetLastError());
+      dense<float, DN_GELU_BACK, true, false>(c->stream, c->pf + c->fo[2 * l], ci, d, co, dn, ci, ci, nv, co, 1,
+                                              nullptr, nullptr, c->z[l - 1]);
       d = dn;
     }
   }
@@ -1368,7 +1567,6 @@ void launch_tb_closure(sc_pmlp* c, const FusedNet& f, int tom, int need, int yba
 // y (nt x nv) = symm(mlp(x)) or, for the TBNN, the basis contraction of the
 // coefficients
 void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
-  const double one = 1.0, zero = 0.0;
   const double* in = x;
   double* out = c->hA;
   FusedNet f;
@@ -1391,14 +1589,19 @@ void infer_chunk(sc_pmlp* c, const double* x, double* y, long nv) {
     for (int l = 0; l < 4; ++l) {
       const int ci = c->cin(l), co = c->cout(l);
       double* dst = l == 3 && c->arch == 0 ? y : out;  // the others end in a head
-      PM_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, co, int(nv), ci, &one,
-                            c->pd + c->fo[2 * l], ci, in, l == 0 ? c->ctot : ci, &zero, dst, co));
+      const double* W = c->pd + c->fo[2 * l];
+      const dim3 grid(unsigned((co + DM_BO - 1) / DM_BO), unsigned((nv + DM_BV - 1) / DM_BV));
+      const long ldx = l == 0 ? c->ctot : ci;
       if (l < 3) {
-        hipLaunchKernelGGL(k_bias_gelu_inf<double>, dim3(nb(nv * co)), dim3(256), 0, c->stream, dst,
-                           (const double*)c->pd + c->fo[2 * l + 1], nv * co, co);
+        hipLaunchKernelGGL(k_dense_mfma64<true>, grid, dim3(256), 0, c->stream, in, ldx, W, long(ci),
+                           (const double*)c->pd + c->fo[2 * l + 1], dst, long(co), nv, co, ci);
         PM_HIP(hipGetLastError());
         in = dst;
         out = dst == c->hA ? c->hB : c->hA;
+      } else {
+        hipLaunchKernelGGL(k_dense_mfma64<false>, grid, dim3(256), 0, c->stream, in, ldx, W, long(ci),
+                           (const double*)nullptr, dst, long(co), nv, co, ci);
+        PM_HIP(hipGetLastError());
       }
     }
   }
@@ -1425,7 +1628,6 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
   const int G = sym ? (c->D == 3 ? 48 : 8) : 1;
   hipStream_t old = c->stream;
   c->stream = s;
-  PM_BLAS(rocblas_set_stream(c->blas, s));
   try {
     // the mlp whose padded network and per-wavefront staging fit in LDS runs
     // as one fused kernel over the field; otherwise chunks of features, the
@@ -1509,11 +1711,9 @@ void closure_tau(sc_pmlp* c, const T* GG, long np, double delta, double visc, do
     }
   } catch (...) {
     c->stream = old;
-    rocblas_set_stream(c->blas, old);
     throw;
   }
   c->stream = old;
-  PM_BLAS(rocblas_set_stream(c->blas, old));
 }
 template void closure_tau<float>(sc_pmlp*, const float*, long, double, double, double, double, int, float*,
                                  hipStream_t);
@@ -1614,8 +1814,6 @@ int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64
   try {
     PM_HIP(hipStreamCreate(&c->stream));
     c->own_stream = true;
-    PM_BLAS(rocblas_create_handle(&c->blas));
-    PM_BLAS(rocblas_set_stream(c->blas, c->stream));
     long o = 0;
     for (int l = 0; l < 4; ++l) {
       c->fo[2 * l] = o;
@@ -1693,9 +1891,6 @@ int create_impl(int arch, int D, int use_redelta, int c1, int c2, int c3, uint64
     }
     PM_HIP(hipMalloc(&c->d0, nv * c->maxw() * sizeof(float)));
     PM_HIP(hipMalloc(&c->d1, nv * c->maxw() * sizeof(float)));
-    PM_HIP(hipMalloc(&c->ones, nv * sizeof(float)));
-    std::vector<float> one(nv, 1.f);
-    PM_HIP(hipMemcpy(c->ones, one.data(), nv * sizeof(float), hipMemcpyHostToDevice));
     PM_HIP(hipMalloc(&c->part, 1024 * sizeof(double)));
     PM_HIP(hipMalloc(&c->gnorm, NORM_DOUBLES * sizeof(double)));
   } catch (std::exception& e) {
@@ -1723,7 +1918,7 @@ int sc_equi_create(int D, int use_redelta, int c1, int c2, int c3, uint64_t seed
 int sc_pmlp_destroy(sc_pmlp* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->p, c->g, c->m, c->v, c->d0, c->d1, c->ones, c->pd, c->zdev, c->xd, c->hA, c->hB, c->yd, c->part, c->gnorm,
+  void* ptrs[] = {c->p, c->g, c->m, c->v, c->d0, c->d1, c->dpart, c->pd, c->zdev, c->xd, c->hA, c->hB, c->yd, c->part, c->gnorm,
                   c->gt};
   for (void* q : ptrs) if (q) (void)hipFree(q);
   if (c->arch == 2) {
@@ -1734,7 +1929,6 @@ int sc_pmlp_destroy(sc_pmlp* c) {
     if (c->z[l]) (void)hipFree(c->z[l]);
     if (c->a[l]) (void)hipFree(c->a[l]);
   }
-  if (c->blas) rocblas_destroy_handle(c->blas);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1756,7 +1950,6 @@ int sc_pmlp_set_stream(sc_pmlp* c, void* s) {
     if (c->own_stream) PM_HIP(hipStreamDestroy(c->stream));
     c->stream = (hipStream_t)s;
     c->own_stream = false;
-    PM_BLAS(rocblas_set_stream(c->blas, c->stream));
   });
 }
 
