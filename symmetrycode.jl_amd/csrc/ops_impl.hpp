@@ -1224,6 +1224,9 @@ template <typename T> void Ops<T>::gradient_phys(Ctx& c, T* GG, const CT* u) {
   c2r(c, GG, G, ng, 1.0, true);
 }
 
+template <typename T>
+static void dynsmag_add_spec(Ctx& c, C<T>* sig, const C<T>* u, double delta);
+
 // les! (src/les.jl:4-33) for the built-in closures:
 // closure 1 = Clark(Delta=p1), 2 = Smagorinsky(C_S=p1, Delta=p2),
 // 3 = dynamic Smagorinsky(Delta=p1), 4 = the attached learned closure,
