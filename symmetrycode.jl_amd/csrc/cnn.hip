@@ -638,13 +638,16 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
 //    15 MFMAs per 32 voxels.
 //  otherwise (CIN = 64, COUT = 16 or 8): the input plane z and the dout planes
 //    z + 1, z, z - 1 (dz = -1, 0, 1) are staged; compute wavefront w owns
-//    input tile w and all 27 taps (27 MFMAs per 32 voxels); COUT = 8 (the
-//    6-channel output layer) uses one output tile with channels 8-15 zero.
+//    input tile w and all 27 taps.  COUT = 16: 27 MFMAs per 32 voxels.
+//    COUT = 8 (the 6-channel output layer): the dout planes dz = -1, 0 share
+//    one 16-row tile (channels 0-7 and 8-15) and dz = 1 has a tile of its
+//    own with channels 8-15 zero, so 18 MFMAs per 32 voxels cover all taps.
 template <int CIN, int COUT, int PD>
 __global__ void __launch_bounds__(512, 1)
     k_conv_wgrad3(const bf16* __restrict__ in, const bf16* __restrict__ dout, float* part, float* db, int n,
                   long nplanes) {
   constexpr bool INPACK = CIN <= 8;
+  constexpr bool PACKO = !INPACK && COUT == 8;  // two dout planes per 16-row tile
   constexpr int NCWV = 4;  // compute wavefronts
   constexpr int NBM = 16;               // x blocks per row (n <= 128)
   constexpr int CP = CIN < 16 ? 16 : CIN;
@@ -652,26 +655,35 @@ __global__ void __launch_bounds__(512, 1)
   constexpr int NW = CP / 16;
   constexpr int NPI = INPACK ? 3 : 1;  // input planes per row step
   constexpr int NPO = INPACK ? 1 : 3;  // dout planes per row step
-  constexpr int DTS = NBM * 16 + 1, XTS = (NBM + 2) * 16 + 1;
+  constexpr int NTD = PACKO ? 2 : NPO;  // dout tiles per row slot and output tile
+  constexpr int NA = INPACK ? 0 : NTD;  // A operands (dout tiles) per x step, CIN = 64
+  // tile strides in 16-byte chunks.  CIN = 64: input tiles start 16 bytes
+  // apart (the staging stores of 8 channel blocks hit distinct 16-byte slots
+  // modulo 128 bytes).  INPACK: the row slots and planes are 128 bytes apart
+  // modulo 256, so the two taps a 16-lane group reads (plane + row offset of
+  // opposite parity) fill the two halves of one 256-byte window
+  constexpr int DTS = NBM * 16 + 1;
+  constexpr int XTS = INPACK ? (NBM + 2) * 16 + 8 : (NBM + 2) * 16 + 1;
+  constexpr int XPS = INPACK ? 4 * XTS + 8 : 4 * NW * XTS;  // input plane stride
   static_assert(NPO * 16 * (COUT / 8) + NPI * 16 * (CIN / 8) <= 256, "staging jobs");
   static_assert(INPACK ? (MT == 4 && NW == 1) : (MT == 1 && NW == 4), "wgrad3 shapes");
   static_assert(COUT % 8 == 0, "dout blocks of 8 channels");
   static_assert(64 % PD == 0, "row sets in flight divide n");
-  __shared__ uint4 dT[NPO * 2 * MT * DTS];  // [dout plane][row slot y & 1][tile]
-  __shared__ uint4 xT[NPI * 4 * NW * XTS];  // [input plane][row slot (r + 1) & 3][tile]
+  __shared__ uint4 dT[NTD * 2 * MT * DTS];  // [dout tile][row slot y & 1][output tile]
+  __shared__ uint4 xT[NPI * XPS];           // [input plane][row slot (r + 1) & 3][tile]
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // w: wave-uniform
   const int li = l & 15, lk = l >> 4;
   const bool producer = w >= NCWV;
   const int pt = int(threadIdx.x) - 64 * NCWV;  // producer thread index
-  constexpr int NACC = INPACK ? 3 * MT + 3 : 27;
+  constexpr int NACC = INPACK ? 3 * MT + 3 : 9 * NA;
   floatx4 acc[NACC];
 #pragma unroll
   for (int i = 0; i < NACC; ++i) acc[i] = floatx4{0, 0, 0, 0};
   if (CIN < CP) {  // the zero-padded channels are never written: clear all input slots once
-    for (int i = threadIdx.x; i < NPI * 4 * NW * XTS; i += 64 * (NCWV + 4)) xT[i] = uint4{0, 0, 0, 0};
+    for (int i = threadIdx.x; i < NPI * XPS; i += 64 * (NCWV + 4)) xT[i] = uint4{0, 0, 0, 0};
   }
   if (COUT < 16 * MT) {  // likewise the zero-padded output channels
-    for (int i = threadIdx.x; i < NPO * 2 * MT * DTS; i += 64 * (NCWV + 4)) dT[i] = uint4{0, 0, 0, 0};
+    for (int i = threadIdx.x; i < NTD * 2 * MT * DTS; i += 64 * (NCWV + 4)) dT[i] = uint4{0, 0, 0, 0};
   }
   float bs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int nb8 = n / 8;
@@ -701,9 +713,11 @@ __global__ void __launch_bounds__(512, 1)
   auto store = [&](int p, bool dvalid, int ds, bool xvalid, int xs) __attribute__((always_inline)) {
     uint32_t o[8][4];
     transpose8x8(v[p], o);
-    const int ch0 = (cb % 2) * 8;
     if (jd && dvalid) {
-      uint4* tile = &dT[((jpl * 2 + ds) * MT + cb / 2) * DTS + xb * 16];
+      // PACKO: dout plane jpl -> tile jpl / 2, channels 8 (jpl % 2) .. + 7
+      const int td = PACKO ? jpl >> 1 : jpl;
+      const int ch0 = PACKO ? (jpl & 1) * 8 : (cb % 2) * 8;
+      uint4* tile = &dT[((td * 2 + ds) * MT + cb / 2) * DTS + xb * 16];
 #pragma unroll
       for (int c = 0; c < 8; ++c) tile[wg_swz(ch0 + c, xb)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
       if (dobias) {
@@ -719,7 +733,8 @@ __global__ void __launch_bounds__(512, 1)
           }
       }
     } else if (jx && xvalid) {
-      uint4* tile = &xT[((jpl * 4 + xs) * NW + cb / 2) * XTS];
+      const int ch0 = (cb % 2) * 8;
+      uint4* tile = &xT[jpl * XPS + (xs * NW + cb / 2) * XTS];
       const int bk = xb + 1;
 #pragma unroll
       for (int c = 0; c < 8; ++c) tile[bk * 16 + wg_swz(ch0 + c, bk)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
@@ -732,8 +747,22 @@ __global__ void __launch_bounds__(512, 1)
   };
   const u32x4* dTv = reinterpret_cast<const u32x4*>(dT);
   const u32x4* xTv = reinterpret_cast<const u32x4*>(xT);
-  // the x shifts: from the chunks of x blocks bx, bx + 1, bx + 2 (P, C, N)
-  auto shifts = [&](const u32x4& P, const u32x4& C, const u32x4& N, u32x4* S) __attribute__((always_inline)) {
+  auto mfma = [&](const u32x4& a, const u32x4& b, floatx4& d) __attribute__((always_inline)) {
+    bf16x8 av, bv;
+    __builtin_memcpy(&av, &a, 16);
+    __builtin_memcpy(&bv, &b, 16);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, d, 0, 0, 0);
+  };
+  // the x shifts of a staged row: from the chunks of x blocks bx - 1, bx,
+  // bx + 1 (X[0..2]; LDS blocks bx, bx + 1, bx + 2, block 0 the left halo)
+  auto chunk = [&](int blk, int ch) __attribute__((always_inline)) { return blk * 16 + wg_swz(ch, blk); };
+  auto read_x = [&](u32x4* X, const u32x4* xr, int k0, int ch) __attribute__((always_inline)) {
+    const int bx = k0 / 8 + lk;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) X[i] = xr[chunk(bx + i, ch)];
+  };
+  auto shifts = [&](const u32x4* X, u32x4* S) __attribute__((always_inline)) {
+    const u32x4 P = X[0], C = X[1], N = X[2];
     asm volatile("" : : "v"(P), "v"(N));  // keeps the reads 16 bytes wide (conflict-free); no instruction
     S[0] = u32x4{__builtin_amdgcn_alignbyte(C.x, P.w, 2), __builtin_amdgcn_alignbyte(C.y, C.x, 2),
                  __builtin_amdgcn_alignbyte(C.z, C.y, 2), __builtin_amdgcn_alignbyte(C.w, C.z, 2)};
@@ -741,61 +770,91 @@ __global__ void __launch_bounds__(512, 1)
     S[2] = u32x4{__builtin_amdgcn_alignbyte(C.y, C.x, 2), __builtin_amdgcn_alignbyte(C.z, C.y, 2),
                  __builtin_amdgcn_alignbyte(C.w, C.z, 2), __builtin_amdgcn_alignbyte(N.x, C.w, 2)};
   };
-  auto mfma = [&](const u32x4& a, const u32x4& b, floatx4& d) __attribute__((always_inline)) {
-    bf16x8 av, bv;
-    __builtin_memcpy(&av, &a, 16);
-    __builtin_memcpy(&bv, &b, 16);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, d, 0, 0, 0);
-  };
   // this lane's rows: INPACK: tap s = min(2 w + li / 8, 8) (plane dz + 1 = s / 3,
   // row y - 1 + s % 3), channel li % 8; otherwise the input tile w rows
   const int sl_tap = INPACK ? 2 * w + li / 8 : 0;
+  // one row: the x steps in pairs (n is a multiple of 64), every operand
+  // chunk read one step ahead, each step's reads kept at its top
+  // (sched_barrier) so that its MFMAs wait only for reads issued a step
+  // earlier; the read after the last step repeats the last step's (in range)
   auto compute_row = [&](int y) __attribute__((always_inline)) {
+    const u32x4* dr = &dTv[(y & 1) * MT * DTS];  // dout row y (slot y & 1), output tile 0
+    if constexpr (INPACK) {
+      const int ci = li & 7;
+      const u32x4* xr = &xTv[(sl_tap / 3) * XPS + ((y + sl_tap % 3) & 3) * XTS];
+      const u32x4* xr8 = &xTv[2 * XPS + ((y + 2) & 3) * XTS];  // tap 8: plane z + 1, row y + 1
+      struct Ops {
+        u32x4 B[MT + 1];  // output tiles 0 .. MT - 1, then tile w (tap 8)
+        u32x4 X[3], X8[3];
+      };
+      auto read_ops = [&](Ops& o, int k0) __attribute__((always_inline)) {
+        const int c = chunk(k0 / 8 + lk, li);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) o.B[m] = dr[m * DTS + c];
+        o.B[MT] = dr[w * DTS + c];
+        read_x(o.X, xr, k0, ci);
+        read_x(o.X8, xr8, k0, ci);
+      };
+      auto step = [&](const Ops& o) __attribute__((always_inline)) {
+        u32x4 S[3], S8[3];
+        shifts(o.X, S);
+        shifts(o.X8, S8);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+#pragma unroll
+          for (int m = 0; m < MT; ++m) mfma(S[d], o.B[m], acc[d * MT + m]);
+          mfma(S8[d], o.B[MT], acc[3 * MT + d]);  // tap 8 against output tile w
+        }
+      };
+      Ops o0, o1;
+      read_ops(o0, 0);
 #pragma unroll 1
-    for (int k0 = 0; k0 < n; k0 += 32) {
-      const int bx = k0 / 8 + lk;
-      if constexpr (INPACK) {
-        const int pi = sl_tap / 3, rs = (y + sl_tap % 3) & 3;
-        const u32x4* xr = &xTv[(pi * 4 + rs) * XTS];
-        const int ci = li & 7;
-        const u32x4 P = xr[bx * 16 + wg_swz(ci, bx)];
-        const u32x4 C = xr[(bx + 1) * 16 + wg_swz(ci, bx + 1)];
-        const u32x4 N = xr[(bx + 2) * 16 + wg_swz(ci, bx + 2)];
-        u32x4 B[MT];
+      for (int k0 = 0; k0 < n; k0 += 64) {
+        __builtin_amdgcn_sched_barrier(0);
+        read_ops(o1, k0 + 32);
+        step(o0);
+        __builtin_amdgcn_sched_barrier(0);
+        read_ops(o0, min(k0 + 64, n - 32));
+        step(o1);
+      }
+    } else {
+      const u32x4* xr[3];  // input rows y - 1, y, y + 1 of tile w
 #pragma unroll
-        for (int m = 0; m < MT; ++m) B[m] = dTv[((y & 1) * MT + m) * DTS + bx * 16 + wg_swz(li, bx)];
-        u32x4 S[3];
-        shifts(P, C, N, S);
+      for (int a = 0; a < 3; ++a) xr[a] = &xTv[(((y + a) & 3) * NW + w) * XTS];
+      // PACKO: tile 0 = dout planes dz = -1 (rows 0-7), 0 (rows 8-15);
+      // tile 1 = dz = 1 (rows 0-7).  Otherwise tile pp = plane pp
+      struct Ops {
+        u32x4 A[NA];
+        u32x4 X[3][3];
+      };
+      auto read_ops = [&](Ops& o, int k0) __attribute__((always_inline)) {
+        const int c = chunk(k0 / 8 + lk, li);
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int pp = 0; pp < NA; ++pp) o.A[pp] = dr[pp * 2 * DTS + c];
 #pragma unroll
-          for (int m = 0; m < MT; ++m) mfma(S[d], B[m], acc[d * MT + m]);
-        // tap 8 (plane z + 1, row y + 1) against output tile w
-        const u32x4* xr8 = &xTv[(2 * 4 + ((y + 2) & 3)) * XTS];
-        const u32x4 P8 = xr8[bx * 16 + wg_swz(ci, bx)];
-        const u32x4 C8 = xr8[(bx + 1) * 16 + wg_swz(ci, bx + 1)];
-        const u32x4 N8 = xr8[(bx + 2) * 16 + wg_swz(ci, bx + 2)];
-        shifts(P8, C8, N8, S);
-        const u32x4 Bw = dTv[((y & 1) * MT + w) * DTS + bx * 16 + wg_swz(li, bx)];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) mfma(S[d], Bw, acc[3 * MT + d]);
-      } else {
-        u32x4 A[3];  // dout planes dz = -1, 0, 1 (output tile 0)
-#pragma unroll
-        for (int po = 0; po < 3; ++po) A[po] = dTv[(po * 2 + (y & 1)) * DTS + bx * 16 + wg_swz(li, bx)];
+        for (int a = 0; a < 3; ++a) read_x(o.X[a], xr[a], k0, li);
+      };
+      auto step = [&](const Ops& o) __attribute__((always_inline)) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const u32x4* xr = &xTv[(((y + a) & 3) * NW + w) * XTS];
-          const u32x4 P = xr[bx * 16 + wg_swz(li, bx)];
-          const u32x4 C = xr[(bx + 1) * 16 + wg_swz(li, bx + 1)];
-          const u32x4 N = xr[(bx + 2) * 16 + wg_swz(li, bx + 2)];
           u32x4 S[3];
-          shifts(P, C, N, S);
+          shifts(o.X[a], S);
 #pragma unroll
           for (int d = 0; d < 3; ++d)
 #pragma unroll
-            for (int po = 0; po < 3; ++po) mfma(A[po], S[d], acc[po * 9 + a * 3 + d]);
+            for (int pp = 0; pp < NA; ++pp) mfma(o.A[pp], S[d], acc[pp * 9 + a * 3 + d]);
         }
+      };
+      Ops o0, o1;
+      read_ops(o0, 0);
+#pragma unroll 1
+      for (int k0 = 0; k0 < n; k0 += 64) {
+        __builtin_amdgcn_sched_barrier(0);
+        read_ops(o1, k0 + 32);
+        step(o0);
+        __builtin_amdgcn_sched_barrier(0);
+        read_ops(o0, min(k0 + 64, n - 32));
+        step(o1);
       }
     }
   };
@@ -882,21 +941,24 @@ __global__ void __launch_bounds__(512, 1)
         part[((pbase + 24 + d) * COUT + co) * CIN + row] = acc[3 * MT + d][r];
       }
   } else {
-    // D[co = 4 lk + r][ci = 16 w + li]
+    // D[row = 4 lk + r][ci = 16 w + li]; PACKO: row -> (dout plane 2 pp + row / 8,
+    // channel row % 8), otherwise (plane pp, channel row)
     const int ci = 16 * w + li;
-    if (4 * lk >= COUT) return;
 #pragma unroll
-    for (int po = 0; po < 3; ++po)
+    for (int pp = 0; pp < NA; ++pp) {
+      const int po = PACKO ? 2 * pp + lk / 2 : pp;
+      if (po > 2 || (!PACKO && 4 * lk >= COUT)) continue;
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int co = 4 * lk + r;
+            const int co = PACKO ? (4 * lk + r) % 8 : 4 * lk + r;
             const int t = po * 9 + a * 3 + d;
-            part[((pbase + t) * COUT + co) * CIN + ci] = acc[po * 9 + a * 3 + d][r];
+            part[((pbase + t) * COUT + co) * CIN + ci] = acc[pp * 9 + a * 3 + d][r];
           }
+    }
   }
 }
 

@@ -126,6 +126,37 @@ def test_cnn_parity_n128():
     assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
 
 
+def test_cnn_parity_wide_output():
+    """12 output channels (16-row output tile, the weight gradient without
+    plane packing) and 3 input channels: forward and gradients"""
+    import symcode as sc
+
+    print_case("cnn parity cout12")
+    net = sc.cnn.ConvClosureNet(64, batch_max=1, cin=3, width=64, cout=12, seed=13)
+    n = net.n
+    assert (net.cin_p, net.cout_p) == (8, 16)
+    Ws, bs = net.unflatten(net.get_params())
+    r = np.random.default_rng(14)
+    x = np.zeros((1, net.nvox, net.cin_p), np.float32)
+    x[..., :3] = r.standard_normal((1, net.nvox, 3))
+    xt = torch.tensor(x).to(torch.bfloat16).cuda()
+    y = net.forward(xt)
+    yo, store = OC.forward(Ws, bs, torch.tensor(x), n)
+    assert rel(y.cpu().numpy(), yo.numpy()) < 1e-2
+    tgt = r.standard_normal((1, net.nvox, 12)).astype(np.float32) * 0.1
+    _, dy = net.mse(y, torch.tensor(tgt).cuda())
+    _, dyo = OC.mse(yo, tgt, 12)
+    net.zero_grad()
+    dx = net.backward(xt, dy, want_dx=True)
+    gW, gb, dxo = OC.backward(store, dyo, n)
+    GW, GB = net.unflatten(net.get_grads())
+    for l in range(4):
+        assert rel(GW[l], gW[l].numpy()) < 3e-2, l
+    for l in range(3):
+        assert rel(GB[l], gb[l].numpy()) < 3e-2, l
+    assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
+
+
 def test_cnn_tracefree_parity():
     """trace removal on the output (the reference nets' last layer): zero
     trace, forward and gradients against the oracle's projected network"""
