@@ -365,7 +365,12 @@ __device__ __forceinline__ void transpose8x8(const uint32_t in[8][4], uint32_t o
 // lanes = 8 channel blocks of one voxel block, or 8 voxel blocks of one
 // channel block) hit 8 distinct 16-byte slots modulo 128 bytes, which
 // ds_write_b128 needs.  Input rows carry one halo block on each side; the x
-// shifts dx = -1, +1 are formed in registers from the neighbouring blocks.
+// shifts dx = -1, +1 are formed in registers from a block's chunk and one
+// dword of each neighbouring block: the producers also write every block's
+// first and last dword per channel into edge arrays xE[slot][tile][first /
+// last][block][channel], where a wavefront's 64 lanes (16 channels x 4
+// consecutive blocks) read 256 contiguous bytes (4-byte reads, conflict-free)
+// instead of the neighbours' whole chunks.
 // Block sums go to part[g][t][co][ci] (k_reduce_partials adds them up); with
 // db set, the dz = 0 blocks also add the bias gradient sum_v dout[v][co].
 __device__ __forceinline__ int wg_swz(int ch, int blk) { return ch ^ (((4 * (ch >> 3)) ^ blk) & 7); }
@@ -386,6 +391,8 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   static_assert(64 % PD == 0, "row sets in flight divide n");
   __shared__ uint4 dT[2 * MT * DTS];  // dout rows y, y + 1
   __shared__ uint4 xT[4 * NW * XTS];  // input rows y - 1 .. y + 2
+  constexpr int EBS = (NBM + 2) * 16;  // dwords per edge array of a row tile (blocks x channels)
+  __shared__ uint32_t xE[4 * NW * 2 * EBS];  // [slot][tile][0: first, 1: last dword][block][channel]
   // block -> (plane group pg, dz): the three dz blocks of a plane group are
   // consecutive multiples of 8 apart in the launch order, so they run on one
   // XCD (workgroups are dealt to the 8 XCDs round-robin) and share its L2
@@ -459,6 +466,16 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
       const int bk = xb + 1;  // LDS block of x block xb (block 0 is the left halo)
 #pragma unroll
       for (int c = 0; c < 8; ++c) tile[bk * 16 + wg_swz(ch0 + c, bk)] = uint4{o[c][0], o[c][1], o[c][2], o[c][3]};
+      // edge dwords: the first dword of blocks 2 .. n / 8 + 1 and the last
+      // of blocks 0 .. n / 8 - 1 are read (halos: x block 0 as block
+      // n / 8 + 1, x block n / 8 - 1 as block 0)
+      uint32_t* ef = &xE[(xs * NW + cb / 2) * 2 * EBS];
+      uint32_t* el = ef + EBS;
+      const int fb = xb == 0 ? nb8 + 1 : bk, lb = xb == nb8 - 1 ? 0 : bk;
+      *reinterpret_cast<uint4*>(&ef[fb * 16 + ch0]) = uint4{o[0][0], o[1][0], o[2][0], o[3][0]};
+      *reinterpret_cast<uint4*>(&ef[fb * 16 + ch0 + 4]) = uint4{o[4][0], o[5][0], o[6][0], o[7][0]};
+      *reinterpret_cast<uint4*>(&el[lb * 16 + ch0]) = uint4{o[0][3], o[1][3], o[2][3], o[3][3]};
+      *reinterpret_cast<uint4*>(&el[lb * 16 + ch0 + 4]) = uint4{o[4][3], o[5][3], o[6][3], o[7][3]};
       if (xb == 0 || xb == nb8 - 1) {
         const int hb = xb == 0 ? nb8 + 1 : 0;  // x + n or x - n
 #pragma unroll
@@ -471,7 +488,8 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   // issued before the MFMAs of the current one (two register sets).  A lane
   // reads x block bx = k0 / 8 + lk.
   struct BOps {
-    u32x4 p, c, x;  // input blocks bx, bx + 1, bx + 2 (x - 8 .., x .., x + 8 ..)
+    u32x4 c;          // input block bx + 1 (x ..)
+    uint32_t pw, nx;  // last dword of block bx (x - 2, x - 1), first of block bx + 2 (x + 8, x + 9)
   };
   const u32x4* dTv = reinterpret_cast<const u32x4*>(dT);
   const u32x4* xTv = reinterpret_cast<const u32x4*>(xT);
@@ -482,24 +500,21 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
   };
   auto read_b = [&](BOps& o, int slot, int k0) __attribute__((always_inline)) {
     const u32x4* xr = &xTv[(slot * NW + wc) * XTS];
+    const uint32_t* er = &xE[(slot * NW + wc) * 2 * EBS];
     const int bx = k0 / 8 + lk;
-    o.p = xr[chunk(bx)];
     o.c = xr[chunk(bx + 1)];
-    o.x = xr[chunk(bx + 2)];
+    o.pw = er[EBS + bx * 16 + li];
+    o.nx = er[(bx + 2) * 16 + li];
   };
   auto mfma_row = [&](const u32x4* am, const BOps& o, floatx4 (&accr)[3][MT]) __attribute__((always_inline)) {
-    // only p.w and x.x enter the MFMAs; using the whole chunks keeps their
-    // LDS reads 16 bytes wide, which these lane patterns serve without bank
-    // conflicts (dword reads would conflict 4-way).  No instruction.
-    asm volatile("" : : "v"(o.p), "v"(o.x));
-    const u32x4 P = o.p, L0 = o.c, N = o.x;
+    const u32x4 L0 = o.c;
     u32x4 S[3];
     // dx = -1: x0 - 1 .. x0 + 6;  dx = 0: x0 .. x0 + 7;  dx = +1: x0 + 1 .. x0 + 8
-    S[0] = u32x4{__builtin_amdgcn_alignbyte(L0.x, P.w, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
+    S[0] = u32x4{__builtin_amdgcn_alignbyte(L0.x, o.pw, 2), __builtin_amdgcn_alignbyte(L0.y, L0.x, 2),
                  __builtin_amdgcn_alignbyte(L0.z, L0.y, 2), __builtin_amdgcn_alignbyte(L0.w, L0.z, 2)};
     S[1] = L0;
     S[2] = u32x4{__builtin_amdgcn_alignbyte(L0.y, L0.x, 2), __builtin_amdgcn_alignbyte(L0.z, L0.y, 2),
-                 __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(N.x, L0.w, 2)};
+                 __builtin_amdgcn_alignbyte(L0.w, L0.z, 2), __builtin_amdgcn_alignbyte(o.nx, L0.w, 2)};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       bf16x8 bv;
@@ -512,33 +527,56 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
       }
     }
   };
+  // A row is 2 n / 32 sections of 12 MFMAs (x step q = s / 3, input row
+  // s % 3).  Each section issues the reads of the section two ahead (B set
+  // (s + 2) % 3; the A set of the next x step one or two sections ahead)
+  // before its MFMAs (sched_barrier / sched_group_barrier: DS reads, then
+  // MFMAs; the scheduler would otherwise move the reads next to their first
+  // use), so a read has two sections of MFMAs to arrive.  The reads after
+  // the last step repeat the last step's, in range.
+#define WG_ORDER(NREADS)                                                         \
+  __builtin_amdgcn_sched_group_barrier(0x100, NREADS, 0); /* DS reads */    \
+  __builtin_amdgcn_sched_group_barrier(0x008, 3 * MT, 0); /* MFMAs */
   auto compute_row = [&](int y) __attribute__((always_inline)) {
     const int sl[3] = {y & 3, (y + 1) & 3, (y + 2) & 3};  // input rows y - 1, y, y + 1
     const int drow0 = (y & 1) * MT * DTS;
     u32x4 a0[MT], a1[MT];
-    BOps b0, b1;
+    BOps b0, b1, b2;
     read_a(a0, drow0, 0);
     read_b(b0, sl[0], 0);
+    read_b(b1, sl[1], 0);
 #pragma unroll 1
     for (int k0 = 0; k0 < n; k0 += 64) {  // n is a multiple of 64: two x steps per iteration
-      read_b(b1, sl[1], k0);
+      const int kn = min(k0 + 64, n - 32);
+      __builtin_amdgcn_sched_barrier(0);
+      read_b(b2, sl[2], k0);
       mfma_row(a0, b0, acc[0]);
-      read_b(b0, sl[2], k0);
-      mfma_row(a0, b1, acc[1]);
+      WG_ORDER(3)
+      __builtin_amdgcn_sched_barrier(0);
       read_a(a1, drow0, k0 + 32);
-      read_b(b1, sl[0], k0 + 32);
-      mfma_row(a0, b0, acc[2]);
-      read_b(b0, sl[1], k0 + 32);
-      mfma_row(a1, b1, acc[0]);
-      read_b(b1, sl[2], k0 + 32);
-      mfma_row(a1, b0, acc[1]);
-      if (k0 + 64 < n) {
-        read_a(a0, drow0, k0 + 64);
-        read_b(b0, sl[0], k0 + 64);
-      }
-      mfma_row(a1, b1, acc[2]);
+      read_b(b0, sl[0], k0 + 32);
+      mfma_row(a0, b1, acc[1]);
+      WG_ORDER(MT + 3)
+      __builtin_amdgcn_sched_barrier(0);
+      read_b(b1, sl[1], k0 + 32);
+      mfma_row(a0, b2, acc[2]);
+      WG_ORDER(3)
+      __builtin_amdgcn_sched_barrier(0);
+      read_b(b2, sl[2], k0 + 32);
+      mfma_row(a1, b0, acc[0]);
+      WG_ORDER(3)
+      __builtin_amdgcn_sched_barrier(0);
+      read_a(a0, drow0, kn);
+      read_b(b0, sl[0], kn);
+      mfma_row(a1, b1, acc[1]);
+      WG_ORDER(MT + 3)
+      __builtin_amdgcn_sched_barrier(0);
+      read_b(b1, sl[1], kn);
+      mfma_row(a1, b2, acc[2]);
+      WG_ORDER(3)
     }
   };
+#undef WG_ORDER
   // Row set s = dout row s + 1 and input row s + 2; it is loaded into
   // v[s % PD] during row s - PD (or before the plane's first row) and staged
   // during row s.  Slot of input row r (r in [-1, n]): (r + 1) & 3.
@@ -628,7 +666,9 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
 // moves every row into three compute units, which bounds these layers by
 // the bytes delivered per unit, not by MFMAs).  Block p takes the (batch, z)
 // planes p, p + gridDim.x, ... whole.  Staging (wavefronts NCWV .. NCWV + 3),
-// LDS layout and row pipeline are those of k_conv_wgrad.
+// the staged rows' chunk layout and the producers' row pipeline are those of
+// k_conv_wgrad; the x neighbours are read as whole chunks (edge arrays as
+// there did not change these kernels' time).
 //  INPACK (CIN <= 8, COUT = 64): the input planes z - 1, z, z + 1 are staged
 //    and (tap, input channel) pairs go on M: compute wavefront w owns the
 //    taps s = 2 w, 2 w + 1 of s = 3 (dz + 1) + dy + 1 (lanes 0-7 and 8-15,
