@@ -69,6 +69,42 @@ def create_data(u, g_dns: Grid, g_les: Grid, visc, filters, nsnap, dt_save, cfl=
     return pairs
 
 
+def save_pairs(path, pairs, g_les: Grid, delta_factor, visc):
+    """The pairs of one filter width as the reference's fieldsfile
+    (src/data.jl:189-191,207-211): inputs = [map(Array, ū)], outputs =
+    [map(Array, τ)] in the grid's precision and the reference's RFFT array
+    layout, redelta, Δ, Δ_factor, visc (written atomically)."""
+    from . import jld2
+
+    if not pairs:
+        raise ValueError("no pairs to save")
+    deltas = {p.delta for p in pairs}
+    if len(deltas) != 1:
+        raise ValueError("a fields file holds one filter width")
+    keys_v = jld2.VECTOR_KEYS[:g_les.D]
+    keys_t = jld2.TENSOR_KEYS if g_les.D == 3 else ("xx", "yy", "xy")
+    inputs = [jld2.field_dict(g_les.download_spectral(p.ubar), keys_v) for p in pairs]
+    outputs = [jld2.field_dict(g_les.download_spectral(p.tau), keys_t) for p in pairs]
+    jld2.save_fields(path, inputs, outputs, [p.redelta for p in pairs], pairs[0].delta, delta_factor, visc)
+
+
+def load_pairs(path, g_les: Grid):
+    """A fieldsfile (the reference's or save_pairs') as device Pairs on g_les"""
+    from . import jld2
+
+    inputs, outputs, redelta, delta = jld2.load(path, "inputs", "outputs", "redelta", "Δ")
+    keys_v = jld2.VECTOR_KEYS[:g_les.D]
+    keys_t = jld2.TENSOR_KEYS if g_les.D == 3 else ("xx", "yy", "xy")
+    shape = g_les.ref_spec_shape(1)[1:]
+    pairs = []
+    for ub, tau, rd in zip(inputs, outputs, np.atleast_1d(redelta)):
+        if tuple(ub[keys_v[0]].shape) != shape:
+            raise ValueError(f"{path}: arrays of shape {ub[keys_v[0]].shape}, the grid needs {shape}")
+        pairs.append(Pair(g_les.upload_spectral(np.stack([ub[k] for k in keys_v])),
+                          g_les.upload_spectral(np.stack([tau[k] for k in keys_t])), float(delta), float(rd)))
+    return pairs
+
+
 def velocity_rms(u, g: Grid):
     return math.sqrt(S.spectraldot(u, u, g).real / g.D)
 
