@@ -1179,8 +1179,8 @@ struct sc_cnn {
   bf16 *wf[4] = {}, *wb[4] = {};
   // activations for the largest batch: z1..z3, a1..a3 and their gradients
   bf16 *z[3] = {}, *a[3] = {}, *dz[3] = {};
-  // one-cube input / output / cotangent buffers of the LES closure (allocated
-  // on first use by the solver)
+  // one-cube input (one per activation slot) / output / cotangent buffers of
+  // the LES closure (allocated on first use by the solver)
   bf16 *a0 = nullptr, *dyb = nullptr;
   float *y = nullptr, *dx = nullptr, *part = nullptr;
   long part_len = 0;
@@ -1326,39 +1326,60 @@ void pack_all(sc_cnn* c) {
 namespace cnnlink {
 using namespace cnn;
 
-void forward(sc_cnn* c, const void* x, float* y, int batch) {
-  if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
+// forward / backward over `batch` samples whose activations start at sample
+// slot s0 of the batch buffers
+static void forward_at(sc_cnn* c, const void* x, float* y, int batch, int s0) {
+  if (batch < 1 || s0 < 0 || s0 + batch > c->bmax) throw std::runtime_error("batch out of range");
+  const long off = long(s0) * c->n * c->n * c->n * c->width;
   const bf16* in = (const bf16*)x;
   for (int l = 0; l < 3; ++l) {
-    launch_conv(c, layer_cin(c, l), c->width, HIDDEN, in, c->wf[l], c->w + c->boff[l], c->a[l], c->z[l], nullptr,
-                nullptr, batch);
-    in = c->a[l];
+    launch_conv(c, layer_cin(c, l), c->width, HIDDEN, in, c->wf[l], c->w + c->boff[l], c->a[l] + off, c->z[l] + off,
+                nullptr, nullptr, batch);
+    in = c->a[l] + off;
   }
   launch_conv(c, c->width, c->cout_p, LAST, in, c->wf[3], nullptr, nullptr, nullptr, y, nullptr, batch);
 }
 
-void backward(sc_cnn* c, const void* x, const void* dy, float* dx, int batch) {
-  if (batch < 1 || batch > c->bmax) throw std::runtime_error("batch out of range");
+static void backward_at(sc_cnn* c, const void* x, const void* dy, float* dx, int batch, int s0) {
+  if (batch < 1 || s0 < 0 || s0 + batch > c->bmax) throw std::runtime_error("batch out of range");
+  const long off = long(s0) * c->n * c->n * c->n * c->width;
   const bf16* dY = (const bf16*)dy;
   // last layer: dW4 (projected when the output is trace-free), then the data
   // gradient into dz[2] with gelu'(z3)
-  launch_wgrad(c, c->width, c->cout_p, c->a[2], dY, c->g + c->woff[3], nullptr, batch);
+  launch_wgrad(c, c->width, c->cout_p, c->a[2] + off, dY, c->g + c->woff[3], nullptr, batch);
   if (c->tracefree) {
     hipLaunchKernelGGL(k_tracefree_rows, dim3(unsigned((27 * c->width + 255) / 256)), dim3(256), 0, c->stream,
                        c->g + c->woff[3], c->cout_p, c->width);
     CNN_HIP(hipGetLastError());
   }
-  launch_conv(c, c->cout_p, c->width, BWD_HIDDEN, dY, c->wb[3], nullptr, c->dz[2], nullptr, nullptr, c->z[2], batch);
+  launch_conv(c, c->cout_p, c->width, BWD_HIDDEN, dY, c->wb[3], nullptr, c->dz[2], nullptr, nullptr,
+              c->z[2] + off, batch);
   for (int l = 2; l >= 0; --l) {
-    const bf16* inl = l == 0 ? (const bf16*)x : c->a[l - 1];
+    const bf16* inl = l == 0 ? (const bf16*)x : c->a[l - 1] + off;
     launch_wgrad(c, layer_cin(c, l), c->width, inl, c->dz[l], c->g + c->woff[l], c->g + c->boff[l], batch);
     if (l > 0)
       launch_conv(c, c->width, c->width, BWD_HIDDEN, c->dz[l], c->wb[l], nullptr, c->dz[l - 1], nullptr, nullptr,
-                  c->z[l - 1], batch);
+                  c->z[l - 1] + off, batch);
     else if (dx)
       launch_conv(c, c->width, c->cin_p, BWD_INPUT, c->dz[0], c->wb[0], nullptr, nullptr, nullptr, dx, nullptr, batch);
   }
 }
+
+void forward(sc_cnn* c, const void* x, float* y, int batch) { forward_at(c, x, y, batch, 0); }
+
+void backward(sc_cnn* c, const void* x, const void* dy, float* dx, int batch) { backward_at(c, x, dy, dx, batch, 0); }
+
+int slots(sc_cnn* c) { return std::min(4, c->bmax); }
+
+void* slot_x(sc_cnn* c, int s) {
+  if (s < 0 || s >= slots(c)) throw std::runtime_error("activation slot out of range");
+  buffers(c);
+  return c->a0 + long(s) * c->n * c->n * c->n * c->cin_p;
+}
+
+void forward_slot(sc_cnn* c, int s, float* y) { forward_at(c, slot_x(c, s), y, 1, s); }
+
+void backward_slot(sc_cnn* c, int s, const void* dy, float* dx) { backward_at(c, slot_x(c, s), dy, dx, 1, s); }
 
 int grid_n(const sc_cnn* c) { return c->n; }
 int in_channels(const sc_cnn* c) { return c->cin; }
@@ -1367,7 +1388,7 @@ int out_channels(const sc_cnn* c) { return c->cout; }
 Bufs buffers(sc_cnn* c) {
   const long nv = long(c->n) * c->n * c->n;
   if (!c->a0) {
-    CNN_HIP(hipMalloc(&c->a0, nv * c->cin_p * 2));
+    CNN_HIP(hipMalloc(&c->a0, nv * c->cin_p * 2 * slots(c)));  // one input per activation slot
     CNN_HIP(hipMalloc(&c->y, nv * c->cout_p * 4));
     CNN_HIP(hipMalloc(&c->dyb, nv * c->cout_p * 2));
     CNN_HIP(hipMalloc(&c->dx, nv * c->cin_p * 4));

@@ -23,6 +23,15 @@ hipStream_t set_stream(sc_cnn* c, hipStream_t s);     // returns the previous st
 void forward(sc_cnn* c, const void* x, float* y, int batch);
 // accumulates the parameter gradients; dx may be null
 void backward(sc_cnn* c, const void* x, const void* dy, float* dx, int batch);
+// Activation slots: one-cube evaluations whose activations stay in sample
+// slot s of the network's batch buffers until that slot is evaluated again,
+// so a backward can follow without recomputing the forward.  slots(c) =
+// min(4, batch_max); slot s's input buffer is slot_x(c, s) (slot 0's is
+// buffers(c).x).
+int slots(sc_cnn* c);
+void* slot_x(sc_cnn* c, int s);
+void forward_slot(sc_cnn* c, int s, float* y);
+void backward_slot(sc_cnn* c, int s, const void* dy, float* dx);
 // x channels 0..2 = scale * v_m (component stride `stride`), channel 3 = extra when cin > 3
 template <typename T> void pack_velocity(sc_cnn* c, const T* v, long stride, double scale, double extra, void* x);
 // tau_c = scale * y[:, c], c < 6, component-major

@@ -71,6 +71,7 @@ struct Ctx {
   // learned closure of les_rhs (closure kind 4): tau = cnn_sout * net(v / cnn_sin, cnn_extra)
   ::sc_cnn* cnn = nullptr;
   double cnn_sin = 1.0, cnn_sout = 1.0, cnn_extra = 0.0;
+  int cnn_slot = 0;  // activation slot the closure's forward writes (the adjoint keeps stages in 1..3)
   // pointwise MLP closure (closure kind 7): tau = Delta^2 |G|^2 mlp(G / |G|, z)
   ::sc_pmlp* pmlp = nullptr;
   double pmlp_delta = 0.0, pmlp_visc = 1.0, pmlp_mu = 0.0, pmlp_sigma = 1.0;

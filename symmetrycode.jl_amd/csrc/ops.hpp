@@ -42,6 +42,7 @@ struct Ops {
   static void lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double b, int ncomp);
   // closure < 0: the DNS right-hand side; 0 or 4: les_rhs with that closure
   static void rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int closure = -1);
+  static void rhs_vjp_kept(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int closure, int kept);
   static void rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu, int closure = -1);
   static void wray3_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, double nu, int closure = -1);
   static void stage_rhs(Ctx& c, CT* du, const CT* u, double nu, int closure);

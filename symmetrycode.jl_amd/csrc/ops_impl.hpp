@@ -1291,8 +1291,9 @@ void Ops<T>::cnn_tau(Ctx& c, T* tau, const CT* u) {
   c2r(c, v, u, c.D, 1.0, true);
   OnStream on(c.cnn, c.stream);
   const cnnlink::Bufs b = cnnlink::buffers(c.cnn);
-  cnnlink::pack_velocity<T>(c.cnn, v, c.phys_elems, 1.0 / c.cnn_sin, c.cnn_extra, b.x);
-  cnnlink::forward(c.cnn, b.x, b.y, 1);
+  void* x = cnnlink::slot_x(c.cnn, c.cnn_slot);
+  cnnlink::pack_velocity<T>(c.cnn, v, c.phys_elems, 1.0 / c.cnn_sin, c.cnn_extra, x);
+  cnnlink::forward_slot(c.cnn, c.cnn_slot, b.y);
   cnnlink::unpack_tau<T>(c.cnn, b.y, c.cnn_sout, tau, c.phys_elems);
 }
 
@@ -1442,6 +1443,14 @@ void Ops<T>::lincomb(Ctx& c, CT* dst, const CT* x, double a, const CT* y, double
 //   out = 2/3 R2C(a)/n^D - nu (|k|^2 w + k (k.w)) + forcing transpose.
 template <typename T>
 void Ops<T>::rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int closure) {
+  rhs_vjp_kept(c, out, u, w, nu, closure, -1);
+}
+
+// rhs_vjp; with kept >= 0 the network's forward on u is already in activation
+// slot `kept` (cnn_tau on u with cnn_slot = kept, nothing evaluated in that
+// slot since), and the backward starts from it
+template <typename T>
+void Ops<T>::rhs_vjp_kept(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int closure, int kept) {
   const int D = c.D, ng = D * D;
   const long np = c.phys_elems;
   if (closure > 0 && closure != 4) throw Error(1, "the transpose exists for closure kinds 0 and 4");
@@ -1461,10 +1470,13 @@ void Ops<T>::rhs_vjp(Ctx& c, CT* out, const CT* u, const CT* w, double nu, int c
   if (closure == 4) {
     OnStream on(c.cnn, c.stream);
     cb = cnnlink::buffers(c.cnn);
-    cnnlink::pack_velocity<T>(c.cnn, v, np, 1.0 / c.cnn_sin, c.cnn_extra, cb.x);
-    cnnlink::forward(c.cnn, cb.x, cb.y, 1);
+    const int s = kept >= 0 ? kept : 0;
+    if (kept < 0) {
+      cnnlink::pack_velocity<T>(c.cnn, v, np, 1.0 / c.cnn_sin, c.cnn_extra, cnnlink::slot_x(c.cnn, 0));
+      cnnlink::forward_slot(c.cnn, 0, cb.y);
+    }
     cnnlink::pack_tau_cotangent<T>(c.cnn, gr, np, c.cnn_sout / nD, cb.dy);
-    cnnlink::backward(c.cnn, cb.x, cb.dy, cb.dx, 1);
+    cnnlink::backward_slot(c.cnn, s, cb.dy, cb.dx);
   }
   if (D == 3) hipLaunchKernelGGL((k_adj_products<T, 3>), dim3(nblk(np)), dim3(256), 0, c.stream, v, gr, np);
   else hipLaunchKernelGGL((k_adj_products<T, 2>), dim3(nblk(np)), dim3(256), 0, c.stream, v, gr, np);
@@ -1507,14 +1519,27 @@ void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, d
   CT* KB = reinterpret_cast<CT*>(c.adj(5, fb));
   CT* Tt = reinterpret_cast<CT*>(c.adj(6, fb));
   CT* acc = reinterpret_cast<CT*>(c.adj(7, fb));
-  // forward recompute of the stage values
-  stage_rhs(c, k, u0, nu, closure);
+  // forward recompute of the stage values.  With the learned closure and
+  // four activation slots, the network's forwards on u0, U1, U2 stay in
+  // slots 1, 2, 3 and the transposes of those stages start from them
+  const bool keep = closure == 4 && c.cnn && cnnlink::slots(c.cnn) >= 4;
+  auto stage = [&](CT* kk, const CT* uu, int slot) {
+    if (keep) c.cnn_slot = slot;
+    try {
+      stage_rhs(c, kk, uu, nu, closure);
+    } catch (...) {
+      c.cnn_slot = 0;
+      throw;
+    }
+    c.cnn_slot = 0;
+  };
+  stage(k, u0, 1);
   lincomb(c, U1, u0, 1.0, k, dt / 2, D);
   project(c, U1);
-  stage_rhs(c, k, U1, nu, closure);
+  stage(k, U1, 2);
   lincomb(c, U2, u0, 1.0, k, dt / 2, D);
   project(c, U2);
-  stage_rhs(c, k, U2, nu, closure);
+  stage(k, U2, 3);
   lincomb(c, U3, u0, 1.0, k, dt, D);
   project(c, U3);
   // reverse sweep
@@ -1526,15 +1551,15 @@ void Ops<T>::rk4_vjp(Ctx& c, CT* out, const CT* u0, const CT* ubar, double dt, d
   project(c, Tt);
   axpy(c, acc, Tt, 1.0, D);
   lincomb(c, KB, Sb, dt / 3, Tt, dt, D);
-  rhs_vjp(c, Tt, U2, KB, nu, closure);
+  rhs_vjp_kept(c, Tt, U2, KB, nu, closure, keep ? 3 : -1);
   project(c, Tt);
   axpy(c, acc, Tt, 1.0, D);
   lincomb(c, KB, Sb, dt / 3, Tt, dt / 2, D);
-  rhs_vjp(c, Tt, U1, KB, nu, closure);
+  rhs_vjp_kept(c, Tt, U1, KB, nu, closure, keep ? 2 : -1);
   project(c, Tt);
   axpy(c, acc, Tt, 1.0, D);
   lincomb(c, KB, Sb, dt / 6, Tt, dt / 2, D);
-  rhs_vjp(c, Tt, u0, KB, nu, closure);
+  rhs_vjp_kept(c, Tt, u0, KB, nu, closure, keep ? 1 : -1);
   lincomb(c, out, acc, 1.0, Tt, 1.0, D);
 }
 

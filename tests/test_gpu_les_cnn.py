@@ -173,3 +173,31 @@ def test_les_rk4_vjp_closure4_tracefree(env):
     ref0 = A.rk4_vjp(u, ub, dt, go, nu)
     assert share_err(gg.download_spectral(out), ref, ref0) < 3e-2
     check_grads(tnet, og)
+
+
+def test_rk4_vjp_kept_activations_equal_recompute(env):
+    """With batch_max >= 4 the RK4 transpose keeps the network activations of
+    its stage recomputes (slots 1-3) and starts three of the four stage
+    transposes from them instead of evaluating the network again: the
+    velocity cotangent and the weight gradients are the same bits as on the
+    recomputing path (batch_max 1); the bias gradients (block sums added with
+    atomics) agree to 1e-6."""
+    sc, go, gg, net, onet, model, u = env
+    print_case("rk4 vjp kept activations", N)
+    net4 = sc.cnn.ConvClosureNet(N, batch_max=4, cin=4, width=64, cout=6, seed=3)
+    net4.set_params(net.get_params())
+    model4 = sc.create_learned_closure(net4, gg, S_IN, S_OUT, EXTRA)
+    ud = gg.upload_spectral(u)
+    wb = gg.upload_spectral(oracle_solenoidal(go, 22, trunc=True))
+    res = []
+    for nn, mm in ((net, model), (net4, model4)):
+        nn.zero_grad()
+        out = sc.les_rk4_vjp(ud, wb, 2e-3, gg, mm, 3e-3)
+        res.append((gg.download_spectral(out), grads_of(nn)))
+    model.attach()  # the module's closure stays attached for later tests
+    (o1, (w1, b1)), (o4, (w4, b4)) = res
+    assert np.array_equal(o1, o4)
+    for l in range(4):
+        assert np.array_equal(w1[l], w4[l]), l
+    for l in range(3):
+        assert rel(b4[l], b1[l]) < 1e-6, l
