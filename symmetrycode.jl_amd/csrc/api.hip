@@ -71,6 +71,12 @@ void need(bool ok, const char* msg) {
     __VA_ARGS__;                                         \
   })
 
+// elements between the spectral row pitch nhp and n / 2 + 1 rounded up to
+// 128 bytes
+#ifndef SC_NHP_PAD
+#define SC_NHP_PAD 0
+#endif
+
 extern "C" {
 
 int sc_create(int D, int n, double l, int precision, int device, sc_ctx** out) {
@@ -84,7 +90,7 @@ int sc_create(int D, int n, double l, int precision, int device, sc_ctx** out) {
   c->D = D; c->n = n; c->l = l; c->prec = precision; c->device = device;
   c->nh = n / 2 + 1;
   const int align = 128 / (precision == 64 ? 16 : 8);
-  c->nhp = (c->nh + align - 1) / align * align;
+  c->nhp = (c->nh + align - 1) / align * align + SC_NHP_PAD;
   c->nz = D == 3 ? n : 1;
   c->spec_elems = long(c->nhp) * n * c->nz;
   c->phys_elems = long(n) * n * c->nz;

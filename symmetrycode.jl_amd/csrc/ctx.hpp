@@ -1,6 +1,7 @@
 // Library context: one grid (D, n, l), one precision, one device, one stream.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 #include <stdexcept>
@@ -13,6 +14,14 @@
 // allocations by the rates of the passes that write them
 #ifndef SC_TUNE_TILED
 #define SC_TUNE_TILED 1
+#endif
+// place the pipeline buffers of a large grid in scattered 1 GiB pieces of
+// device memory (Ctx::scattered_allocs)
+#ifndef SC_SCATTER
+#define SC_SCATTER 1
+#endif
+#ifndef SC_SCATTER_PIECE_MB
+#define SC_SCATTER_PIECE_MB 256
 #endif
 
 struct sc_cnn;   // learned closure (cnn.hip)
@@ -37,6 +46,9 @@ struct Ctx {
   int ncu = 256;  // compute units of the device (persistent grids)
   double l = 0;
   long spec_elems = 0, phys_elems = 0;  // per component
+  // per component of the buffers that hold either a spectral field or the
+  // z-pencil-tiled state of the multi-step path (>= spec_elems)
+  long state_elems = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool sync_calls = true;
@@ -49,6 +61,7 @@ struct Ctx {
   bool tiled_tuned = false;  // bufT chosen among candidate allocations (choose_placement)
   bool inv_tuned = false;    // bufA likewise
   bool state_tuned = false;  // ust, acc chosen among candidate allocations (tune_state)
+  bool scattered = false;    // the pipeline buffers were placed by scattered_allocs
   void* scr = nullptr;    // general spectral scratch
   int scr_comp = 0;
   void* ustart = nullptr; // RK storage (D comps each)
@@ -119,6 +132,50 @@ struct Ctx {
     *p = static_cast<char*>(base) + off;
     blocks.push_back({*p, base});
   }
+  // Run allocs(), which allocates about `need` bytes, while every other
+  // piece (SC_SCATTER_PIECE_MB) of a ballast spread over the device's free
+  // memory is held,
+  // so that those allocations are made of separate pieces.  The streaming
+  // passes of a large grid run slower when their buffers sit in long
+  // physically contiguous ranges, which is what a device whose memory has not
+  // been allocated before hands out (measured at 810^3 FP64: the forward
+  // y-pass at 11.3-11.4 instead of 10.1-10.4 ms per launch, as with
+  // hipDeviceMallocContiguous); processes that start on memory earlier
+  // processes have used get separated pieces and the fast rates.  With
+  // 256 MiB pieces the first process on a fresh device ran every pass at its
+  // fast rate (1 GiB pieces left the fused z-stage 3 % slow).  The
+  // ballast B is bounded so that allocs() still finds need + 8 GiB free
+  // (free memory during allocs(): F - B / 2).  Everything held is freed
+  // before returning.
+  template <class F> void scattered_allocs(size_t need, F&& allocs) {
+    std::vector<void*> keep;
+    const size_t piece = size_t(SC_SCATTER_PIECE_MB) << 20, margin = size_t(8) << 30;
+    size_t freeb = 0, totalb = 0;
+    SC_HIP(hipMemGetInfo(&freeb, &totalb));
+    size_t ballast = 0;
+    if (freeb > need + margin) ballast = std::min(freeb - margin, 2 * (freeb - need - margin));
+    try {
+      for (size_t held = 0; held + piece <= ballast; held += piece) {
+        void* p = nullptr;
+        if (hipMalloc(&p, piece) != hipSuccess) {
+          (void)hipGetLastError();
+          break;
+        }
+        keep.push_back(p);
+      }
+      for (size_t i = 1; i < keep.size(); i += 2) {
+        SC_HIP(hipFree(keep[i]));
+        keep[i] = nullptr;
+      }
+      allocs();
+    } catch (...) {
+      for (void* p : keep)
+        if (p) (void)hipFree(p);
+      throw;
+    }
+    for (void* p : keep)
+      if (p) SC_HIP(hipFree(p));
+  }
   void dfree(void* p) {
     if (!p) return;
     for (size_t i = 0; i < blocks.size(); ++i)
@@ -139,10 +196,13 @@ struct Ctx {
     g.kf = 3.14159265358979323846 / l * 2;
     return g;
   }
+  long state_or_spec() const { return state_elems > spec_elems ? state_elems : spec_elems; }
+  // spectral buffers; sized for the tiled state as well, since the state
+  // buffers (du, ust, acc) hold either layout
   void* ensure_spec(void*& p, int ncomp) {
     if (!p) {
-      dmalloc(&p, size_t(ncomp) * spec_elems * celem());
-      SC_HIP(hipMemsetAsync(p, 0, size_t(ncomp) * spec_elems * celem(), stream));
+      dmalloc(&p, size_t(ncomp) * state_or_spec() * celem());
+      SC_HIP(hipMemsetAsync(p, 0, size_t(ncomp) * state_or_spec() * celem(), stream));
     }
     return p;
   }

@@ -55,6 +55,12 @@ template <typename T, int N> constexpr int tile_x() {
   while (tx > 1 && tx * N * int(sizeof(C<T>)) > 65536) tx /= 2;
   return tx;
 }
+// elements from one line of the tiled layouts (the n pencil positions of a
+// tile of TX x-values) to the next: n TX plus SC_TILE_PAD elements
+#ifndef SC_TILE_PAD
+#define SC_TILE_PAD 0
+#endif
+template <typename T, int N> constexpr long tile_line() { return long(N) * tile_x<T, N>() + SC_TILE_PAD; }
 
 // ------------------------------------------------------------------
 // Strided complex pass along y or z.  Each block transforms TX consecutive-x

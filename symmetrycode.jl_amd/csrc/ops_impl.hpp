@@ -277,9 +277,13 @@ static StridedGeom full_geom(const Ctx& c, int axis) {
 // component stride of the tiled buffers between strided passes (x tiles of
 // TX, each holding n x n pencil positions)
 template <typename T>
+static long tile_line_rt(const Ctx& c) {
+  return long(c.n) * tile_x_rt<T>(c.n) + SC_TILE_PAD;
+}
+template <typename T>
 static long tiled_slot(const Ctx& c) {
   const int TX = tile_x_rt<T>(c.n);
-  return long((c.nh + TX - 1) / TX) * TX * c.n * c.n;
+  return long((c.nh + TX - 1) / TX) * c.n * tile_line_rt<T>(c);
 }
 
 template <typename T>
@@ -334,6 +338,7 @@ static void fwd_rhs_last(Ctx& c, const C<T>* buf, const C<T>* u, C<T>* du, RhsPa
 template <typename T>
 void Ops<T>::init_tables(Ctx& c) {
   const int n = c.n, M = n / 2;
+  c.state_elems = c.D == 3 ? std::max(c.spec_elems, tiled_slot<T>(c)) : c.spec_elems;
   std::vector<C<T>> hn(n), hm(M);
   for (int k = 0; k < n; ++k) {
     long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
@@ -503,6 +508,22 @@ struct TuneScope {
 // its candidate, so the kept buffer holds the launch's output, and the
 // results are the same whichever is kept.  Returns false, having launched
 // nothing, when the buffer is small or no candidate fits.
+// The first allocation of a large 3D grid's pipeline buffers (one spectral
+// component of 512 MiB or more) goes through Ctx::scattered_allocs; allocs()
+// allocates whatever of those buffers does not exist yet.
+template <typename T, class F>
+static void scatter_first(Ctx& c, F&& allocs) {
+  const size_t comp = size_t(c.state_or_spec()) * c.celem();
+  if (!SC_SCATTER || c.scattered || c.D != 3 || comp < (size_t(512) << 20)) {
+    allocs();
+    return;
+  }
+  c.scattered = true;
+  // bufA (3), bufB (6), bufT (6 tiled slots), du, ust, acc (3 each)
+  const size_t need = size_t(24) * comp;
+  c.scattered_allocs(need, allocs);
+}
+
 template <class Launch>
 static bool choose_placement(Ctx& c, void*& buf, size_t bytes, int max_extra, Launch launch) {
   if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return false;
@@ -552,6 +573,12 @@ static bool choose_placement(Ctx& c, void*& buf, size_t bytes, int max_extra, La
 // (whose coefficient is left in c.forceA).
 template <typename T>
 void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long long* vmax, bool lin) {
+  if (c.D == 3)
+    scatter_first<T>(c, [&] {
+      c.ensure_spec(c.bufA, c.D);
+      c.ensure_spec(c.bufB, c.tdim());
+      if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT));
+    });
   CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, c.D));
   CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, c.tdim()));
   const int kc = c.n / 3;
@@ -563,18 +590,18 @@ void Ops<T>::rhs_fused(Ctx& c, CT* du, const CT* u, double nu, unsigned long lon
     // stay within few address-translation pages
     if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * tiled_slot<T>(c) * sizeof(CT));
     CT* AT = reinterpret_cast<CT*>(c.bufT);
-    const long n = c.n, TX = tile_x_rt<T>(c.n), ts = tiled_slot<T>(c);
+    const long n = c.n, TX = tile_x_rt<T>(c.n), ts = tiled_slot<T>(c), LP = tile_line_rt<T>(c);
     gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
-    gz.o_tile = n * n * TX;
-    gz.o_axis = n * TX;
+    gz.o_tile = n * LP;
+    gz.o_axis = LP;
     gz.o_other = TX;
     gz.o_comp = ts;
     gz.pair = 1;
     c2c_launch<T>(c, gz, 2, +1, u, AT, 3);
     StridedGeom gy = axis_geom<T>(c, 1, kc + 1, false, kc, false, 1.0);
-    gy.s_tile = n * n * TX;
+    gy.s_tile = n * LP;
     gy.s_axis = TX;
-    gy.s_other = n * TX;
+    gy.s_other = LP;
     gy.s_comp = ts;
     auto inv_y = [&](void* dst) { c2c_launch<T>(c, gy, 1, +1, AT, reinterpret_cast<CT*>(dst), 3); };
     bool done = false;
@@ -680,7 +707,7 @@ double Ops<T>::wray3_dev(Ctx& c, CT* u, double dt, double nu, double cfl, int le
 template <typename T>
 static void tune_state(Ctx& c, const C<T>* u, const C<T>* du) {
   c.state_tuned = true;
-  const size_t bytes = size_t(c.D) * c.spec_elems * c.celem();
+  const size_t bytes = size_t(c.D) * c.state_or_spec() * c.celem();
   if (bytes < (size_t(1) << 30) || !SC_TUNE_TILED) return;
   size_t freeb = 0, totalb = 0;
   SC_HIP(hipMemGetInfo(&freeb, &totalb));
@@ -783,10 +810,10 @@ static void ztile_convert(Ctx& c, const C<T>* src, C<T>* dst, int ncomp, bool to
   if (TX == TXV) {                                                                                            \
     if (to_tiles)                                                                                             \
       hipLaunchKernelGGL((k_ztile_convert<T, TXV, true>), dim3(nblocks, unsigned(ncomp)), dim3(256), 0, c.stream, \
-                         src, dst, n, c.nhp, c.spec_elems, ts);                                               \
+                         src, dst, n, c.nhp, c.spec_elems, ts, tile_line_rt<T>(c));                          \
     else                                                                                                      \
       hipLaunchKernelGGL((k_ztile_convert<T, TXV, false>), dim3(nblocks, unsigned(ncomp)), dim3(256), 0,       \
-                         c.stream, src, dst, n, c.nhp, c.spec_elems, ts);                                     \
+                         c.stream, src, dst, n, c.nhp, c.spec_elems, ts, tile_line_rt<T>(c));                \
   }
   SC_CONV(2) SC_CONV(4) SC_CONV(8) SC_CONV(16)
 #undef SC_CONV
@@ -838,13 +865,21 @@ void Ops<T>::rk4_steps(Ctx& c, CT* u, int nsteps, double cfl, double nu, double 
     return;
   }
   const int n = c.n, kc = n / 3;
-  const long TX = tile_x_rt<T>(n), ts = tiled_slot<T>(c);
-  CT* A = reinterpret_cast<CT*>(c.ensure_spec(c.bufA, 3));
-  CT* B = reinterpret_cast<CT*>(c.ensure_spec(c.bufB, 6));
-  if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * ts * sizeof(CT));
-  CT* U = reinterpret_cast<CT*>(c.ensure_spec(c.du, 3));   // u0, tiled
-  CT* US = reinterpret_cast<CT*>(c.ensure_spec(c.ust, 3)); // stage value, tiled
-  CT* AC = reinterpret_cast<CT*>(c.ensure_spec(c.acc, 3)); // accumulator, tiled
+  const long TX = tile_x_rt<T>(n), ts = tiled_slot<T>(c), LP = tile_line_rt<T>(c);
+  auto allocs = [&] {
+    c.ensure_spec(c.bufA, 3);
+    c.ensure_spec(c.bufB, 6);
+    if (!c.bufT) c.dmalloc(&c.bufT, size_t(6) * ts * sizeof(CT));
+    c.ensure_spec(c.du, 3);
+    c.ensure_spec(c.ust, 3);
+    c.ensure_spec(c.acc, 3);
+  };
+  scatter_first<T>(c, allocs);
+  CT* A = reinterpret_cast<CT*>(c.bufA);
+  CT* B = reinterpret_cast<CT*>(c.bufB);
+  CT* U = reinterpret_cast<CT*>(c.du);   // u0, tiled
+  CT* US = reinterpret_cast<CT*>(c.ust); // stage value, tiled
+  CT* AC = reinterpret_cast<CT*>(c.acc); // accumulator, tiled
   double* dtp = dt_slot<T>(c);
   // the placement choice of the pipeline's buffers happens at the first RHS
   // (its output goes to the stage buffer, which stage 0 overwrites)
@@ -861,19 +896,19 @@ void Ops<T>::rk4_steps(Ctx& c, CT* u, int nsteps, double cfl, double nu, double 
   double* dlog = c.dtlog;
   ztile_convert<T>(c, u, U, 3, true);
   StridedGeom gz = axis_geom<T>(c, 2, kc + 1, true, kc, true, 1.0);
-  gz.s_tile = n * n * TX;
+  gz.s_tile = n * LP;
   gz.s_axis = TX;
-  gz.s_other = n * TX;
+  gz.s_other = LP;
   gz.s_comp = ts;
-  gz.o_tile = n * n * TX;
-  gz.o_axis = n * TX;
+  gz.o_tile = n * LP;
+  gz.o_axis = LP;
   gz.o_other = TX;
   gz.o_comp = ts;
   gz.pair = 1;
   StridedGeom gy = axis_geom<T>(c, 1, kc + 1, false, kc, false, 1.0);
-  gy.s_tile = n * n * TX;
+  gy.s_tile = n * LP;
   gy.s_axis = TX;
-  gy.s_other = n * TX;
+  gy.s_other = LP;
   gy.s_comp = ts;
   SpecGeom gt = c.geom();
   gt.s_comp = ts;
@@ -901,7 +936,7 @@ void Ops<T>::rk4_steps(Ctx& c, CT* u, int nsteps, double cfl, double nu, double 
       if (c.force_kind == 1) {
         c.prof_begin("band_coeff");
         hipLaunchKernelGGL(k_band_coeff<T>, dim3(1), dim3(256), 0, c.stream, sin, 3, gt, c.kband, c.injection,
-                           c.forceA, int(TX));
+                           c.forceA, int(TX), LP);
         c.prof_end();
         SC_HIP(hipGetLastError());
       }

@@ -72,7 +72,8 @@ k_y_fwd_div(const C<T>* buf, C<T>* outT, long s_slot, const C<T>* __restrict__ t
   const int x = tile * TX + xl;
   const bool xok = x < g.xs;
   const C<T>* base = buf + long(oi) * g.s_other + x;
-  C<T>* tbase = outT + (long(tile) * N * N + oi) * TX + xl;  // (tile, y = 0, z = oi)
+  constexpr long LP = tile_line<T, N>();
+  C<T>* tbase = outT + long(tile) * N * LP + long(oi) * TX + xl;  // (tile, y = 0, z = oi)
   const T kx = T(kf) * T(x);
   C<T> v[PE::MAXE];
   C<T> nx[PE::E0];
@@ -91,7 +92,7 @@ k_y_fwd_div(const C<T>* buf, C<T>* outT, long s_slot, const C<T>* __restrict__ t
   auto put = [&](int slot, const C<T>* val) {
     if (!xok) return;
     C<T>* dst = tbase + long(slot) * s_slot;
-    PE::for_out(p, [&](int e, int k) { dst[long(k) * (N * TX)] = val[e]; });
+    PE::for_out(p, [&](int e, int k) { dst[long(k) * LP] = val[e]; });
   };
   auto kyv = [&](int k) { return T(kf) * T(freq(k, N)); };
   fetch(0);
@@ -155,7 +156,7 @@ k_z_fwd_rhs(const C<T>* bufT, long s_slot, const C<T>* __restrict__ u, C<T>* du,
   if (rp.forceA) fA = T(*rp.forceA);
   C<T> v[PE::MAXE];
   C<T> nx[PE::E0];
-  const C<T>* tsrc = bufT + (long(tile) * N + oi) * N * TX + xl;  // (tile, y = oi, z = 0)
+  const C<T>* tsrc = bufT + (long(tile) * N + oi) * tile_line<T, N>() + xl;  // (tile, y = oi, z = 0)
   auto fetch = [&](int slot) {
     const C<T>* src = tsrc + long(slot) * s_slot;
     PE::load(nx, p, [&](int j) { return xok ? src[j * TX] : mk<T>(T(0), T(0)); });
@@ -246,7 +247,7 @@ k_z_fwd_rk4(const C<T>* bufT, long s_slot, C<T>* u0, C<T>* ust, C<T>* acc, const
   const int xl = t % TX, p = t / TX;
   const int x = tile * TX + xl;
   const bool xok = x < g.xs;
-  const long toff = (long(tile) * N + oi) * N * TX + xl;  // (tile, y = oi, z = 0) of a slot or component
+  const long toff = (long(tile) * N + oi) * tile_line<T, N>() + xl;  // (tile, y = oi, z = 0) of a slot or component
   const T kf = T(rp.kf);
   const int kyi = freq(oi, g.n_other);
   const T kx = kf * T(x), ky = kf * T(kyi);
@@ -338,13 +339,14 @@ k_z_fwd_rk4(const C<T>* bufT, long s_slot, C<T>* u0, C<T>* ust, C<T>* acc, const
 
 // Layout conversion of ncomp spectral components between the x-contiguous
 // layout (element (x, y, z) at (z n + y) nhp + x, component stride s_spec)
-// and z-pencil tiles (((x/TX n + y) n + z) TX + x%TX, component stride ts).
+// and z-pencil tiles ((x/TX n + y) lp + z TX + x%TX, lp = n TX + padding,
+// component stride ts).
 // A block moves a tile of TX x by 8 y by 32 z through LDS, so the spectral
 // side is accessed in runs of TX elements and the tiled side in runs of
 // 32 TX.  Grid: (ntile * ceil(n/8) * ceil(n/32), ncomp).
 template <typename T, int TX, bool TO_TILES>
 __global__ void __launch_bounds__(256) k_ztile_convert(const C<T>* __restrict__ src, C<T>* dst, int n, int nhp,
-                                                       long s_spec, long ts) {
+                                                       long s_spec, long ts, long lp) {
   constexpr int YB = 8, ZB = 32;
   constexpr int NE = YB * ZB * TX;
   __shared__ C<T> buf[YB][ZB][TX + 1];
@@ -368,7 +370,7 @@ __global__ void __launch_bounds__(256) k_ztile_convert(const C<T>* __restrict__ 
     const int xl = e % TX;
     zz = (e / TX) % ZB;
     yy = e / (TX * ZB);
-    off = ((long(tile) * n + (yb * YB + yy)) * n + (zb * ZB + zz)) * TX + xl;
+    off = (long(tile) * n + (yb * YB + yy)) * lp + long(zb * ZB + zz) * TX + xl;
   };
   for (int e = threadIdx.x; e < NE; e += 256) {
     int yy, zz;

@@ -666,7 +666,7 @@ __global__ void k_shell_scale(C<T>* u, int ncomp, SpecGeom g, int nshell, const 
 // one block over the (small) band.
 template <typename T>
 __global__ void k_band_coeff(const C<T>* u, int ncomp, SpecGeom g, int kband, double injection, double* A,
-                             int tx = 0) {
+                             int tx = 0, long lp = 0) {
   __shared__ double sm[256];
   double acc = 0.0;
   const int kb2 = kband * kband;
@@ -681,8 +681,9 @@ __global__ void k_band_coeff(const C<T>* u, int ncomp, SpecGeom g, int kband, do
     if (k2 == 0 || k2 > kb2) continue;
     int y = ky < 0 ? ky + g.n : ky;
     int z = kz < 0 ? kz + g.n : kz;
-    // tx > 0: u in z-pencil tiles of tx x-values (component stride g.s_comp)
-    long off = tx > 0 ? ((long(kx / tx) * g.n + y) * g.n + z) * tx + kx % tx : kx + long(g.nhp) * (y + long(g.n) * z);
+    // tx > 0: u in z-pencil tiles of tx x-values, lines lp elements apart
+    // (component stride g.s_comp)
+    long off = tx > 0 ? (long(kx / tx) * g.n + y) * lp + long(z) * tx + kx % tx : kx + long(g.nhp) * (y + long(g.n) * z);
     double w = (kx >= 1 && kx <= g.nh - 2) ? 2.0 : 1.0;
     for (int c = 0; c < ncomp; ++c) {
       C<T> v = u[c * g.s_comp + off];
