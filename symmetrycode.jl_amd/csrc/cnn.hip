@@ -140,10 +140,18 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   // half the weight-fragment traffic through L1 (8 wavefronts of a block
   // request the same fragments), twice the A reads from LDS
   constexpr bool SPLIT = SC_CNN_SPLIT && COUTT == 64 && RPW == 1;
+  // PAIR (the last layer with at most 8 output channels): the 16 rows of an
+  // output tile hold 8 channels of two consecutive output rows; a k step
+  // pairs one staged input row with the taps dy (rows 0-7) and dy - 1 (rows
+  // 8-15) that map it onto those two rows, so one plane takes 4 input rows x
+  // 3 dx x 2 channel halves = 24 k steps for two rows (36 for two rows with
+  // 8 of 16 MFMA rows used).  Wavefront w takes rows 2 (w / 2), + 1 and the
+  // x half w % 2 of the block's 64 voxels (two M tiles)
+  constexpr bool PAIR = MODE == LAST && COUTS == 8 && CIN == 64 && RPW == 1;
   constexpr int NT = SPLIT ? 2 : COUTT / 16;   // output tiles per wavefront
   constexpr int NTA = COUTT / 16;              // output tiles in the weight layout
   constexpr int CH8 = CIN / 8;
-  constexpr int MT = SPLIT ? 8 : 4 * RPW;      // M tiles of 16 voxels: 4 per row
+  constexpr int MT = SPLIT ? 8 : PAIR ? 2 : 4 * RPW;  // M tiles of 16 voxels: 4 per row
   constexpr int NPH = BYPLANE ? 3 : 1;
   constexpr int NCH = NPL * RW * XW * CH8;
   constexpr int NPT = (NCH + CONV_THREADS - 1) / CONV_THREADS;
@@ -151,8 +159,9 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   const int segs = n / VPB, ytiles = n / TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
-  const int wr = SPLIT ? 2 * (w / 2) : w * RPW;  // the wavefront's first row
-  const int wn = SPLIT ? 2 * (w % 2) : 0;        // and first output tile
+  const int wr = (SPLIT || PAIR) ? 2 * (w / 2) : w * RPW;  // the wavefront's first row
+  const int wn = SPLIT ? 2 * (w % 2) : 0;                  // and first output tile
+  const int wx = PAIR ? 32 * (w % 2) : 0;                  // and first voxel of its x range
   struct Tile { long b; int z, y0, x0; };
   auto decode = [&](long t) {
     Tile T;
@@ -176,6 +185,14 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   const bf16* lane_a = &win[lk * XWP * 8];
   auto mt_off = [](int mt) { return ((mt / 4) * CH8 * XWP + 16 * (mt % 4)) * 8; };
   auto load_a = [&](int kc, bf16x8* a) {
+    if constexpr (PAIR) {  // k step within the plane: (staged row offset ro, dx, channel half)
+      const int ks = kc % 24, ro = ks / 6, dx = (ks / 2) % 3 - 1, cb = (ks % 2) * 32;
+      const int c8 = cb / 8 + lk;
+      const bf16* base = lane_a + (((wr + ro) * CH8 + cb / 8) * XWP + ((1 + dx + li) ^ (c8 & 7)) + wx) * 8;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(base + mt_off(mt));
+      return;
+    }
     if constexpr (CIN >= 32) {  // one tap per k step: the row offsets are uniform
       const int t = (kc * 32) / CIN, cb = (kc * 32) % CIN;
       const int dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
@@ -241,14 +258,14 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
     if (SC_CNN_PREFETCH && ph + 1 < NPH) {
       CONV_FETCH(cur, ph + 1)
     }
-    const int kc0 = BYPLANE ? ph * (NKC / 3) : 0;
+    constexpr int NS = PAIR ? 24 : BYPLANE ? NKC / 3 : NKC;  // k steps of the phase
+    const int kc0 = ph * NS;
     // software pipeline over the phase's k steps, fully unrolled (the step
     // count is a compile-time constant, so the code is straight-line): B
     // fragments are requested four steps ahead, A fragments one step ahead.
     // With the loads under run-time conditions the compiler either waits
     // with vmcnt(0) (draining the fragments requested for later steps too)
     // or sinks a load into the block that uses it
-    constexpr int NS = BYPLANE ? NKC / 3 : NKC;
     bf16x8 af[2][MT], bf[4][NT];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -280,10 +297,12 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   // D[co = 16 nt + 4 lk + r][voxel = 16 mt + li]: four consecutive channels of
   // one voxel, stored (and, for the data gradient, the pre-activation read)
   // as one 8- or 16-byte access
-  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + wr) * unsigned(n) + cur.x0 + li;
+  // (PAIR: rows 0-7 of the tile are output row wr, rows 8-15 row wr + 1)
+  const unsigned vrow0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + wr + (PAIR ? lk / 2 : 0)) * unsigned(n) +
+                         cur.x0 + wx + li;
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
-    const int co0 = (wn + nt) * 16 + 4 * lk;
+    const int co0 = PAIR ? 4 * (lk % 2) : (wn + nt) * 16 + 4 * lk;
     if (co0 >= COUTS) continue;
     float bco[4] = {0.f, 0.f, 0.f, 0.f};
     if constexpr (MODE == HIDDEN) {
@@ -293,7 +312,7 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       // voxel and element offsets fit in 32 bits (at most 2^31 elements per tensor)
-      const unsigned v = vrow0 + (mt / 4) * unsigned(n) + 16 * (mt % 4);
+      const unsigned v = vrow0 + (PAIR ? 0u : (mt / 4) * unsigned(n)) + 16 * (mt % 4);
       const floatx4 d = acc[mt][nt];
       if constexpr (MODE == HIDDEN) {
         bf16x4 zz, aa;
@@ -1088,12 +1107,25 @@ __device__ __forceinline__ long frag_index(int row, int k, int nrows) {
   const int kc = k >> 5, j = (k >> 3) & 3, e = k & 7, nt = row >> 4, i = row & 15;
   return ((long(kc) * (nrows / 16) + nt) * 64 + 16 * j + i) * 8 + e;
 }
-__global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cop, int cip, int tracefree) {
+// With paired (the last layer, cop = 8, cip = 64; see PAIR in k_conv) the
+// forward fragments take the paired layout: k step kc = 24 pz + (ro 3 + dxi)
+// 2 + ch (plane pz = dz + 1, staged row offset ro, dxi = dx + 1, channel half
+// ch), row r = 8 h + co holding the tap (dz, dy = ro - 1 - h, dx) of output
+// channel co, zero where dy leaves [-1, 1].
+__device__ __forceinline__ int pair_kpad() { return 72 * 32; }
+__global__ void k_pack_weights(const float* w, bf16* wf, bf16* wb, int cop, int cip, int tracefree, int paired) {
   const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int kpf = kpad(cip), kpb = kpad(cop);
-  const int rf = (cop + 15) / 16 * 16, rb = (cip + 15) / 16 * 16;
+  const int kpf = paired ? pair_kpad() : kpad(cip), kpb = kpad(cop);
+  const int rf = paired ? 16 : (cop + 15) / 16 * 16, rb = (cip + 15) / 16 * 16;
   const long nf = long(rf) * kpf, nb = long(rb) * kpb;
-  if (i < nf) {
+  if (i < nf && paired) {
+    const int r = int(i / kpf), k = int(i % kpf);
+    const int kc = k / 32, ci = (kc % 2) * 32 + k % 32;
+    const int pz = kc / 24, ro = (kc % 24) / 6, dxi = (kc / 2) % 3;
+    const int h = r / 8, co = r % 8, dy = ro - 1 - h;
+    const int t = pz * 9 + (dy + 1) * 3 + dxi;
+    wf[frag_index(r, k, rf)] = bf16(dy >= -1 && dy <= 1 && co < cop ? w[(long(t) * cop + co) * cip + ci] : 0.f);
+  } else if (i < nf) {
     const int co = int(i / kpf), k = int(i % kpf);
     const int t = k / cip, ci = k % cip;
     wf[frag_index(co, k, rf)] = bf16(t < 27 && co < cop ? w[(long(t) * cop + co) * cip + ci] : 0.f);
@@ -1244,7 +1276,9 @@ void launch_conv(sc_cnn* c, int cinp, int coutp, int mode, const bf16* in, const
   const long rows = long(batch) * c->n * c->n;
 
   static const char* names[4] = {"conv_fwd", "conv_last", "conv_bwd_data", "conv_bwd_input"};
-  const double flop = 2.0 * double(rows) * c->n * kpad(cinp) * ((coutp + 15) / 16 * 16);
+  const bool pair = mode == LAST && coutp == 8 && cinp == 64;
+  const double flop = pair ? double(rows) * c->n * 16 * 72 * 32  // half the rows, 72 k steps of 32
+                           : 2.0 * double(rows) * c->n * kpad(cinp) * ((coutp + 15) / 16 * 16);
 #define CONV(CI, COT, COS, MO)                                                                             \
   if (cinp == CI && coutp == COS && mode == MO) {                                                         \
     c->pbeg(names[MO], flop);                                                                             \
@@ -1309,14 +1343,20 @@ void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* do
   c->pend();
 }
 
-long wf_len(const sc_cnn* c, int l) { return long((layer_cout(c, l) + 15) / 16 * 16) * kpad(layer_cin(c, l)); }
+// the last layer's forward fragments take the paired layout (k_conv PAIR)
+// when it has at most 8 output channels
+bool paired_last(const sc_cnn* c, int l) { return l == 3 && c->cout_p == 8 && layer_cin(c, l) == 64; }
+long wf_len(const sc_cnn* c, int l) {
+  return paired_last(c, l) ? 16L * 72 * 32 : long((layer_cout(c, l) + 15) / 16 * 16) * kpad(layer_cin(c, l));
+}
 long wb_len(const sc_cnn* c, int l) { return long((layer_cin(c, l) + 15) / 16 * 16) * kpad(layer_cout(c, l)); }
 
 void pack_all(sc_cnn* c) {
   for (int l = 0; l < 4; ++l) {
     const long len = wf_len(c, l) + wb_len(c, l);
     hipLaunchKernelGGL(k_pack_weights, dim3(unsigned((len + 255) / 256)), dim3(256), 0, c->stream, c->w + c->woff[l],
-                       c->wf[l], c->wb[l], layer_cout(c, l), layer_cin(c, l), (l == 3 && c->tracefree) ? 1 : 0);
+                       c->wf[l], c->wb[l], layer_cout(c, l), layer_cin(c, l), (l == 3 && c->tracefree) ? 1 : 0,
+                       paired_last(c, l) ? 1 : 0);
     CNN_HIP(hipGetLastError());
   }
 }
