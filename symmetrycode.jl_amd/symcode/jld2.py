@@ -17,7 +17,8 @@ artefacts need, from the HDF5 file-format specification (version 3.0):
 * superblock version 2 at byte 512 (base address 512; every other address
   is relative to it, the end-of-file address is absolute);
 * version-2 object headers (``OHDR``, one chunk, Jenkins lookup3 checksum;
-  continuation chunks ``OCHK`` are followed on read);
+  continuation chunks ``OCHK`` and shared messages, e.g. committed
+  datatypes, are followed on read);
 * groups in compact form: Link Info, Group Info and hard-link messages;
 * datasets with Dataspace (v2), Datatype, Fill Value (v3) and contiguous
   Data Layout (v3) messages; compact layouts are read as well;
@@ -444,8 +445,25 @@ class _Reader:
                         raise ValueError("continuation chunk signature")
                     chunks.append((cp + 4, clen - 8, cp))
                 elif mtype != 0:
-                    out.append((mtype, body))
+                    out.append((mtype, body) if not mflags & 0x02 else (mtype, self.shared(mtype, body)))
         return out
+
+    def shared(self, mtype, body):
+        """The body of a shared message (flag bit 1): a committed datatype
+        (JLD2 commits the datatypes of Julia structs such as Complex) or a
+        message kept in another object header; returns that header's message
+        of the same type."""
+        ver = body[0]
+        if ver == 1:
+            addr = struct.unpack_from("<Q", body, 8)[0]
+        elif ver == 2 or (ver == 3 and body[1] in (2, 3)):
+            addr = struct.unpack_from("<Q", body, 2)[0]
+        else:
+            raise ValueError("shared messages in the shared-message heap are not supported")
+        for t, b in self.messages(addr):
+            if t == mtype:
+                return b
+        raise ValueError("shared message target holds no message of its type")
 
     def links(self, addr):
         res = []
@@ -506,6 +524,8 @@ class _Reader:
                 return ("group", addr)
         if dt is None:
             return ("group", addr)
+        if layout is None:  # a committed (named) datatype
+            return ("datatype", dt)
         nd = _np_of_dt(dt)
         count = int(np.prod(shape)) if shape else 1
         if layout[0] == "compact":
@@ -547,7 +567,8 @@ class _Reader:
         return v
 
     def value(self, addr):
-        """a dataset's value, or a group as a dict of its members"""
+        """a dataset's value, a group as a dict of its members, a committed
+        datatype as ("datatype", description)"""
         v = self.dataset(addr)
         if isinstance(v, tuple) and v and v[0] == "group":
             return {k: self.value(a) for k, a in self.links(addr)}
@@ -570,8 +591,9 @@ def load(path, *names):
     (Julia-reversed) index order, NamedTuples and groups as dicts, vectors as
     lists (vectors of plain numbers as numpy arrays)."""
     r = _Reader(path)
-    if not names:
-        return {k: r.value(a) for k, a in r.links(r.root)}
+    if not names:  # JLD2's _types group (committed Julia types) is not data
+        out = {k: r.value(a) for k, a in r.links(r.root) if k != "_types"}
+        return {k: v for k, v in out.items() if not (isinstance(v, tuple) and v and v[0] == "datatype")}
     vals = tuple(r.value(r.lookup(path, n)) for n in names)
     return vals[0] if len(vals) == 1 else vals
 

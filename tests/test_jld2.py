@@ -261,3 +261,29 @@ def test_h5dump_parses_our_files(tmp_path):
     # values through the library: the scalar visc
     r = subprocess.run([H5DUMP, "-d", "/visc", p], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "0.00025" in r.stdout
+
+
+WRITE_COMMITTED = r"""
+import sys, h5py, numpy as np
+with h5py.File(sys.argv[1], "w", libver="latest", userblock_size=512) as f:
+    f["ComplexF32"] = np.dtype([("re", "<f4"), ("im", "<f4")])  # a committed (named) datatype
+    a = np.zeros(6, dtype=f["ComplexF32"].dtype)
+    a["re"] = np.arange(6); a["im"] = -np.arange(6)
+    f.create_dataset("z", data=a, dtype=f["ComplexF32"])
+    rec = np.dtype([("e", "<f8"), ("n", "<i8")])
+    f["Stats"] = rec
+    r = np.zeros(2, dtype=rec); r["e"] = [0.5, 1.5]; r["n"] = [3, 4]
+    f.create_dataset("stats", data=r, dtype=f["Stats"])
+"""
+
+
+@needs_h5py
+def test_reader_follows_committed_datatypes(tmp_path):
+    """datasets whose datatype is a committed (named) one, as JLD2 writes the
+    element types of Julia structs: the datatype message is shared"""
+    p = str(tmp_path / "c.h5")
+    _h5py(WRITE_COMMITTED, p)
+    z = jld2.load(p, "z")
+    assert z.dtype == np.complex64 and np.array_equal(z, np.arange(6) - 1j * np.arange(6))
+    assert jld2.load(p, "stats") == [{"e": 0.5, "n": 3}, {"e": 1.5, "n": 4}]
+    assert sorted(jld2.load(p)) == ["stats", "z"]  # the committed datatypes are not data
