@@ -15,8 +15,8 @@
 #ifndef SC_TUNE_TILED
 #define SC_TUNE_TILED 1
 #endif
-// place the pipeline buffers of a large grid in scattered 1 GiB pieces of
-// device memory (Ctx::scattered_allocs)
+// place the pipeline buffers of a large grid in scattered pieces of device
+// memory, SC_SCATTER_PIECE_MB each (Ctx::scattered_allocs)
 #ifndef SC_SCATTER
 #define SC_SCATTER 1
 #endif
