@@ -155,7 +155,12 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
   constexpr int NPH = BYPLANE ? 3 : 1;
   constexpr int NCH = NPL * RW * XW * CH8;
   constexpr int NPT = (NCH + CONV_THREADS - 1) / CONV_THREADS;
-  __shared__ __attribute__((aligned(16))) bf16 win[NPL * RW * CH8 * XWP * 8];
+  // HIDDEN and BWD_HIDDEN with SPLIT: the epilogue passes the block's
+  // outputs through the same LDS (two row pairs x 128 voxels x 64 channels)
+  constexpr bool LINES = (MODE == HIDDEN || MODE == BWD_HIDDEN) && SPLIT;
+  constexpr int WIN = NPL * RW * CH8 * XWP * 8;
+  constexpr int EPI = LINES ? 2 * 128 * 64 : 0;
+  __shared__ __attribute__((aligned(16))) bf16 win[WIN > EPI ? WIN : EPI];
   const int segs = n / VPB, ytiles = n / TROWS;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
@@ -292,6 +297,82 @@ __global__ void __launch_bounds__(conv_threads(COUTT), 8 / (conv_threads(COUTT) 
       CONV_COMMIT()
       __syncthreads();
     }
+  }
+  if constexpr (LINES) {
+    // The block's 4 rows x 64 voxels x 64 channels go through LDS so that
+    // every store of the epilogue is a whole 128-byte voxel line (8 lanes x
+    // 16 bytes); from the MFMA layout each lane would store 8 bytes, a
+    // quarter line per 16 lanes.  HIDDEN: z, then gelu(z).  BWD_HIDDEN:
+    // conv * gelu'(zprev), zprev read in the MFMA layout (loading its lines
+    // through LDS as well, after the last k step, was slower: the barrier
+    // then waits for the loads).  Row pair wr / 2: voxel vloc = 64 (row - wr) + x of 128, a
+    // line of 8 chunks of 8 channels, chunk q at position q ^ (vloc & 7)
+    static_assert(TROWS == 4 && NT == 2 && MT == 8 && COUTS == 64, "block epilogue shape");
+    uint2* ew = reinterpret_cast<uint2*>(win) + (w / 2) * 128 * 16;
+    uint4* el = reinterpret_cast<uint4*>(win) + (w / 2) * 128 * 8;
+    float bco[NT][4] = {};
+    if constexpr (MODE == HIDDEN) {
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bco[nt][r] = bias[(wn + nt) * 16 + 4 * lk + r];
+    }
+    // this lane's 8 bytes of (M tile mt, output tile nt)
+    auto slot = [&](int mt, int nt) __attribute__((always_inline)) {
+      const int vloc = 64 * (mt / 4) + 16 * (mt % 4) + li;
+      const int q = ((wn + nt) * 2 + lk / 2) ^ (vloc & 7);
+      return vloc * 16 + q * 2 + (lk & 1);
+    };
+    // the lane's first voxel in the MFMA layout (row wr, x li)
+    const unsigned v0w = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + wr) * unsigned(n) + cur.x0 + li;
+    auto put = [&](int act) __attribute__((always_inline)) {  // 0: z, 1: gelu(z), 2: conv * gelu'(zprev)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int e = slot(mt, nt);
+          bf16x4 o, zp;
+          if (act == 2) {
+            const unsigned v = v0w + (mt / 4) * unsigned(n) + 16 * (mt % 4);
+            zp = *reinterpret_cast<const bf16x4*>(&zprev[v * 64u + (wn + nt) * 16 + 4 * lk]);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float d = acc[mt][nt][r];
+            const float z = d + bco[nt][r];
+            o[r] = bf16(act == 2 ? d * gelu_grad(float(zp[r])) : act == 1 ? gelu(z) : z);
+          }
+          uint2 u;
+          __builtin_memcpy(&u, &o, 8);
+          ew[e] = u;
+        }
+    };
+    // wavefront w % 2 of the pair moves the lines of row wr + w % 2, 8 voxels per instruction
+    const int rr = w % 2, p = l % 8;
+    const unsigned v0 = ((unsigned(cur.b) * n + cur.z) * n + cur.y0 + wr + rr) * unsigned(n) + cur.x0;
+    auto drain = [&](bf16* out) __attribute__((always_inline)) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int x = 8 * j + l / 8, vloc = 64 * rr + x;
+        *reinterpret_cast<uint4*>(&out[(v0 + x) * 64u + p * 8]) = el[vloc * 8 + (p ^ (vloc & 7))];
+      }
+    };
+    if constexpr (MODE == HIDDEN) {
+      __syncthreads();  // every wavefront is done with the staged input
+      put(0);
+      __syncthreads();
+      drain(out_z);
+      __syncthreads();
+      put(1);
+      __syncthreads();
+      drain(out_a);
+    } else {
+      __syncthreads();  // every wavefront is done with the staged input
+      put(2);
+      __syncthreads();
+      drain(out_a);
+    }
+    return;
   }
   // epilogue: with the weights as the MFMA's A operand, lane (li, lk) holds
   // D[co = 16 nt + 4 lk + r][voxel = 16 mt + li]: four consecutive channels of
