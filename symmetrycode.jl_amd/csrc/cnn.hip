@@ -17,6 +17,7 @@
 // N = output channels, K = 27 taps x input channels.  The 9 (dz, dy) input
 // rows of a 64-voxel output segment (with one halo voxel each side) are
 // staged in LDS; weights stream from L2.
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cfloat>
@@ -776,18 +777,25 @@ __global__ void __launch_bounds__(512, 1) k_conv_wgrad(const bf16* __restrict__ 
 //    fragments, each against the 4 output tiles, and tap s = 8 (lanes 0-7;
 //    lanes 8-15 repeat it and their rows are dropped) against output tile w:
 //    15 MFMAs per 32 voxels.
+//    C4 (at most 4 real input channels): 16-row tiles of 4 taps x 4
+//    channels; compute wavefront w owns the taps s = 4 (w & 1) .. + 3 (lane
+//    group li / 4 one tap, lane li % 4 its channel) against output tiles
+//    2 (w / 2), + 1, and tap 8 against output tile w: 9 MFMAs per 32 voxels.
+//    The row slot and plane strides are 64 and 192 bytes modulo 256, so the
+//    four taps a 16-lane group reads lie in four distinct 64-byte quarters.
 //  otherwise (CIN = 64, COUT = 16 or 8): the input plane z and the dout planes
 //    z + 1, z, z - 1 (dz = -1, 0, 1) are staged; compute wavefront w owns
 //    input tile w and all 27 taps.  COUT = 16: 27 MFMAs per 32 voxels.
 //    COUT = 8 (the 6-channel output layer): the dout planes dz = -1, 0 share
 //    one 16-row tile (channels 0-7 and 8-15) and dz = 1 has a tile of its
 //    own with channels 8-15 zero, so 18 MFMAs per 32 voxels cover all taps.
-template <int CIN, int COUT, int PD>
+template <int CIN, int COUT, int PD, bool C4 = false>
 __global__ void __launch_bounds__(512, 1)
     k_conv_wgrad3(const bf16* __restrict__ in, const bf16* __restrict__ dout, float* part, float* db, int n,
                   long nplanes) {
   constexpr bool INPACK = CIN <= 8;
   constexpr bool PACKO = !INPACK && COUT == 8;  // two dout planes per 16-row tile
+  static_assert(!C4 || INPACK, "C4: the input layer");
   constexpr int NCWV = 4;  // compute wavefronts
   constexpr int NBM = 16;               // x blocks per row (n <= 128)
   constexpr int CP = CIN < 16 ? 16 : CIN;
@@ -803,8 +811,8 @@ __global__ void __launch_bounds__(512, 1)
   // modulo 256, so the two taps a 16-lane group reads (plane + row offset of
   // opposite parity) fill the two halves of one 256-byte window
   constexpr int DTS = NBM * 16 + 1;
-  constexpr int XTS = INPACK ? (NBM + 2) * 16 + 8 : (NBM + 2) * 16 + 1;
-  constexpr int XPS = INPACK ? 4 * XTS + 8 : 4 * NW * XTS;  // input plane stride
+  constexpr int XTS = C4 ? (NBM + 2) * 16 + 4 : INPACK ? (NBM + 2) * 16 + 8 : (NBM + 2) * 16 + 1;
+  constexpr int XPS = C4 ? 4 * XTS + 12 : INPACK ? 4 * XTS + 8 : 4 * NW * XTS;  // input plane stride
   static_assert(NPO * 16 * (COUT / 8) + NPI * 16 * (CIN / 8) <= 256, "staging jobs");
   static_assert(INPACK ? (MT == 4 && NW == 1) : (MT == 1 && NW == 4), "wgrad3 shapes");
   static_assert(COUT % 8 == 0, "dout blocks of 8 channels");
@@ -815,7 +823,7 @@ __global__ void __launch_bounds__(512, 1)
   const int li = l & 15, lk = l >> 4;
   const bool producer = w >= NCWV;
   const int pt = int(threadIdx.x) - 64 * NCWV;  // producer thread index
-  constexpr int NACC = INPACK ? 3 * MT + 3 : 9 * NA;
+  constexpr int NACC = C4 ? 9 : INPACK ? 3 * MT + 3 : 9 * NA;
   floatx4 acc[NACC];
 #pragma unroll
   for (int i = 0; i < NACC; ++i) acc[i] = floatx4{0, 0, 0, 0};
@@ -912,14 +920,59 @@ __global__ void __launch_bounds__(512, 1)
   };
   // this lane's rows: INPACK: tap s = min(2 w + li / 8, 8) (plane dz + 1 = s / 3,
   // row y - 1 + s % 3), channel li % 8; otherwise the input tile w rows
-  const int sl_tap = INPACK ? 2 * w + li / 8 : 0;
+  const int sl_tap = C4 ? 4 * (w & 1) + li / 4 : INPACK ? 2 * w + li / 8 : 0;
   // one row: the x steps in pairs (n is a multiple of 64), every operand
   // chunk read one step ahead, each step's reads kept at its top
   // (sched_barrier) so that its MFMAs wait only for reads issued a step
   // earlier; the read after the last step repeats the last step's (in range)
   auto compute_row = [&](int y) __attribute__((always_inline)) {
     const u32x4* dr = &dTv[(y & 1) * MT * DTS];  // dout row y (slot y & 1), output tile 0
-    if constexpr (INPACK) {
+    if constexpr (C4) {
+      const int ci = li & 3, m0 = 2 * (w >> 1);
+      const u32x4* xr = &xTv[(sl_tap / 3) * XPS + ((y + sl_tap % 3) & 3) * XTS];
+      const u32x4* xr8 = &xTv[2 * XPS + ((y + 2) & 3) * XTS];  // tap 8: plane z + 1, row y + 1
+      struct Ops {
+        u32x4 B[2];  // output tiles m0, m0 + 1 (tile w is B[w & 1])
+        u32x4 X[3], X8[3];
+      };
+      auto read_ops = [&](Ops& o, int k0) __attribute__((always_inline)) {
+        const int c = chunk(k0 / 8 + lk, li);
+        o.B[0] = dr[m0 * DTS + c];
+        o.B[1] = dr[(m0 + 1) * DTS + c];
+        read_x(o.X, xr, k0, ci);
+        read_x(o.X8, xr8, k0, ci);
+      };
+      // M8 = w & 1 as a constant: tile w is B[M8] (a run-time index would
+      // move the operand sets to memory)
+      auto step = [&](const Ops& o, auto M8) __attribute__((always_inline)) {
+        u32x4 S[3], S8[3];
+        shifts(o.X, S);
+        shifts(o.X8, S8);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+#pragma unroll
+          for (int m = 0; m < 2; ++m) mfma(S[d], o.B[m], acc[d * 2 + m]);
+          mfma(S8[d], o.B[decltype(M8)::value], acc[6 + d]);  // tap 8 against output tile w
+        }
+      };
+      auto run = [&](auto M8) __attribute__((always_inline)) {
+        Ops o0, o1;
+        read_ops(o0, 0);
+#pragma unroll 1
+        for (int k0 = 0; k0 < n; k0 += 64) {
+          __builtin_amdgcn_sched_barrier(0);
+          read_ops(o1, k0 + 32);
+          step(o0, M8);
+          __builtin_amdgcn_sched_barrier(0);
+          read_ops(o0, min(k0 + 64, n - 32));
+          step(o1, M8);
+        }
+      };
+      if (w & 1)
+        run(std::integral_constant<int, 1>{});
+      else
+        run(std::integral_constant<int, 0>{});
+    } else if constexpr (INPACK) {
       const int ci = li & 7;
       const u32x4* xr = &xTv[(sl_tap / 3) * XPS + ((y + sl_tap % 3) & 3) * XTS];
       const u32x4* xr8 = &xTv[2 * XPS + ((y + 2) & 3) * XTS];  // tap 8: plane z + 1, row y + 1
@@ -1057,7 +1110,26 @@ __global__ void __launch_bounds__(512, 1)
   }
   if (producer) return;
   const long pbase = long(blockIdx.x) * 27;
-  if constexpr (INPACK) {
+  if constexpr (C4) {
+    // D[row = 4 lk + r][co = 16 (m0 + m) + li]: tap 4 (w & 1) + lk, channel r;
+    // tap 8 in lane group lk = 0.  Channels 4-7 of every tap get zeros.
+    const int m0 = 2 * (w >> 1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const int t = (4 * (w & 1) + lk) * 3 + d;  // (dz + 1) * 9 + (dy + 1) * 3 + dx + 1
+        float* dst = &part[((pbase + t) * COUT + 16 * (m0 + m) + li) * CIN];
+        *reinterpret_cast<floatx4*>(dst) = acc[d * 2 + m];
+        *reinterpret_cast<floatx4*>(dst + 4) = floatx4{0, 0, 0, 0};
+      }
+      if (lk == 0) {
+        float* dst = &part[((pbase + 24 + d) * COUT + 16 * w + li) * CIN];
+        *reinterpret_cast<floatx4*>(dst) = acc[6 + d];
+        *reinterpret_cast<floatx4*>(dst + 4) = floatx4{0, 0, 0, 0};
+      }
+    }
+  } else if constexpr (INPACK) {
     // D[row = 4 lk + r][co = 16 m + li]; row -> (tap 2 w + row / 8, channel row % 8)
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -1403,6 +1475,9 @@ void launch_wgrad(sc_cnn* c, int cinp, int coutp, const bf16* in, const bf16* do
   if (cinp == 64 && coutp == 64) {
     hipLaunchKernelGGL((k_conv_wgrad<64, 64, SC_WG_PD>), dim3(unsigned(3 * G)), dim3(512), 0, c->stream, in, dout,
                        c->part, db, c->n, nplanes);
+  } else if (cinp == 8 && coutp == 64 && c->cin <= 4) {
+    hipLaunchKernelGGL((k_conv_wgrad3<8, 64, SC_WG3_PD, true>), dim3(unsigned(G)), dim3(64 * 8), 0, c->stream, in,
+                       dout, c->part, db, c->n, nplanes);
   } else if (cinp == 8 && coutp == 64) {
     hipLaunchKernelGGL((k_conv_wgrad3<8, 64, SC_WG3_PD>), dim3(unsigned(G)), dim3(64 * 8), 0, c->stream, in, dout,
                        c->part, db, c->n, nplanes);

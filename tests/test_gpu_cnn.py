@@ -126,19 +126,22 @@ def test_cnn_parity_n128():
     assert rel(dx.cpu().numpy(), dxo.numpy()) < 3e-2
 
 
-def test_cnn_parity_wide_output():
+@pytest.mark.parametrize("cin", [3, 7])
+def test_cnn_parity_wide_output(cin):
     """12 output channels (16-row output tile, the weight gradient without
-    plane packing) and 3 input channels: forward and gradients"""
+    plane packing) and 3 or 7 input channels (the input layer's weight
+    gradient with 4 taps x 4 channels per MFMA tile, or 2 taps x 8):
+    forward and gradients"""
     import symcode as sc
 
-    print_case("cnn parity cout12")
-    net = sc.cnn.ConvClosureNet(64, batch_max=1, cin=3, width=64, cout=12, seed=13)
+    print_case(f"cnn parity cout12 cin{cin}")
+    net = sc.cnn.ConvClosureNet(64, batch_max=1, cin=cin, width=64, cout=12, seed=13)
     n = net.n
     assert (net.cin_p, net.cout_p) == (8, 16)
     Ws, bs = net.unflatten(net.get_params())
     r = np.random.default_rng(14)
     x = np.zeros((1, net.nvox, net.cin_p), np.float32)
-    x[..., :3] = r.standard_normal((1, net.nvox, 3))
+    x[..., :cin] = r.standard_normal((1, net.nvox, cin))
     xt = torch.tensor(x).to(torch.bfloat16).cuda()
     y = net.forward(xt)
     yo, store = OC.forward(Ws, bs, torch.tensor(x), n)
