@@ -72,13 +72,13 @@ constexpr int VPB = 64;   // output voxels per x segment (n must be a multiple)
 #ifndef SC_CNN_TROWS
 #define SC_CNN_TROWS 4
 #endif
-// output y rows per block: for 64 output channels 4 rows (4 wavefronts, 57 KB
-// of LDS for a 64-channel input plane) let two blocks share a compute unit,
-// so one block's plane fetch, plane commits and epilogue overlap the other's
-// MFMAs (with 8 rows, 95 KB, a unit holds one block and those stalls are
-// exposed); the 16-column output layers, short on MFMAs, keep 8 rows and
-// the smaller y halo (10 staged rows per 8 instead of 6 per 4)
-constexpr int conv_trows(int coutt) { return coutt == 16 ? 8 : SC_CNN_TROWS; }
+// output y rows per block: 4 rows (4 wavefronts, 57-61 KB of LDS for a
+// 64-channel input plane) let two blocks share a compute unit, so one
+// block's plane fetch, plane commits and epilogue overlap the other's MFMAs
+// (with 8 rows, 95-102 KB, a unit holds one block and those stalls are
+// exposed: the 16-column last layer took 2.99 against 2.78 ms with 8 rows,
+// for all its smaller y halo)
+constexpr int conv_trows(int) { return SC_CNN_TROWS; }
 // y rows per wavefront.  RPW = 2 halves the weight-fragment traffic per flop
 // but leaves one wavefront per SIMD (the LDS tile allows one block per CU):
 // batch-16 x 128^3 training steps took 87.8 ms against 80.3 ms with RPW = 1
@@ -109,7 +109,7 @@ constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 
 // One convolution layer (or its data gradient) as an implicit GEMM with the
 // reduction index k = t * CIN + c (tap-major; narrow layers pack several taps
-// into one 32-wide MFMA step).  Block: 8 consecutive y rows of one z plane,
+// into one 32-wide MFMA step).  Block: 4 consecutive y rows of one z plane,
 // 64 x voxels each, RPW rows per wavefront; a wavefront holds its
 // (64 RPW)-voxel x COUTT output tile in 4 RPW x (COUTT/16) MFMA accumulators,
 // so every B fragment feeds 4 RPW MFMAs and every A fragment COUTT/16 (with
