@@ -1191,7 +1191,33 @@ __global__ void k_mse(const float* y, const float* tgt, bf16* dy, float* dyf, do
   __shared__ double sm[256];
   long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
   double acc = 0;
-  if (v < nvox) {
+  const bool vec = cp == 8 && nout == 6 && reinterpret_cast<uintptr_t>(y) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(tgt) % 8 == 0 && reinterpret_cast<uintptr_t>(dy) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(dyf) % 16 == 0;
+  if (v < nvox && vec) {
+    // the network's 6-channel output: a voxel's record in 16-, 8- and
+    // 16-byte accesses (per-channel accesses would touch 4 or 2 bytes of a
+    // 32-, 24- or 16-byte record per lane); the same arithmetic in the same order
+    const float4 y0 = reinterpret_cast<const float4*>(y + v * 8)[0];
+    const float4 y1 = reinterpret_cast<const float4*>(y + v * 8)[1];
+    const float2* tv = reinterpret_cast<const float2*>(tgt + v * 6);
+    const float2 t0 = tv[0], t1 = tv[1], t2 = tv[2];
+    const float e[6] = {y0.x - t0.x, y0.y - t0.y, y0.z - t1.x, y0.w - t1.y, y1.x - t2.x, y1.y - t2.y};
+    bf16x8 o;
+    float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      acc += double(e[c]) * e[c];
+      g[c] = float(2.0 * inv) * e[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o[c] = bf16(g[c]);
+    *reinterpret_cast<bf16x8*>(dy + v * 8) = o;
+    if (dyf) {
+      reinterpret_cast<float4*>(dyf + v * 8)[0] = float4{g[0], g[1], g[2], g[3]};
+      reinterpret_cast<float4*>(dyf + v * 8)[1] = float4{g[4], g[5], g[6], g[7]};
+    }
+  } else if (v < nvox) {
     for (int c = 0; c < cp; ++c) {
       float g = 0.f;
       if (c < nout) {
@@ -1216,11 +1242,22 @@ __global__ void k_mse(const float* y, const float* tgt, bf16* dy, float* dyf, do
 // sum of squares of x[0, len) in block partials (FP64)
 __global__ void k_sumsq(const float* x, long len, double* part) {
   __shared__ double sm[256];
-  double acc = 0;
-  for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += long(gridDim.x) * blockDim.x) {
-    const double e = x[i];
-    acc += e * e;
+  // 16-byte loads into four independent sums where x is 16-byte aligned
+  const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x, stride = long(gridDim.x) * blockDim.x;
+  const long n4 = reinterpret_cast<uintptr_t>(x) % 16 == 0 ? len / 4 : 0;
+  double a[4] = {0, 0, 0, 0};
+  for (long i = gid; i < n4; i += stride) {
+    const float4 q = reinterpret_cast<const float4*>(x)[i];
+    a[0] += double(q.x) * q.x;
+    a[1] += double(q.y) * q.y;
+    a[2] += double(q.z) * q.z;
+    a[3] += double(q.w) * q.w;
   }
+  for (long i = 4 * n4 + gid; i < len; i += stride) {
+    const double e = x[i];
+    a[0] += e * e;
+  }
+  double acc = (a[0] + a[1]) + (a[2] + a[3]);
   sm[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
